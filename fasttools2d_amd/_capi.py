@@ -1,0 +1,84 @@
+"""ctypes binding to ``libse2p.so`` (the C ABI declared in ``include/se2p.h``).
+
+The library is built in-tree into ``fasttools2d_amd/lib/``.  Loading fails
+loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libse2p.so")
+CSRC = os.path.join(_HERE, "csrc")
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_P = ctypes.c_void_p
+_PP = ctypes.POINTER(ctypes.c_void_p)
+
+# status codes (include/se2p.h)
+SE2P_OK = 0
+SE2P_ERR_ARG = 1
+SE2P_ERR_GRID = 2
+SE2P_ERR_P = 3
+SE2P_ERR_W = 4
+SE2P_ERR_NSIDE = 5
+SE2P_ERR_PARAM = 6
+SE2P_ERR_NOCONV = 7
+SE2P_ERR_HIP = 10
+SE2P_ERR_FFT = 11
+
+# every symbol include/se2p.h declares, with (restype, argtypes)
+SIGNATURES = {
+    "se2p_create": (_I, [_PP, _I]),
+    "se2p_destroy": (_I, [_P]),
+    "se2p_last_error": (ctypes.c_char_p, [_P]),
+    "se2p_set_stream": (_I, [_P, _P]),
+    "se2p_get_stream": (_P, [_P]),
+    "se2p_set_async": (_I, [_P, _I]),
+    "se2p_set_timing": (_I, [_P, _I]),
+    "se2p_stage_times": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
+    "se2p_slp_real": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _I, _I, _D, _D, _P]),
+    "se2p_slp_kspace": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _I, _I, _D, _D, _D, _I, _P]),
+    "se2p_dlp_real": (_I, [_P, _P, _I64, _P, _I64, _P, _P, _D, _I, _I, _D, _D, _P]),
+    "se2p_dlp_kspace": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _P, _I, _I, _D, _D, _D, _I, _P]),
+    "se2p_count_pairs": (_I, [_P, _P, _I64, _P, _I64, _I, _I, _D, _D,
+                              ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+}
+
+STAGES = ("sort", "real", "spread", "fft_fwd", "filter", "fft_inv", "interp")
+
+_lib = None
+
+
+class Se2pError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"se2p error {code}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+def load():
+    """Load libse2p.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C {CSRC}` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def build() -> None:
+    import subprocess
+
+    subprocess.run(["make", "-s", "-j8", "-C", CSRC], check=True)

@@ -1,0 +1,281 @@
+// se2p_api.cpp -- the C ABI (include/se2p.h): contexts, argument validation
+// and the four MEX-equivalent entry points.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+#include "se2p_internal.h"
+
+namespace se2p {
+
+int fail(se2p_ctx *ctx, int code, const char *fmt, ...)
+{
+    if (ctx) {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int ensure_workspace(se2p_ctx *ctx, size_t bytes)
+{
+    if (bytes <= ctx->ws_bytes) return 0;
+    if (ctx->ws) {
+        SE2P_HIP(hipStreamSynchronize(ctx->stream));
+        SE2P_HIP(hipFree(ctx->ws));
+        ctx->ws = nullptr;
+        ctx->ws_bytes = 0;
+    }
+    size_t grow = bytes + bytes / 4;
+    SE2P_HIP(hipMalloc((void **)&ctx->ws, grow));
+    ctx->ws_bytes = grow;
+    return 0;
+}
+
+void stage_begin(se2p_ctx *ctx, Stage s)
+{
+    if (!ctx->timing) return;
+    if (!ctx->ev_used[s]) {
+        hipEventRecord(ctx->ev[2 * s], ctx->stream);
+        ctx->ev_used[s] = true;
+    }
+}
+
+void stage_end(se2p_ctx *ctx, Stage s)
+{
+    if (!ctx->timing) return;
+    hipEventRecord(ctx->ev[2 * s + 1], ctx->stream);
+}
+
+int finish(se2p_ctx *ctx)
+{
+    SE2P_HIP(hipGetLastError());
+    if (!ctx->async || ctx->timing) SE2P_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->timing) {
+        for (int s = 0; s < ST_COUNT; ++s) {
+            ctx->stage_ms[s] = -1.0f;
+            if (ctx->ev_used[s]) {
+                float ms = 0.0f;
+                hipEventElapsedTime(&ms, ctx->ev[2 * s], ctx->ev[2 * s + 1]);
+                ctx->stage_ms[s] = ms;
+            }
+        }
+    }
+    return SE2P_OK;
+}
+
+static void begin_call(se2p_ctx *ctx)
+{
+    ctx->err.clear();
+    for (int s = 0; s < ST_COUNT; ++s) ctx->ev_used[s] = false;
+}
+
+static int set_device(se2p_ctx *ctx)
+{
+    SE2P_HIP(hipSetDevice(ctx->device));
+    return 0;
+}
+
+static int check_common(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                        int64_t nt, const double *u)
+{
+    if (ns < 0 || nt < 0) return fail(ctx, SE2P_ERR_ARG, "negative point count");
+    if (ns > 0x7fffffffLL || nt > 0x7fffffffLL)
+        return fail(ctx, SE2P_ERR_ARG, "point counts above 2^31-1 are not supported");
+    if ((ns > 0 && !psrc) || (nt > 0 && (!ptar || !u)))
+        return fail(ctx, SE2P_ERR_ARG, "null array pointer");
+    return 0;
+}
+
+static int check_real(se2p_ctx *ctx, double xi, int nsx, int nsy, double Lx, double Ly)
+{
+    if (nsx < 1 || nsy < 1) return fail(ctx, SE2P_ERR_NSIDE, "nside_x and nside_y must be >= 1");
+    if ((long long)nsx * nsy > (1LL << 28)) return fail(ctx, SE2P_ERR_NSIDE, "too many cells");
+    if (!(xi > 0) || !(Lx > 0) || !(Ly > 0) || !std::isfinite(xi) || !std::isfinite(Lx) ||
+        !std::isfinite(Ly))
+        return fail(ctx, SE2P_ERR_PARAM, "xi, Lx and Ly must be positive and finite");
+    return 0;
+}
+
+}  // namespace se2p
+
+using namespace se2p;
+
+extern "C" {
+
+int se2p_create(se2p_ctx **out, int device)
+{
+    if (!out) return SE2P_ERR_ARG;
+    *out = nullptr;
+    static std::once_flag fft_once;
+    std::call_once(fft_once, [] { rocfft_setup(); });
+    se2p_ctx *ctx = new se2p_ctx();
+    ctx->device = device;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        delete ctx;
+        return SE2P_ERR_HIP;
+    }
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return SE2P_ERR_HIP;
+    }
+    ctx->own_stream = true;
+    for (int i = 0; i < 2 * ST_COUNT; ++i) hipEventCreate(&ctx->ev[i]);
+    *out = ctx;
+    return SE2P_OK;
+}
+
+int se2p_destroy(se2p_ctx *ctx)
+{
+    if (!ctx) return SE2P_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->plans) {
+        FftPlans &p = kv.second;
+        if (p.fwd) rocfft_plan_destroy(p.fwd);
+        if (p.inv) rocfft_plan_destroy(p.inv);
+        if (p.fwd_info) rocfft_execution_info_destroy(p.fwd_info);
+        if (p.inv_info) rocfft_execution_info_destroy(p.inv_info);
+        if (p.work) hipFree(p.work);
+    }
+    if (ctx->ws) hipFree(ctx->ws);
+    for (int i = 0; i < 2 * ST_COUNT; ++i)
+        if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
+    if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SE2P_OK;
+}
+
+const char *se2p_last_error(const se2p_ctx *ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+int se2p_set_stream(se2p_ctx *ctx, void *s)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    if (ctx->own_stream && ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (s) {
+        if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
+        ctx->stream = (hipStream_t)s;
+        ctx->own_stream = false;
+    } else if (!ctx->own_stream) {
+        hipSetDevice(ctx->device);
+        if (hipStreamCreate(&ctx->stream) != hipSuccess) return SE2P_ERR_HIP;
+        ctx->own_stream = true;
+    }
+    return SE2P_OK;
+}
+
+void *se2p_get_stream(se2p_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int se2p_set_async(se2p_ctx *ctx, int a)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    ctx->async = a != 0;
+    return SE2P_OK;
+}
+
+int se2p_set_timing(se2p_ctx *ctx, int enable)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    ctx->timing = enable != 0;
+    return SE2P_OK;
+}
+
+int se2p_stage_times(se2p_ctx *ctx, float *ms, int n)
+{
+    if (!ctx || !ms) return SE2P_ERR_ARG;
+    for (int i = 0; i < n && i < ST_COUNT; ++i) ms[i] = ctx->timing ? ctx->stage_ms[i] : -1.0f;
+    return SE2P_OK;
+}
+
+int se2p_slp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                  const double *f, double xi, int nside_x, int nside_y, double Lx, double Ly,
+                  double *u)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
+    if (ns > 0 && !f) return fail(ctx, SE2P_ERR_ARG, "null density pointer");
+    if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
+    if (int rc = real_sum(ctx, 0, g, f, nullptr, 1, 2 * ns, 2 * nt, u, nullptr)) return rc;
+    return finish(ctx);
+}
+
+int se2p_dlp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                  const double *f, const double *n, double xi, int nside_x, int nside_y,
+                  double Lx, double Ly, double *u)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
+    if (ns > 0 && (!f || !n)) return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
+    if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
+    if (int rc = real_sum(ctx, 1, g, f, n, 1, 2 * ns, 2 * nt, u, nullptr)) return rc;
+    return finish(ctx);
+}
+
+int se2p_slp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                    double xi, double eta, const double *f, int Mx, int My, double Lx, double Ly,
+                    double w, int P, double *u)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
+    if (ns > 0 && !f) return fail(ctx, SE2P_ERR_ARG, "null density pointer");
+    KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
+    if (int rc = check_kspace_params(ctx, g)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    if (int rc = kspace_sum(ctx, 0, g, f, nullptr, 1, 2 * ns, 2 * nt, u)) return rc;
+    return finish(ctx);
+}
+
+int se2p_dlp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                    double xi, double eta, const double *f, const double *n, int Mx, int My,
+                    double Lx, double Ly, double w, int P, double *u)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
+    if (ns > 0 && (!f || !n)) return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
+    KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
+    if (int rc = check_kspace_params(ctx, g)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    if (int rc = kspace_sum(ctx, 1, g, f, n, 1, 2 * ns, 2 * nt, u)) return rc;
+    return finish(ctx);
+}
+
+int se2p_count_pairs(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                     int nside_x, int nside_y, double Lx, double Ly, int64_t *pairs_in,
+                     int64_t *pairs_out)
+{
+    if (!ctx || !pairs_in || !pairs_out) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    double dummy_u = 0.0;
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, nt > 0 ? &dummy_u : nullptr)) return rc;
+    if (int rc = check_real(ctx, 1.0, nside_x, nside_y, Lx, Ly)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    *pairs_in = *pairs_out = 0;
+    if (nt == 0) return SE2P_OK;
+    RealGeom g{psrc, ptar, ns, nt, 1.0, Lx, Ly, nside_x, nside_y};
+    int64_t counts[2] = {0, 0};
+    // densities are not read when counting; pass the positions as a stand-in
+    if (int rc = real_sum(ctx, 1, g, psrc, psrc, 1, 2 * ns, 2 * nt, nullptr, counts)) return rc;
+    *pairs_in = counts[0];
+    *pairs_out = counts[1];
+    return finish(ctx);
+}
+
+}  // extern "C"

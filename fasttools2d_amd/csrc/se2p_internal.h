@@ -1,0 +1,118 @@
+// se2p_internal.h -- context, workspace and launch helpers shared by the
+// spectral Ewald HIP sources.  Not part of the public C ABI (include/se2p.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/se2p.h"
+
+namespace se2p {
+
+// Number of timed stages recorded when timing is enabled.
+enum Stage {
+    ST_SORT = 0,      // cell lists / gridding bins
+    ST_REAL,          // real-space pair sum
+    ST_SPREAD,        // Gaussian spreading
+    ST_FFT_FWD,       // forward transforms
+    ST_FILTER,        // k-space filter
+    ST_FFT_INV,       // inverse transforms
+    ST_INTERP,        // Gaussian interpolation
+    ST_COUNT
+};
+
+struct FftPlans {
+    rocfft_plan fwd = nullptr;
+    rocfft_plan inv = nullptr;
+    rocfft_execution_info fwd_info = nullptr;
+    rocfft_execution_info inv_info = nullptr;
+    size_t fwd_work = 0, inv_work = 0;
+    void *work = nullptr;   // shared work buffer (hipMalloc, freed with the context)
+};
+
+}  // namespace se2p
+
+struct se2p_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool async = false;
+    bool timing = false;
+    char *ws = nullptr;       // device workspace
+    size_t ws_bytes = 0;
+    std::string err;
+    std::map<std::tuple<int, int, int, int>, se2p::FftPlans> plans;  // (Mx, My, nfwd, ninv)
+    hipEvent_t ev[2 * se2p::ST_COUNT] = {};
+    bool ev_used[se2p::ST_COUNT] = {};
+    float stage_ms[se2p::ST_COUNT] = {};
+    int64_t pairs_in = -1, pairs_out = -1;   // counted by se2p_count_pairs
+};
+
+namespace se2p {
+
+int fail(se2p_ctx *ctx, int code, const char *fmt, ...);
+
+// Ensure the workspace holds at least `bytes`.
+int ensure_workspace(se2p_ctx *ctx, size_t bytes);
+
+// Timing helpers (no-ops unless ctx->timing).
+void stage_begin(se2p_ctx *ctx, Stage s);
+void stage_end(se2p_ctx *ctx, Stage s);
+
+int finish(se2p_ctx *ctx);   // error check + optional synchronisation
+
+// ---- cell lists / bins (se2p_sort.hip) ---------------------------------
+// Sorted order of points by an integer key in [0, nkeys): perm[k] = original
+// index of the k-th point, off[0..nkeys] = start of each key's run.  Stable.
+struct KeySort {
+    int *keys_in, *keys_out, *idx_in, *perm, *off;
+    void *temp;
+    size_t temp_bytes;
+};
+size_t keysort_bytes(int64_t n, int nkeys, size_t *temp_bytes);
+void keysort_carve(char *base, int64_t n, int nkeys, size_t temp_bytes, KeySort *ks);
+int keysort_run(se2p_ctx *ctx, KeySort *ks, int64_t n, int nkeys);
+
+// Keys for real-space cell lists (ewald_tools.cpp:28-35).
+void launch_cell_keys(hipStream_t st, const double *p, int64_t n, double Lx, double Ly,
+                      int nsx, int nsy, int *keys, int *idx);
+
+// ---- real space (se2p_real.hip) -----------------------------------------
+struct RealGeom {
+    const double *psrc, *ptar;
+    int64_t ns, nt;
+    double xi, Lx, Ly;
+    int nsx, nsy;
+};
+int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
+             int nrhs, int64_t dstride, int64_t ustride, double *u, int64_t *pair_counts);
+
+// ---- Fourier space (se2p_kspace.hip) ------------------------------------
+struct KGeom {
+    const double *psrc, *ptar;
+    int64_t ns, nt;
+    double xi, eta, Lx, Ly, w;
+    int Mx, My, P;
+};
+int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *n,
+               int nrhs, int64_t dstride, int64_t ustride, double *u);
+
+// Validation of the k-space preconditions (returns 0 or an SE2P_ERR code).
+int check_kspace_params(se2p_ctx *ctx, const KGeom &g);
+
+}  // namespace se2p
+
+#define SE2P_HIP(call)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return se2p::fail(ctx, SE2P_ERR_HIP, "%s failed: %s (%s:%d)", #call,    \
+                              hipGetErrorString(e_), __FILE__, __LINE__);           \
+    } while (0)

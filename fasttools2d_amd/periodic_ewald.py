@@ -1,0 +1,209 @@
+"""Host-side mirror of the reference's PeriodicEwald interface.
+
+The four ``mex_stokes_*`` functions take the same arguments, in the same
+order, as the reference MEX kernels and call the HIP implementation through
+the C ABI (``include/se2p.h``):
+
+* ``mex_stokes_slp_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly)``
+  -- PeriodicEwald/src/mex/StokesSLP/mex_stokes_slp_real.cpp:58-83
+* ``mex_stokes_slp_kspace(psrc, ptar, xi, eta, f, Mx, My, Lx, Ly, w, P)``
+  -- PeriodicEwald/src/mex/StokesSLP/mex_stokes_slp_kspace.cpp:8-42
+* ``mex_stokes_dlp_real(psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly)``
+  -- PeriodicEwald/src/mex/StokesDLP/mex_stokes_dlp_real.cpp:15-52
+* ``mex_stokes_dlp_kspace(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w, P)``
+  -- PeriodicEwald/src/mex/StokesDLP/mex_stokes_dlp_kspace.cpp:10-53
+
+Arrays: a MATLAB ``2 x N`` matrix is an ``(N, 2)`` float64 array here (the
+same interleaved memory).  Device tensors (``torch`` on ``cuda``) are used in
+place; numpy arrays or CPU tensors are copied to the current device and the
+result is returned as a device tensor.  Errors raise ``ValueError`` with the
+reference's message for the checks the MEX performs, and ``Se2pError`` for
+codes returned by the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _capi
+
+
+class Context:
+    """Owns an ``se2p_ctx`` (stream, workspace, FFT plans) on one device."""
+
+    def __init__(self, device: Optional[int] = None):
+        if device is None:
+            device = torch.cuda.current_device()
+        self.device = int(device)
+        self._lib = _capi.load()
+        h = ctypes.c_void_p()
+        rc = self._lib.se2p_create(ctypes.byref(h), self.device)
+        if rc != 0:
+            raise _capi.Se2pError(rc, f"se2p_create failed on device {self.device}")
+        self.handle = h
+        self._stream_bound = None
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.se2p_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stream / timing -----------------------------------------------------
+    def bind_torch_stream(self, stream: Optional[torch.cuda.Stream] = None) -> None:
+        """Run on torch's stream so inputs written by torch are ordered before
+        the library's kernels and outputs are visible to torch afterwards."""
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        ptr = stream.cuda_stream
+        if ptr == 0:
+            # torch is on the legacy null stream: make its pending writes
+            # visible, then keep the context's own stream
+            stream.synchronize()
+        if ptr != self._stream_bound:
+            self._check(self._lib.se2p_set_stream(self.handle, ctypes.c_void_p(ptr)))
+            self._stream_bound = ptr
+
+    def set_async(self, flag: bool) -> None:
+        self._check(self._lib.se2p_set_async(self.handle, 1 if flag else 0))
+
+    def set_timing(self, flag: bool) -> None:
+        self._check(self._lib.se2p_set_timing(self.handle, 1 if flag else 0))
+
+    def stage_times(self) -> dict:
+        buf = (ctypes.c_float * len(_capi.STAGES))()
+        self._check(self._lib.se2p_stage_times(self.handle, buf, len(_capi.STAGES)))
+        return {name: float(buf[i]) for i, name in enumerate(_capi.STAGES)}
+
+    def last_error(self) -> str:
+        return (self._lib.se2p_last_error(self.handle) or b"").decode()
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise _capi.Se2pError(rc, self.last_error())
+
+
+_contexts: dict = {}
+
+
+def get_context(device: Optional[int] = None) -> Context:
+    if device is None:
+        device = torch.cuda.current_device()
+    ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        _contexts[device] = ctx
+    return ctx
+
+
+# -- argument handling ---------------------------------------------------------
+def _as_points(a, name: str, device) -> torch.Tensor:
+    """(N, 2) float64 contiguous device tensor; MATLAB's 2xN check."""
+    if isinstance(a, np.ndarray):
+        a = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64))
+    if not isinstance(a, torch.Tensor):
+        a = torch.as_tensor(a, dtype=torch.float64)
+    if a.dim() != 2 or a.shape[1] != 2:
+        raise ValueError(f"{name} must be a 2xn matrix.")
+    a = a.to(device=device, dtype=torch.float64).contiguous()
+    return a
+
+
+def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr() if t.numel() > 0 else 0)
+
+
+def _device_of(*arrays) -> torch.device:
+    for a in arrays:
+        if isinstance(a, torch.Tensor) and a.is_cuda:
+            return a.device
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _prepare(ctx: Optional[Context], dev: torch.device) -> Context:
+    if ctx is None:
+        ctx = get_context(dev.index)
+    ctx.bind_torch_stream(torch.cuda.current_stream(dev))
+    return ctx
+
+
+# -- the four MEX equivalents -------------------------------------------------
+def mex_stokes_slp_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly, *, ctx=None, out=None):
+    dev = _device_of(psrc, ptar, f)
+    ps, pt = _as_points(psrc, "psrc", dev), _as_points(ptar, "ptar", dev)
+    ff = _as_points(f, "f", dev)
+    if ff.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and f must be the same size.")
+    ctx = _prepare(ctx, dev)
+    u = out if out is not None else torch.empty((pt.shape[0], 2), dtype=torch.float64, device=dev)
+    ctx._check(ctx._lib.se2p_slp_real(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0],
+                                      _ptr(ff), float(xi), int(nside_x), int(nside_y),
+                                      float(Lx), float(Ly), _ptr(u)))
+    return u
+
+
+def mex_stokes_slp_kspace(psrc, ptar, xi, eta, f, Mx, My, Lx, Ly, w, P, *, ctx=None, out=None):
+    dev = _device_of(psrc, ptar, f)
+    ps, pt = _as_points(psrc, "psrc", dev), _as_points(ptar, "ptar", dev)
+    ff = _as_points(f, "f", dev)
+    if ff.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and f must be the same size.")
+    ctx = _prepare(ctx, dev)
+    u = out if out is not None else torch.empty((pt.shape[0], 2), dtype=torch.float64, device=dev)
+    ctx._check(ctx._lib.se2p_slp_kspace(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0],
+                                        float(xi), float(eta), _ptr(ff), int(Mx), int(My),
+                                        float(Lx), float(Ly), float(w), int(P), _ptr(u)))
+    return u
+
+
+def mex_stokes_dlp_real(psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly, *, ctx=None, out=None):
+    dev = _device_of(psrc, ptar, f, n)
+    ps, pt = _as_points(psrc, "psrc", dev), _as_points(ptar, "ptar", dev)
+    ff, nn = _as_points(f, "f", dev), _as_points(n, "n", dev)
+    if ff.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and f must be the same size.")
+    if nn.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and n must be the same size.")
+    ctx = _prepare(ctx, dev)
+    u = out if out is not None else torch.empty((pt.shape[0], 2), dtype=torch.float64, device=dev)
+    ctx._check(ctx._lib.se2p_dlp_real(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0],
+                                      _ptr(ff), _ptr(nn), float(xi), int(nside_x), int(nside_y),
+                                      float(Lx), float(Ly), _ptr(u)))
+    return u
+
+
+def mex_stokes_dlp_kspace(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w, P, *, ctx=None,
+                          out=None):
+    dev = _device_of(psrc, ptar, f, n)
+    ps, pt = _as_points(psrc, "psrc", dev), _as_points(ptar, "ptar", dev)
+    ff, nn = _as_points(f, "f", dev), _as_points(n, "n", dev)
+    if ff.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and f must be the same size.")
+    if nn.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and n must be the same size.")
+    ctx = _prepare(ctx, dev)
+    u = out if out is not None else torch.empty((pt.shape[0], 2), dtype=torch.float64, device=dev)
+    ctx._check(ctx._lib.se2p_dlp_kspace(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0],
+                                        float(xi), float(eta), _ptr(ff), _ptr(nn), int(Mx), int(My),
+                                        float(Lx), float(Ly), float(w), int(P), _ptr(u)))
+    return u
+
+
+def count_pairs(psrc, ptar, nside_x, nside_y, Lx, Ly, *, ctx=None):
+    """(pairs_in, pairs_out) of the real-space sum for this geometry."""
+    dev = _device_of(psrc, ptar)
+    ps, pt = _as_points(psrc, "psrc", dev), _as_points(ptar, "ptar", dev)
+    ctx = _prepare(ctx, dev)
+    a, b = ctypes.c_int64(), ctypes.c_int64()
+    ctx._check(ctx._lib.se2p_count_pairs(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0],
+                                         int(nside_x), int(nside_y), float(Lx), float(Ly),
+                                         ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value

@@ -1,0 +1,100 @@
+/*
+ * se2p.h -- C ABI of the MI355X doubly periodic 2D Stokes spectral Ewald
+ * library (libse2p.so).
+ *
+ * The four forward entry points replace the reference's four MEX kernels one
+ * for one, with the same argument order and meaning:
+ *
+ *   se2p_slp_real    <- mex_stokes_slp_real(psrc,ptar,f,xi,nside_x,nside_y,Lx,Ly)
+ *                       PeriodicEwald/src/mex/StokesSLP/mex_stokes_slp_real.cpp:58-83
+ *   se2p_slp_kspace  <- mex_stokes_slp_kspace(psrc,ptar,xi,eta,f,Mx,My,Lx,Ly,w,P)
+ *                       PeriodicEwald/src/mex/StokesSLP/mex_stokes_slp_kspace.cpp:8-42
+ *   se2p_dlp_real    <- mex_stokes_dlp_real(psrc,ptar,f,n,xi,nside_x,nside_y,Lx,Ly)
+ *                       PeriodicEwald/src/mex/StokesDLP/mex_stokes_dlp_real.cpp:15-52
+ *   se2p_dlp_kspace  <- mex_stokes_dlp_kspace(psrc,ptar,xi,eta,f,n,Mx,My,Lx,Ly,w,P)
+ *                       PeriodicEwald/src/mex/StokesDLP/mex_stokes_dlp_kspace.cpp:10-53
+ *
+ * Conventions
+ *  - All arrays are FP64 DEVICE pointers (HBM), in the reference's layout: a
+ *    point set, density or normal field of N entries is the MATLAB 2xN
+ *    column-major matrix, i.e. interleaved [x0, y0, x1, y1, ...].
+ *  - Outputs are caller-owned and fully overwritten (2 x Nt doubles).
+ *  - Every call returns SE2P_OK (0) or an SE2P_ERR_* code; the message is
+ *    available from se2p_last_error(ctx).  The reference aborts through
+ *    mexErrMsgTxt where this returns a code.
+ *  - A context owns a HIP stream, a device workspace and rocFFT plans.  One
+ *    context per host thread.  Calls are synchronous (they return after the
+ *    device work has finished) unless se2p_set_async(ctx, 1) was called, in
+ *    which case they only enqueue work on the context's stream.
+ *  - Integer parameters the reference reads as doubles and truncates
+ *    (nside_x, nside_y, Mx, My, P) are passed as int.
+ */
+#ifndef SE2P_H
+#define SE2P_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct se2p_ctx se2p_ctx;
+
+/* status codes */
+#define SE2P_OK 0
+#define SE2P_ERR_ARG 1     /* null pointer, negative count, non-finite scalar */
+#define SE2P_ERR_GRID 2    /* Mx/My odd, not >= P/2+2, or Lx/Mx != Ly/My */
+#define SE2P_ERR_P 3       /* P odd or negative */
+#define SE2P_ERR_W 4       /* w != P*Lx/(2*Mx) */
+#define SE2P_ERR_NSIDE 5   /* nside_x or nside_y < 1 */
+#define SE2P_ERR_PARAM 6   /* xi <= 0, eta <= 0, Lx or Ly <= 0 */
+#define SE2P_ERR_NOCONV 7  /* parameter Newton iteration did not converge */
+#define SE2P_ERR_HIP 10    /* HIP runtime error */
+#define SE2P_ERR_FFT 11    /* rocFFT error */
+
+/* ---- context ----------------------------------------------------------- */
+int se2p_create(se2p_ctx **ctx, int device);
+int se2p_destroy(se2p_ctx *ctx);
+const char *se2p_last_error(const se2p_ctx *ctx);
+/* Use a caller-provided hipStream_t (passed as void*); NULL restores the
+ * context's own stream. */
+int se2p_set_stream(se2p_ctx *ctx, void *hip_stream);
+void *se2p_get_stream(se2p_ctx *ctx);
+int se2p_set_async(se2p_ctx *ctx, int async_flag);
+/* Per-stage device timing with HIP events on the context stream.  When
+ * enabled, se2p_stage_times fills up to n floats (milliseconds) for the
+ * most recent call: sort, real, spread, fft_fwd, filter, fft_inv, interp
+ * (-1 for a stage the call did not run). */
+int se2p_set_timing(se2p_ctx *ctx, int enable);
+int se2p_stage_times(se2p_ctx *ctx, float *ms, int n);
+
+/* ---- forward entry points (MEX replacements) ---------------------------- */
+int se2p_slp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                  int64_t nt, const double *f, double xi, int nside_x, int nside_y,
+                  double Lx, double Ly, double *u);
+
+int se2p_slp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                    int64_t nt, double xi, double eta, const double *f, int Mx, int My,
+                    double Lx, double Ly, double w, int P, double *u);
+
+int se2p_dlp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                  int64_t nt, const double *f, const double *n, double xi, int nside_x,
+                  int nside_y, double Lx, double Ly, double *u);
+
+int se2p_dlp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                    int64_t nt, double xi, double eta, const double *f, const double *n,
+                    int Mx, int My, double Lx, double Ly, double w, int P, double *u);
+
+/* ---- pair statistics (no reference counterpart) ------------------------- */
+/* Number of source-target pairs the real-space sum evaluates (pairs_in: same
+ * cell, or neighbour cell with r^2 < cutoff^2) and rejects (pairs_out).  Used
+ * to state the real-space sum's algorithmic flops. */
+int se2p_count_pairs(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                     int64_t nt, int nside_x, int nside_y, double Lx, double Ly,
+                     int64_t *pairs_in, int64_t *pairs_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SE2P_H */

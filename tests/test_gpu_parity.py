@@ -1,0 +1,148 @@
+"""HIP path vs the CPU oracle, through the C ABI, on seeded inputs.
+
+Bar: max-norm relative error <= 1e-12 against the oracle (same parameters on
+both sides).  Sizes are such that the oracle finishes in seconds; the
+full-size stresslet case compares a target subset (targets are independent).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import cref
+from fasttools2d_amd import params as Pm
+from fasttools2d_amd import workloads as W
+from tests.conftest import relerr
+
+TOL = 1e-12
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _run_all(pe, ps, pt, f, nv, Lx, Ly, pr, pd, label):
+    out = {}
+    _say(f"[{label}] slp_real Ns={len(ps)} Nt={len(pt)} nside={pr.nside_x}x{pr.nside_y}")
+    g = pe.mex_stokes_slp_real(_dev(ps), _dev(pt), _dev(f), pr.xi, pr.nside_x, pr.nside_y, Lx, Ly).cpu().numpy()
+    c = cref.slp_real(ps, pt, f, pr.xi, pr.nside_x, pr.nside_y, Lx, Ly)
+    out["slp_real"] = relerr(g, c)
+    _say(f"[{label}] slp_kspace M={pr.Mx}x{pr.My}")
+    g = pe.mex_stokes_slp_kspace(_dev(ps), _dev(pt), pr.xi, pr.eta, _dev(f), pr.Mx, pr.My, Lx, Ly, pr.w, pr.P).cpu().numpy()
+    c = cref.slp_kspace(ps, pt, pr.xi, pr.eta, f, pr.Mx, pr.My, Lx, Ly, pr.w, pr.P)
+    out["slp_kspace"] = relerr(g, c)
+    _say(f"[{label}] dlp_real")
+    g = pe.mex_stokes_dlp_real(_dev(ps), _dev(pt), _dev(f), _dev(nv), pd.xi, pd.nside_x, pd.nside_y, Lx, Ly).cpu().numpy()
+    c = cref.dlp_real(ps, pt, f, nv, pd.xi, pd.nside_x, pd.nside_y, Lx, Ly)
+    out["dlp_real"] = relerr(g, c)
+    _say(f"[{label}] dlp_kspace M={pd.Mx}x{pd.My}")
+    g = pe.mex_stokes_dlp_kspace(_dev(ps), _dev(pt), pd.xi, pd.eta, _dev(f), _dev(nv), pd.Mx, pd.My, Lx, Ly, pd.w, pd.P).cpu().numpy()
+    c = cref.dlp_kspace(ps, pt, pd.xi, pd.eta, f, nv, pd.Mx, pd.My, Lx, Ly, pd.w, pd.P)
+    out["dlp_kspace"] = relerr(g, c)
+    _say(f"[{label}] errors {out}")
+    return out
+
+
+def test_plain_unit_box_N1000(gpu):
+    w = W.uniform_box(1000, seed=1)
+    pr = Pm.select_params("slp", 1000, 1000, w.f, 1.0, 1.0, tol=1e-10, M=64)
+    pd = Pm.select_params("dlp", 1000, 1000, w.f, 1.0, 1.0, tol=1e-10, M=64)
+    errs = _run_all(gpu, w.psrc, w.ptar, w.f, w.n, 1.0, 1.0, pr, pd, "N1000")
+    for k, v in errs.items():
+        assert v <= TOL, (k, v)
+
+
+def test_rectangular_box_separate_targets(gpu):
+    rng = np.random.default_rng(7)
+    Lx, Ly = 1.0, 2.0
+    Ns, Nt = 700, 333
+    ps = np.stack([Pm.wrap(Lx * rng.random(Ns), Lx), Pm.wrap(Ly * rng.random(Ns), Ly)], 1)
+    pt = np.stack([Pm.wrap(Lx * rng.random(Nt), Lx), Pm.wrap(Ly * rng.random(Nt), Ly)], 1)
+    f = 10 * rng.random((Ns, 2))
+    n1 = rng.random(Ns)
+    nv = np.stack([n1, np.sqrt(1 - n1 ** 2)], 1)
+    pr = Pm.select_params("slp", Ns, Nt, f, Lx, Ly, tol=1e-12)
+    pd = Pm.select_params("dlp", Ns, Nt, f, Lx, Ly, tol=1e-12)
+    errs = _run_all(gpu, ps, pt, f, nv, Lx, Ly, pr, pd, "rect")
+    for k, v in errs.items():
+        assert v <= TOL, (k, v)
+
+
+def test_small_nside_and_tiny_N(gpu):
+    # nside = 1 and 2 revisit the same cell under several image offsets
+    rng = np.random.default_rng(11)
+    for N, nside in ((1, 1), (2, 1), (5, 2), (40, 2)):
+        ps = Pm.wrap(rng.random((N, 2)), 1.0)
+        f = rng.standard_normal((N, 2))
+        nv = rng.standard_normal((N, 2))
+        nv /= np.linalg.norm(nv, axis=1, keepdims=True)
+        xi = 6.0
+        _say(f"[tiny N={N} nside={nside}]")
+        g = gpu.mex_stokes_slp_real(_dev(ps), _dev(ps), _dev(f), xi, nside, nside, 1.0, 1.0).cpu().numpy()
+        c = cref.slp_real(ps, ps, f, xi, nside, nside, 1.0, 1.0)
+        assert relerr(g, c) <= TOL
+        g = gpu.mex_stokes_dlp_real(_dev(ps), _dev(ps), _dev(f), _dev(nv), xi, nside, nside, 1.0, 1.0).cpu().numpy()
+        c = cref.dlp_real(ps, ps, f, nv, xi, nside, nside, 1.0, 1.0)
+        if N > 1:
+            assert relerr(g, c) <= TOL
+        else:
+            assert np.all(g == 0) and np.all(c == 0)
+
+
+def test_grid_node_points_and_small_grid(gpu):
+    # points exactly on grid nodes / cell edges exercise the anchor snapping
+    M, P, L = 32, 24, 1.0
+    h = L / M
+    idx = np.array([[0, 0], [1, 5], [31, 31], [16, 7], [3, 30]])
+    ps = np.concatenate([-L / 2 + h * idx, Pm.wrap(np.random.default_rng(3).random((50, 2)), L)])
+    ps = np.concatenate([ps, ps[:3] + 1e-14])
+    f = np.random.default_rng(4).standard_normal((len(ps), 2))
+    nv = np.random.default_rng(5).standard_normal((len(ps), 2))
+    nv /= np.linalg.norm(nv, axis=1, keepdims=True)
+    xi = 10.0
+    m = 0.95 * np.sqrt(np.pi * P)
+    for MM in (32, 24, 14):   # 24 and 14 make the support wrap onto itself
+        hh = L / MM
+        w = P * hh / 2
+        eta = (2 * xi * w / m) ** 2
+        _say(f"[grid M={MM}]")
+        g = gpu.mex_stokes_slp_kspace(_dev(ps), _dev(ps), xi, eta, _dev(f), MM, MM, L, L, w, P).cpu().numpy()
+        c = cref.slp_kspace(ps, ps, xi, eta, f, MM, MM, L, L, w, P)
+        assert relerr(g, c) <= TOL, ("slp", MM, relerr(g, c))
+        g = gpu.mex_stokes_dlp_kspace(_dev(ps), _dev(ps), xi, eta, _dev(f), _dev(nv), MM, MM, L, L, w, P).cpu().numpy()
+        c = cref.dlp_kspace(ps, ps, xi, eta, f, nv, MM, MM, L, L, w, P)
+        assert relerr(g, c) <= TOL, ("dlp", MM, relerr(g, c))
+
+
+def test_empty_sources(gpu):
+    pt = Pm.wrap(np.random.default_rng(9).random((10, 2)), 1.0)
+    e = np.zeros((0, 2))
+    u = gpu.mex_stokes_slp_real(_dev(e), _dev(pt), _dev(e), 5.0, 3, 3, 1.0, 1.0).cpu().numpy()
+    assert np.all(u == 0)
+    u = gpu.mex_stokes_dlp_kspace(_dev(e), _dev(pt), 5.0, 0.5, _dev(e), _dev(e), 64, 64, 1.0, 1.0,
+                                  24 / 64 / 2, 24).cpu().numpy()
+    assert np.all(u == 0)
+
+
+def test_ellipses_full_size_subset(gpu):
+    """Stresslet on 200 ellipses x 500 nodes, grid 512^2, tol 1e-10: the full
+    GPU run checked on 256 targets against the oracle run on those targets."""
+    w, p = W.dlp_ellipses_config()
+    rng = np.random.default_rng(55)
+    sub = rng.choice(len(w.ptar), 256, replace=False)
+    _say(f"[ellipses] N={len(w.psrc)} nside={p.nside_x} xi={p.xi:.4f} M={p.Mx}")
+    g = gpu.mex_stokes_dlp_real(_dev(w.psrc), _dev(w.ptar), _dev(w.f), _dev(w.n), p.xi, p.nside_x,
+                                p.nside_y, w.Lx, w.Ly).cpu().numpy()[sub]
+    c = cref.dlp_real(w.psrc, w.ptar[sub], w.f, w.n, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly)
+    e1 = relerr(g, c)
+    g = gpu.mex_stokes_dlp_kspace(_dev(w.psrc), _dev(w.ptar), p.xi, p.eta, _dev(w.f), _dev(w.n),
+                                  p.Mx, p.My, w.Lx, w.Ly, p.w, p.P).cpu().numpy()[sub]
+    c = cref.dlp_kspace(w.psrc, w.ptar[sub], p.xi, p.eta, w.f, w.n, p.Mx, p.My, w.Lx, w.Ly, p.w, p.P)
+    e2 = relerr(g, c)
+    _say(f"[ellipses] real {e1:.3e} kspace {e2:.3e}")
+    assert e1 <= TOL and e2 <= TOL
