@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Throughput of the doubly periodic stresslet spectral Ewald sum on one MI355X.
+
+Workload (default): the stresslet double-layer potential on 200 non-overlapping
+ellipses x 500 nodes (N = 1e5 sources = targets, unit box), trapezoid-weighted
+smooth densities, Ewald parameters from the reference driver's formulas at
+tol 1e-10 with the grid fixed at 512^2 (P = 24).  One step = the real-space sum
+plus the Fourier-space sum (the two kernels the reference driver calls,
+mex_stokes_dlp_real + mex_stokes_dlp_kspace) over all targets, inputs resident
+in HBM.
+
+Prints ONE JSON line: target evaluations per second (one evaluation = one
+2-vector u(x_t) for one density), the roofline of the dominant kernel (the
+real-space pair sum, FP64 vector ALU bound) and the CPU oracle timed on this
+host's cores on the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (and matrix) peak
+PEAK_HBM_GBS = 8000.0
+# flop convention per real-space pair: exp = 20, divide = 1 (x2), FMA = 2;
+# an evaluated stresslet pair costs 80, a pair rejected by the cutoff 6.
+F_PAIR_IN = 80
+F_PAIR_OUT = 6
+
+
+def _pmc_traffic(kernel_tag: str):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated),
+    or None when no summary is committed."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        return data.get(kernel_tag, {}).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
+                          "n_gpus": args.gpus}), flush=True)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from fasttools2d_amd import periodic_ewald as pe
+    from fasttools2d_amd import workloads as W
+
+    w, p = W.dlp_ellipses_config(seed=2, n_ell=200, n_nodes=500, tol=1e-10, M=512)
+    N = len(w.psrc)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    ps, pt, f, nv = t(w.psrc), t(w.ptar), t(w.f), t(w.n)
+    ur = torch.empty((N, 2), dtype=torch.float64, device=dev)
+    uk = torch.empty((N, 2), dtype=torch.float64, device=dev)
+
+    ctx = pe.Context(0)
+    stream = torch.cuda.Stream(dev)
+    ctx.bind_torch_stream(stream)
+
+    def step():
+        pe.mex_stokes_dlp_real(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly, ctx=ctx, out=ur)
+        pe.mex_stokes_dlp_kspace(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, w.Lx, w.Ly, p.w, p.P,
+                                 ctx=ctx, out=uk)
+
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        ctx.bind_torch_stream(stream)
+        ctx.set_async(True)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        print(f"# warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        wall0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            step()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - wall0
+        gpu_ms = ev0.elapsed_time(ev1)
+        print(f"# timed {args.steps} steps: {gpu_ms:.3f} ms (wall {wall * 1e3:.3f} ms)",
+              file=sys.stderr, flush=True)
+
+        # per-stage kernel durations (HIP events on the context stream)
+        ctx.set_timing(True)
+        stages = {}
+        nstage = max(3, min(args.steps, 10))
+        for _ in range(nstage):
+            pe.mex_stokes_dlp_real(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly, ctx=ctx, out=ur)
+            for k, v in ctx.stage_times().items():
+                if v >= 0:
+                    stages.setdefault("real_call." + k, []).append(v)
+            pe.mex_stokes_dlp_kspace(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, w.Lx, w.Ly, p.w, p.P,
+                                     ctx=ctx, out=uk)
+            for k, v in ctx.stage_times().items():
+                if v >= 0:
+                    stages.setdefault("kspace_call." + k, []).append(v)
+        ctx.set_timing(False)
+        ctx.set_async(False)
+    torch.cuda.synchronize()
+    stage_ms = {k: statistics.mean(v) for k, v in stages.items()}
+
+    pairs_in, pairs_out = pe.count_pairs(ps, pt, p.nside_x, p.nside_y, w.Lx, w.Ly, ctx=ctx)
+    flops_real = pairs_in * F_PAIR_IN + pairs_out * F_PAIR_OUT
+    real_ms = stage_ms["real_call.real"]
+    achieved_tflops = flops_real / (real_ms * 1e-3) / 1e12
+
+    ms_per_step = gpu_ms / args.steps
+    value = N * args.steps / (gpu_ms * 1e-3)
+
+    # parity of the measured result against the CPU oracle on a target subset
+    from oracle import cref
+
+    rng = np.random.default_rng(55)
+    sub = rng.choice(N, 256, replace=False)
+    u_gpu = (ur + uk).cpu().numpy()[sub]
+    c_r = cref.dlp_real(w.psrc, w.ptar[sub], w.f, w.n, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly)
+    c_k = cref.dlp_kspace(w.psrc, w.ptar[sub], p.xi, p.eta, w.f, w.n, p.Mx, p.My, w.Lx, w.Ly,
+                          p.w, p.P)
+    c = c_r + c_k
+    parity = float(np.abs(u_gpu - c).max() / np.abs(c).max())
+
+    cpu = None
+    if not args.no_cpu:
+        threads = cref.get_threads()
+
+        def cpu_step():
+            cref.dlp_real(w.psrc, w.ptar, w.f, w.n, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly)
+            cref.dlp_kspace(w.psrc, w.ptar, p.xi, p.eta, w.f, w.n, p.Mx, p.My, w.Lx, w.Ly, p.w, p.P)
+
+        cpu_step()   # warm-up
+        times = []
+        for _ in range(args.cpu_runs):
+            t0 = time.perf_counter()
+            cpu_step()
+            times.append(time.perf_counter() - t0)
+            print(f"# cpu run {len(times)}: {times[-1]:.3f} s", file=sys.stderr, flush=True)
+        med = statistics.median(times)
+        cpu = {"value": N / med, "unit": "target evals/s", "cores": threads, "kind": "port",
+               "sample": f"full workload (N={N} targets, real + Fourier sums), median of "
+                         f"{args.cpu_runs} runs after 1 warm-up, {threads} OpenMP threads",
+               "seconds_per_step": med}
+
+    traffic = _pmc_traffic("k_real_dlp")
+    out = {
+        "metric": "target evals/s at 1e-10 rel err, 1 GPU (stresslet DLP, plain forward)",
+        "value": value,
+        "unit": "target evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded ellipses, seed 2)",
+        "config": {"workload": "stresslet DLP, 200 ellipses x 500 nodes (N=1e5), grid 512^2, "
+                               "P=24, tol 1e-10",
+                   "N": N, "Mx": p.Mx, "My": p.My, "P": p.P, "xi": p.xi,
+                   "nside": [p.nside_x, p.nside_y], "tol": p.tol},
+        "roofline": {"bound": "valu", "kernel": "k_real<1,1,false> (real-space stresslet pair sum)",
+                     "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "flops_per_launch": flops_real, "pairs_in": pairs_in, "pairs_out": pairs_out,
+                     "avg_kernel_ms": real_ms},
+        "stage_ms": stage_ms,
+        "parity_vs_oracle": parity,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -104,11 +104,11 @@ def test_grid_node_points_and_small_grid(gpu):
     f = np.random.default_rng(4).standard_normal((len(ps), 2))
     nv = np.random.default_rng(5).standard_normal((len(ps), 2))
     nv /= np.linalg.norm(nv, axis=1, keepdims=True)
-    xi = 10.0
     m = 0.95 * np.sqrt(np.pi * P)
     for MM in (32, 24, 14):   # 24 and 14 make the support wrap onto itself
         hh = L / MM
         w = P * hh / 2
+        xi = np.sqrt(0.5) * m / (2 * w)   # eta = 0.5, as the drivers' choice keeps eta < 1
         eta = (2 * xi * w / m) ** 2
         _say(f"[grid M={MM}]")
         g = gpu.mex_stokes_slp_kspace(_dev(ps), _dev(ps), xi, eta, _dev(f), MM, MM, L, L, w, P).cpu().numpy()
