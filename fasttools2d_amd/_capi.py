@@ -46,7 +46,12 @@ SIGNATURES = {
     "se2p_dlp_kspace": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _P, _I, _I, _D, _D, _D, _I, _P]),
     "se2p_count_pairs": (_I, [_P, _P, _I64, _P, _I64, _I, _I, _D, _D,
                               ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "se2p_selftest_e1": (_D, [_D]),
+    "se2p_selftest_anchor": (None, [_D, _D, _D, _D, _D, _I, _P, _P, _P, _P]),
+    "se2p_selftest_cell": (_I, [_D, _D, _D, _D, _I, _I]),
 }
+
+SE2P_HOST_ONLY = -1
 
 STAGES = ("sort", "real", "spread", "fft_fwd", "filter", "fft_inv", "interp")
 

@@ -5,7 +5,10 @@
 #include <cstring>
 #include <mutex>
 
+#include "se2p_device.h"
 #include "se2p_internal.h"
+
+#include <limits>
 
 namespace se2p {
 
@@ -77,6 +80,8 @@ static void begin_call(se2p_ctx *ctx)
 
 static int set_device(se2p_ctx *ctx)
 {
+    if (ctx->device == SE2P_HOST_ONLY)
+        return fail(ctx, SE2P_ERR_HIP, "host-only context: no device to run on");
     SE2P_HIP(hipSetDevice(ctx->device));
     return 0;
 }
@@ -116,6 +121,11 @@ int se2p_create(se2p_ctx **out, int device)
     std::call_once(fft_once, [] { rocfft_setup(); });
     se2p_ctx *ctx = new se2p_ctx();
     ctx->device = device;
+    if (device == SE2P_HOST_ONLY) {
+        // argument validation only; every compute call fails with SE2P_ERR_HIP
+        *out = ctx;
+        return SE2P_OK;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
         delete ctx;
@@ -134,6 +144,10 @@ int se2p_create(se2p_ctx **out, int device)
 int se2p_destroy(se2p_ctx *ctx)
 {
     if (!ctx) return SE2P_OK;
+    if (ctx->device == SE2P_HOST_ONLY) {
+        delete ctx;
+        return SE2P_OK;
+    }
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->plans) {
@@ -161,6 +175,7 @@ const char *se2p_last_error(const se2p_ctx *ctx)
 int se2p_set_stream(se2p_ctx *ctx, void *s)
 {
     if (!ctx) return SE2P_ERR_ARG;
+    if (ctx->device == SE2P_HOST_ONLY) return fail(ctx, SE2P_ERR_HIP, "host-only context");
     if (ctx->own_stream && ctx->stream) hipStreamSynchronize(ctx->stream);
     if (s) {
         if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
@@ -276,6 +291,34 @@ int se2p_count_pairs(se2p_ctx *ctx, const double *psrc, int64_t ns, const double
     *pairs_in = counts[0];
     *pairs_out = counts[1];
     return finish(ctx);
+}
+
+double se2p_selftest_e1(double x)
+{
+    static double tab[SE2P_E1_NPIECE * SE2P_E1_ROW];
+    static std::once_flag once;
+    std::call_once(once, [] {
+        for (int p = 0; p < SE2P_E1_NPIECE; ++p) {
+            double *row = tab + p * SE2P_E1_ROW;
+            row[0] = se2p_e1_scale_shift[p][0];
+            row[1] = se2p_e1_scale_shift[p][1];
+            for (int k = 0; k < SE2P_E1_NCOEF; ++k) row[2 + k] = se2p_e1_coef[p][k];
+            for (int k = 2 + SE2P_E1_NCOEF; k < SE2P_E1_ROW; ++k) row[k] = 0.0;
+        }
+    });
+    if (!(x > 0)) return std::numeric_limits<double>::infinity();
+    return se2p_e1(x, std::exp(-x), tab);
+}
+
+void se2p_selftest_anchor(double x, double y, double Lx, double Ly, double h, int P, int *mx,
+                          int *my, double *px, double *py)
+{
+    se2p_anchor(x, y, Lx, Ly, h, P, mx, my, px, py);
+}
+
+int se2p_selftest_cell(double x, double y, double Lx, double Ly, int nside_x, int nside_y)
+{
+    return se2p_cell_of(x, y, Lx, Ly, nside_x, nside_y);
 }
 
 }  // extern "C"

@@ -70,19 +70,22 @@ int finish(se2p_ctx *ctx);   // error check + optional synchronisation
 
 // ---- cell lists / bins (se2p_sort.hip) ---------------------------------
 // Sorted order of points by an integer key in [0, nkeys): perm[k] = original
-// index of the k-th point, off[0..nkeys] = start of each key's run.  Stable.
+// index of the k-th point, off[0..nkeys] = start of each key's run.
 struct KeySort {
-    int *keys_in, *keys_out, *idx_in, *perm, *off;
-    void *temp;
-    size_t temp_bytes;
+    int *key, *slot, *tmp, *perm, *count, *off;
 };
-size_t keysort_bytes(int64_t n, int nkeys, size_t *temp_bytes);
-void keysort_carve(char *base, int64_t n, int nkeys, size_t temp_bytes, KeySort *ks);
-int keysort_run(se2p_ctx *ctx, KeySort *ks, int64_t n, int nkeys);
-
-// Keys for real-space cell lists (ewald_tools.cpp:28-35).
-void launch_cell_keys(hipStream_t st, const double *p, int64_t n, double Lx, double Ly,
-                      int nsx, int nsy, int *keys, int *idx);
+size_t keysort_bytes(int64_t n, int nkeys);
+void keysort_carve(char *base, int64_t n, int nkeys, KeySort *ks);
+// Zero the key counters (must precede the kernel that fills key/slot/count).
+void keysort_clear(hipStream_t st, KeySort *ks, int nkeys);
+// Scan counts, scatter; stable = restore input order inside each key run.
+int keysort_finish(se2p_ctx *ctx, KeySort *ks, int64_t n, int nkeys, bool stable);
+// Cell lists (ewald_tools.cpp:28-35); sublog > 0 appends a Morton sub-cell of
+// 2^sublog x 2^sublog to the key (nkeys = nsx*nsy*4^sublog).
+int cell_sort(se2p_ctx *ctx, const double *p, int64_t n, double Lx, double Ly, int nsx, int nsy,
+              int sublog, bool stable, KeySort *ks);
+// Exclusive scan of n ints into out[0..n] with one workgroup.
+void scan_ints(hipStream_t st, const int *a, int n, int *out);
 
 // ---- real space (se2p_real.hip) -----------------------------------------
 struct RealGeom {

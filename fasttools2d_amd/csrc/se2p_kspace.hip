@@ -54,7 +54,7 @@ struct SpreadSrc {
 // ---- per-source gridding data -------------------------------------------
 __global__ void k_bin_keys(const double *__restrict__ p, int n, double Lx, double Ly, double h,
                            int P, int Mx, int My, int nby, int *__restrict__ keys,
-                           int *__restrict__ idx)
+                           int *__restrict__ slot, int *__restrict__ count)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -62,8 +62,9 @@ __global__ void k_bin_keys(const double *__restrict__ p, int n, double Lx, doubl
     double px, py;
     se2p_anchor(p[2 * i], p[2 * i + 1], Lx, Ly, h, P, &mx, &my, &px, &py);
     int ax = ((mx % Mx) + Mx) % Mx, ay = ((my % My) + My) % My;
-    keys[i] = (ax / kBin) * nby + ay / kBin;
-    idx[i] = i;
+    const int k = (ax / kBin) * nby + ay / kBin;
+    keys[i] = k;
+    slot[i] = atomicAdd(&count[k], 1);
 }
 
 // FGG factors of mex_stokes_slp_kspace.cpp:95-99 with c = -2 xi^2 / eta.
@@ -507,8 +508,7 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         if (int rc = get_plans(ctx, Mx, My, nq, 2 * RB, &plans)) return rc;
 
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
-    size_t tb = 0;
-    size_t ks = keysort_bytes(ns, nbins, &tb);
+    size_t ks = keysort_bytes(ns, nbins);
     const size_t nsz = (size_t)(ns > 0 ? ns : 1);
     size_t need = ks + up(sizeof(int2) * nsz) + up(sizeof(double) * 3 * nsz) +
                   up(sizeof(double) * nq * nsz) + up(sizeof(double) * nq * (size_t)G) +
@@ -517,7 +517,7 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     char *base = ctx->ws;
     size_t o = 0;
     KeySort kss;
-    keysort_carve(base + o, ns, nbins, tb, &kss); o += ks;
+    keysort_carve(base + o, ns, nbins, &kss); o += ks;
     int2 *anchor = (int2 *)(base + o); o += up(sizeof(int2) * nsz);
     double *fgg = (double *)(base + o); o += up(sizeof(double) * 3 * nsz);
     double *q = (double *)(base + o); o += up(sizeof(double) * nq * nsz);
@@ -535,9 +535,11 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     }
 
     stage_begin(ctx, ST_SORT);
+    keysort_clear(st, &kss, nbins);
     hipLaunchKernelGGL(k_bin_keys, dim3((unsigned)((ns + threads - 1) / threads)), dim3(threads), 0,
-                       st, g.psrc, (int)ns, g.Lx, g.Ly, h, P, Mx, My, nby, kss.keys_in, kss.idx_in);
-    if (int rc = keysort_run(ctx, &kss, ns, nbins)) return rc;
+                       st, g.psrc, (int)ns, g.Lx, g.Ly, h, P, Mx, My, nby, kss.key, kss.slot,
+                       kss.count);
+    if (int rc = keysort_finish(ctx, &kss, ns, nbins, true)) return rc;
     stage_end(ctx, ST_SORT);
 
     const bool wrap = (Mx <= P) || (My <= P);

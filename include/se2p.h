@@ -53,6 +53,10 @@ typedef struct se2p_ctx se2p_ctx;
 #define SE2P_ERR_FFT 11    /* rocFFT error */
 
 /* ---- context ----------------------------------------------------------- */
+/* device >= 0: a HIP device.  device == SE2P_HOST_ONLY: a context that only
+ * validates arguments (every call that would launch work returns
+ * SE2P_ERR_HIP); for hosts without a GPU. */
+#define SE2P_HOST_ONLY (-1)
 int se2p_create(se2p_ctx **ctx, int device);
 int se2p_destroy(se2p_ctx *ctx);
 const char *se2p_last_error(const se2p_ctx *ctx);
@@ -92,6 +96,16 @@ int se2p_dlp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double 
 int se2p_count_pairs(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
                      int64_t nt, int nside_x, int nside_y, double Lx, double Ly,
                      int64_t *pairs_in, int64_t *pairs_out);
+
+/* ---- self-test hooks (no reference counterpart) -------------------------- */
+/* Host builds of the device functions the kernels use, so that a CPU-only test
+ * suite can pin their arithmetic: the E1 approximation, the gridding anchor
+ * (same arithmetic as FindClosestNode, ewald_tools.cpp:86-114) and the cell
+ * index (ewald_tools.cpp:28-35). */
+double se2p_selftest_e1(double x);
+void se2p_selftest_anchor(double x, double y, double Lx, double Ly, double h, int P, int *mx,
+                          int *my, double *px, double *py);
+int se2p_selftest_cell(double x, double y, double Lx, double Ly, int nside_x, int nside_y);
 
 #ifdef __cplusplus
 }

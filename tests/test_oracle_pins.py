@@ -1,0 +1,211 @@
+"""Pin the CPU oracle (oracle/se2p_oracle.c) before trusting it.
+
+Each check is chosen so that a plausible mistake fails it:
+* E1 against mpmath (series/continued-fraction cut-over, missing gamma);
+* the grid anchor against brute force and the snapping special cases of
+  ewald_tools.cpp:86-114;
+* cell lists against a stable numpy argsort (ewald_tools.cpp:7-80);
+* the 2D DFT against numpy.fft (sign, normalisation, non-power-of-two);
+* real-space sums against the closed-form image sums of
+  PeriodicEwald/tests/matlab_sums/stokes_{slp,dlp}_real_ds.m (self term,
+  kernel signs, 1/4pi) at tol 1e-16 where the cutoff truncation is below
+  1e-15;
+* Fourier-space sums against the direct Fourier sums of
+  PeriodicEwald/tests/matlab_sums/stokes_{slp,dlp}_kspace_ds.m (filter,
+  wavenumber convention, scaling, Re/Im);
+* the stresslet identity of PeriodicEwald/tests/stresslet_identity_test.m;
+* the invariants of PeriodicEwald/tests/consistency_test.m (independence of
+  the cell count Nb, periodic replication of the box).
+"""
+import mpmath
+import numpy as np
+import pytest
+
+from oracle import cref
+from oracle import direct_sums as ds
+from fasttools2d_amd import params as Pm
+from fasttools2d_amd import workloads as W
+from tests.conftest import relerr
+
+
+def test_e1_against_mpmath():
+    xs = np.concatenate([np.logspace(-14, -0.01, 60), [0.5, 0.999999, 1.0, 1.000001],
+                         np.linspace(1.01, 60, 120), [200.0, 700.0]])
+    worst = 0.0
+    for x in xs:
+        ref = float(mpmath.e1(mpmath.mpf(float(x))))
+        worst = max(worst, abs(cref.e1(x) - ref) / ref)
+    assert worst < 2e-14, worst
+
+
+def test_anchor_brute_force():
+    rng = np.random.default_rng(0)
+    for (L, M, P) in ((1.0, 64, 24), (2.0, 48, 24), (0.7, 30, 8)):
+        h = L / M
+        for x in Pm.wrap(rng.uniform(0, L, 200), L):
+            mx, my, px, py = cref.find_closest_node(x, x, L, L, h, P)
+            # node mx + P/2 is the grid line at or below x (grid starts at -L/2)
+            node = -L / 2 + (mx + P // 2) * h
+            assert -1e-12 <= x - node < h + 1e-12
+            assert abs((x - node) - px) < 1e-12 or px == 0.0
+            assert 0.0 <= px < h
+
+
+def test_anchor_snapping_cases():
+    L, M, P = 1.0, 32, 24
+    h = L / M
+    for k in (0, 1, 7, 16, 31):
+        x = -L / 2 + k * h
+        for xx in (x, x + 5e-14, x - 5e-14):
+            mx, my, px, py = cref.find_closest_node(xx, xx, L, L, h, P)
+            assert px == 0.0 and py == 0.0
+            assert mx + P // 2 == k          # anchor refers to the node nearest x
+            assert my == mx
+    # product's host build of the same arithmetic agrees bit for bit
+    from fasttools2d_amd import _capi
+    import ctypes
+    lib = _capi.load()
+    rng = np.random.default_rng(1)
+    pts = np.concatenate([Pm.wrap(rng.uniform(0, 1, 300), 1.0),
+                          -0.5 + h * np.arange(M), -0.5 + h * np.arange(M) - 3e-14])
+    for x in pts:
+        a = cref.find_closest_node(x, -x, L, L, h, P)
+        mx, my, px, py = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        lib.se2p_selftest_anchor(x, -x, L, L, h, P, ctypes.byref(mx), ctypes.byref(my),
+                                 ctypes.byref(px), ctypes.byref(py))
+        assert (mx.value, my.value, px.value, py.value) == a
+
+
+def test_cell_lists_are_stable_counting_sort():
+    rng = np.random.default_rng(2)
+    L = 1.0
+    p = Pm.wrap(rng.uniform(0, L, (500, 2)), L)
+    p[:5] = [[-0.5, -0.5], [0.4999999999999999, 0.0], [0.0, 0.0], [-0.5, 0.49], [0.2, -0.5]]
+    nsx, nsy = 5, 3
+    perm, off = cref.cell_lists(p, L, L, nsx, nsy)
+    bx = np.clip(np.floor(nsx * (p[:, 0] / L + 0.5)).astype(int), 0, nsx - 1)
+    by = np.clip(np.floor(nsy * (p[:, 1] / L + 0.5)).astype(int), 0, nsy - 1)
+    box = by * nsx + bx
+    order = np.argsort(box, kind="stable")
+    assert np.array_equal(perm, order)
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum(np.bincount(box, minlength=nsx * nsy))]))
+
+
+@pytest.mark.parametrize("shape", [(8, 8), (12, 20), (45, 48), (64, 32)])
+def test_fft2_against_numpy(shape):
+    rng = np.random.default_rng(3)
+    z = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    # memory layout x*My + y == numpy array indexed [x, y]
+    ref = np.fft.fft2(z)
+    assert np.abs(cref.fft2(z) - ref).max() <= 1e-12 * np.abs(ref).max()
+    ref_i = np.fft.ifft2(z)
+    assert np.abs(cref.fft2(z, inverse=True) - ref_i).max() <= 1e-12 * np.abs(ref_i).max()
+
+
+def _small_case(seed=0, N=10, Lx=1.0, Ly=2.0):
+    # direct_sums_test.m:17-38
+    rng = np.random.default_rng(seed)
+    xs = np.stack([Pm.wrap(Lx * rng.random(N), Lx), Pm.wrap(Ly * rng.random(N), Ly)], 1)
+    xt = np.stack([Pm.wrap(Lx * rng.random(N), Lx), Pm.wrap(Ly * rng.random(N), Ly)], 1)
+    f = 10 * rng.random((N, 2))
+    n1 = rng.random(N)
+    nv = np.stack([n1, np.sqrt(1 - n1 ** 2)], 1)
+    return xs, xt, f, nv, Lx, Ly
+
+
+def test_slp_real_against_direct_sum():
+    xs, xt, f, nv, Lx, Ly = _small_case()
+    xt[0] = xs[3]                    # a coincident pair exercises the self term
+    p = Pm.select_params("slp", len(xs), len(xt), f, Lx, Ly, tol=1e-16)
+    u = cref.slp_real(xs, xt, f, p.xi, p.nside_x, p.nside_y, Lx, Ly)
+    ref = ds.slp_real_ds(xs, xt, f, Lx, Ly, p.xi)
+    assert relerr(u, ref) < 1e-13
+
+
+def test_dlp_real_against_direct_sum():
+    xs, xt, f, nv, Lx, Ly = _small_case(seed=5)
+    xt[0] = xs[3]
+    p = Pm.select_params("dlp", len(xs), len(xt), f, Lx, Ly, tol=1e-16)
+    u = cref.dlp_real(xs, xt, f, nv, p.xi, p.nside_x, p.nside_y, Lx, Ly)
+    ref = ds.dlp_real_ds(xs, xt, f, nv, Lx, Ly, p.xi)
+    assert relerr(u, ref) < 1e-13
+
+
+def test_slp_kspace_against_direct_fourier_sum():
+    xs, xt, f, nv, Lx, Ly = _small_case(seed=7)
+    p = Pm.select_params("slp", len(xs), len(xt), f, Lx, Ly, tol=1e-16)
+    u = cref.slp_kspace(xs, xt, p.xi, p.eta, f, p.Mx, p.My, Lx, Ly, p.w, p.P)
+    ref = ds.slp_kspace_ds(xs, xt, f, Lx, Ly, p.xi, 100)
+    assert relerr(u, ref) < 1e-12
+
+
+def test_dlp_kspace_against_direct_fourier_sum():
+    xs, xt, f, nv, Lx, Ly = _small_case(seed=9)
+    p = Pm.select_params("dlp", len(xs), len(xt), f, Lx, Ly, tol=1e-16)
+    u = cref.dlp_kspace(xs, xt, p.xi, p.eta, f, nv, p.Mx, p.My, Lx, Ly, p.w, p.P)
+    ref = ds.dlp_kspace_ds(xs, xt, f, nv, Lx, Ly, p.xi, 100, zero_mode=False)
+    assert relerr(u, ref) < 1e-12
+
+
+def _dlp_full_oracle(w, tol=1e-14, Nb=None):
+    """Driver composition of StokesDLP_ewald_2p.m:140-158 on the oracle."""
+    Ns, Nt = len(w.psrc), len(w.ptar)
+    p = Pm.select_params("dlp", Ns, Nt, w.f, w.Lx, w.Ly, tol=tol, Nb=Nb)
+    ur = cref.dlp_real(w.psrc, w.ptar, w.f, w.n, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly)
+    uk = cref.dlp_kspace(w.psrc, w.ptar, p.xi, p.eta, w.f, w.n, p.Mx, p.My, w.Lx, w.Ly, p.w, p.P)
+    xu = w.psrc_unwrapped
+    fn = w.f[:, 0] * w.n[:, 0] + w.f[:, 1] * w.n[:, 1]
+    uk[:, 0] -= np.sum(fn * xu[:, 0]) / (w.Lx * w.Ly)
+    uk[:, 1] -= np.sum(fn * xu[:, 1]) / (w.Lx * w.Ly)
+    return -(ur + uk)
+
+
+@pytest.mark.parametrize("comp", [0, 1])
+def test_stresslet_identity(comp):
+    # u = e_comp inside the circle, 0 outside (stresslet_identity_test.m:60-78)
+    w = W.stresslet_circle(comp=comp)
+    u = _dlp_full_oracle(w)
+    e = np.zeros(2)
+    e[comp] = 1.0
+    assert np.abs(u[:16] - e).max() < 1e-10
+    assert np.abs(u[16:]).max() < 1e-10
+
+
+def test_slp_independent_of_cell_count():
+    # consistency_test.m:44-66 (Nb in {3, 9, 27} changes rc, xi and M)
+    w = W.uniform_box(400, seed=3, targets_equal_sources=False, Nt=50)
+    us = []
+    for Nb in (3, 9, 27):
+        p = Pm.select_params("slp", 400, 50, w.f, 1.0, 1.0, tol=1e-13, Nb=Nb)
+        ur = cref.slp_real(w.psrc, w.ptar, w.f, p.xi, p.nside_x, p.nside_y, 1.0, 1.0)
+        uk = cref.slp_kspace(w.psrc, w.ptar, p.xi, p.eta, w.f, p.Mx, p.My, 1.0, 1.0, p.w, p.P)
+        us.append(ur + uk)
+    for u in us[1:]:
+        assert relerr(u, us[0]) < 1e-10
+
+
+def test_slp_periodic_replication():
+    # consistency_test.m:68-84: doubling the box with a copy of the sources
+    w = W.uniform_box(300, seed=4, targets_equal_sources=False, Nt=40)
+    p = Pm.select_params("slp", 300, 40, w.f, 1.0, 1.0, tol=1e-13)
+    u1 = (cref.slp_real(w.psrc, w.ptar, w.f, p.xi, p.nside_x, p.nside_y, 1.0, 1.0) +
+          cref.slp_kspace(w.psrc, w.ptar, p.xi, p.eta, w.f, p.Mx, p.My, 1.0, 1.0, p.w, p.P))
+    xs2 = np.concatenate([w.psrc_unwrapped, w.psrc_unwrapped + [1.0, 0.0]])
+    f2 = np.concatenate([w.f, w.f])
+    Lx, Ly = 2.0, 1.0
+    ps2 = np.stack([Pm.wrap(xs2[:, 0], Lx), Pm.wrap(xs2[:, 1], Ly)], 1)
+    q = Pm.select_params("slp", 600, 40, f2, Lx, Ly, tol=1e-13)
+    u2 = (cref.slp_real(ps2, w.ptar, f2, q.xi, q.nside_x, q.nside_y, Lx, Ly) +
+          cref.slp_kspace(ps2, w.ptar, q.xi, q.eta, f2, q.Mx, q.My, Lx, Ly, q.w, q.P))
+    assert relerr(u2, u1) < 1e-10
+
+
+def test_planted_sign_error_is_caught():
+    """A sign slip in the stresslet real-space kernel (a plausible porting
+    mistake) must fail the direct-sum pin that the oracle passes."""
+    xs, xt, f, nv, Lx, Ly = _small_case(seed=5)
+    p = Pm.select_params("dlp", len(xs), len(xt), f, Lx, Ly, tol=1e-16)
+    good = cref.dlp_real(xs, xt, f, nv, p.xi, p.nside_x, p.nside_y, Lx, Ly)
+    bad = cref.dlp_real(xs, xt, f, -nv, p.xi, p.nside_x, p.nside_y, Lx, Ly)   # n -> -n
+    ref = ds.dlp_real_ds(xs, xt, f, nv, Lx, Ly, p.xi)
+    assert relerr(good, ref) < 1e-13 and relerr(bad, ref) > 1e-2

@@ -1,0 +1,172 @@
+"""Host-side logic without a GPU: parameter selection of the reference drivers,
+the C ABI's exports and its argument validation, and the product's host
+builds of the device E1 and cell-index arithmetic."""
+import ctypes
+import math
+import os
+import re
+
+import mpmath
+import numpy as np
+import pytest
+
+from fasttools2d_amd import _capi
+from fasttools2d_amd import params as Pm
+from fasttools2d_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_matlab_rat_and_round():
+    assert Pm.matlab_rat(1.0) == (1, 1)
+    assert Pm.matlab_rat(0.5) == (1, 2)
+    assert Pm.matlab_rat(2.0) == (2, 1)
+    assert Pm.matlab_rat(1.5) == (3, 2)
+    assert Pm.matlab_rat(4 / 3) == (4, 3)
+    n, d = Pm.matlab_rat(math.pi)
+    assert abs(n / d - math.pi) < 1e-6 * math.pi
+    assert Pm.matlab_round(2.5) == 3 and Pm.matlab_round(-2.5) == -3 and Pm.matlab_round(0.49) == 0
+
+
+def test_wrap():
+    x = np.array([0.0, 0.49, 0.5, 0.99, 1.0, -0.5, -0.51])
+    assert np.allclose(Pm.wrap(x, 1.0), [0.0, 0.49, -0.5, -0.01, 0.0, -0.5, 0.49])
+
+
+def test_driver_parameters_spot_values():
+    # 1000 uniform points, targets = sources, tol 1e-10 (unit box)
+    w = W.uniform_box(1000, seed=1)
+    p = Pm.select_params("slp", 1000, 1000, w.f, 1.0, 1.0, tol=1e-10)
+    assert (p.nside_x, p.nside_y) == (3, 3)
+    assert abs(p.xi - 15.04) < 0.01 and p.kinf == 24 and p.Mx == 48
+    q = Pm.select_params("dlp", 1000, 1000, w.f, 1.0, 1.0, tol=1e-10)
+    assert abs(q.xi - 16.15) < 0.02 and q.kinf == 28 and q.Mx == 56
+    # the error estimates are at the tolerance after the Newton solve
+    Q = float(np.sum(w.f ** 2)) + 1
+    f_slp = math.sqrt(Q * math.pi / (4 * p.xi)) * math.exp(-p.xi ** 2 * p.rc ** 2)
+    assert abs(f_slp - 1e-10) <= 1e-10
+    f_dlp = math.exp(-q.xi ** 2 * q.rc ** 2) * q.xi * q.rc * math.sqrt(2 * math.pi * Q)
+    assert abs(f_dlp - 1e-10) <= 1e-10
+
+
+def test_grid_override_keeps_w_and_eta_consistent():
+    w = W.uniform_box(200, seed=2)
+    p = Pm.select_params("slp", 200, 200, w.f, 1.0, 1.0, tol=1e-10, M=64)
+    assert p.Mx == 64 and p.w == 24 * 1.0 / 64 / 2
+    m = 0.95 * math.sqrt(math.pi * 24)
+    assert abs(p.eta - (2 * p.xi * p.w / m) ** 2) < 1e-15
+
+
+def test_ellipse_workload_geometry():
+    w = W.ellipses(n_ell=20, n_nodes=100, seed=3)
+    assert w.psrc.shape == (2000, 2) and np.all(np.abs(w.psrc) <= 0.5)
+    assert np.allclose(np.linalg.norm(w.n, axis=1), 1.0)
+    c = w.meta["centres"]
+    # outward normals: (x - centre).n > 0
+    xu = w.psrc_unwrapped.reshape(20, 100, 2)
+    nn = w.n.reshape(20, 100, 2)
+    assert np.all(((xu - c[:, None, :]) * nn).sum(axis=2) > 0)
+    # bounding circles at least the gap apart (periodic)
+    R = w.meta["semi_axes"].max(axis=1)
+    for i in range(20):
+        d = c - c[i]
+        d -= np.round(d)
+        dist = np.sqrt((d ** 2).sum(axis=1))
+        dist[i] = np.inf
+        assert np.all(dist >= R + R[i] + 0.005 - 1e-12)
+
+
+def _declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "se2p.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(se2p_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _capi.load()
+    decl = _declared_symbols()
+    assert len(decl) >= 15
+    for name in decl:
+        assert hasattr(lib, name), name
+        assert name in _capi.SIGNATURES, name
+
+
+def test_status_codes_match_header():
+    txt = open(os.path.join(ROOT, "include", "se2p.h")).read()
+    for name in ("SE2P_OK", "SE2P_ERR_ARG", "SE2P_ERR_GRID", "SE2P_ERR_P", "SE2P_ERR_W",
+                 "SE2P_ERR_NSIDE", "SE2P_ERR_PARAM", "SE2P_ERR_NOCONV", "SE2P_ERR_HIP",
+                 "SE2P_ERR_FFT"):
+        m = re.search(r"#define\s+" + name + r"\s+(\d+)", txt)
+        assert m and int(m.group(1)) == getattr(_capi, name), name
+
+
+@pytest.fixture()
+def hctx():
+    lib = _capi.load()
+    h = ctypes.c_void_p()
+    assert lib.se2p_create(ctypes.byref(h), _capi.SE2P_HOST_ONLY) == 0
+    yield lib, h
+    lib.se2p_destroy(h)
+
+
+def _fake(n):
+    # host pointers are never dereferenced by the validation paths
+    return ctypes.c_void_p(0x1000) if n else None
+
+
+def test_kspace_validation_codes(hctx):
+    lib, h = hctx
+    P, M, L = 24, 64, 1.0
+    w = P * L / M / 2
+    x, u = _fake(1), _fake(1)
+
+    def call(**kw):
+        a = dict(xi=10.0, eta=0.5, Mx=M, My=M, Lx=L, Ly=L, w=w, P=P)
+        a.update(kw)
+        return lib.se2p_slp_kspace(h, x, 4, x, 4, a["xi"], a["eta"], x, a["Mx"], a["My"],
+                                   a["Lx"], a["Ly"], a["w"], a["P"], u)
+
+    assert call(Mx=63) == _capi.SE2P_ERR_GRID          # odd grid
+    assert call(Mx=64, My=32) == _capi.SE2P_ERR_GRID   # Lx/Mx != Ly/My
+    assert call(Mx=12, My=12, w=P * L / 12 / 2) == _capi.SE2P_ERR_GRID   # M < P/2 + 2
+    assert call(P=23) == _capi.SE2P_ERR_P
+    assert call(w=w * 1.01) == _capi.SE2P_ERR_W
+    assert call(xi=-1.0) == _capi.SE2P_ERR_PARAM
+    assert call(eta=0.0) == _capi.SE2P_ERR_PARAM
+    # valid arguments reach the device step, which a host-only context refuses
+    assert call() == _capi.SE2P_ERR_HIP
+    assert lib.se2p_dlp_kspace(h, x, -1, x, 4, 10.0, 0.5, x, x, M, M, L, L, w, P, u) == _capi.SE2P_ERR_ARG
+    assert lib.se2p_dlp_kspace(h, x, 4, x, 4, 10.0, 0.5, x, None, M, M, L, L, w, P, u) == _capi.SE2P_ERR_ARG
+
+
+def test_real_validation_codes(hctx):
+    lib, h = hctx
+    x = _fake(1)
+    assert lib.se2p_slp_real(h, x, 4, x, 4, x, 10.0, 0, 3, 1.0, 1.0, x) == _capi.SE2P_ERR_NSIDE
+    assert lib.se2p_dlp_real(h, x, 4, x, 4, x, x, 10.0, 3, 3, 1.0, -1.0, x) == _capi.SE2P_ERR_PARAM
+    assert lib.se2p_dlp_real(h, None, 4, x, 4, x, x, 10.0, 3, 3, 1.0, 1.0, x) == _capi.SE2P_ERR_ARG
+    assert lib.se2p_slp_real(h, x, 4, x, 4, x, 10.0, 3, 3, 1.0, 1.0, x) == _capi.SE2P_ERR_HIP
+
+
+def test_product_e1_against_mpmath():
+    """The kernels' piecewise E1 (host build of the same code) to a few ulp."""
+    lib = _capi.load()
+    xs = np.concatenate([np.logspace(-14, -0.01, 80), [0.5, 1.0, 1.25, 2.0, 600.0],
+                         np.linspace(1.001, 80, 300)])
+    worst = 0.0
+    for x in xs:
+        ref = float(mpmath.e1(mpmath.mpf(float(x))))
+        worst = max(worst, abs(lib.se2p_selftest_e1(float(x)) - ref) / ref)
+    assert worst < 2e-15, worst
+    assert lib.se2p_selftest_e1(2000.0) == 0.0
+
+
+def test_product_cell_index_matches_oracle_cells():
+    from oracle import cref
+    lib = _capi.load()
+    rng = np.random.default_rng(4)
+    p = Pm.wrap(rng.uniform(0, 1, (300, 2)), 1.0)
+    perm, off = cref.cell_lists(p, 1.0, 1.0, 7, 5)
+    cells = np.array([lib.se2p_selftest_cell(x, y, 1.0, 1.0, 7, 5) for x, y in p])
+    for c in range(35):
+        assert set(perm[off[c]:off[c + 1]]) == set(np.nonzero(cells == c)[0])
