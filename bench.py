@@ -56,6 +56,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (scales HBM counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -127,6 +129,16 @@ def main() -> int:
         ctx.set_async(False)
     torch.cuda.synchronize()
     stage_ms = {k: statistics.mean(v) for k, v in stages.items()}
+    if args.calib_copy_mb > 0:
+        # reference kernel for HBM counters: reads and writes exactly this many bytes
+        nbytes = args.calib_copy_mb << 20
+        a = torch.empty(nbytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     pairs_in, pairs_out = pe.count_pairs(ps, pt, p.nside_x, p.nside_y, w.Lx, w.Ly, ctx=ctx)
     flops_real = pairs_in * F_PAIR_IN + pairs_out * F_PAIR_OUT

@@ -49,6 +49,7 @@ SIGNATURES = {
     "se2p_selftest_e1": (_D, [_D]),
     "se2p_selftest_anchor": (None, [_D, _D, _D, _D, _D, _I, _P, _P, _P, _P]),
     "se2p_selftest_cell": (_I, [_D, _D, _D, _D, _I, _I]),
+    "se2p_selftest_keysort": (_I, [_P, _P, _I64, _I, _P, _P]),
 }
 
 SE2P_HOST_ONLY = -1

@@ -293,6 +293,28 @@ int se2p_count_pairs(se2p_ctx *ctx, const double *psrc, int64_t ns, const double
     return finish(ctx);
 }
 
+int se2p_selftest_keysort(se2p_ctx *ctx, const int *keys, int64_t n, int nkeys, int *perm,
+                          int *off)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (n < 0 || n > 0x7fffffffLL || nkeys < 1 || (n > 0 && (!keys || !perm)) || !off)
+        return fail(ctx, SE2P_ERR_ARG, "bad keysort arguments");
+    if (int rc = set_device(ctx)) return rc;
+    const size_t need = keysort_bytes(n, nkeys);
+    if (int rc = ensure_workspace(ctx, need)) return rc;
+    KeySort ks;
+    keysort_carve(ctx->ws, n, nkeys, &ks);
+    if (n > 0)
+        SE2P_HIP(hipMemcpyAsync(ks.key, keys, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    if (int rc = keysort_run(ctx, &ks, n, nkeys)) return rc;
+    if (n > 0)
+        SE2P_HIP(hipMemcpyAsync(perm, ks.perm, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    SE2P_HIP(hipMemcpyAsync(off, ks.off, sizeof(int) * (nkeys + 1), hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    return finish(ctx);
+}
+
 double se2p_selftest_e1(double x)
 {
     static double tab[SE2P_E1_NPIECE * SE2P_E1_ROW];

@@ -69,23 +69,28 @@ void stage_end(se2p_ctx *ctx, Stage s);
 int finish(se2p_ctx *ctx);   // error check + optional synchronisation
 
 // ---- cell lists / bins (se2p_sort.hip) ---------------------------------
-// Sorted order of points by an integer key in [0, nkeys): perm[k] = original
+// Stable sort of points by an integer key in [0, nkeys): perm[k] = original
 // index of the k-th point, off[0..nkeys] = start of each key's run.
 struct KeySort {
-    int *key, *slot, *tmp, *perm, *count, *off;
+    int *key;          // keys (input; also a ping-pong buffer)
+    int *key2, *tmp;   // ping-pong buffers
+    int *perm;         // sorted order -> original index
+    int *off;          // [nkeys+1] run offsets
+    int *hist, *hoff;  // per-block digit histograms and their scan
+    int *sorted_key;   // the buffer holding the sorted keys after the sort
 };
 size_t keysort_bytes(int64_t n, int nkeys);
 void keysort_carve(char *base, int64_t n, int nkeys, KeySort *ks);
-// Zero the key counters (must precede the kernel that fills key/slot/count).
-void keysort_clear(hipStream_t st, KeySort *ks, int nkeys);
-// Scan counts, scatter; stable = restore input order inside each key run.
-int keysort_finish(se2p_ctx *ctx, KeySort *ks, int64_t n, int nkeys, bool stable);
+// Sort ks->key[0..n) (filled by the caller), producing perm and off.
+int keysort_run(se2p_ctx *ctx, KeySort *ks, int64_t n, int nkeys);
 // Cell lists (ewald_tools.cpp:28-35); sublog > 0 appends a Morton sub-cell of
 // 2^sublog x 2^sublog to the key (nkeys = nsx*nsy*4^sublog).
 int cell_sort(se2p_ctx *ctx, const double *p, int64_t n, double Lx, double Ly, int nsx, int nsy,
-              int sublog, bool stable, KeySort *ks);
+              int sublog, KeySort *ks);
 // Exclusive scan of n ints into out[0..n] with one workgroup.
 void scan_ints(hipStream_t st, const int *a, int n, int *out);
+// map[k] = c for every k in [off[c], off[c+1]), c < n.
+void expand_map(hipStream_t st, const int *off, int n, int *map);
 
 // ---- real space (se2p_real.hip) -----------------------------------------
 struct RealGeom {

@@ -53,8 +53,7 @@ struct SpreadSrc {
 
 // ---- per-source gridding data -------------------------------------------
 __global__ void k_bin_keys(const double *__restrict__ p, int n, double Lx, double Ly, double h,
-                           int P, int Mx, int My, int nby, int *__restrict__ keys,
-                           int *__restrict__ slot, int *__restrict__ count)
+                           int P, int Mx, int My, int nby, int *__restrict__ keys)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -62,9 +61,7 @@ __global__ void k_bin_keys(const double *__restrict__ p, int n, double Lx, doubl
     double px, py;
     se2p_anchor(p[2 * i], p[2 * i + 1], Lx, Ly, h, P, &mx, &my, &px, &py);
     int ax = ((mx % Mx) + Mx) % Mx, ay = ((my % My) + My) % My;
-    const int k = (ax / kBin) * nby + ay / kBin;
-    keys[i] = k;
-    slot[i] = atomicAdd(&count[k], 1);
+    keys[i] = (ax / kBin) * nby + ay / kBin;
 }
 
 // FGG factors of mex_stokes_slp_kspace.cpp:95-99 with c = -2 xi^2 / eta.
@@ -135,28 +132,49 @@ struct SpreadArgs {
     double ch2;        // c h^2
     double *grids;     // [nq][Mx][My]
     int64_t gstride;   // Mx*My
+    // work split of the strip kernel: tile t owns chunks
+    // [chunk_off[t], chunk_off[t+1]) of up to `chunk` candidate sources each
+    const int *chunk_off;
+    int ntx, nty, chunk;
+    double *partial;   // [chunk id][256 nodes][nq] for tiles with several chunks
+    const int *block_tile;                 // workgroup -> tile
+    const int *rng_start, *rng_len;        // [tile][kTR] candidate runs
 };
 
-// Cyclic list of anchor bins whose anchors a satisfy (t - a) mod M in [0, P]
-// for some node t of [t0, t0 + kTile): anchors in [t0 - P, t0 + kTile - 1].
-__device__ inline int bins_for_tile(int t0, int M, int P, int nb, int *out)
+// Anchor bins whose anchors a satisfy (t - a) mod M in [0, P] for some node t
+// of [t0, t0 + kTile), i.e. anchors in the cyclic range [t0 - P, t0 + kTile - 1]:
+// at most two runs of consecutive bins, (b0[0], n[0]) and (b0[1], n[1]), in
+// cyclic order, each bin once.  Returns the number of runs.
+__host__ __device__ inline int bin_runs(int t0, int M, int P, int nb, int *b0, int *n)
 {
-    int n = 0;
-    int span = kTile + P;
-    if (span > M) span = M;
-    int start = t0 - P;
-    start = ((start % M) + M) % M;
-    for (int d = 0; d < span; ++d) {
-        int a = start + d;
-        if (a >= M) a -= M;
-        int b = a / kBin;
-        bool seen = false;
-        for (int k = 0; k < n; ++k) seen |= (out[k] == b);
-        if (!seen) out[n++] = b;
+    const int span = kTile + P < M ? kTile + P : M;
+    const int s = ((t0 - P) % M + M) % M;
+    const int e = s + span - 1;   // last anchor, possibly >= M (wraps)
+    if (e < M) {
+        b0[0] = s / kBin;
+        n[0] = e / kBin - s / kBin + 1;
+        return 1;
     }
-    (void)nb;
-    return n;
+    const int b1 = s / kBin;
+    b0[0] = b1;
+    n[0] = nb - b1;
+    const int e2 = (e - M) / kBin;
+    const int n2 = (e2 + 1 < b1 ? e2 + 1 : b1);
+    if (n2 <= 0) return 1;
+    b0[1] = 0;
+    n[1] = n2;
+    return 2;
 }
+
+// Bin list in cyclic order: position p of the runs above.
+__host__ __device__ inline int bin_at(const int *b0, const int *n, int p)
+{
+    return p < n[0] ? b0[0] + p : b0[1] + (p - n[0]);
+}
+
+// Per-tile candidate ranges of the bin-sorted sources: for x-bin position xp
+// and y-run yr, slot xp*2 + yr holds (start, length) of a run of sources.
+constexpr int kTR = 2 * 20;   // slots per tile (x-bins <= 20 for P <= 126)
 
 template <int NQ, bool WRAP>
 __global__ __launch_bounds__(256) void k_spread(SpreadArgs a)
@@ -176,25 +194,22 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a)
     const int X0 = blockIdx.x * kTile, Y0 = blockIdx.y * kTile;
     fill_e1(E1, a.P, a.ch2);
     if (tid == 0) {
-        int bxl[32], byl[32];
-        int nbxl = bins_for_tile(X0, a.Mx, a.P, a.nbx, bxl);
-        int nbyl = bins_for_tile(Y0, a.My, a.P, a.nby, byl);
+        int xb0[2], xn[2], yb0[2], yn[2];
+        const int nxr = bin_runs(X0, a.Mx, a.P, a.nbx, xb0, xn);
+        const int nyr = bin_runs(Y0, a.My, a.P, a.nby, yb0, yn);
+        const int nbxl = xn[0] + (nxr > 1 ? xn[1] : 0);
         int nr = 0, tot = 0;
         for (int i = 0; i < nbxl; ++i) {
-            // y bins for this x bin form runs of consecutive keys
-            int k = 0;
-            while (k < nbyl) {
-                int b0 = byl[k], len = 1;
-                while (k + len < nbyl && byl[k + len] == b0 + len) ++len;
-                int key0 = bxl[i] * a.nby + b0;
-                int s0 = a.boff[key0], s1 = a.boff[key0 + len];
+            const int bx = bin_at(xb0, xn, i);
+            for (int r = 0; r < nyr; ++r) {
+                const int key0 = bx * a.nby + yb0[r];
+                const int s0 = a.boff[key0], s1 = a.boff[key0 + yn[r]];
                 if (s1 > s0 && nr < kMaxRanges) {
                     rstart[nr] = s0;
                     rpref[nr] = tot;
                     tot += s1 - s0;
                     ++nr;
                 }
-                k += len;
             }
         }
         rpref[nr] = tot;
@@ -274,6 +289,260 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a)
         const int64_t idx = (int64_t)X * a.My + Y;
 #pragma unroll
         for (int c = 0; c < NQ; ++c) a.grids[c * a.gstride + idx] = acc[c];
+    }
+}
+
+
+// Strip variant (grids with Mx > P and My > P + kPad): a lane owns a column
+// strip of 4 nodes (X, Ys..Ys+3) of the 16x16 tile, so the x factor and the
+// window test are shared by 4 nodes, and the y factors are zero-padded by
+// kPad on both sides so the y overlap needs no test.  The candidate list of
+// the tile is cut into batches of kSB sources; wavefront w takes batches
+// w, w+4, ... into its own LDS area, and the four partial tiles are added in
+// a fixed order at the end (deterministic).
+constexpr int kPad = 3;
+
+// sources per wavefront batch: fewer when many grids share the LDS
+__host__ __device__ constexpr int strip_batch(int NQ) { return NQ <= 4 ? 32 : 16; }
+
+__host__ __device__ inline size_t strip_lds_doubles(int P, int NQ)
+{
+    const size_t sb = strip_batch(NQ);
+    const size_t P1 = P + 1, PW = P1 + 2 * kPad;
+    const size_t per_wave = sb * (P1 + PW + NQ) + sb;   // + sb doubles holding 2*sb ints
+    const size_t batch = 4 * per_wave + P1;
+    const size_t red = 64 * 4 * (size_t)NQ;
+    return batch > red ? batch : red;
+}
+
+template <int NQ>
+__global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
+{
+    constexpr int kSB = strip_batch(NQ);
+    extern __shared__ double lds[];
+    const int P1 = a.P + 1, PW = P1 + 2 * kPad;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const size_t per_wave = (size_t)kSB * (P1 + PW + NQ) + kSB;
+    double *WX = lds + wave * per_wave;       // [kSB][P1]
+    double *WY = WX + kSB * P1;               // [kSB][PW]
+    double *Q = WY + kSB * PW;                // [kSB][NQ]
+    int *AXY = (int *)(Q + kSB * NQ);         // [kSB][2]
+    double *E1 = lds + 4 * per_wave;          // [P1]
+    __shared__ int rstart[kMaxRanges], rpref[kMaxRanges + 1];
+    __shared__ int nrange;
+
+    // block -> (tile, chunk of the tile's candidate list)
+    const int ntiles = a.ntx * a.nty;
+    const int b = blockIdx.x;
+    if (b >= a.chunk_off[ntiles]) return;   // uniform over the workgroup
+    const int tile = a.block_tile[b];
+    const int chunk = b - a.chunk_off[tile];
+    const int nchunk = a.chunk_off[tile + 1] - a.chunk_off[tile];
+    const int X0 = (tile / a.nty) * kTile, Y0 = (tile % a.nty) * kTile;
+    fill_e1(E1, a.P, a.ch2);
+    if (threadIdx.x < 64) {
+        // candidate runs of the tile (precomputed), prefix sums by wave 0
+        const int l = threadIdx.x;
+        const int st = l < kTR ? a.rng_start[tile * kTR + l] : 0;
+        const int ln = l < kTR ? a.rng_len[tile * kTR + l] : 0;
+        int x = ln;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d);
+            if (l >= d) x += y;
+        }
+        if (l < kTR) {
+            rstart[l] = st;
+            rpref[l + 1] = x;
+        }
+        if (l == 0) {
+            rpref[0] = 0;
+            nrange = kTR;
+        }
+    }
+    __syncthreads();
+    const int c0 = chunk * a.chunk;
+    const int c1 = min(rpref[nrange], c0 + a.chunk);
+    const int X = X0 + (lane & 15);
+    const int Ys = Y0 + 4 * (lane >> 4);
+    double acc[4][NQ];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) acc[k][c] = 0.0;
+
+    const int nbatch = (c1 - c0 + kSB - 1) / kSB;
+    const int rounds = (nbatch + 3) / 4;
+    for (int round = 0; round < rounds; ++round) {
+        const int base = c0 + (round * 4 + wave) * kSB;
+        if (base < c1 && lane < 2 * kSB) {
+            const int sl = lane & (kSB - 1);
+            const int v = base + sl;
+            if (v < c1) {
+                int r = 0;
+                while (rpref[r + 1] <= v) ++r;
+                const int k = rstart[r] + (v - rpref[r]);
+                if (lane < kSB) {
+                    double w0 = a.s.fgg[k];
+                    const double e3x = a.s.fgg[a.s.ns + k];
+                    for (int i = 0; i < P1; ++i) {
+                        WX[sl * P1 + i] = w0 * E1[i];
+                        w0 *= e3x;
+                    }
+                    const int2 an = a.s.anchor[k];
+                    AXY[2 * sl] = an.x;
+                    AXY[2 * sl + 1] = an.y;
+#pragma unroll
+                    for (int c = 0; c < NQ; ++c) Q[sl * NQ + c] = a.s.q[(int64_t)c * a.s.ns + k];
+                } else {
+                    double w0 = 1.0;
+                    const double e3y = a.s.fgg[2 * a.s.ns + k];
+                    double *row = WY + sl * PW;
+                    for (int j = 0; j < kPad; ++j) row[j] = 0.0;
+                    for (int j = 0; j < P1; ++j) {
+                        row[kPad + j] = w0 * E1[j];
+                        w0 *= e3y;
+                    }
+                    for (int j = kPad + P1; j < PW; ++j) row[j] = 0.0;
+                }
+            }
+        }
+        __syncthreads();
+        if (base < c1) {
+            const int nb = min(kSB, c1 - base);
+#pragma unroll 2
+            for (int s = 0; s < nb; ++s) {
+                int i = X - AXY[2 * s];
+                i += (i < 0) ? a.Mx : 0;
+                int j0 = Ys - AXY[2 * s + 1];
+                j0 += (j0 < -kPad) ? a.My : 0;
+                if (i <= a.P && j0 <= a.P) {
+                    const double wx = WX[s * P1 + i];
+                    const double *wy = WY + s * PW + kPad + j0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const double wgt = wx * wy[k];
+#pragma unroll
+                        for (int c = 0; c < NQ; ++c) acc[k][c] += wgt * Q[s * NQ + c];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // fixed-order reduction of the four wavefronts' tiles, one wave at a time
+    double *red = lds;   // [64][4*NQ]; the batch areas are dead now
+    for (int w = 1; w < 4; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int c = 0; c < NQ; ++c) red[lane * 4 * NQ + k * NQ + c] = acc[k][c];
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int c = 0; c < NQ; ++c) acc[k][c] += red[lane * 4 * NQ + k * NQ + c];
+        }
+        __syncthreads();
+    }
+    if (wave != 0) return;
+    if (nchunk == 1) {
+        if (X < a.Mx) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int Y = Ys + k;
+                if (Y >= a.My) break;
+                const int64_t idx = (int64_t)X * a.My + Y;
+#pragma unroll
+                for (int c = 0; c < NQ; ++c) a.grids[c * a.gstride + idx] = acc[k][c];
+            }
+        }
+    } else {
+        // partial tile, added to the others in chunk order by k_spread_combine
+        double *pt = a.partial + (int64_t)b * 256 * NQ;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) pt[(c * 4 + k) * 64 + lane] = acc[k][c];
+    }
+}
+
+// Tiles whose candidates were split over several workgroups: add the partial
+// tiles in chunk order (deterministic) and write the grid.
+template <int NQ>
+__global__ __launch_bounds__(64) void k_spread_combine(SpreadArgs a)
+{
+    const int tile = blockIdx.x;
+    const int b0 = a.chunk_off[tile], b1 = a.chunk_off[tile + 1];
+    if (b1 - b0 <= 1) return;
+    const int lane = threadIdx.x;
+    const int X = (tile / a.nty) * kTile + (lane & 15);
+    const int Ys = (tile % a.nty) * kTile + 4 * (lane >> 4);
+    if (X >= a.Mx) return;
+    double acc[4][NQ];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) acc[k][c] = 0.0;
+    for (int b = b0; b < b1; ++b) {
+        const double *pt = a.partial + (int64_t)b * 256 * NQ;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) acc[k][c] += pt[(c * 4 + k) * 64 + lane];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int Y = Ys + k;
+        if (Y >= a.My) break;
+        const int64_t idx = (int64_t)X * a.My + Y;
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) a.grids[c * a.gstride + idx] = acc[k][c];
+    }
+}
+
+// Work split per tile (one wavefront per tile): the candidate runs of the
+// bin-sorted sources, the number of spreading chunks (at least one per tile,
+// so every grid node is written) and of interpolation chunks of 256 targets.
+__global__ __launch_bounds__(64) void k_tile_work(SpreadArgs a, const int *__restrict__ tboff,
+                                                  int *__restrict__ scnt, int *__restrict__ icnt,
+                                                  int *__restrict__ rng_start,
+                                                  int *__restrict__ rng_len)
+{
+    const int tile = blockIdx.x;
+    const int l = threadIdx.x;
+    const int tx = tile / a.nty, ty = tile % a.nty;
+    int xb0[2], xn[2], yb0[2], yn[2];
+    const int nxr = bin_runs(tx * kTile, a.Mx, a.P, a.nbx, xb0, xn);
+    const int nyr = bin_runs(ty * kTile, a.My, a.P, a.nby, yb0, yn);
+    const int nbxl = xn[0] + (nxr > 1 ? xn[1] : 0);
+    int st = 0, ln = 0;
+    if (l < kTR) {
+        const int xp = l >> 1, yr = l & 1;
+        if (xp < nbxl && yr < nyr) {
+            const int key0 = bin_at(xb0, xn, xp) * a.nby + yb0[yr];
+            st = a.boff[key0];
+            ln = a.boff[key0 + yn[yr]] - st;
+        }
+        rng_start[tile * kTR + l] = st;
+        rng_len[tile * kTR + l] = ln;
+    }
+    int tot = ln;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) tot += __shfl_xor(tot, d);
+    if (l == 0) {
+        scnt[tile] = max(1, (tot + a.chunk - 1) / a.chunk);
+        // targets of the tile: bins (2tx..2tx+1, 2ty..2ty+1)
+        const int bx0 = 2 * tx, by0 = 2 * ty;
+        const int nyb = min(2, a.nby - by0);
+        int nt = tboff[bx0 * a.nby + by0 + nyb] - tboff[bx0 * a.nby + by0];
+        if (bx0 + 1 < a.nbx)
+            nt += tboff[(bx0 + 1) * a.nby + by0 + nyb] - tboff[(bx0 + 1) * a.nby + by0];
+        icnt[tile] = (nt + 255) / 256;
     }
 }
 
@@ -422,6 +691,104 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs a)
     }
 }
 
+
+// Tile variant: one workgroup per 16x16 block of target anchors.  The grid
+// values the block's targets can touch, a (16+P)^2 region per grid, are staged
+// in LDS once (as (G1, G2) pairs), then each thread interpolates one target at
+// a time from LDS.  Targets come sorted by 8x8 anchor bin (the tile's targets
+// are two runs of that order).
+struct InterpTileArgs {
+    const double *ptar;
+    const int *perm;       // bin order -> caller's target index
+    const int *boff;       // [nbx*nby+1] bin offsets of the targets
+    int nbx, nby;
+    const int *chunk_off;  // tile t owns chunks [chunk_off[t], chunk_off[t+1]) of 256 targets
+    const int *block_tile; // workgroup -> tile
+    int ntx, nty;
+    const double *grids;   // [2*nr][Mx][My] real
+    int64_t gstride;
+    int Mx, My, P, nr;
+    double Lx, Ly, h, c, w, ch2, scale;
+    double *u;
+    int64_t ustride;
+};
+
+__global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
+{
+    extern __shared__ double lds[];
+    const int P1 = a.P + 1;
+    const int R = kTile + a.P;            // region edge
+    double2 *G = (double2 *)lds;          // [R][R]
+    double *E1 = (double *)(G + R * R);   // [P1]
+    const int ntiles = a.ntx * a.nty;
+    const int b = blockIdx.x;
+    if (b >= a.chunk_off[ntiles]) return;   // uniform over the workgroup
+    const int tile = a.block_tile[b];
+    const int chunk = b - a.chunk_off[tile];
+    const int tx = tile / a.nty, ty = tile % a.nty;
+    const int bx0 = 2 * tx, by0 = 2 * ty;
+    const int nyb = min(2, a.nby - by0);
+    int r0s = 0, r0e = 0, r1s = 0, r1e = 0;
+    {
+        const int k0 = bx0 * a.nby + by0;
+        r0s = a.boff[k0];
+        r0e = a.boff[k0 + nyb];
+        if (bx0 + 1 < a.nbx) {
+            const int k1 = (bx0 + 1) * a.nby + by0;
+            r1s = a.boff[k1];
+            r1e = a.boff[k1 + nyb];
+        }
+    }
+    const int n0 = r0e - r0s, total = n0 + (r1e - r1s);
+    const int v0 = chunk * 256, v1 = min(total, v0 + 256);
+    fill_e1(E1, a.P, a.ch2);
+    const int X0 = tx * kTile, Y0 = ty * kTile;
+    for (int r = 0; r < a.nr; ++r) {
+        const double *g1 = a.grids + (int64_t)(2 * r) * a.gstride;
+        const double *g2 = g1 + a.gstride;
+        __syncthreads();   // previous RHS done with G
+        for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
+            const int i = e / R, j = e % R;
+            int X = X0 + i, Y = Y0 + j;
+            X %= a.Mx;
+            Y %= a.My;
+            const int64_t idx = (int64_t)X * a.My + Y;
+            G[e] = make_double2(g1[idx], g2[idx]);
+        }
+        __syncthreads();
+        for (int v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+            const int k = v < n0 ? r0s + v : r1s + (v - n0);
+            const int t = a.perm[k];
+            int mx, my;
+            double px, py;
+            se2p_anchor(a.ptar[2 * t], a.ptar[2 * t + 1], a.Lx, a.Ly, a.h, a.P, &mx, &my, &px,
+                        &py);
+            double bx, e3x, e3y;
+            fgg_factors(px, py, a.c, a.w, a.h, &bx, &e3x, &e3y);
+            int li = ((mx % a.Mx) + a.Mx) % a.Mx - X0;
+            int lj = ((my % a.My) + a.My) % a.My - Y0;
+            double s1 = 0.0, s2 = 0.0;
+            double ex = bx;
+            const double2 *row = G + li * R + lj;
+            for (int i = 0; i < P1; ++i) {
+                double ey = ex * E1[i];
+                for (int j = 0; j < P1; ++j) {
+                    const double wgt = ey * E1[j];
+                    const double2 gv = row[j];
+                    s1 += wgt * gv.x;
+                    s2 += wgt * gv.y;
+                    ey *= e3y;
+                }
+                ex *= e3x;
+                row += R;
+            }
+            double *ur = a.u + (int64_t)r * a.ustride;
+            ur[2 * t] = s1 * a.scale;
+            ur[2 * t + 1] = s2 * a.scale;
+        }
+    }
+}
+
 __global__ void k_zero(double *__restrict__ u, int64_t n)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -502,27 +869,61 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     const int64_t GH = (int64_t)Mx * Mh;
     const int nbx = (Mx + kBin - 1) / kBin, nby = (My + kBin - 1) / kBin;
     const int nbins = nbx * nby;
+    // targets are binned separately unless they are the source array itself
+    const bool same_points = (g.ptar == g.psrc) && (nt == ns);
+
+    // kernel variants
+    const size_t lds_max = 65536;
+    const bool strip = (Mx > P) && (My > P + kPad) && sizeof(double) * strip_lds_doubles(P, nq) <= lds_max;
+    const size_t lds_strip = sizeof(double) * strip_lds_doubles(P, nq);
+    const size_t lds_gen = sizeof(double) * ((size_t)2 * kBatch * (P + 1) + (size_t)kBatch * nq + (P + 1)) +
+                           sizeof(int) * 2 * kBatch;
+    const bool wrap = (Mx <= P) || (My <= P);
+    const size_t lds_itile = sizeof(double2) * (size_t)(kTile + P) * (kTile + P) + sizeof(double) * (P + 1);
+    const bool itile = lds_itile <= lds_max;
 
     FftPlans *plans = nullptr;
     if (ns > 0)
         if (int rc = get_plans(ctx, Mx, My, nq, 2 * RB, &plans)) return rc;
 
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
-    size_t ks = keysort_bytes(ns, nbins);
+    const size_t ks = keysort_bytes(ns, nbins);
+    const size_t kt = (itile && !same_points) ? keysort_bytes(nt, nbins) : 0;
     const size_t nsz = (size_t)(ns > 0 ? ns : 1);
-    size_t need = ks + up(sizeof(int2) * nsz) + up(sizeof(double) * 3 * nsz) +
+    // work split of the tile kernels
+    const int ntx = (Mx + kTile - 1) / kTile, nty = (My + kTile - 1) / kTile;
+    const int ntiles = ntx * nty;
+    const int chunk = 4 * strip_batch(nq) * 4;   // 4 wavefronts x 4 batches
+    const int64_t tps_x = std::min<int64_t>(ntx, (kTile + P + kBin - 1) / kTile + 1);
+    const int64_t tps_y = std::min<int64_t>(nty, (kTile + P + kBin - 1) / kTile + 1);
+    const int64_t max_spread_chunks = ntiles + (tps_x * tps_y * ns + chunk - 1) / chunk;
+    const int64_t max_interp_chunks = ntiles + (nt + 255) / 256;
+    size_t need = ks + kt + up(sizeof(int2) * nsz) + up(sizeof(double) * 3 * nsz) +
                   up(sizeof(double) * nq * nsz) + up(sizeof(double) * nq * (size_t)G) +
-                  up(sizeof(double2) * nq * (size_t)GH);
+                  up(sizeof(double2) * nq * (size_t)GH) + 4 * up(sizeof(int) * (size_t)(ntiles + 1)) +
+                  2 * up(sizeof(int) * (size_t)ntiles * kTR) +
+                  up(sizeof(int) * (size_t)max_spread_chunks) + up(sizeof(int) * (size_t)max_interp_chunks) +
+                  (strip ? up(sizeof(double) * 256 * nq * (size_t)max_spread_chunks) : 0);
     if (int rc = ensure_workspace(ctx, need)) return rc;
     char *base = ctx->ws;
     size_t o = 0;
-    KeySort kss;
+    KeySort kss, kst;
     keysort_carve(base + o, ns, nbins, &kss); o += ks;
+    if (kt) { keysort_carve(base + o, nt, nbins, &kst); o += kt; }
     int2 *anchor = (int2 *)(base + o); o += up(sizeof(int2) * nsz);
     double *fgg = (double *)(base + o); o += up(sizeof(double) * 3 * nsz);
     double *q = (double *)(base + o); o += up(sizeof(double) * nq * nsz);
     double *grids = (double *)(base + o); o += up(sizeof(double) * nq * (size_t)G);
     double2 *spec = (double2 *)(base + o); o += up(sizeof(double2) * nq * (size_t)GH);
+    int *scnt = (int *)(base + o); o += up(sizeof(int) * (size_t)(ntiles + 1));
+    int *soffc = (int *)(base + o); o += up(sizeof(int) * (size_t)(ntiles + 1));
+    int *icnt = (int *)(base + o); o += up(sizeof(int) * (size_t)(ntiles + 1));
+    int *ioffc = (int *)(base + o); o += up(sizeof(int) * (size_t)(ntiles + 1));
+    int *rng_start = (int *)(base + o); o += up(sizeof(int) * (size_t)ntiles * kTR);
+    int *rng_len = (int *)(base + o); o += up(sizeof(int) * (size_t)ntiles * kTR);
+    int *smap = (int *)(base + o); o += up(sizeof(int) * (size_t)max_spread_chunks);
+    int *imap = (int *)(base + o); o += up(sizeof(int) * (size_t)max_interp_chunks);
+    double *partial = strip ? (double *)(base + o) : nullptr;
 
     const int threads = 256;
     if (ns == 0) {
@@ -535,20 +936,48 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     }
 
     stage_begin(ctx, ST_SORT);
-    keysort_clear(st, &kss, nbins);
     hipLaunchKernelGGL(k_bin_keys, dim3((unsigned)((ns + threads - 1) / threads)), dim3(threads), 0,
-                       st, g.psrc, (int)ns, g.Lx, g.Ly, h, P, Mx, My, nby, kss.key, kss.slot,
-                       kss.count);
-    if (int rc = keysort_finish(ctx, &kss, ns, nbins, true)) return rc;
+                       st, g.psrc, (int)ns, g.Lx, g.Ly, h, P, Mx, My, nby, kss.key);
+    if (int rc = keysort_run(ctx, &kss, ns, nbins)) return rc;
+    if (kt) {
+        hipLaunchKernelGGL(k_bin_keys, dim3((unsigned)((nt + threads - 1) / threads)), dim3(threads),
+                           0, st, g.ptar, (int)nt, g.Lx, g.Ly, h, P, Mx, My, nby, kst.key);
+        if (int rc = keysort_run(ctx, &kst, nt, nbins)) return rc;
+    }
+    const KeySort &tbins = same_points ? kss : kst;
+    SpreadArgs sa{};
+    sa.boff = kss.off;
+    sa.nbx = nbx;
+    sa.nby = nby;
+    sa.Mx = Mx;
+    sa.My = My;
+    sa.P = P;
+    sa.gstride = G;
+    sa.chunk_off = soffc;
+    sa.ntx = ntx;
+    sa.nty = nty;
+    sa.chunk = chunk;
+    sa.partial = partial;
+    sa.block_tile = smap;
+    sa.rng_start = rng_start;
+    sa.rng_len = rng_len;
+    if (strip || itile) {
+        hipLaunchKernelGGL(k_tile_work, dim3(ntiles), dim3(64), 0, st, sa, tbins.off, scnt, icnt,
+                           rng_start, rng_len);
+        scan_ints(st, scnt, ntiles, soffc);
+        scan_ints(st, icnt, ntiles, ioffc);
+        const unsigned mb = (unsigned)((ntiles + threads - 1) / threads);
+        (void)mb;
+        expand_map(st, soffc, ntiles, smap);
+        expand_map(st, ioffc, ntiles, imap);
+    }
     stage_end(ctx, ST_SORT);
 
-    const bool wrap = (Mx <= P) || (My <= P);
-    const size_t lds = sizeof(double) * ((size_t)2 * kBatch * (P + 1) + (size_t)kBatch * nq + (P + 1)) +
-                       sizeof(int) * 2 * kBatch;
     const double ch2 = c * h * h;
     double s_int = 4 * g.xi * g.xi / g.eta;                  // :325-328
     s_int = s_int * s_int * h * h / SE2P_PI;
     const double interp_scale = s_int / (4 * SE2P_PI);
+    const dim3 tiles((Mx + kTile - 1) / kTile, (My + kTile - 1) / kTile);
 
     for (int r0 = 0; r0 < nrhs; r0 += RB) {
         stage_begin(ctx, ST_SPREAD);
@@ -560,24 +989,19 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
             hipLaunchKernelGGL(k_gather_grid_src<1>, dim3((unsigned)((ns + threads - 1) / threads)),
                                dim3(threads), 0, st, g.psrc, f, nv, kss.perm, (int)ns, r0, RB,
                                dstride, g.Lx, g.Ly, h, P, Mx, My, c, g.w, anchor, fgg, q);
-        SpreadArgs sa;
         sa.s.anchor = anchor;
         sa.s.fgg = fgg;
         sa.s.q = q;
         sa.s.ns = ns;
-        sa.boff = kss.off;
-        sa.nbx = nbx;
-        sa.nby = nby;
-        sa.Mx = Mx;
-        sa.My = My;
-        sa.P = P;
         sa.ch2 = ch2;
         sa.grids = grids;
-        sa.gstride = G;
-        dim3 sg((Mx + kTile - 1) / kTile, (My + kTile - 1) / kTile);
 #define SE2P_SPREAD(NQV)                                                                        \
-    if (wrap) hipLaunchKernelGGL((k_spread<NQV, true>), sg, dim3(256), lds, st, sa);            \
-    else hipLaunchKernelGGL((k_spread<NQV, false>), sg, dim3(256), lds, st, sa);
+    if (strip) {                                                                                \
+        hipLaunchKernelGGL((k_spread_strip<NQV>), dim3((unsigned)max_spread_chunks), dim3(256),  \
+                           lds_strip, st, sa);                                                  \
+        hipLaunchKernelGGL((k_spread_combine<NQV>), dim3(ntiles), dim3(64), 0, st, sa);         \
+    } else if (wrap) hipLaunchKernelGGL((k_spread<NQV, true>), tiles, dim3(256), lds_gen, st, sa); \
+    else hipLaunchKernelGGL((k_spread<NQV, false>), tiles, dim3(256), lds_gen, st, sa);
         switch (nq) {
         case 2: SE2P_SPREAD(2); break;
         case 3: SE2P_SPREAD(3); break;
@@ -628,27 +1052,57 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         stage_end(ctx, ST_FFT_INV);
 
         stage_begin(ctx, ST_INTERP);
-        InterpArgs ia;
-        ia.ptar = g.ptar;
-        ia.nt = nt;
-        ia.grids = grids;
-        ia.gstride = G;
-        ia.Mx = Mx;
-        ia.My = My;
-        ia.P = P;
-        ia.nr = RB;
-        ia.Lx = g.Lx;
-        ia.Ly = g.Ly;
-        ia.h = h;
-        ia.c = c;
-        ia.w = g.w;
-        ia.ch2 = ch2;
-        ia.scale = interp_scale;
-        ia.u = u + (int64_t)r0 * ustride;
-        ia.ustride = ustride;
-        const unsigned iblocks = (unsigned)((nt + threads - 1) / threads);
-        if (RB == 1) hipLaunchKernelGGL(k_interp<1>, dim3(iblocks), dim3(threads), 0, st, ia);
-        else hipLaunchKernelGGL(k_interp<4>, dim3(iblocks), dim3(threads), 0, st, ia);
+        if (itile) {
+            InterpTileArgs ia;
+            ia.ptar = g.ptar;
+            ia.perm = tbins.perm;
+            ia.boff = tbins.off;
+            ia.nbx = nbx;
+            ia.nby = nby;
+            ia.chunk_off = ioffc;
+            ia.block_tile = imap;
+            ia.ntx = ntx;
+            ia.nty = nty;
+            ia.grids = grids;
+            ia.gstride = G;
+            ia.Mx = Mx;
+            ia.My = My;
+            ia.P = P;
+            ia.nr = RB;
+            ia.Lx = g.Lx;
+            ia.Ly = g.Ly;
+            ia.h = h;
+            ia.c = c;
+            ia.w = g.w;
+            ia.ch2 = ch2;
+            ia.scale = interp_scale;
+            ia.u = u + (int64_t)r0 * ustride;
+            ia.ustride = ustride;
+            hipLaunchKernelGGL(k_interp_tile, dim3((unsigned)max_interp_chunks), dim3(256), lds_itile,
+                               st, ia);
+        } else {
+            InterpArgs ia;
+            ia.ptar = g.ptar;
+            ia.nt = nt;
+            ia.grids = grids;
+            ia.gstride = G;
+            ia.Mx = Mx;
+            ia.My = My;
+            ia.P = P;
+            ia.nr = RB;
+            ia.Lx = g.Lx;
+            ia.Ly = g.Ly;
+            ia.h = h;
+            ia.c = c;
+            ia.w = g.w;
+            ia.ch2 = ch2;
+            ia.scale = interp_scale;
+            ia.u = u + (int64_t)r0 * ustride;
+            ia.ustride = ustride;
+            const unsigned iblocks = (unsigned)((nt + threads - 1) / threads);
+            if (RB == 1) hipLaunchKernelGGL(k_interp<1>, dim3(iblocks), dim3(threads), 0, st, ia);
+            else hipLaunchKernelGGL(k_interp<4>, dim3(iblocks), dim3(threads), 0, st, ia);
+        }
         SE2P_HIP(hipGetLastError());
         stage_end(ctx, ST_INTERP);
     }

@@ -54,9 +54,10 @@ struct RealArgs {
     const int *tperm;      // that order -> caller's index
     const int *toff;       // [ncell*16+1] run offsets of (cell, sub-cell) keys
     const int *chunk_off;  // [ncell+1] exclusive scan of ceil(targets/64)
+    const int *block_cell; // workgroup -> cell
     const double2 *spos;   // sources in cell order
     const double *q;       // densities in cell order, [nrhs][ncomp][ns]
-    const int *soff;       // [ncell+1]
+    const int *soff;       // [ncell*16+1] run offsets of the sources' (cell, sub-cell) keys
     int64_t ns;
     int nsx, nsy, ncell, nrhs;
     double Lx, Ly, xi, xi2, cut2, self;
@@ -131,17 +132,6 @@ __device__ __forceinline__ void diff(double sx, double sy, double tx, double ty,
     }
 }
 
-__device__ __forceinline__ int find_chunk_cell(const int *chunk_off, int ncell, int b)
-{
-    int lo = 0, hi = ncell;   // largest c with chunk_off[c] <= b
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (chunk_off[mid] <= b) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // ---- the cell-list kernel ------------------------------------------------
 template <int KIND, int RB>
 __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restrict__ spos,
@@ -160,7 +150,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
 
     const int b = blockIdx.x;
     if (b >= a.chunk_off[a.ncell]) return;   // uniform over the workgroup
-    const int cell = find_chunk_cell(a.chunk_off, a.ncell, b);
+    const int cell = a.block_cell[b];
     const int sub = 1 << (2 * kSubLog);
     const int t0 = a.toff[cell * sub] + 64 * (b - a.chunk_off[cell]);
     const int t1 = a.toff[(cell + 1) * sub];
@@ -186,9 +176,9 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
 
         // own cell: every pair, no cutoff
         {
-            const int s1 = soff[cell + 1];
+            const int s1 = soff[(cell + 1) * sub];
 #pragma unroll 2
-            for (int s = soff[cell] + wave; s < s1; s += kWaves) {
+            for (int s = soff[cell * sub] + wave; s < s1; s += kWaves) {
                 const double2 sp = spos[s];
                 double d1, d2;
                 diff<KIND>(sp.x, sp.y, tx, ty, 0.0, 0.0, d1, d2);
@@ -211,11 +201,11 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
             const int qx = cx + nb_dx(j), qy = cy + nb_dy(j);
             const int wx = (qx + a.nsx) % a.nsx, wy = (qy + a.nsy) % a.nsy;
             const int sb = wy * a.nsx + wx;
-            const int s1 = soff[sb + 1];
+            const int s1 = soff[(sb + 1) * sub];
             const double zx = (a.Lx * (qx - wx)) / a.nsx;   // :248-249
             const double zy = (a.Ly * (qy - wy)) / a.nsy;
 #pragma unroll 2
-            for (int s = soff[sb] + wave; s < s1; s += kWaves) {
+            for (int s = soff[sb * sub] + wave; s < s1; s += kWaves) {
                 const double2 sp = spos[s];
                 double d1, d2;
                 diff<KIND>(sp.x, sp.y, tx, ty, zx, zy, d1, d2);
@@ -265,14 +255,14 @@ __global__ __launch_bounds__(64) void k_real_count(RealArgs a)
     const int lane = threadIdx.x;
     const int b = blockIdx.x;
     if (b >= a.chunk_off[a.ncell]) return;
-    const int cell = find_chunk_cell(a.chunk_off, a.ncell, b);
+    const int cell = a.block_cell[b];
     const int sub = 1 << (2 * kSubLog);
     const int t = a.toff[cell * sub] + 64 * (b - a.chunk_off[cell]) + lane;
     if (t >= a.toff[(cell + 1) * sub]) return;
     const double2 p = a.tpos[t];
     unsigned long long n_in = 0, n_out = 0;
     const int cx = cell % a.nsx, cy = cell / a.nsx;
-    for (int s = a.soff[cell]; s < a.soff[cell + 1]; ++s) {
+    for (int s = a.soff[cell * sub]; s < a.soff[(cell + 1) * sub]; ++s) {
         double d1, d2;
         diff<1>(a.spos[s].x, a.spos[s].y, p.x, p.y, 0.0, 0.0, d1, d2);
         n_in += se2p_sqsum(d1, d2) != 0.0 ? 1 : 0;
@@ -283,7 +273,7 @@ __global__ __launch_bounds__(64) void k_real_count(RealArgs a)
         const int sb = wy * a.nsx + wx;
         const double zx = (a.Lx * (qx - wx)) / a.nsx;
         const double zy = (a.Ly * (qy - wy)) / a.nsy;
-        for (int s = a.soff[sb]; s < a.soff[sb + 1]; ++s) {
+        for (int s = a.soff[sb * sub]; s < a.soff[(sb + 1) * sub]; ++s) {
             double d1, d2;
             diff<1>(a.spos[s].x, a.spos[s].y, p.x, p.y, zx, zy, d1, d2);
             if (se2p_sqsum(d1, d2) < a.cut2) ++n_in;
@@ -369,35 +359,44 @@ int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const 
     hipStream_t st = ctx->stream;
     const int ncell = g.nsx * g.nsy;
     const int sub = 1 << (2 * kSubLog);
-    const int ntkeys = ncell * sub;
+    const int nkeys = ncell * sub;
     const int64_t ns = g.ns, nt = g.nt;
     const bool counting = pair_counts != nullptr;
     if (nt == 0) return 0;
     if (kind == 0 && upload_table(ctx)) return SE2P_ERR_HIP;
+    // targets that are the source array itself share the sources' cell lists
+    const bool same_points = (g.ptar == g.psrc) && (nt == ns);
 
     const int ncomp = kind == 0 ? 2 : 3;
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t nsz = (size_t)(ns > 0 ? ns : 1);
-    const size_t ks_s = keysort_bytes(ns, ncell), ks_t = keysort_bytes(nt, ntkeys);
+    const size_t ks_s = keysort_bytes(ns, nkeys);
+    const size_t ks_t = same_points ? 0 : keysort_bytes(nt, nkeys);
     size_t need = ks_s + ks_t + up(sizeof(double2) * nsz) + up(sizeof(double2) * (size_t)nt) +
                   up(sizeof(double) * (size_t)ncomp * nrhs * nsz) +
-                  2 * up(sizeof(int) * (size_t)(ncell + 1)) + up(2 * sizeof(unsigned long long));
+                  2 * up(sizeof(int) * (size_t)(ncell + 1)) + up(2 * sizeof(unsigned long long)) +
+                  up(sizeof(int) * (size_t)((nt + 63) / 64 + ncell));
     if (int rc = ensure_workspace(ctx, need)) return rc;
     char *base = ctx->ws;
     size_t o = 0;
     KeySort kss, kst;
-    keysort_carve(base + o, ns, ncell, &kss); o += ks_s;
-    keysort_carve(base + o, nt, ntkeys, &kst); o += ks_t;
+    keysort_carve(base + o, ns, nkeys, &kss); o += ks_s;
+    if (!same_points) { keysort_carve(base + o, nt, nkeys, &kst); o += ks_t; }
     double2 *spos = (double2 *)(base + o); o += up(sizeof(double2) * nsz);
     double2 *tpos = (double2 *)(base + o); o += up(sizeof(double2) * (size_t)nt);
     double *q = (double *)(base + o); o += up(sizeof(double) * (size_t)ncomp * nrhs * nsz);
     int *ccnt = (int *)(base + o); o += up(sizeof(int) * (size_t)(ncell + 1));
     int *coff = (int *)(base + o); o += up(sizeof(int) * (size_t)(ncell + 1));
-    unsigned long long *cnt = (unsigned long long *)(base + o);
+    unsigned long long *cnt = (unsigned long long *)(base + o); o += up(2 * sizeof(unsigned long long));
+    int *bmap = (int *)(base + o);
 
+    // Sources and targets in (cell, Morton sub-cell) order, stable: a cell's
+    // sources are summed in a fixed order on every run.
     stage_begin(ctx, ST_SORT);
-    if (int rc = cell_sort(ctx, g.psrc, ns, g.Lx, g.Ly, g.nsx, g.nsy, 0, true, &kss)) return rc;
-    if (int rc = cell_sort(ctx, g.ptar, nt, g.Lx, g.Ly, g.nsx, g.nsy, kSubLog, false, &kst)) return rc;
+    if (int rc = cell_sort(ctx, g.psrc, ns, g.Lx, g.Ly, g.nsx, g.nsy, kSubLog, &kss)) return rc;
+    if (!same_points)
+        if (int rc = cell_sort(ctx, g.ptar, nt, g.Lx, g.Ly, g.nsx, g.nsy, kSubLog, &kst)) return rc;
+    const KeySort &tks = same_points ? kss : kst;
     const int threads = 256;
     if (ns > 0) {
         const unsigned blocks = (unsigned)((ns + threads - 1) / threads);
@@ -408,18 +407,24 @@ int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const 
             hipLaunchKernelGGL(k_gather_sources<1>, dim3(blocks), dim3(threads), 0, st, g.psrc, f,
                                n, kss.perm, (int)ns, nrhs, dstride, spos, q);
     }
-    hipLaunchKernelGGL(k_gather_targets, dim3((unsigned)((nt + threads - 1) / threads)),
-                       dim3(threads), 0, st, g.ptar, kst.perm, (int)nt, tpos);
+    if (same_points) {
+        tpos = spos;
+    } else {
+        hipLaunchKernelGGL(k_gather_targets, dim3((unsigned)((nt + threads - 1) / threads)),
+                           dim3(threads), 0, st, g.ptar, tks.perm, (int)nt, tpos);
+    }
     hipLaunchKernelGGL(k_chunk_counts, dim3((ncell + threads - 1) / threads), dim3(threads), 0, st,
-                       kst.off, ncell, sub, ccnt);
+                       tks.off, ncell, sub, ccnt);
     scan_ints(st, ccnt, ncell, coff);
+    expand_map(st, coff, ncell, bmap);
     stage_end(ctx, ST_SORT);
 
     RealArgs a;
     a.tpos = tpos;
-    a.tperm = kst.perm;
-    a.toff = kst.off;
+    a.tperm = tks.perm;
+    a.toff = tks.off;
     a.chunk_off = coff;
+    a.block_cell = bmap;
     a.spos = spos;
     a.q = q;
     a.soff = kss.off;

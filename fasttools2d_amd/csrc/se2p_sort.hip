@@ -1,130 +1,254 @@
-// se2p_sort.hip -- counting sort of point sets by an integer key (cell lists
+// se2p_sort.hip -- stable sort of point sets by an integer key (cell lists
 // and gridding bins).
 //
 // The reference builds its cell lists with a serial, stable counting sort
-// (ewald_tools.cpp:7-80): points of a box keep their input order.  Here:
-//   1. key per point, slot = atomicAdd(count[key])      (slot order arbitrary)
-//   2. exclusive scan of the counts                      (one workgroup)
-//   3. scatter i to off[key] + slot
-//   4. (stable sorts only) rank each element inside its key run by input
-//      index and scatter again, which restores input order within a run.
-// Step 4 makes the order -- and hence every floating-point summation order
-// downstream -- independent of atomic timing.  Sorts whose order cannot
-// change any result (targets of the real-space sum) skip it.
+// (ewald_tools.cpp:7-80): points of a box keep their input order.  Here a
+// least-significant-digit radix sort with 8-bit digits does the same job on
+// the GPU and is stable by construction, so the order -- and every
+// floating-point summation order downstream -- is reproducible run to run.
+// One pass is three launches:
+//   hist    : per-block digit histogram (LDS counters), stored digit-major;
+//   scan    : exclusive scan of the histograms -> global digit/block offsets;
+//   scatter : each block ranks its elements stably (wavefront ballots decide
+//             equal digits, LDS prefix over the four wavefronts and over the
+//             block's four rounds) and writes them to their final slots.
+// A final launch turns the sorted keys into run offsets (lower bound per key).
 #include "se2p_device.h"
 #include "se2p_internal.h"
 
 namespace se2p {
 
+constexpr int kRadixBlock = 1024;   // elements per block (4 rounds of 256)
+
 static size_t up256(size_t b) { return (b + 255) & ~size_t(255); }
+
+static int key_bits(int nkeys)
+{
+    int b = 1;
+    while ((1LL << b) < (long long)nkeys) ++b;
+    return b;
+}
+
+static int radix_blocks(int64_t n) { return (int)((n + kRadixBlock - 1) / kRadixBlock); }
 
 size_t keysort_bytes(int64_t n, int nkeys)
 {
-    size_t nn = (size_t)(n > 0 ? n : 1);
-    return 4 * up256(sizeof(int) * nn) + 2 * up256(sizeof(int) * ((size_t)nkeys + 1));
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    const size_t nb = (size_t)(radix_blocks(n) > 0 ? radix_blocks(n) : 1);
+    return 4 * up256(sizeof(int) * nn) + up256(sizeof(int) * ((size_t)nkeys + 1)) +
+           2 * up256(sizeof(int) * (256 * nb + 256));
 }
 
 void keysort_carve(char *base, int64_t n, int nkeys, KeySort *ks)
 {
-    size_t nn = (size_t)(n > 0 ? n : 1);
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    const size_t nb = (size_t)(radix_blocks(n) > 0 ? radix_blocks(n) : 1);
     size_t o = 0;
-    ks->key = (int *)(base + o);   o += up256(sizeof(int) * nn);
-    ks->slot = (int *)(base + o);  o += up256(sizeof(int) * nn);
-    ks->tmp = (int *)(base + o);   o += up256(sizeof(int) * nn);
-    ks->perm = (int *)(base + o);  o += up256(sizeof(int) * nn);
-    ks->count = (int *)(base + o); o += up256(sizeof(int) * ((size_t)nkeys + 1));
-    ks->off = (int *)(base + o);   o += up256(sizeof(int) * ((size_t)nkeys + 1));
+    ks->key = (int *)(base + o);    o += up256(sizeof(int) * nn);
+    ks->key2 = (int *)(base + o);   o += up256(sizeof(int) * nn);
+    ks->tmp = (int *)(base + o);    o += up256(sizeof(int) * nn);
+    ks->perm = (int *)(base + o);   o += up256(sizeof(int) * nn);
+    ks->off = (int *)(base + o);    o += up256(sizeof(int) * ((size_t)nkeys + 1));
+    ks->hist = (int *)(base + o);   o += up256(sizeof(int) * (256 * nb + 256));   // + digit totals
+    ks->hoff = (int *)(base + o);   o += up256(sizeof(int) * (256 * nb + 256));
 }
 
-__global__ void k_fill_int(int *__restrict__ a, int n, int v)
+// Block-wide exclusive scan of one value per thread (256 threads); returns
+// the exclusive prefix and writes the block total to *total.
+__device__ inline int block_exclusive_scan_256(int v, int *wsum, int *total)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
-}
-
-// Exclusive scan of a[0..n) into out[0..n] (out[n] = total), one workgroup.
-__global__ __launch_bounds__(1024) void k_scan_one_block(const int *__restrict__ a, int n,
-                                                         int *__restrict__ out)
-{
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int b = t * per, e = min(n, b + per);
-    int s = 0;
-    for (int i = b; i < e; ++i) s += a[i];
-    part[t] = s;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partials
-    for (int d = 1; d < 1024; d <<= 1) {
-        int v = (t >= d) ? part[t - d] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    int run = (t > 0) ? part[t - 1] : 0;
-    for (int i = b; i < e; ++i) {
-        out[i] = run;
-        run += a[i];
-    }
-    if (t == 1023) out[n] = part[1023];
+    int base = 0;
+    for (int w = 0; w < wave; ++w) base += wsum[w];
+    const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    *total = tot;
+    return base + x - v;
 }
 
-__global__ void k_scatter(const int *__restrict__ key, const int *__restrict__ slot,
-                          const int *__restrict__ off, int n, int *__restrict__ dst)
+// Exclusive scan of a[0..n) into out[0..n] (out[n] = total), one 256-thread
+// workgroup walking the array in coalesced chunks of 256 with a carry.
+__global__ __launch_bounds__(256) void k_scan_one_block(const int *__restrict__ a, int n,
+                                                        int *__restrict__ out)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    dst[off[key[i]] + slot[i]] = i;
-}
-
-// Stable order inside each key run: rank = number of run members with a
-// smaller input index.
-__global__ void k_rank_in_run(const int *__restrict__ key, const int *__restrict__ tmp,
-                              const int *__restrict__ off, int n, int *__restrict__ perm)
-{
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int e = tmp[p];
-    const int k = key[e];
-    const int s0 = off[k], s1 = off[k + 1];
-    int r = 0;
-    for (int q = s0; q < s1; ++q) r += (tmp[q] < e) ? 1 : 0;
-    perm[s0 + r] = e;
+    __shared__ int wsum[4];
+    int carry = 0;
+    for (int b = 0; b < n; b += 256) {
+        const int i = b + threadIdx.x;
+        const int v = i < n ? a[i] : 0;
+        int tot;
+        const int ex = block_exclusive_scan_256(v, wsum, &tot);
+        if (i < n) out[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
 }
 
 void scan_ints(hipStream_t st, const int *a, int n, int *out)
 {
-    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, st, a, n, out);
+    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(256), 0, st, a, n, out);
 }
 
-int keysort_finish(se2p_ctx *ctx, KeySort *ks, int64_t n, int nkeys, bool stable)
+__global__ __launch_bounds__(256) void k_radix_hist(const int *__restrict__ key, int n, int shift,
+                                                    int nblocks, int *__restrict__ hist)
+{
+    __shared__ int cnt[256];
+    const int t = threadIdx.x;
+    cnt[t] = 0;
+    __syncthreads();
+    const int b = blockIdx.x;
+    for (int j = 0; j < kRadixBlock / 256; ++j) {
+        const int i = b * kRadixBlock + j * 256 + t;
+        if (i < n) atomicAdd(&cnt[(key[i] >> shift) & 255], 1);   // integer counts
+    }
+    __syncthreads();
+    hist[t * nblocks + b] = cnt[t];
+}
+
+// Per digit d (one workgroup each): exclusive scan of hist[d][0..nb) into
+// hoff[d][...] and the digit total into dtot[d].
+__global__ __launch_bounds__(256) void k_radix_digit_scan(const int *__restrict__ hist, int nb,
+                                                          int *__restrict__ hoff,
+                                                          int *__restrict__ dtot)
+{
+    __shared__ int wsum[4];
+    const int d = blockIdx.x;
+    int carry = 0;
+    for (int b = 0; b < nb; b += 256) {
+        const int i = b + threadIdx.x;
+        const int v = i < nb ? hist[d * nb + i] : 0;
+        int tot;
+        const int ex = block_exclusive_scan_256(v, wsum, &tot);
+        if (i < nb) hoff[d * nb + i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) dtot[d] = carry;
+}
+
+// Stable scatter of one radix pass.  vin == nullptr means the identity
+// (original indices).
+__global__ __launch_bounds__(256) void k_radix_scatter(const int *__restrict__ kin,
+                                                       const int *__restrict__ vin, int n,
+                                                       int shift, int nblocks,
+                                                       const int *__restrict__ hoff,
+                                                       const int *__restrict__ dtot,
+                                                       int *__restrict__ kout,
+                                                       int *__restrict__ vout)
+{
+    __shared__ int run[256];
+    __shared__ int wcnt[4][256];
+    __shared__ int wsum[4];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int b = blockIdx.x;
+    {
+        int tot;
+        const int dbase = block_exclusive_scan_256(dtot[t], wsum, &tot);
+        run[t] = dbase + hoff[t * nblocks + b];
+    }
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int j = 0; j < kRadixBlock / 256; ++j) {
+        const int i = b * kRadixBlock + j * 256 + t;
+        const bool valid = i < n;
+        const int k = valid ? kin[i] : 0;
+        const int v = valid ? (vin ? vin[i] : i) : 0;
+        const int d = (k >> shift) & 255;
+        for (int w = 0; w < 4; ++w) wcnt[w][t] = 0;
+        __syncthreads();
+        // lanes of this wavefront holding the same digit
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool on = (d >> bit) & 1;
+            const unsigned long long bal = __ballot(on);
+            m &= on ? bal : ~bal;
+        }
+        const int rank = __popcll(m & lt);
+        if (valid && rank == 0) wcnt[wave][d] = __popcll(m);
+        __syncthreads();
+        if (valid) {
+            int pos = run[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        run[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+        __syncthreads();
+    }
+}
+
+// off[k] = first position whose sorted key is >= k (k = 0..nkeys).
+__global__ void k_run_offsets(const int *__restrict__ keys, int n, int nkeys, int *__restrict__ off)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nkeys) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (keys[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    off[k] = lo;
+}
+
+int keysort_run(se2p_ctx *ctx, KeySort *ks, int64_t n, int nkeys)
 {
     hipStream_t st = ctx->stream;
-    const int threads = 256;
-    hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(1024), 0, st, ks->count, nkeys, ks->off);
+    const int nb = radix_blocks(n);
+    const int passes = (key_bits(nkeys) + 7) / 8;
     if (n > 0) {
-        const unsigned blocks = (unsigned)((n + threads - 1) / threads);
-        hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(threads), 0, st, ks->key, ks->slot,
-                           ks->off, (int)n, stable ? ks->tmp : ks->perm);
-        if (stable)
-            hipLaunchKernelGGL(k_rank_in_run, dim3(blocks), dim3(threads), 0, st, ks->key,
-                               ks->tmp, ks->off, (int)n, ks->perm);
+        // ping-pong: key -> key2 -> key ...; values: identity -> tmp -> perm -> ...
+        const int *kin = ks->key;
+        const int *vin = nullptr;
+        for (int p = 0; p < passes; ++p) {
+            int *kout = (kin == ks->key) ? ks->key2 : ks->key;
+            // values alternate so that the last pass lands in perm
+            int *vout = ((passes - 1 - p) % 2 == 0) ? ks->perm : ks->tmp;
+            hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, st, kin, (int)n, 8 * p, nb,
+                               ks->hist);
+            hipLaunchKernelGGL(k_radix_digit_scan, dim3(256), dim3(256), 0, st, ks->hist, nb,
+                               ks->hoff, ks->hist + 256 * nb);
+            hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(256), 0, st, kin, vin, (int)n, 8 * p,
+                               nb, ks->hoff, ks->hist + 256 * nb, kout, vout);
+            kin = kout;
+            vin = vout;
+        }
+        ks->sorted_key = const_cast<int *>(kin);
+    } else {
+        ks->sorted_key = ks->key;
     }
+    hipLaunchKernelGGL(k_run_offsets, dim3((nkeys + 1 + 255) / 256), dim3(256), 0, st,
+                       ks->sorted_key, (int)n, nkeys, ks->off);
     return 0;
 }
 
-void keysort_clear(hipStream_t st, KeySort *ks, int nkeys)
+// map[k] = c for k in [off[c], off[c+1]): workgroup index -> owning item.
+__global__ void k_expand_map(const int *__restrict__ off, int n, int *__restrict__ map)
 {
-    hipLaunchKernelGGL(k_fill_int, dim3((nkeys + 1 + 255) / 256), dim3(256), 0, st, ks->count,
-                       nkeys + 1, 0);
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    for (int k = off[c]; k < off[c + 1]; ++k) map[k] = c;
 }
 
-// cell key (ewald_tools.cpp:28-35); with sub > 0 the key also carries a
-// sub x sub Morton-ordered sub-cell so that consecutive targets of a cell are
-// spatially close.
+void expand_map(hipStream_t st, const int *off, int n, int *map)
+{
+    hipLaunchKernelGGL(k_expand_map, dim3((n + 255) / 256), dim3(256), 0, st, off, n, map);
+}
+
+// cell key (ewald_tools.cpp:28-35); with sublog > 0 the key also carries a
+// 2^sublog x 2^sublog Morton-ordered sub-cell so that consecutive points of
+// a cell are spatially close.
 __global__ void k_cell_keys(const double *__restrict__ p, int n, double Lx, double Ly, int nsx,
-                            int nsy, int sublog, int *__restrict__ key, int *__restrict__ slot,
-                            int *__restrict__ count)
+                            int nsy, int sublog, int *__restrict__ key)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -143,19 +267,17 @@ __global__ void k_cell_keys(const double *__restrict__ p, int n, double Lx, doub
         k = (k << (2 * sublog)) | m;
     }
     key[i] = k;
-    slot[i] = atomicAdd(&count[k], 1);
 }
 
 int cell_sort(se2p_ctx *ctx, const double *p, int64_t n, double Lx, double Ly, int nsx, int nsy,
-              int sublog, bool stable, KeySort *ks)
+              int sublog, KeySort *ks)
 {
     hipStream_t st = ctx->stream;
     const int nkeys = nsx * nsy << (2 * sublog);
-    keysort_clear(st, ks, nkeys);
     if (n > 0)
         hipLaunchKernelGGL(k_cell_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p,
-                           (int)n, Lx, Ly, nsx, nsy, sublog, ks->key, ks->slot, ks->count);
-    return keysort_finish(ctx, ks, n, nkeys, stable);
+                           (int)n, Lx, Ly, nsx, nsy, sublog, ks->key);
+    return keysort_run(ctx, ks, n, nkeys);
 }
 
 }  // namespace se2p

@@ -106,6 +106,11 @@ double se2p_selftest_e1(double x);
 void se2p_selftest_anchor(double x, double y, double Lx, double Ly, double h, int P, int *mx,
                           int *my, double *px, double *py);
 int se2p_selftest_cell(double x, double y, double Lx, double Ly, int nside_x, int nside_y);
+/* The device sort behind the cell lists and gridding bins: stable order of
+ * keys[0..n) (device array, values in [0, nkeys)) into perm[0..n) and run
+ * offsets off[0..nkeys] (device arrays). */
+int se2p_selftest_keysort(se2p_ctx *ctx, const int *keys, int64_t n, int nkeys, int *perm,
+                          int *off);
 
 #ifdef __cplusplus
 }

@@ -146,3 +146,28 @@ def test_ellipses_full_size_subset(gpu):
     e2 = relerr(g, c)
     _say(f"[ellipses] real {e1:.3e} kspace {e2:.3e}")
     assert e1 <= TOL and e2 <= TOL
+
+
+def test_device_sort_is_stable_counting_order(gpu):
+    """The radix sort behind cell lists and bins equals numpy's stable argsort
+    (ewald_tools.cpp:71-74 keeps input order inside a box)."""
+    import ctypes
+
+    ctx = gpu.get_context(0)
+    rng = np.random.default_rng(21)
+    for n, nkeys in ((1, 1), (1000, 9), (5000, 300), (70001, 7744), (200000, 65536), (3000, 234256)):
+        keys = rng.integers(0, nkeys, n).astype(np.int32)
+        if n > 10:
+            keys[: n // 3] = keys[0]   # a heavily repeated key
+        kd = torch.from_numpy(keys).cuda()
+        perm = torch.empty(n, dtype=torch.int32, device="cuda")
+        off = torch.empty(nkeys + 1, dtype=torch.int32, device="cuda")
+        _say(f"[sort n={n} nkeys={nkeys}]")
+        torch.cuda.synchronize()
+        rc = ctx._lib.se2p_selftest_keysort(ctx.handle, ctypes.c_void_p(kd.data_ptr()), n, nkeys,
+                                           ctypes.c_void_p(perm.data_ptr()),
+                                           ctypes.c_void_p(off.data_ptr()))
+        assert rc == 0, ctx.last_error()
+        assert np.array_equal(perm.cpu().numpy(), np.argsort(keys, kind="stable"))
+        ref_off = np.concatenate([[0], np.cumsum(np.bincount(keys, minlength=nkeys))])
+        assert np.array_equal(off.cpu().numpy(), ref_off)
