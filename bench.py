@@ -49,6 +49,62 @@ def _pmc_traffic(kernel_tag: str):
         return None
 
 
+def _batched_variant(args, pe, W, np, torch, dev, nrhs=64):
+    """Per-RHS throughput of 64 densities on the 1000-point unit-box geometry
+    (grid 64^2, tol 1e-10), Stokeslet + stresslet, as 64 plain calls of the
+    four entry points versus one call of each batched entry point."""
+    from fasttools2d_amd import params as Pm
+
+    w = W.uniform_box(1000, seed=1)
+    pr = Pm.select_params("slp", 1000, 1000, w.f, 1.0, 1.0, tol=1e-10, M=64)
+    pd = Pm.select_params("dlp", 1000, 1000, w.f, 1.0, 1.0, tol=1e-10, M=64)
+    rng = np.random.default_rng(100)
+    F = torch.from_numpy(rng.standard_normal((nrhs, 1000, 2))).to(dev)
+    ps = torch.from_numpy(w.psrc).to(dev)
+    nv = torch.from_numpy(w.n).to(dev)
+    ctx = pe.Context(0)
+    stream = torch.cuda.Stream(dev)
+    outs = [torch.empty((1000, 2), dtype=torch.float64, device=dev) for _ in range(4)]
+
+    def plain():
+        for r in range(nrhs):
+            pe.mex_stokes_slp_real(ps, ps, F[r], pr.xi, pr.nside_x, pr.nside_y, 1.0, 1.0, ctx=ctx, out=outs[0])
+            pe.mex_stokes_slp_kspace(ps, ps, pr.xi, pr.eta, F[r], pr.Mx, pr.My, 1.0, 1.0, pr.w, pr.P, ctx=ctx, out=outs[1])
+            pe.mex_stokes_dlp_real(ps, ps, F[r], nv, pd.xi, pd.nside_x, pd.nside_y, 1.0, 1.0, ctx=ctx, out=outs[2])
+            pe.mex_stokes_dlp_kspace(ps, ps, pd.xi, pd.eta, F[r], nv, pd.Mx, pd.My, 1.0, 1.0, pd.w, pd.P, ctx=ctx, out=outs[3])
+
+    def batched():
+        pe.stokes_slp_real_batched(ps, ps, F, pr.xi, pr.nside_x, pr.nside_y, 1.0, 1.0, ctx=ctx)
+        pe.stokes_slp_kspace_batched(ps, ps, pr.xi, pr.eta, F, pr.Mx, pr.My, 1.0, 1.0, pr.w, pr.P, ctx=ctx)
+        pe.stokes_dlp_real_batched(ps, ps, F, nv, pd.xi, pd.nside_x, pd.nside_y, 1.0, 1.0, ctx=ctx)
+        pe.stokes_dlp_kspace_batched(ps, ps, pd.xi, pd.eta, F, nv, pd.Mx, pd.My, 1.0, 1.0, pd.w, pd.P, ctx=ctx)
+
+    res = {}
+    with torch.cuda.stream(stream):
+        ctx.bind_torch_stream(stream)
+        ctx.set_async(True)
+        for name, fn in (("plain", plain), ("batched", batched)):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 3
+            e0.record(stream)
+            for _ in range(reps):
+                fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            # one evaluation = one target x one density, Stokeslet and stresslet counted once each
+            res[name] = {"ms_per_call_set": ms, "target_evals_per_s": 2 * 1000 * nrhs / (ms * 1e-3)}
+        ctx.set_async(False)
+    res["batched_over_plain"] = res["batched"]["target_evals_per_s"] / res["plain"]["target_evals_per_s"]
+    res["workload"] = (f"1000 uniform points (unit box, targets = sources), grid 64^2, tol 1e-10, "
+                       f"{nrhs} densities, Stokeslet + stresslet (real + Fourier each)")
+    print(f"# batched variant: plain {res['plain']['ms_per_call_set']:.3f} ms, batched "
+          f"{res['batched']['ms_per_call_set']:.3f} ms", file=sys.stderr, flush=True)
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +112,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the plain-vs-batched right-hand-side comparison")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (scales HBM counters)")
     args = ap.parse_args()
@@ -181,6 +239,8 @@ def main() -> int:
                          f"{args.cpu_runs} runs after 1 warm-up, {threads} OpenMP threads",
                "seconds_per_step": med}
 
+    variants = None if args.no_variants else _batched_variant(args, pe, W, np, torch, dev)
+
     traffic = _pmc_traffic("k_real_dlp")
     out = {
         "metric": "target evals/s at 1e-10 rel err, 1 GPU (stresslet DLP, plain forward)",
@@ -207,6 +267,7 @@ def main() -> int:
         "stage_ms": stage_ms,
         "parity_vs_oracle": parity,
         "cpu_baseline": cpu,
+        "variants": variants,
     }
     print(json.dumps(out), flush=True)
     return 0

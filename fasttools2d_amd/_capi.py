@@ -44,6 +44,14 @@ SIGNATURES = {
     "se2p_slp_kspace": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _I, _I, _D, _D, _D, _I, _P]),
     "se2p_dlp_real": (_I, [_P, _P, _I64, _P, _I64, _P, _P, _D, _I, _I, _D, _D, _P]),
     "se2p_dlp_kspace": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _P, _I, _I, _D, _D, _D, _I, _P]),
+    "se2p_slp_real_batched": (_I, [_P, _P, _I64, _P, _I64, _P, _I, _I64, _D, _I, _I, _D, _D,
+                                   _P, _I64]),
+    "se2p_slp_kspace_batched": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _I, _I64, _I, _I, _D,
+                                     _D, _D, _I, _P, _I64]),
+    "se2p_dlp_real_batched": (_I, [_P, _P, _I64, _P, _I64, _P, _P, _I, _I64, _D, _I, _I, _D, _D,
+                                   _P, _I64]),
+    "se2p_dlp_kspace_batched": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _P, _I, _I64, _I, _I,
+                                     _D, _D, _D, _I, _P, _I64]),
     "se2p_count_pairs": (_I, [_P, _P, _I64, _P, _I64, _I, _I, _D, _D,
                               ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "se2p_selftest_e1": (_D, [_D]),
@@ -53,6 +61,23 @@ SIGNATURES = {
 }
 
 SE2P_HOST_ONLY = -1
+
+class Se2pParams(ctypes.Structure):
+    """se2p_params of include/se2p.h."""
+    _fields_ = [("P", ctypes.c_int), ("Nb", ctypes.c_double), ("tol", ctypes.c_double),
+                ("M", ctypes.c_int), ("nside_x", ctypes.c_int), ("nside_y", ctypes.c_int),
+                ("Mx", ctypes.c_int), ("My", ctypes.c_int), ("kinf", ctypes.c_int),
+                ("converged", ctypes.c_int), ("rc", ctypes.c_double), ("xi", ctypes.c_double),
+                ("eta", ctypes.c_double), ("w", ctypes.c_double)]
+
+
+SIGNATURES.update({
+    "se2p_select_params": (_I, [_I, _I64, _I64, _D, _D, _D, ctypes.POINTER(Se2pParams)]),
+    "se2p_slp_ewald": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _D, ctypes.POINTER(Se2pParams),
+                            _P, _P, _P]),
+    "se2p_dlp_ewald": (_I, [_P, _P, _I64, _P, _I64, _P, _P, _D, _D, ctypes.POINTER(Se2pParams),
+                            _P, _P, _P]),
+})
 
 STAGES = ("sort", "real", "spread", "fft_fwd", "filter", "fft_inv", "interp")
 

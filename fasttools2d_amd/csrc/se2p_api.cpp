@@ -159,6 +159,7 @@ int se2p_destroy(se2p_ctx *ctx)
         if (p.work) hipFree(p.work);
     }
     if (ctx->ws) hipFree(ctx->ws);
+    if (ctx->drv) hipFree(ctx->drv);
     for (int i = 0; i < 2 * ST_COUNT; ++i)
         if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
@@ -270,6 +271,177 @@ int se2p_dlp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double 
     if (int rc = set_device(ctx)) return rc;
     if (int rc = kspace_sum(ctx, 1, g, f, n, 1, 2 * ns, 2 * nt, u)) return rc;
     return finish(ctx);
+}
+
+static int check_batch(se2p_ctx *ctx, int nrhs, int64_t f_stride, int64_t ns, int64_t u_stride,
+                       int64_t nt)
+{
+    if (nrhs < 1) return fail(ctx, SE2P_ERR_ARG, "nrhs must be >= 1");
+    if (f_stride < 2 * ns || u_stride < 2 * nt)
+        return fail(ctx, SE2P_ERR_ARG, "strides must be at least 2*Ns (densities) and 2*Nt (outputs)");
+    return 0;
+}
+
+int se2p_slp_real_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                          int64_t nt, const double *f, int nrhs, int64_t f_stride, double xi,
+                          int nside_x, int nside_y, double Lx, double Ly, double *u,
+                          int64_t u_stride)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
+    if (ns > 0 && !f) return fail(ctx, SE2P_ERR_ARG, "null density pointer");
+    if (int rc = check_batch(ctx, nrhs, f_stride, ns, u_stride, nt)) return rc;
+    if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
+    if (int rc = real_sum(ctx, 0, g, f, nullptr, nrhs, f_stride, u_stride, u, nullptr)) return rc;
+    return finish(ctx);
+}
+
+int se2p_dlp_real_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                          int64_t nt, const double *f, const double *n, int nrhs, int64_t f_stride,
+                          double xi, int nside_x, int nside_y, double Lx, double Ly, double *u,
+                          int64_t u_stride)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
+    if (ns > 0 && (!f || !n)) return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
+    if (int rc = check_batch(ctx, nrhs, f_stride, ns, u_stride, nt)) return rc;
+    if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
+    if (int rc = real_sum(ctx, 1, g, f, n, nrhs, f_stride, u_stride, u, nullptr)) return rc;
+    return finish(ctx);
+}
+
+int se2p_slp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                            int64_t nt, double xi, double eta, const double *f, int nrhs,
+                            int64_t f_stride, int Mx, int My, double Lx, double Ly, double w, int P,
+                            double *u, int64_t u_stride)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
+    if (ns > 0 && !f) return fail(ctx, SE2P_ERR_ARG, "null density pointer");
+    if (int rc = check_batch(ctx, nrhs, f_stride, ns, u_stride, nt)) return rc;
+    KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
+    if (int rc = check_kspace_params(ctx, g)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    if (int rc = kspace_sum(ctx, 0, g, f, nullptr, nrhs, f_stride, u_stride, u)) return rc;
+    return finish(ctx);
+}
+
+int se2p_dlp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                            int64_t nt, double xi, double eta, const double *f, const double *n,
+                            int nrhs, int64_t f_stride, int Mx, int My, double Lx, double Ly,
+                            double w, int P, double *u, int64_t u_stride)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
+    if (ns > 0 && (!f || !n)) return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
+    if (int rc = check_batch(ctx, nrhs, f_stride, ns, u_stride, nt)) return rc;
+    KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
+    if (int rc = check_kspace_params(ctx, g)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    if (int rc = kspace_sum(ctx, 1, g, f, n, nrhs, f_stride, u_stride, u)) return rc;
+    return finish(ctx);
+}
+
+int se2p_select_params(int kind, int64_t ns, int64_t nt, double Q, double Lx, double Ly,
+                       se2p_params *p)
+{
+    if (!p || (kind != 0 && kind != 1) || ns < 0 || nt < 0 || ns + nt < 2 || !(Lx > 0) || !(Ly > 0) ||
+        !(Q >= 1) || !std::isfinite(Q))
+        return SE2P_ERR_ARG;
+    if (p->P < 0 || (p->P % 2) != 0) return SE2P_ERR_P;
+    select_params(kind, ns, nt, Q, Lx, Ly, p);
+    return SE2P_OK;
+}
+
+static int ensure_driver_buffer(se2p_ctx *ctx, size_t bytes)
+{
+    if (bytes <= ctx->drv_bytes) return 0;
+    if (ctx->drv) {
+        SE2P_HIP(hipStreamSynchronize(ctx->stream));
+        SE2P_HIP(hipFree(ctx->drv));
+        ctx->drv = nullptr;
+        ctx->drv_bytes = 0;
+    }
+    SE2P_HIP(hipMalloc((void **)&ctx->drv, bytes));
+    ctx->drv_bytes = bytes;
+    return 0;
+}
+
+// Shared body of the two driver entry points.
+static int ewald_full(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const double *ptar,
+                      int64_t nt, const double *f, const double *n, double Lx, double Ly,
+                      se2p_params *prm, double *u, double *ur_out, double *uk_out)
+{
+    if (!prm) return fail(ctx, SE2P_ERR_ARG, "null parameter struct");
+    if (ns + nt < 2) return fail(ctx, SE2P_ERR_ARG, "need at least two points in total");
+    if (!(Lx > 0) || !(Ly > 0) || !std::isfinite(Lx) || !std::isfinite(Ly))
+        return fail(ctx, SE2P_ERR_PARAM, "Lx and Ly must be positive and finite");
+    if (prm->P < 0 || (prm->P % 2) != 0) return fail(ctx, SE2P_ERR_P, "P must be even");
+    if (int rc = set_device(ctx)) return rc;
+    hipStream_t st = ctx->stream;
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const bool same = (psrc == ptar) && (ns == nt);
+    const int64_t nb = (ns + 255) / 256;
+    const size_t need = up(16 * (size_t)(ns > 0 ? ns : 1)) + (same ? 0 : up(16 * (size_t)(nt > 0 ? nt : 1))) +
+                        2 * up(16 * (size_t)(nt > 0 ? nt : 1)) + up(24 * (size_t)(nb + 1)) + up(24);
+    if (int rc = ensure_driver_buffer(ctx, need)) return rc;
+    size_t o = 0;
+    double *ws_src = (double *)(ctx->drv + o); o += up(16 * (size_t)(ns > 0 ? ns : 1));
+    double *ws_tar = ws_src;
+    if (!same) { ws_tar = (double *)(ctx->drv + o); o += up(16 * (size_t)(nt > 0 ? nt : 1)); }
+    double *ur = (double *)(ctx->drv + o); o += up(16 * (size_t)(nt > 0 ? nt : 1));
+    double *uk = (double *)(ctx->drv + o); o += up(16 * (size_t)(nt > 0 ? nt : 1));
+    double *partial = (double *)(ctx->drv + o); o += up(24 * (size_t)(nb + 1));
+    double *sums = (double *)(ctx->drv + o);
+    if (ur_out) ur = ur_out;
+    if (uk_out) uk = uk_out;
+
+    // Q = sum(f.^2) + 1 and, for the DLP, sum((f.n) x) over the caller's points
+    launch_driver_sums(st, f, kind == 1 ? n : nullptr, psrc, ns, partial, sums);
+    double hs[3];
+    SE2P_HIP(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, st));
+    SE2P_HIP(hipStreamSynchronize(st));
+    select_params(kind, ns, nt, hs[0] + 1.0, Lx, Ly, prm);
+
+    launch_wrap(st, psrc, ns, Lx, Ly, ws_src);
+    if (!same) launch_wrap(st, ptar, nt, Lx, Ly, ws_tar);
+    RealGeom rg{ws_src, ws_tar, ns, nt, prm->xi, Lx, Ly, prm->nside_x, prm->nside_y};
+    if (int rc = real_sum(ctx, kind, rg, f, n, 1, 2 * ns, 2 * nt, ur, nullptr)) return rc;
+    KGeom kg{ws_src, ws_tar, ns, nt, prm->xi, prm->eta, Lx, Ly, prm->w, prm->Mx, prm->My, prm->P};
+    if (int rc = check_kspace_params(ctx, kg)) return rc;
+    if (int rc = kspace_sum(ctx, kind, kg, f, n, 1, 2 * ns, 2 * nt, uk)) return rc;
+    launch_compose(st, ur, uk, nt, kind, sums, Lx * Ly, u);
+    return finish(ctx);
+}
+
+int se2p_slp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                   const double *f, double Lx, double Ly, se2p_params *prm, double *u, double *ur,
+                   double *uk)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
+    if (ns > 0 && !f) return fail(ctx, SE2P_ERR_ARG, "null density pointer");
+    return ewald_full(ctx, 0, psrc, ns, ptar, nt, f, nullptr, Lx, Ly, prm, u, ur, uk);
+}
+
+int se2p_dlp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                   const double *f, const double *n, double Lx, double Ly, se2p_params *prm,
+                   double *u, double *ur, double *uk)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
+    if (ns > 0 && (!f || !n)) return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
+    return ewald_full(ctx, 1, psrc, ns, ptar, nt, f, n, Lx, Ly, prm, u, ur, uk);
 }
 
 int se2p_count_pairs(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,

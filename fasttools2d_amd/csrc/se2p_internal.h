@@ -53,6 +53,8 @@ struct se2p_ctx {
     bool ev_used[se2p::ST_COUNT] = {};
     float stage_ms[se2p::ST_COUNT] = {};
     int64_t pairs_in = -1, pairs_out = -1;   // counted by se2p_count_pairs
+    char *drv = nullptr;      // driver-level buffers (kept apart from ws)
+    size_t drv_bytes = 0;
 };
 
 namespace se2p {
@@ -114,6 +116,16 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
 
 // Validation of the k-space preconditions (returns 0 or an SE2P_ERR code).
 int check_kspace_params(se2p_ctx *ctx, const KGeom &g);
+
+// ---- driver entry points (se2p_driver.cpp, se2p_compose.hip) ------------
+int select_params(int kind, int64_t ns, int64_t nt, double Q, double Lx, double Ly,
+                  se2p_params *p);
+void launch_wrap(hipStream_t st, const double *p, int64_t n, double Lx, double Ly, double *out);
+// out[0] = sum f.f, out[1..2] = sum (f.n) x (nv may be null: then 0)
+void launch_driver_sums(hipStream_t st, const double *f, const double *nv, const double *x,
+                        int64_t n, double *partial, double *out);
+void launch_compose(hipStream_t st, const double *ur, double *uk, int64_t nt, int dlp,
+                    const double *zm, double LxLy, double *u);
 
 }  // namespace se2p
 

@@ -79,7 +79,7 @@ __device__ inline void fgg_factors(double px, double py, double c, double w, dou
 template <int KIND>
 __global__ void k_gather_grid_src(const double *__restrict__ p, const double *__restrict__ f,
                                   const double *__restrict__ nv, const int *__restrict__ perm,
-                                  int n, int r0, int nr, int64_t dstride, double Lx, double Ly,
+                                  int n, int nreal, int nr, int64_t dstride, double Lx, double Ly,
                                   double h, int P, int Mx, int My, double c, double w,
                                   int2 *__restrict__ anchor, double *__restrict__ fgg,
                                   double *__restrict__ q)
@@ -96,17 +96,18 @@ __global__ void k_gather_grid_src(const double *__restrict__ p, const double *__
     fgg[i] = bx;
     fgg[n + i] = e3x;
     fgg[2 * n + i] = e3y;
+    // right-hand sides nreal..nr-1 only pad the last group: zero densities
     if (KIND == 0) {
         for (int r = 0; r < nr; ++r) {
-            const double *fr = f + (int64_t)(r0 + r) * dstride;
-            q[(int64_t)(2 * r) * n + i] = fr[2 * o];
-            q[(int64_t)(2 * r + 1) * n + i] = fr[2 * o + 1];
+            const double *fr = f + (int64_t)r * dstride;
+            q[(int64_t)(2 * r) * n + i] = r < nreal ? fr[2 * o] : 0.0;
+            q[(int64_t)(2 * r + 1) * n + i] = r < nreal ? fr[2 * o + 1] : 0.0;
         }
     } else {
         double n1 = nv[2 * o], n2 = nv[2 * o + 1];
         for (int r = 0; r < nr; ++r) {
-            const double *fr = f + (int64_t)(r0 + r) * dstride;
-            double f1 = fr[2 * o], f2 = fr[2 * o + 1];
+            const double *fr = f + (int64_t)r * dstride;
+            double f1 = r < nreal ? fr[2 * o] : 0.0, f2 = r < nreal ? fr[2 * o + 1] : 0.0;
             q[(int64_t)(3 * r) * n + i] = n1 * f1;
             q[(int64_t)(3 * r + 1) * n + i] = n2 * f1 + n1 * f2;
             q[(int64_t)(3 * r + 2) * n + i] = n2 * f2;
@@ -139,6 +140,7 @@ struct SpreadArgs {
     double *partial;   // [chunk id][256 nodes][nq] for tiles with several chunks
     const int *block_tile;                 // workgroup -> tile
     const int *rng_start, *rng_len;        // [tile][kTR] candidate runs
+    int64_t max_chunks;                    // partial-buffer stride per group
 };
 
 // Anchor bins whose anchors a satisfy (t - a) mod M in [0, P] for some node t
@@ -179,6 +181,7 @@ constexpr int kTR = 2 * 20;   // slots per tile (x-bins <= 20 for P <= 126)
 template <int NQ, bool WRAP>
 __global__ __launch_bounds__(256) void k_spread(SpreadArgs a)
 {
+    const int grp = blockIdx.z;   // group of right-hand sides
     extern __shared__ double lds[];
     const int P1 = a.P + 1;
     double *WX = lds;                       // [kBatch][P1]
@@ -257,7 +260,8 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a)
                     AX[sl] = an.x;
                     AY[sl] = an.y;
 #pragma unroll
-                    for (int c = 0; c < NQ; ++c) Q[sl * NQ + c] = a.s.q[(int64_t)c * a.s.ns + k];
+                    for (int c = 0; c < NQ; ++c)
+                        Q[sl * NQ + c] = a.s.q[((int64_t)grp * NQ + c) * a.s.ns + k];
                 }
             }
         }
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a)
     if (X < a.Mx && Y < a.My) {
         const int64_t idx = (int64_t)X * a.My + Y;
 #pragma unroll
-        for (int c = 0; c < NQ; ++c) a.grids[c * a.gstride + idx] = acc[c];
+        for (int c = 0; c < NQ; ++c) a.grids[((int64_t)grp * NQ + c) * a.gstride + idx] = acc[c];
     }
 }
 
@@ -332,6 +336,9 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
     __shared__ int rstart[kMaxRanges], rpref[kMaxRanges + 1];
     __shared__ int nrange;
 
+    // blockIdx.y: group of right-hand sides (grids NQ*y .. NQ*y+NQ-1)
+    const int grp = blockIdx.y;
+    const double *qg = a.s.q + (int64_t)grp * NQ * a.s.ns;
     // block -> (tile, chunk of the tile's candidate list)
     const int ntiles = a.ntx * a.nty;
     const int b = blockIdx.x;
@@ -394,7 +401,7 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
                     AXY[2 * sl] = an.x;
                     AXY[2 * sl + 1] = an.y;
 #pragma unroll
-                    for (int c = 0; c < NQ; ++c) Q[sl * NQ + c] = a.s.q[(int64_t)c * a.s.ns + k];
+                    for (int c = 0; c < NQ; ++c) Q[sl * NQ + c] = qg[(int64_t)c * a.s.ns + k];
                 } else {
                     double w0 = 1.0;
                     const double e3y = a.s.fgg[2 * a.s.ns + k];
@@ -450,6 +457,7 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
         __syncthreads();
     }
     if (wave != 0) return;
+    double *gg = a.grids + (int64_t)grp * NQ * a.gstride;
     if (nchunk == 1) {
         if (X < a.Mx) {
 #pragma unroll
@@ -458,12 +466,12 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
                 if (Y >= a.My) break;
                 const int64_t idx = (int64_t)X * a.My + Y;
 #pragma unroll
-                for (int c = 0; c < NQ; ++c) a.grids[c * a.gstride + idx] = acc[k][c];
+                for (int c = 0; c < NQ; ++c) gg[c * a.gstride + idx] = acc[k][c];
             }
         }
     } else {
         // partial tile, added to the others in chunk order by k_spread_combine
-        double *pt = a.partial + (int64_t)b * 256 * NQ;
+        double *pt = a.partial + ((int64_t)grp * a.max_chunks + b) * 256 * NQ;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -477,6 +485,8 @@ template <int NQ>
 __global__ __launch_bounds__(64) void k_spread_combine(SpreadArgs a)
 {
     const int tile = blockIdx.x;
+    const int grp = blockIdx.y;
+    double *gg = a.grids + (int64_t)grp * NQ * a.gstride;
     const int b0 = a.chunk_off[tile], b1 = a.chunk_off[tile + 1];
     if (b1 - b0 <= 1) return;
     const int lane = threadIdx.x;
@@ -489,7 +499,7 @@ __global__ __launch_bounds__(64) void k_spread_combine(SpreadArgs a)
 #pragma unroll
         for (int c = 0; c < NQ; ++c) acc[k][c] = 0.0;
     for (int b = b0; b < b1; ++b) {
-        const double *pt = a.partial + (int64_t)b * 256 * NQ;
+        const double *pt = a.partial + ((int64_t)grp * a.max_chunks + b) * 256 * NQ;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -501,7 +511,7 @@ __global__ __launch_bounds__(64) void k_spread_combine(SpreadArgs a)
         if (Y >= a.My) break;
         const int64_t idx = (int64_t)X * a.My + Y;
 #pragma unroll
-        for (int c = 0; c < NQ; ++c) a.grids[c * a.gstride + idx] = acc[k][c];
+        for (int c = 0; c < NQ; ++c) gg[c * a.gstride + idx] = acc[k][c];
     }
 }
 
@@ -743,7 +753,8 @@ __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
     const int v0 = chunk * 256, v1 = min(total, v0 + 256);
     fill_e1(E1, a.P, a.ch2);
     const int X0 = tx * kTile, Y0 = ty * kTile;
-    for (int r = 0; r < a.nr; ++r) {
+    // blockIdx.y selects the right-hand side
+    for (int r = blockIdx.y; r < blockIdx.y + 1 && r < a.nr; ++r) {
         const double *g1 = a.grids + (int64_t)(2 * r) * a.gstride;
         const double *g2 = g1 + a.gstride;
         __syncthreads();   // previous RHS done with G
@@ -861,8 +872,11 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     const double h = g.Lx / Mx;                  // also used for y (:45)
     const double c = -2 * g.xi * g.xi / g.eta;
     const int ncomp = kind == 0 ? 2 : 3;
-    // right-hand sides per pass (grids per pass = ncomp * RB)
-    const int RB = (nrhs >= 4 && nrhs % 4 == 0) ? 4 : 1;
+    // right-hand sides per spreading group (grids per group nq = ncomp*RB);
+    // the last group is padded with zero densities
+    const int RB = nrhs >= 2 ? 4 : 1;
+    const int ngrp = (nrhs + RB - 1) / RB;
+    const int nrp = ngrp * RB;                   // padded right-hand sides
     const int nq = ncomp * RB;
     const int64_t G = (int64_t)Mx * My;
     const int Mh = My / 2 + 1;
@@ -884,12 +898,8 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
 
     FftPlans *plans = nullptr;
     if (ns > 0)
-        if (int rc = get_plans(ctx, Mx, My, nq, 2 * RB, &plans)) return rc;
+        if (int rc = get_plans(ctx, Mx, My, ncomp * nrp, 2 * nrp, &plans)) return rc;
 
-    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t ks = keysort_bytes(ns, nbins);
-    const size_t kt = (itile && !same_points) ? keysort_bytes(nt, nbins) : 0;
-    const size_t nsz = (size_t)(ns > 0 ? ns : 1);
     // work split of the tile kernels
     const int ntx = (Mx + kTile - 1) / kTile, nty = (My + kTile - 1) / kTile;
     const int ntiles = ntx * nty;
@@ -898,12 +908,18 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     const int64_t tps_y = std::min<int64_t>(nty, (kTile + P + kBin - 1) / kTile + 1);
     const int64_t max_spread_chunks = ntiles + (tps_x * tps_y * ns + chunk - 1) / chunk;
     const int64_t max_interp_chunks = ntiles + (nt + 255) / 256;
+
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t ks = keysort_bytes(ns, nbins);
+    const size_t kt = (itile && !same_points) ? keysort_bytes(nt, nbins) : 0;
+    const size_t nsz = (size_t)(ns > 0 ? ns : 1);
+    const size_t ngrid_all = (size_t)ncomp * nrp;
     size_t need = ks + kt + up(sizeof(int2) * nsz) + up(sizeof(double) * 3 * nsz) +
-                  up(sizeof(double) * nq * nsz) + up(sizeof(double) * nq * (size_t)G) +
-                  up(sizeof(double2) * nq * (size_t)GH) + 4 * up(sizeof(int) * (size_t)(ntiles + 1)) +
-                  2 * up(sizeof(int) * (size_t)ntiles * kTR) +
+                  up(sizeof(double) * ngrid_all * nsz) + up(sizeof(double) * ngrid_all * (size_t)G) +
+                  up(sizeof(double2) * ngrid_all * (size_t)GH) +
+                  4 * up(sizeof(int) * (size_t)(ntiles + 1)) + 2 * up(sizeof(int) * (size_t)ntiles * kTR) +
                   up(sizeof(int) * (size_t)max_spread_chunks) + up(sizeof(int) * (size_t)max_interp_chunks) +
-                  (strip ? up(sizeof(double) * 256 * nq * (size_t)max_spread_chunks) : 0);
+                  (strip ? up(sizeof(double) * 256 * nq * (size_t)max_spread_chunks * ngrp) : 0);
     if (int rc = ensure_workspace(ctx, need)) return rc;
     char *base = ctx->ws;
     size_t o = 0;
@@ -912,9 +928,9 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     if (kt) { keysort_carve(base + o, nt, nbins, &kst); o += kt; }
     int2 *anchor = (int2 *)(base + o); o += up(sizeof(int2) * nsz);
     double *fgg = (double *)(base + o); o += up(sizeof(double) * 3 * nsz);
-    double *q = (double *)(base + o); o += up(sizeof(double) * nq * nsz);
-    double *grids = (double *)(base + o); o += up(sizeof(double) * nq * (size_t)G);
-    double2 *spec = (double2 *)(base + o); o += up(sizeof(double2) * nq * (size_t)GH);
+    double *q = (double *)(base + o); o += up(sizeof(double) * ngrid_all * nsz);
+    double *grids = (double *)(base + o); o += up(sizeof(double) * ngrid_all * (size_t)G);
+    double2 *spec = (double2 *)(base + o); o += up(sizeof(double2) * ngrid_all * (size_t)GH);
     int *scnt = (int *)(base + o); o += up(sizeof(int) * (size_t)(ntiles + 1));
     int *soffc = (int *)(base + o); o += up(sizeof(int) * (size_t)(ntiles + 1));
     int *icnt = (int *)(base + o); o += up(sizeof(int) * (size_t)(ntiles + 1));
@@ -961,13 +977,12 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     sa.block_tile = smap;
     sa.rng_start = rng_start;
     sa.rng_len = rng_len;
+    sa.max_chunks = max_spread_chunks;
     if (strip || itile) {
         hipLaunchKernelGGL(k_tile_work, dim3(ntiles), dim3(64), 0, st, sa, tbins.off, scnt, icnt,
                            rng_start, rng_len);
         scan_ints(st, scnt, ntiles, soffc);
         scan_ints(st, icnt, ntiles, ioffc);
-        const unsigned mb = (unsigned)((ntiles + threads - 1) / threads);
-        (void)mb;
         expand_map(st, soffc, ntiles, smap);
         expand_map(st, ioffc, ntiles, imap);
     }
@@ -977,119 +992,119 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     double s_int = 4 * g.xi * g.xi / g.eta;                  // :325-328
     s_int = s_int * s_int * h * h / SE2P_PI;
     const double interp_scale = s_int / (4 * SE2P_PI);
-    const dim3 tiles((Mx + kTile - 1) / kTile, (My + kTile - 1) / kTile);
 
-    for (int r0 = 0; r0 < nrhs; r0 += RB) {
-        stage_begin(ctx, ST_SPREAD);
-        if (kind == 0)
-            hipLaunchKernelGGL(k_gather_grid_src<0>, dim3((unsigned)((ns + threads - 1) / threads)),
-                               dim3(threads), 0, st, g.psrc, f, nv, kss.perm, (int)ns, r0, RB,
-                               dstride, g.Lx, g.Ly, h, P, Mx, My, c, g.w, anchor, fgg, q);
-        else
-            hipLaunchKernelGGL(k_gather_grid_src<1>, dim3((unsigned)((ns + threads - 1) / threads)),
-                               dim3(threads), 0, st, g.psrc, f, nv, kss.perm, (int)ns, r0, RB,
-                               dstride, g.Lx, g.Ly, h, P, Mx, My, c, g.w, anchor, fgg, q);
-        sa.s.anchor = anchor;
-        sa.s.fgg = fgg;
-        sa.s.q = q;
-        sa.s.ns = ns;
-        sa.ch2 = ch2;
-        sa.grids = grids;
-#define SE2P_SPREAD(NQV)                                                                        \
-    if (strip) {                                                                                \
-        hipLaunchKernelGGL((k_spread_strip<NQV>), dim3((unsigned)max_spread_chunks), dim3(256),  \
-                           lds_strip, st, sa);                                                  \
-        hipLaunchKernelGGL((k_spread_combine<NQV>), dim3(ntiles), dim3(64), 0, st, sa);         \
-    } else if (wrap) hipLaunchKernelGGL((k_spread<NQV, true>), tiles, dim3(256), lds_gen, st, sa); \
-    else hipLaunchKernelGGL((k_spread<NQV, false>), tiles, dim3(256), lds_gen, st, sa);
-        switch (nq) {
-        case 2: SE2P_SPREAD(2); break;
-        case 3: SE2P_SPREAD(3); break;
-        case 8: SE2P_SPREAD(8); break;
-        case 12: SE2P_SPREAD(12); break;
-        default: return fail(ctx, SE2P_ERR_ARG, "unsupported grid count %d", nq);
-        }
+    stage_begin(ctx, ST_SPREAD);
+    if (kind == 0)
+        hipLaunchKernelGGL(k_gather_grid_src<0>, dim3((unsigned)((ns + threads - 1) / threads)),
+                           dim3(threads), 0, st, g.psrc, f, nv, kss.perm, (int)ns, nrhs, nrp,
+                           dstride, g.Lx, g.Ly, h, P, Mx, My, c, g.w, anchor, fgg, q);
+    else
+        hipLaunchKernelGGL(k_gather_grid_src<1>, dim3((unsigned)((ns + threads - 1) / threads)),
+                           dim3(threads), 0, st, g.psrc, f, nv, kss.perm, (int)ns, nrhs, nrp,
+                           dstride, g.Lx, g.Ly, h, P, Mx, My, c, g.w, anchor, fgg, q);
+    sa.s.anchor = anchor;
+    sa.s.fgg = fgg;
+    sa.s.q = q;
+    sa.s.ns = ns;
+    sa.ch2 = ch2;
+    sa.grids = grids;
+    const dim3 tiles2d(ntx, nty, ngrp);
+#define SE2P_SPREAD(NQV)                                                                          \
+    if (strip) {                                                                                  \
+        hipLaunchKernelGGL((k_spread_strip<NQV>), dim3((unsigned)max_spread_chunks, ngrp),         \
+                           dim3(256), lds_strip, st, sa);                                         \
+        hipLaunchKernelGGL((k_spread_combine<NQV>), dim3(ntiles, ngrp), dim3(64), 0, st, sa);     \
+    } else if (wrap) hipLaunchKernelGGL((k_spread<NQV, true>), tiles2d, dim3(256), lds_gen, st, sa); \
+    else hipLaunchKernelGGL((k_spread<NQV, false>), tiles2d, dim3(256), lds_gen, st, sa);
+    switch (nq) {
+    case 2: SE2P_SPREAD(2); break;
+    case 3: SE2P_SPREAD(3); break;
+    case 8: SE2P_SPREAD(8); break;
+    case 12: SE2P_SPREAD(12); break;
+    default: return fail(ctx, SE2P_ERR_ARG, "unsupported grid count %d", nq);
+    }
 #undef SE2P_SPREAD
-        SE2P_HIP(hipGetLastError());
-        stage_end(ctx, ST_SPREAD);
+    SE2P_HIP(hipGetLastError());
+    stage_end(ctx, ST_SPREAD);
 
-        stage_begin(ctx, ST_FFT_FWD);
-        rocfft_execution_info_set_stream(plans->fwd_info, st);
-        {
-            void *in[1] = {grids};
-            void *out[1] = {spec};
-            rocfft_status s = rocfft_execute(plans->fwd, in, out, plans->fwd_info);
-            if (s != rocfft_status_success) return fail(ctx, SE2P_ERR_FFT, "rocfft_execute (fwd) = %d", (int)s);
-        }
-        stage_end(ctx, ST_FFT_FWD);
+    stage_begin(ctx, ST_FFT_FWD);
+    rocfft_execution_info_set_stream(plans->fwd_info, st);
+    {
+        void *in[1] = {grids};
+        void *out[1] = {spec};
+        rocfft_status s = rocfft_execute(plans->fwd, in, out, plans->fwd_info);
+        if (s != rocfft_status_success) return fail(ctx, SE2P_ERR_FFT, "rocfft_execute (fwd) = %d", (int)s);
+    }
+    stage_end(ctx, ST_FFT_FWD);
 
-        stage_begin(ctx, ST_FILTER);
-        FilterArgs fa;
-        fa.spec = spec;
-        fa.sstride = GH;
-        fa.Mx = Mx;
-        fa.My = My;
-        fa.nrhs = RB;
-        fa.Lx = g.Lx;
-        fa.Ly = g.Ly;
-        fa.xi = g.xi;
-        fa.eta = g.eta;
-        fa.scale = 1.0 / ((double)Mx * (double)My);
-        const unsigned fblocks = (unsigned)((GH + threads - 1) / threads);
-        if (kind == 0) hipLaunchKernelGGL(k_filter_slp, dim3(fblocks), dim3(threads), 0, st, fa);
-        else hipLaunchKernelGGL(k_filter_dlp, dim3(fblocks), dim3(threads), 0, st, fa);
-        SE2P_HIP(hipGetLastError());
-        stage_end(ctx, ST_FILTER);
+    stage_begin(ctx, ST_FILTER);
+    FilterArgs fa;
+    fa.spec = spec;
+    fa.sstride = GH;
+    fa.Mx = Mx;
+    fa.My = My;
+    fa.nrhs = nrp;
+    fa.Lx = g.Lx;
+    fa.Ly = g.Ly;
+    fa.xi = g.xi;
+    fa.eta = g.eta;
+    fa.scale = 1.0 / ((double)Mx * (double)My);
+    const unsigned fblocks = (unsigned)((GH + threads - 1) / threads);
+    if (kind == 0) hipLaunchKernelGGL(k_filter_slp, dim3(fblocks), dim3(threads), 0, st, fa);
+    else hipLaunchKernelGGL(k_filter_dlp, dim3(fblocks), dim3(threads), 0, st, fa);
+    SE2P_HIP(hipGetLastError());
+    stage_end(ctx, ST_FILTER);
 
-        stage_begin(ctx, ST_FFT_INV);
-        rocfft_execution_info_set_stream(plans->inv_info, st);
-        {
-            void *in[1] = {spec};
-            void *out[1] = {grids};
-            rocfft_status s = rocfft_execute(plans->inv, in, out, plans->inv_info);
-            if (s != rocfft_status_success) return fail(ctx, SE2P_ERR_FFT, "rocfft_execute (inv) = %d", (int)s);
-        }
-        stage_end(ctx, ST_FFT_INV);
+    stage_begin(ctx, ST_FFT_INV);
+    rocfft_execution_info_set_stream(plans->inv_info, st);
+    {
+        void *in[1] = {spec};
+        void *out[1] = {grids};
+        rocfft_status s = rocfft_execute(plans->inv, in, out, plans->inv_info);
+        if (s != rocfft_status_success) return fail(ctx, SE2P_ERR_FFT, "rocfft_execute (inv) = %d", (int)s);
+    }
+    stage_end(ctx, ST_FFT_INV);
 
-        stage_begin(ctx, ST_INTERP);
-        if (itile) {
-            InterpTileArgs ia;
-            ia.ptar = g.ptar;
-            ia.perm = tbins.perm;
-            ia.boff = tbins.off;
-            ia.nbx = nbx;
-            ia.nby = nby;
-            ia.chunk_off = ioffc;
-            ia.block_tile = imap;
-            ia.ntx = ntx;
-            ia.nty = nty;
-            ia.grids = grids;
-            ia.gstride = G;
-            ia.Mx = Mx;
-            ia.My = My;
-            ia.P = P;
-            ia.nr = RB;
-            ia.Lx = g.Lx;
-            ia.Ly = g.Ly;
-            ia.h = h;
-            ia.c = c;
-            ia.w = g.w;
-            ia.ch2 = ch2;
-            ia.scale = interp_scale;
-            ia.u = u + (int64_t)r0 * ustride;
-            ia.ustride = ustride;
-            hipLaunchKernelGGL(k_interp_tile, dim3((unsigned)max_interp_chunks), dim3(256), lds_itile,
-                               st, ia);
-        } else {
+    stage_begin(ctx, ST_INTERP);
+    if (itile) {
+        InterpTileArgs ia;
+        ia.ptar = g.ptar;
+        ia.perm = tbins.perm;
+        ia.boff = tbins.off;
+        ia.nbx = nbx;
+        ia.nby = nby;
+        ia.chunk_off = ioffc;
+        ia.block_tile = imap;
+        ia.ntx = ntx;
+        ia.nty = nty;
+        ia.grids = grids;
+        ia.gstride = G;
+        ia.Mx = Mx;
+        ia.My = My;
+        ia.P = P;
+        ia.nr = nrhs;
+        ia.Lx = g.Lx;
+        ia.Ly = g.Ly;
+        ia.h = h;
+        ia.c = c;
+        ia.w = g.w;
+        ia.ch2 = ch2;
+        ia.scale = interp_scale;
+        ia.u = u;
+        ia.ustride = ustride;
+        hipLaunchKernelGGL(k_interp_tile, dim3((unsigned)max_interp_chunks, nrhs), dim3(256),
+                           lds_itile, st, ia);
+    } else {
+        for (int r = 0; r < nrhs; ++r) {
             InterpArgs ia;
             ia.ptar = g.ptar;
             ia.nt = nt;
-            ia.grids = grids;
+            ia.grids = grids + (int64_t)(2 * r) * G;
             ia.gstride = G;
             ia.Mx = Mx;
             ia.My = My;
             ia.P = P;
-            ia.nr = RB;
+            ia.nr = 1;
             ia.Lx = g.Lx;
             ia.Ly = g.Ly;
             ia.h = h;
@@ -1097,15 +1112,14 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
             ia.w = g.w;
             ia.ch2 = ch2;
             ia.scale = interp_scale;
-            ia.u = u + (int64_t)r0 * ustride;
+            ia.u = u + (int64_t)r * ustride;
             ia.ustride = ustride;
             const unsigned iblocks = (unsigned)((nt + threads - 1) / threads);
-            if (RB == 1) hipLaunchKernelGGL(k_interp<1>, dim3(iblocks), dim3(threads), 0, st, ia);
-            else hipLaunchKernelGGL(k_interp<4>, dim3(iblocks), dim3(threads), 0, st, ia);
+            hipLaunchKernelGGL(k_interp<1>, dim3(iblocks), dim3(threads), 0, st, ia);
         }
-        SE2P_HIP(hipGetLastError());
-        stage_end(ctx, ST_INTERP);
     }
+    SE2P_HIP(hipGetLastError());
+    stage_end(ctx, ST_INTERP);
     return 0;
 }
 

@@ -60,6 +60,7 @@ struct RealArgs {
     const int *soff;       // [ncell*16+1] run offsets of the sources' (cell, sub-cell) keys
     int64_t ns;
     int nsx, nsy, ncell, nrhs;
+    int r_begin, r_end;    // right-hand sides handled by this launch
     double Lx, Ly, xi, xi2, cut2, self;
     double *u;             // rhs r at u + r*ustride, interleaved, caller's order
     int64_t ustride;
@@ -168,7 +169,9 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
     const int64_t qs = a.ns;
     const int cx = cell % a.nsx, cy = cell / a.nsx;
 
-    for (int r0 = 0; r0 < a.nrhs; r0 += RB) {
+    // blockIdx.y selects this workgroup's group of RB right-hand sides
+    for (int r0 = a.r_begin + blockIdx.y * RB; r0 < a.r_begin + (blockIdx.y + 1) * RB &&
+                                               r0 + RB <= a.r_end; r0 += RB) {
         double acc[RB][2];
 #pragma unroll
         for (int r = 0; r < RB; ++r) acc[r][0] = acc[r][1] = 0.0;
@@ -449,12 +452,24 @@ int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const 
     if (counting) {
         SE2P_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), st));
         hipLaunchKernelGGL(k_real_count, dim3(max_chunks), dim3(64), 0, st, a);
-    } else if (kind == 0) {
-        if (nrhs % 4 == 0) hipLaunchKernelGGL((k_real<0, 4>), dim3(max_chunks), dim3(256), 0, st, a, spos, q, kss.off, u);
-        else hipLaunchKernelGGL((k_real<0, 1>), dim3(max_chunks), dim3(256), 0, st, a, spos, q, kss.off, u);
     } else {
-        if (nrhs % 4 == 0) hipLaunchKernelGGL((k_real<1, 4>), dim3(max_chunks), dim3(256), 0, st, a, spos, q, kss.off, u);
-        else hipLaunchKernelGGL((k_real<1, 1>), dim3(max_chunks), dim3(256), 0, st, a, spos, q, kss.off, u);
+        // groups of four right-hand sides share every pair evaluation; the
+        // remainder goes one at a time
+        const int n4 = nrhs / 4 * 4;
+        if (n4 > 0) {
+            a.r_begin = 0;
+            a.r_end = n4;
+            const dim3 grid(max_chunks, n4 / 4);
+            if (kind == 0) hipLaunchKernelGGL((k_real<0, 4>), grid, dim3(256), 0, st, a, spos, q, kss.off, u);
+            else hipLaunchKernelGGL((k_real<1, 4>), grid, dim3(256), 0, st, a, spos, q, kss.off, u);
+        }
+        if (n4 < nrhs) {
+            a.r_begin = n4;
+            a.r_end = nrhs;
+            const dim3 grid(max_chunks, nrhs - n4);
+            if (kind == 0) hipLaunchKernelGGL((k_real<0, 1>), grid, dim3(256), 0, st, a, spos, q, kss.off, u);
+            else hipLaunchKernelGGL((k_real<1, 1>), grid, dim3(256), 0, st, a, spos, q, kss.off, u);
+        }
     }
     stage_end(ctx, ST_REAL);
     SE2P_HIP(hipGetLastError());

@@ -207,3 +207,148 @@ def count_pairs(psrc, ptar, nside_x, nside_y, Lx, Ly, *, ctx=None):
                                          int(nside_x), int(nside_y), float(Lx), float(Ly),
                                          ctypes.byref(a), ctypes.byref(b)))
     return a.value, b.value
+
+
+# -- batched right-hand sides (one geometry, many densities) -----------------
+def _as_batch(F, name, ns, device):
+    """(nrhs, N, 2) float64 contiguous device tensor."""
+    if isinstance(F, np.ndarray):
+        F = torch.from_numpy(np.ascontiguousarray(F, dtype=np.float64))
+    if F.dim() != 3 or F.shape[2] != 2 or F.shape[1] != ns:
+        raise ValueError(f"{name} must be (nrhs, N, 2) with N = number of sources.")
+    return F.to(device=device, dtype=torch.float64).contiguous()
+
+
+def stokes_slp_real_batched(psrc, ptar, F, xi, nside_x, nside_y, Lx, Ly, *, ctx=None):
+    """mex_stokes_slp_real for every density F[r]; returns (nrhs, Nt, 2)."""
+    dev = _device_of(psrc, ptar, F)
+    ps, pt = _as_points(psrc, "psrc", dev), _as_points(ptar, "ptar", dev)
+    FF = _as_batch(F, "F", ps.shape[0], dev)
+    ctx = _prepare(ctx, dev)
+    nr = FF.shape[0]
+    u = torch.empty((nr, pt.shape[0], 2), dtype=torch.float64, device=dev)
+    ctx._check(ctx._lib.se2p_slp_real_batched(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt),
+                                              pt.shape[0], _ptr(FF), nr, 2 * ps.shape[0],
+                                              float(xi), int(nside_x), int(nside_y), float(Lx),
+                                              float(Ly), _ptr(u), 2 * pt.shape[0]))
+    return u
+
+
+def stokes_slp_kspace_batched(psrc, ptar, xi, eta, F, Mx, My, Lx, Ly, w, P, *, ctx=None):
+    """mex_stokes_slp_kspace for every density F[r]; returns (nrhs, Nt, 2)."""
+    dev = _device_of(psrc, ptar, F)
+    ps, pt = _as_points(psrc, "psrc", dev), _as_points(ptar, "ptar", dev)
+    FF = _as_batch(F, "F", ps.shape[0], dev)
+    ctx = _prepare(ctx, dev)
+    nr = FF.shape[0]
+    u = torch.empty((nr, pt.shape[0], 2), dtype=torch.float64, device=dev)
+    ctx._check(ctx._lib.se2p_slp_kspace_batched(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt),
+                                                pt.shape[0], float(xi), float(eta), _ptr(FF), nr,
+                                                2 * ps.shape[0], int(Mx), int(My), float(Lx),
+                                                float(Ly), float(w), int(P), _ptr(u),
+                                                2 * pt.shape[0]))
+    return u
+
+
+def stokes_dlp_real_batched(psrc, ptar, F, n, xi, nside_x, nside_y, Lx, Ly, *, ctx=None):
+    """mex_stokes_dlp_real for every density F[r] (normals shared)."""
+    dev = _device_of(psrc, ptar, F, n)
+    ps, pt = _as_points(psrc, "psrc", dev), _as_points(ptar, "ptar", dev)
+    nn = _as_points(n, "n", dev)
+    FF = _as_batch(F, "F", ps.shape[0], dev)
+    if nn.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and n must be the same size.")
+    ctx = _prepare(ctx, dev)
+    nr = FF.shape[0]
+    u = torch.empty((nr, pt.shape[0], 2), dtype=torch.float64, device=dev)
+    ctx._check(ctx._lib.se2p_dlp_real_batched(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt),
+                                              pt.shape[0], _ptr(FF), _ptr(nn), nr,
+                                              2 * ps.shape[0], float(xi), int(nside_x),
+                                              int(nside_y), float(Lx), float(Ly), _ptr(u),
+                                              2 * pt.shape[0]))
+    return u
+
+
+def stokes_dlp_kspace_batched(psrc, ptar, xi, eta, F, n, Mx, My, Lx, Ly, w, P, *, ctx=None):
+    """mex_stokes_dlp_kspace for every density F[r] (normals shared)."""
+    dev = _device_of(psrc, ptar, F, n)
+    ps, pt = _as_points(psrc, "psrc", dev), _as_points(ptar, "ptar", dev)
+    nn = _as_points(n, "n", dev)
+    FF = _as_batch(F, "F", ps.shape[0], dev)
+    if nn.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and n must be the same size.")
+    ctx = _prepare(ctx, dev)
+    nr = FF.shape[0]
+    u = torch.empty((nr, pt.shape[0], 2), dtype=torch.float64, device=dev)
+    ctx._check(ctx._lib.se2p_dlp_kspace_batched(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt),
+                                                pt.shape[0], float(xi), float(eta), _ptr(FF),
+                                                _ptr(nn), nr, 2 * ps.shape[0], int(Mx), int(My),
+                                                float(Lx), float(Ly), float(w), int(P), _ptr(u),
+                                                2 * pt.shape[0]))
+    return u
+
+
+# -- the reference drivers as single calls --------------------------------------
+def _driver(kind, psrc, ptar, f, n, Lx, Ly, P, Nb, tol, M, ctx, same):
+    dev = _device_of(psrc, ptar, f, n)
+    ps = _as_points(psrc, "psrc", dev)
+    pt = ps if same else _as_points(ptar, "ptar", dev)
+    ff = _as_points(f, "f", dev)
+    nn = _as_points(n, "n", dev) if kind == 1 else None
+    ctx = _prepare(ctx, dev)
+    prm = _capi.Se2pParams()
+    prm.P = int(P)
+    prm.Nb = float(Nb) if Nb is not None else 0.0
+    prm.tol = float(tol)
+    prm.M = int(M) if M is not None else 0
+    nt = pt.shape[0]
+    u = torch.empty((nt, 2), dtype=torch.float64, device=dev)
+    ur = torch.empty_like(u)
+    uk = torch.empty_like(u)
+    if kind == 0:
+        rc = ctx._lib.se2p_slp_ewald(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), nt, _ptr(ff),
+                                     float(Lx), float(Ly), ctypes.byref(prm), _ptr(u), _ptr(ur),
+                                     _ptr(uk))
+    else:
+        rc = ctx._lib.se2p_dlp_ewald(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), nt, _ptr(ff),
+                                     _ptr(nn), float(Lx), float(Ly), ctypes.byref(prm), _ptr(u),
+                                     _ptr(ur), _ptr(uk))
+    ctx._check(rc)
+    if not prm.converged:
+        import warnings
+
+        warnings.warn("Couldn't find suitable xi/k: Newton iteration cap reached", RuntimeWarning)
+    return u, ur, uk, prm
+
+
+def _cols(*arrays):
+    return np.stack([np.asarray(a, dtype=np.float64).ravel() for a in arrays], axis=1)
+
+
+def StokesSLP_ewald_2p(xsrc, ysrc, xtar, ytar, f1, f2, Lx, Ly, *, P=24, Nb=None, tol=1e-16,
+                       M=None, ctx=None, return_params=False):
+    """[u1, u2, ur, uk, xi] of PeriodicEwald/src/matlab/StokesSLP_ewald_2p.m:1-138.
+
+    Inputs are 1-D arrays (MATLAB column vectors); ur and uk are (Nt, 2)
+    device tensors, u1/u2 device tensors.  ``M`` overrides the grid the
+    driver would choose; ``return_params`` appends the se2p_params used."""
+    same = (xsrc is xtar) and (ysrc is ytar)
+    ps = _cols(xsrc, ysrc)
+    pt = ps if same else _cols(xtar, ytar)
+    u, ur, uk, prm = _driver(0, ps, pt, _cols(f1, f2), None, Lx, Ly, P, Nb, tol, M, ctx, same)
+    out = (u[:, 0], u[:, 1], ur, uk, prm.xi)
+    return out + (prm,) if return_params else out
+
+
+def StokesDLP_ewald_2p(xsrc, ysrc, xtar, ytar, n1, n2, f1, f2, Lx, Ly, *, P=24, Nb=None, tol=1e-16,
+                       M=None, ctx=None, return_params=False):
+    """[u1, u2, ur, uk, xi] of PeriodicEwald/src/matlab/StokesDLP_ewald_2p.m:1-163
+    (uk includes the k = 0 mode; u = -(ur + uk)).  Normals are used as given
+    (no renormalisation)."""
+    same = (xsrc is xtar) and (ysrc is ytar)
+    ps = _cols(xsrc, ysrc)
+    pt = ps if same else _cols(xtar, ytar)
+    u, ur, uk, prm = _driver(1, ps, pt, _cols(f1, f2), _cols(n1, n2), Lx, Ly, P, Nb, tol, M, ctx,
+                             same)
+    out = (u[:, 0], u[:, 1], ur, uk, prm.xi)
+    return out + (prm,) if return_params else out

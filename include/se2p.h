@@ -89,6 +89,69 @@ int se2p_dlp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double 
                     int64_t nt, double xi, double eta, const double *f, const double *n,
                     int Mx, int My, double Lx, double Ly, double w, int P, double *u);
 
+/* ---- batched right-hand sides -------------------------------------------- */
+/* nrhs density vectors on one geometry (same sources, targets and, for the
+ * DLP, normals).  Density r is at f + r*f_stride and result r at
+ * u + r*u_stride (strides in doubles, f_stride >= 2*Ns, u_stride >= 2*Nt).
+ * The cell lists, gridding bins, Gaussian factors and every pair kernel
+ * evaluation are shared by the right-hand sides, and the Fourier-space sum
+ * runs one batched transform pair for all of them.  Result r equals the
+ * single-density entry point's result for density r (to rounding).  The
+ * reference has no batched form; its callers loop over the MEX kernels. */
+int se2p_slp_real_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                          int64_t nt, const double *f, int nrhs, int64_t f_stride, double xi,
+                          int nside_x, int nside_y, double Lx, double Ly, double *u,
+                          int64_t u_stride);
+int se2p_slp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                            int64_t nt, double xi, double eta, const double *f, int nrhs,
+                            int64_t f_stride, int Mx, int My, double Lx, double Ly, double w, int P,
+                            double *u, int64_t u_stride);
+int se2p_dlp_real_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                          int64_t nt, const double *f, const double *n, int nrhs, int64_t f_stride,
+                          double xi, int nside_x, int nside_y, double Lx, double Ly, double *u,
+                          int64_t u_stride);
+int se2p_dlp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                            int64_t nt, double xi, double eta, const double *f, const double *n,
+                            int nrhs, int64_t f_stride, int Mx, int My, double Lx, double Ly,
+                            double w, int P, double *u, int64_t u_stride);
+
+/* ---- the reference drivers as single calls --------------------------------- */
+/* Parameters of the spectral Ewald split.  Inputs left at 0 take the
+ * reference driver's defaults: P = 24, Nb = min(P*round(log2(Ns+Nt)), (Ns+Nt)/4),
+ * tol = 1e-16, grid M = min(2*kinf, 10000) (Mx = A*M, My = B*M with
+ * [A,B] = rat(Lx/Ly)).  The other fields are outputs. */
+typedef struct se2p_params {
+    int P;
+    double Nb;
+    double tol;
+    int M;
+    int nside_x, nside_y, Mx, My, kinf;
+    int converged;   /* 0 if a Newton solve hit its iteration cap (the reference warns) */
+    double rc, xi, eta, w;
+} se2p_params;
+
+/* Parameter selection of StokesSLP_ewald_2p.m:27-104 (kind 0) or
+ * StokesDLP_ewald_2p.m:28-125 (kind 1); Q = sum(f.^2) + 1.  Host only. */
+int se2p_select_params(int kind, int64_t ns, int64_t nt, double Q, double Lx, double Ly,
+                       se2p_params *p);
+
+/* [u1,u2,ur,uk,xi] = StokesSLP_ewald_2p(xsrc,ysrc,xtar,ytar,f1,f2,Lx,Ly,...)
+ * (PeriodicEwald/src/matlab/StokesSLP_ewald_2p.m:1-138): parameters from
+ * *prm (filled on return), points wrapped into [-L/2, L/2), real + Fourier
+ * sums, u = ur + uk.  psrc/ptar are the caller's (unwrapped) points; ur and
+ * uk may be NULL.  All arrays are device arrays. */
+int se2p_slp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                   const double *f, double Lx, double Ly, se2p_params *prm, double *u, double *ur,
+                   double *uk);
+/* [u1,u2,ur,uk,xi] = StokesDLP_ewald_2p(xsrc,ysrc,xtar,ytar,n1,n2,f1,f2,Lx,Ly,...)
+ * (StokesDLP_ewald_2p.m:1-163): as above, then uk -= sum((f.n) x_src)/(Lx Ly)
+ * with the caller's unwrapped source points, and u = -(ur + uk).  The
+ * driver's renormalisation of the normals on MATLAB releases after R2017a
+ * (:69-80) is not applied: normals are used as given. */
+int se2p_dlp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                   const double *f, const double *n, double Lx, double Ly, se2p_params *prm,
+                   double *u, double *ur, double *uk);
+
 /* ---- pair statistics (no reference counterpart) ------------------------- */
 /* Number of source-target pairs the real-space sum evaluates (pairs_in: same
  * cell, or neighbour cell with r^2 < cutoff^2) and rejects (pairs_out).  Used

@@ -170,3 +170,24 @@ def test_product_cell_index_matches_oracle_cells():
     cells = np.array([lib.se2p_selftest_cell(x, y, 1.0, 1.0, 7, 5) for x, y in p])
     for c in range(35):
         assert set(perm[off[c]:off[c + 1]]) == set(np.nonzero(cells == c)[0])
+
+
+@pytest.mark.parametrize("kind", ["slp", "dlp"])
+def test_c_parameter_selection_matches_host_restatement(kind):
+    """se2p_select_params (C++) against fasttools2d_amd.params (Python): both
+    restate the drivers' Newton solves; they must pick the same cells and
+    grids and the same xi to rounding."""
+    lib = _capi.load()
+    for (N, Lx, Ly, tol, seed) in ((1000, 1.0, 1.0, 1e-10, 1), (700, 1.0, 2.0, 1e-12, 2),
+                                   (5000, 3.0, 1.0, 1e-8, 3)):
+        rng = np.random.default_rng(seed)
+        f = rng.standard_normal((N, 2))
+        Q = float(np.sum(f * f)) + 1
+        ref = Pm.select_params(kind, N, N, f, Lx, Ly, tol=tol)
+        prm = _capi.Se2pParams()
+        prm.tol = tol
+        assert lib.se2p_select_params(0 if kind == "slp" else 1, N, N, Q, Lx, Ly, ctypes.byref(prm)) == 0
+        assert (prm.nside_x, prm.nside_y, prm.Mx, prm.My, prm.kinf) == \
+            (ref.nside_x, ref.nside_y, ref.Mx, ref.My, ref.kinf)
+        assert abs(prm.xi - ref.xi) <= 1e-12 * ref.xi
+        assert abs(prm.eta - ref.eta) <= 1e-12 * ref.eta and prm.converged == 1
