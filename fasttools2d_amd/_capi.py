@@ -52,6 +52,14 @@ SIGNATURES = {
                                    _P, _I64]),
     "se2p_dlp_kspace_batched": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _P, _I, _I64, _I, _I,
                                      _D, _D, _D, _I, _P, _I64]),
+    "se2p_slp_real_backward": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _I, _I, _D, _D,
+                                    _P, _P, _P, _P]),
+    "se2p_slp_kspace_backward": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _I, _I, _D, _D, _D,
+                                      _I, _P, _P, _P, _P]),
+    "se2p_dlp_real_backward": (_I, [_P, _P, _I64, _P, _I64, _P, _P, _D, _I, _I, _D, _D,
+                                    _P, _P, _P, _P, _P]),
+    "se2p_dlp_kspace_backward": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _P, _I, _I, _D, _D,
+                                      _D, _I, _P, _P, _P, _P, _P]),
     "se2p_count_pairs": (_I, [_P, _P, _I64, _P, _I64, _I, _I, _D, _D,
                               ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "se2p_selftest_e1": (_D, [_D]),

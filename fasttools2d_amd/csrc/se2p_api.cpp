@@ -40,19 +40,19 @@ int ensure_workspace(se2p_ctx *ctx, size_t bytes)
     return 0;
 }
 
+// A stage may run several times in one call (the adjoint's two passes); each
+// visit gets its own event pair and the reported time is the sum of the visits.
 void stage_begin(se2p_ctx *ctx, Stage s)
 {
-    if (!ctx->timing) return;
-    if (!ctx->ev_used[s]) {
-        hipEventRecord(ctx->ev[2 * s], ctx->stream);
-        ctx->ev_used[s] = true;
-    }
+    if (!ctx->timing || ctx->ev_used[s] >= se2p_ctx::kStageVisits) return;
+    hipEventRecord(ctx->ev[s][ctx->ev_used[s]][0], ctx->stream);
 }
 
 void stage_end(se2p_ctx *ctx, Stage s)
 {
-    if (!ctx->timing) return;
-    hipEventRecord(ctx->ev[2 * s + 1], ctx->stream);
+    if (!ctx->timing || ctx->ev_used[s] >= se2p_ctx::kStageVisits) return;
+    hipEventRecord(ctx->ev[s][ctx->ev_used[s]][1], ctx->stream);
+    ++ctx->ev_used[s];
 }
 
 int finish(se2p_ctx *ctx)
@@ -62,10 +62,14 @@ int finish(se2p_ctx *ctx)
     if (ctx->timing) {
         for (int s = 0; s < ST_COUNT; ++s) {
             ctx->stage_ms[s] = -1.0f;
-            if (ctx->ev_used[s]) {
-                float ms = 0.0f;
-                hipEventElapsedTime(&ms, ctx->ev[2 * s], ctx->ev[2 * s + 1]);
-                ctx->stage_ms[s] = ms;
+            if (ctx->ev_used[s] > 0) {
+                float total = 0.0f;
+                for (int v = 0; v < ctx->ev_used[s]; ++v) {
+                    float ms = 0.0f;
+                    hipEventElapsedTime(&ms, ctx->ev[s][v][0], ctx->ev[s][v][1]);
+                    total += ms;
+                }
+                ctx->stage_ms[s] = total;
             }
         }
     }
@@ -75,7 +79,7 @@ int finish(se2p_ctx *ctx)
 static void begin_call(se2p_ctx *ctx)
 {
     ctx->err.clear();
-    for (int s = 0; s < ST_COUNT; ++s) ctx->ev_used[s] = false;
+    for (int s = 0; s < ST_COUNT; ++s) ctx->ev_used[s] = 0;
 }
 
 static int set_device(se2p_ctx *ctx)
@@ -104,6 +108,18 @@ static int check_real(se2p_ctx *ctx, double xi, int nsx, int nsy, double Lx, dou
     if (!(xi > 0) || !(Lx > 0) || !(Ly > 0) || !std::isfinite(xi) || !std::isfinite(Lx) ||
         !std::isfinite(Ly))
         return fail(ctx, SE2P_ERR_PARAM, "xi, Lx and Ly must be positive and finite");
+    return 0;
+}
+
+// Backward entries: du is required, every output may be null (skipped).
+static int check_adjoint(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                         int64_t nt, const double *f, const double *n, bool dlp, const double *du)
+{
+    if (ns < 0 || nt < 0) return fail(ctx, SE2P_ERR_ARG, "negative point count");
+    if (ns > 0x7fffffffLL || nt > 0x7fffffffLL)
+        return fail(ctx, SE2P_ERR_ARG, "point counts above 2^31-1 are not supported");
+    if ((ns > 0 && (!psrc || !f || (dlp && !n))) || (nt > 0 && (!ptar || !du)))
+        return fail(ctx, SE2P_ERR_ARG, "null array pointer");
     return 0;
 }
 
@@ -136,7 +152,9 @@ int se2p_create(se2p_ctx **out, int device)
         return SE2P_ERR_HIP;
     }
     ctx->own_stream = true;
-    for (int i = 0; i < 2 * ST_COUNT; ++i) hipEventCreate(&ctx->ev[i]);
+    for (auto &stage : ctx->ev)
+        for (auto &visit : stage)
+            for (hipEvent_t &e : visit) hipEventCreate(&e);
     *out = ctx;
     return SE2P_OK;
 }
@@ -160,8 +178,10 @@ int se2p_destroy(se2p_ctx *ctx)
     }
     if (ctx->ws) hipFree(ctx->ws);
     if (ctx->drv) hipFree(ctx->drv);
-    for (int i = 0; i < 2 * ST_COUNT; ++i)
-        if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
+    for (auto &stage : ctx->ev)
+        for (auto &visit : stage)
+            for (hipEvent_t &e : visit)
+                if (e) hipEventDestroy(e);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return SE2P_OK;
@@ -347,6 +367,67 @@ int se2p_dlp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
     if (int rc = kspace_sum(ctx, 1, g, f, n, nrhs, f_stride, u_stride, u)) return rc;
+    return finish(ctx);
+}
+
+int se2p_slp_real_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                           int64_t nt, const double *f, double xi, int nside_x, int nside_y,
+                           double Lx, double Ly, const double *du, double *dpsrc, double *dptar,
+                           double *df)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_adjoint(ctx, psrc, ns, ptar, nt, f, nullptr, false, du)) return rc;
+    if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
+    if (int rc = real_adjoint(ctx, 0, g, f, nullptr, du, dpsrc, dptar, df, nullptr)) return rc;
+    return finish(ctx);
+}
+
+int se2p_dlp_real_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                           int64_t nt, const double *f, const double *n, double xi, int nside_x,
+                           int nside_y, double Lx, double Ly, const double *du, double *dpsrc,
+                           double *dptar, double *df, double *dn)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_adjoint(ctx, psrc, ns, ptar, nt, f, n, true, du)) return rc;
+    if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
+    if (int rc = real_adjoint(ctx, 1, g, f, n, du, dpsrc, dptar, df, dn)) return rc;
+    return finish(ctx);
+}
+
+int se2p_slp_kspace_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                             int64_t nt, double xi, double eta, const double *f, int Mx, int My,
+                             double Lx, double Ly, double w, int P, const double *du,
+                             double *dpsrc, double *dptar, double *df)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_adjoint(ctx, psrc, ns, ptar, nt, f, nullptr, false, du)) return rc;
+    KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
+    if (int rc = check_kspace_params(ctx, g)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    if (int rc = kspace_adjoint(ctx, 0, g, f, nullptr, du, dpsrc, dptar, df, nullptr)) return rc;
+    return finish(ctx);
+}
+
+int se2p_dlp_kspace_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                             int64_t nt, double xi, double eta, const double *f, const double *n,
+                             int Mx, int My, double Lx, double Ly, double w, int P,
+                             const double *du, double *dpsrc, double *dptar, double *df,
+                             double *dn)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_adjoint(ctx, psrc, ns, ptar, nt, f, n, true, du)) return rc;
+    KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
+    if (int rc = check_kspace_params(ctx, g)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    if (int rc = kspace_adjoint(ctx, 1, g, f, n, du, dpsrc, dptar, df, dn)) return rc;
     return finish(ctx);
 }
 

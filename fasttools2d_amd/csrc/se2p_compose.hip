@@ -95,6 +95,18 @@ __global__ void k_compose(const double *__restrict__ ur, double *__restrict__ uk
     }
 }
 
+__global__ void k_fill_zero(double *__restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+void launch_zero(hipStream_t st, double *p, int64_t n)
+{
+    if (p == nullptr || n <= 0) return;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+
 void launch_wrap(hipStream_t st, const double *p, int64_t n, double Lx, double Ly, double *out)
 {
     if (n > 0)

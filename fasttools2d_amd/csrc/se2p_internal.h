@@ -49,8 +49,10 @@ struct se2p_ctx {
     size_t ws_bytes = 0;
     std::string err;
     std::map<std::tuple<int, int, int, int>, se2p::FftPlans> plans;  // (Mx, My, nfwd, ninv)
-    hipEvent_t ev[2 * se2p::ST_COUNT] = {};
-    bool ev_used[se2p::ST_COUNT] = {};
+    // stage timing: up to kStageVisits (start, end) event pairs per stage and call
+    static constexpr int kStageVisits = 4;
+    hipEvent_t ev[se2p::ST_COUNT][kStageVisits][2] = {};
+    int ev_used[se2p::ST_COUNT] = {};   // visits recorded in this call
     float stage_ms[se2p::ST_COUNT] = {};
     int64_t pairs_in = -1, pairs_out = -1;   // counted by se2p_count_pairs
     char *drv = nullptr;      // driver-level buffers (kept apart from ws)
@@ -69,6 +71,24 @@ void stage_begin(se2p_ctx *ctx, Stage s);
 void stage_end(se2p_ctx *ctx, Stage s);
 
 int finish(se2p_ctx *ctx);   // error check + optional synchronisation
+
+// Bump allocator over the workspace, 256-byte aligned pieces.  With a null
+// base it only measures: run the same sequence of take() calls once to size
+// the workspace and once to hand out pointers.
+struct Carve {
+    char *base = nullptr;
+    size_t off = 0;
+    template <class T>
+    T *take(size_t count)
+    {
+        T *p = base ? (T *)(base + off) : nullptr;
+        off += (sizeof(T) * (count > 0 ? count : 1) + 255) & ~size_t(255);
+        return p;
+    }
+};
+
+// Zero n doubles with a fill kernel (no memset node, so capturable).
+void launch_zero(hipStream_t st, double *p, int64_t n);
 
 // ---- cell lists / bins (se2p_sort.hip) ---------------------------------
 // Stable sort of points by an integer key in [0, nkeys): perm[k] = original
@@ -103,6 +123,11 @@ struct RealGeom {
 };
 int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
              int nrhs, int64_t dstride, int64_t ustride, double *u, int64_t *pair_counts);
+// Adjoint of real_sum (one right-hand side): with L = sum_t du_t . u_t, the
+// gradients dL/dpsrc, dL/dptar, dL/df and (stresslet) dL/dn; null outputs are
+// skipped.
+int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
+                 const double *du, double *dpsrc, double *dptar, double *df, double *dn);
 
 // ---- Fourier space (se2p_kspace.hip) ------------------------------------
 struct KGeom {
@@ -113,6 +138,9 @@ struct KGeom {
 };
 int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *n,
                int nrhs, int64_t dstride, int64_t ustride, double *u);
+// Adjoint of kspace_sum (one right-hand side), outputs as for real_adjoint.
+int kspace_adjoint(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *n,
+                   const double *du, double *dpsrc, double *dptar, double *df, double *dn);
 
 // Validation of the k-space preconditions (returns 0 or an SE2P_ERR code).
 int check_kspace_params(se2p_ctx *ctx, const KGeom &g);

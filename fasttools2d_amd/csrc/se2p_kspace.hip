@@ -599,6 +599,34 @@ __global__ void k_filter_slp(FilterArgs a)
     }
 }
 
+// Stresslet filter coefficients at half-spectrum index (j, k), the inverse
+// transform's 1/(Mx My) included (mex_stokes_dlp_kspace.cpp:232-247).  Only the
+// anti-Hermitian part is kept: terms odd in k1 vanish on the x Nyquist line,
+// terms odd in k2 on the y Nyquist line.
+struct DlpCoef {
+    double et, b111, b112, b122, b222;
+};
+__device__ inline DlpCoef dlp_coef(const FilterArgs &a, int j, int k)
+{
+    DlpCoef c{0.0, 0.0, 0.0, 0.0, 0.0};
+    if (j == 0 && k == 0) return c;
+    const double k1 = wavenumber(j, a.Mx, a.Lx);
+    const double k2 = 2.0 * SE2P_PI / a.Ly * k;
+    const double K2 = k1 * k1 + k2 * k2;
+    const double e = exp(-0.25 * (1 - a.eta) / (a.xi * a.xi) * K2) / K2 / K2;
+    const double t = 1 + 0.25 * K2 / a.xi / a.xi;
+    c.et = e * t * a.scale;
+    if (2 * j != a.Mx) {
+        c.b111 = 3.0 * k1 * K2 - 2.0 * k1 * k1 * k1;
+        c.b122 = k1 * K2 - 2.0 * k1 * k2 * k2;
+    }
+    if (2 * k != a.My) {
+        c.b112 = k2 * K2 - 2.0 * k1 * k1 * k2;
+        c.b222 = 3.0 * k2 * K2 - 2.0 * k2 * k2 * k2;
+    }
+    return c;
+}
+
 // DLP: input grids [3r], [3r+1], [3r+2] (A = n1f1, B = n2f1+n1f2, C = n2f2)
 // -> output grids [2r], [2r+1], multiplied by -i so that the real inverse
 // transform yields the imaginary part of the reference's inverse.
@@ -608,41 +636,44 @@ __global__ void k_filter_dlp(FilterArgs a)
     const int64_t n = (int64_t)a.Mx * Mh;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const int j = (int)(p / Mh), k = (int)(p % Mh);
-    const double k1 = wavenumber(j, a.Mx, a.Lx);
-    const double k2 = 2.0 * SE2P_PI / a.Ly * k;
-    const double K2 = k1 * k1 + k2 * k2;
-    double et = 0.0, b111 = 0.0, b112 = 0.0, b122 = 0.0, b222 = 0.0;
-    if (p != 0) {
-        // mex_stokes_dlp_kspace.cpp:232-247
-        const double e = exp(-0.25 * (1 - a.eta) / (a.xi * a.xi) * K2) / K2 / K2;
-        const double t = 1 + 0.25 * K2 / a.xi / a.xi;
-        et = e * t * a.scale;
-        const bool nyx = (2 * j == a.Mx), nyy = (2 * k == a.My);
-        // anti-Hermitian part: terms odd in k1 vanish on the x Nyquist line,
-        // terms odd in k2 on the y Nyquist line
-        if (!nyx) {
-            b111 = 3.0 * k1 * K2 - 2.0 * k1 * k1 * k1;
-            b122 = k1 * K2 - 2.0 * k1 * k2 * k2;
-        }
-        if (!nyy) {
-            b112 = k2 * K2 - 2.0 * k1 * k1 * k2;
-            b222 = 3.0 * k2 * K2 - 2.0 * k2 * k2 * k2;
-        }
-    }
+    const DlpCoef c = dlp_coef(a, (int)(p / Mh), (int)(p % Mh));
     for (int r = 0; r < a.nrhs; ++r) {
         const double2 A = a.spec[(int64_t)(3 * r) * a.sstride + p];
         const double2 B = a.spec[(int64_t)(3 * r + 1) * a.sstride + p];
         const double2 C = a.spec[(int64_t)(3 * r + 2) * a.sstride + p];
-        const double o1r = et * (b111 * A.x + b112 * B.x + b122 * C.x);
-        const double o1i = et * (b111 * A.y + b112 * B.y + b122 * C.y);
-        const double o2r = et * (b112 * A.x + b122 * B.x + b222 * C.x);
-        const double o2i = et * (b112 * A.y + b122 * B.y + b222 * C.y);
+        const double o1r = c.et * (c.b111 * A.x + c.b112 * B.x + c.b122 * C.x);
+        const double o1i = c.et * (c.b111 * A.y + c.b112 * B.y + c.b122 * C.y);
+        const double o2r = c.et * (c.b112 * A.x + c.b122 * B.x + c.b222 * C.x);
+        const double o2i = c.et * (c.b112 * A.y + c.b122 * B.y + c.b222 * C.y);
         // all threads read their own element before writing it; grid 2r and
         // 2r+1 never alias an unread input of another thread
         a.spec[(int64_t)(2 * r) * a.sstride + p] = make_double2(o1i, -o1r);
         a.spec[(int64_t)(2 * r + 1) * a.sstride + p] = make_double2(o2i, -o2r);
     }
+}
+
+// Transpose of k_filter_dlp (one right-hand side): the filter is a real
+// convolution with an odd kernel, so its transpose multiplies by the complex
+// conjugate of -i et b, i.e. +i et b, with the 2x3 coefficient matrix
+// transposed.  Input grids [0], [1] (spectra of the spread du), output grids
+// [0], [1], [2] (the three stresslet densities).
+__global__ void k_filter_dlp_adj(FilterArgs a)
+{
+    const int Mh = a.My / 2 + 1;
+    const int64_t n = (int64_t)a.Mx * Mh;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const DlpCoef c = dlp_coef(a, (int)(p / Mh), (int)(p % Mh));
+    const double2 Z1 = a.spec[p], Z2 = a.spec[a.sstride + p];
+    const double o1r = c.et * (c.b111 * Z1.x + c.b112 * Z2.x);
+    const double o1i = c.et * (c.b111 * Z1.y + c.b112 * Z2.y);
+    const double o2r = c.et * (c.b112 * Z1.x + c.b122 * Z2.x);
+    const double o2i = c.et * (c.b112 * Z1.y + c.b122 * Z2.y);
+    const double o3r = c.et * (c.b122 * Z1.x + c.b222 * Z2.x);
+    const double o3i = c.et * (c.b122 * Z1.y + c.b222 * Z2.y);
+    a.spec[p] = make_double2(-o1i, o1r);
+    a.spec[a.sstride + p] = make_double2(-o2i, o2r);
+    a.spec[2 * a.sstride + p] = make_double2(-o3i, o3r);
 }
 
 // ---- interpolation ---------------------------------------------------------
@@ -800,10 +831,212 @@ __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
     }
 }
 
-__global__ void k_zero(double *__restrict__ u, int64_t n)
+
+// ---- adjoint interpolation -------------------------------------------------
+// For each point and each of NR grids g_m: V_m = sum W g_m and the sums
+// X_m = sum (i - P/2) W g_m, Y_m = sum (j - P/2) W g_m over the (P+1)^2 support
+// nodes (i, j).  The weight is W = exp(c (h i - w - px)^2 + c (h j - w - py)^2)
+// (the factored form above, px the offset from the anchor node), so
+//   dV_m/dx = -2c sum (h i - w - px) W g_m = -2c (h X_m + (h P/2 - w - px) V_m),
+// the same for y.  The gradient is recomputed here, never stored.
+enum AdjMode {
+    ADJ_TARGET = 0,   // dL/dptar = scale * sum_m du_m grad V_m   (NR = 2)
+    ADJ_SOURCE_SLP,   // dL/df = scale * V, dL/dpsrc = scale * sum_m f_m grad V_m
+    ADJ_SOURCE_DLP    // V = dL/d(n1f1, n2f1 + n1f2, n2f2): dL/df, dL/dn, dL/dpsrc
+};
+
+struct InterpAdjArgs {
+    const double *pts;     // points, caller order
+    const int *perm;       // bin order -> caller's index (tile kernel)
+    const int *boff;       // [nbx*nby+1] bin offsets of the points (tile kernel)
+    int nbx, nby;
+    const int *chunk_off;  // tile t owns chunks [chunk_off[t], chunk_off[t+1]) of 256 points
+    const int *block_tile; // workgroup -> tile
+    int ntx, nty;
+    int64_t n;             // point count (direct kernel)
+    const double *grids;   // [NR][Mx][My] real
+    int64_t gstride;
+    int Mx, My, P;
+    double Lx, Ly, h, c, w, ch2, scale;
+    const double *du;      // ADJ_TARGET: du at the points
+    const double *f, *nv;  // ADJ_SOURCE_*: densities and normals at the points
+    double *dpos, *df, *dn;   // outputs, caller order; may be null
+};
+
+template <int NR, class GET>
+__device__ inline void interp_sums(const GET &get, int P, const double *E1, double bx, double e3x,
+                                   double e3y, double (&V)[NR], double (&X)[NR], double (&Y)[NR])
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) u[i] = 0.0;
+#pragma unroll
+    for (int m = 0; m < NR; ++m) V[m] = X[m] = Y[m] = 0.0;
+    const double hp = 0.5 * P;
+    double ex = bx;
+    for (int i = 0; i <= P; ++i) {
+        double ey = ex * E1[i];
+        double r0[NR], ry[NR];
+#pragma unroll
+        for (int m = 0; m < NR; ++m) r0[m] = ry[m] = 0.0;
+        double jj = -hp;
+        for (int j = 0; j <= P; ++j) {
+            const double wgt = ey * E1[j];
+#pragma unroll
+            for (int m = 0; m < NR; ++m) {
+                const double wg = wgt * get(i, j, m);
+                r0[m] += wg;
+                ry[m] = fma(jj, wg, ry[m]);
+            }
+            ey *= e3y;
+            jj += 1.0;
+        }
+        const double ii = i - hp;
+#pragma unroll
+        for (int m = 0; m < NR; ++m) {
+            V[m] += r0[m];
+            X[m] = fma(ii, r0[m], X[m]);
+            Y[m] += ry[m];
+        }
+        ex *= e3x;
+    }
+}
+
+template <int NR, int MODE>
+__device__ inline void interp_adj_store(const InterpAdjArgs &a, int o, double px, double py,
+                                        const double (&V)[NR], const double (&X)[NR],
+                                        const double (&Y)[NR])
+{
+    const double k = -2.0 * a.c * a.scale;
+    const double ox = a.h * (0.5 * a.P) - a.w - px, oy = a.h * (0.5 * a.P) - a.w - py;
+    double gx[NR], gy[NR];
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+        gx[m] = k * (a.h * X[m] + ox * V[m]);
+        gy[m] = k * (a.h * Y[m] + oy * V[m]);
+    }
+    if (MODE == ADJ_TARGET) {
+        const double a1 = a.du[2 * o], a2 = a.du[2 * o + 1];
+        a.dpos[2 * o] = a1 * gx[0] + a2 * gx[1];
+        a.dpos[2 * o + 1] = a1 * gy[0] + a2 * gy[1];
+    } else if (MODE == ADJ_SOURCE_SLP) {
+        const double f1 = a.f[2 * o], f2 = a.f[2 * o + 1];
+        if (a.dpos) {
+            a.dpos[2 * o] = f1 * gx[0] + f2 * gx[1];
+            a.dpos[2 * o + 1] = f1 * gy[0] + f2 * gy[1];
+        }
+        if (a.df) {
+            a.df[2 * o] = a.scale * V[0];
+            a.df[2 * o + 1] = a.scale * V[1];
+        }
+    } else {
+        const double f1 = a.f[2 * o], f2 = a.f[2 * o + 1];
+        const double n1 = a.nv[2 * o], n2 = a.nv[2 * o + 1];
+        const double q1 = n1 * f1, q2 = n2 * f1 + n1 * f2, q3 = n2 * f2;
+        const double m1 = a.scale * V[0], m2 = a.scale * V[NR > 1 ? 1 : 0],
+                     m3 = a.scale * V[NR > 2 ? 2 : 0];
+        if (a.dpos) {
+            a.dpos[2 * o] = q1 * gx[0] + q2 * gx[NR > 1 ? 1 : 0] + q3 * gx[NR > 2 ? 2 : 0];
+            a.dpos[2 * o + 1] = q1 * gy[0] + q2 * gy[NR > 1 ? 1 : 0] + q3 * gy[NR > 2 ? 2 : 0];
+        }
+        if (a.df) {
+            a.df[2 * o] = m1 * n1 + m2 * n2;
+            a.df[2 * o + 1] = m2 * n1 + m3 * n2;
+        }
+        if (a.dn) {
+            a.dn[2 * o] = m1 * f1 + m2 * f2;
+            a.dn[2 * o + 1] = m2 * f1 + m3 * f2;
+        }
+    }
+}
+
+__host__ __device__ inline size_t interp_adj_lds(int P, int NR)
+{
+    const size_t R = kTile + P;
+    return sizeof(double) * (R * R * NR + P + 1);
+}
+
+// Tile kernel: the (16+P)^2 region of the NR grids that a 16x16 block of
+// anchors can touch is staged in LDS, as in k_interp_tile.
+template <int NR, int MODE>
+__global__ __launch_bounds__(256) void k_interp_adj_tile(InterpAdjArgs a)
+{
+    extern __shared__ double lds[];
+    const int P1 = a.P + 1;
+    const int R = kTile + a.P;
+    double *G = lds;                           // [R*R][NR]
+    double *E1 = G + (size_t)R * R * NR;       // [P1]
+    const int ntiles = a.ntx * a.nty;
+    const int b = blockIdx.x;
+    if (b >= a.chunk_off[ntiles]) return;   // uniform over the workgroup
+    const int tile = a.block_tile[b];
+    const int chunk = b - a.chunk_off[tile];
+    const int tx = tile / a.nty, ty = tile % a.nty;
+    const int bx0 = 2 * tx, by0 = 2 * ty;
+    const int nyb = min(2, a.nby - by0);
+    int r0s = 0, r0e = 0, r1s = 0, r1e = 0;
+    {
+        const int k0 = bx0 * a.nby + by0;
+        r0s = a.boff[k0];
+        r0e = a.boff[k0 + nyb];
+        if (bx0 + 1 < a.nbx) {
+            const int k1 = (bx0 + 1) * a.nby + by0;
+            r1s = a.boff[k1];
+            r1e = a.boff[k1 + nyb];
+        }
+    }
+    const int n0 = r0e - r0s, total = n0 + (r1e - r1s);
+    const int v0 = chunk * 256, v1 = min(total, v0 + 256);
+    fill_e1(E1, a.P, a.ch2);
+    const int X0 = tx * kTile, Y0 = ty * kTile;
+    for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
+        const int i = e / R, j = e % R;
+        const int64_t idx = (int64_t)((X0 + i) % a.Mx) * a.My + (Y0 + j) % a.My;
+#pragma unroll
+        for (int m = 0; m < NR; ++m) G[e * NR + m] = a.grids[m * a.gstride + idx];
+    }
+    __syncthreads();
+    for (int v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+        const int k = v < n0 ? r0s + v : r1s + (v - n0);
+        const int o = a.perm[k];
+        int mx, my;
+        double px, py;
+        se2p_anchor(a.pts[2 * o], a.pts[2 * o + 1], a.Lx, a.Ly, a.h, a.P, &mx, &my, &px, &py);
+        double bx, e3x, e3y;
+        fgg_factors(px, py, a.c, a.w, a.h, &bx, &e3x, &e3y);
+        const int li = ((mx % a.Mx) + a.Mx) % a.Mx - X0;
+        const int lj = ((my % a.My) + a.My) % a.My - Y0;
+        const double *base = G + (size_t)(li * R + lj) * NR;
+        auto get = [&](int i, int j, int m) { return base[(i * R + j) * NR + m]; };
+        double V[NR], X[NR], Y[NR];
+        interp_sums<NR>(get, a.P, E1, bx, e3x, e3y, V, X, Y);
+        interp_adj_store<NR, MODE>(a, o, px, py, V, X, Y);
+    }
+    (void)P1;
+}
+
+// Direct kernel (regions too large for LDS): one thread per point, grid
+// values read from global memory with periodic wrap.
+template <int NR, int MODE>
+__global__ __launch_bounds__(256) void k_interp_adj_direct(InterpAdjArgs a)
+{
+    __shared__ double E1[128];   // P <= 126
+    fill_e1(E1, a.P, a.ch2);
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const int o = (int)t;
+    int mx, my;
+    double px, py;
+    se2p_anchor(a.pts[2 * o], a.pts[2 * o + 1], a.Lx, a.Ly, a.h, a.P, &mx, &my, &px, &py);
+    double bx, e3x, e3y;
+    fgg_factors(px, py, a.c, a.w, a.h, &bx, &e3x, &e3y);
+    const int col0 = ((mx % a.Mx) + a.Mx) % a.Mx;
+    const int row0 = ((my % a.My) + a.My) % a.My;
+    auto get = [&](int i, int j, int m) {
+        const int64_t idx = (int64_t)((col0 + i) % a.Mx) * a.My + (row0 + j) % a.My;
+        return a.grids[m * a.gstride + idx];
+    };
+    double V[NR], X[NR], Y[NR];
+    interp_sums<NR>(get, a.P, E1, bx, e3x, e3y, V, X, Y);
+    interp_adj_store<NR, MODE>(a, o, px, py, V, X, Y);
 }
 
 // ---- rocFFT plans ------------------------------------------------------------
@@ -861,172 +1094,171 @@ int check_kspace_params(se2p_ctx *ctx, const KGeom &g)
     return 0;
 }
 
-// ---- host driver -------------------------------------------------------------
-int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *nv,
-               int nrhs, int64_t dstride, int64_t ustride, double *u)
+// ---- host building blocks ----------------------------------------------------
+constexpr size_t kLdsMax = 65536;
+
+// Gridding geometry of one call.
+struct KLayout {
+    int Mx, My, P;
+    int nbx, nby, nbins;     // 8x8 anchor bins
+    int ntx, nty, ntiles;    // 16x16 node tiles
+    int64_t G, GH;           // real grid / half spectrum sizes
+    double h, c, ch2, interp_scale;
+    bool wrap;               // the support is wider than the grid
+    size_t lds_itile;        // forward interpolation tile (2 grids)
+    bool itile;
+};
+
+static KLayout make_layout(const KGeom &g)
+{
+    KLayout L;
+    L.Mx = g.Mx;
+    L.My = g.My;
+    L.P = g.P;
+    L.nbx = (L.Mx + kBin - 1) / kBin;
+    L.nby = (L.My + kBin - 1) / kBin;
+    L.nbins = L.nbx * L.nby;
+    L.ntx = (L.Mx + kTile - 1) / kTile;
+    L.nty = (L.My + kTile - 1) / kTile;
+    L.ntiles = L.ntx * L.nty;
+    L.G = (int64_t)L.Mx * L.My;
+    L.GH = (int64_t)L.Mx * (L.My / 2 + 1);
+    L.h = g.Lx / L.Mx;   // also used for y (mex_stokes_slp_kspace.cpp:45)
+    L.c = -2 * g.xi * g.xi / g.eta;
+    L.ch2 = L.c * L.h * L.h;
+    double s = 4 * g.xi * g.xi / g.eta;   // :325-328
+    s = s * s * L.h * L.h / SE2P_PI;
+    L.interp_scale = s / (4 * SE2P_PI);
+    L.wrap = (L.Mx <= L.P) || (L.My <= L.P);
+    L.lds_itile = sizeof(double2) * (size_t)(kTile + L.P) * (kTile + L.P) + sizeof(double) * (L.P + 1);
+    L.itile = L.lds_itile <= kLdsMax;
+    return L;
+}
+
+// One spreading pass (nsp points into ngrp groups of nq grids) and the
+// interpolation at nip points that reads its result: kernel choice, work
+// split and buffers.
+struct GridPass {
+    int nq, ngrp;
+    bool strip;
+    size_t lds_strip, lds_gen;
+    int chunk;
+    int64_t max_spread_chunks, max_interp_chunks;
+    int *scnt, *soffc, *icnt, *ioffc, *rng_start, *rng_len, *smap, *imap;
+    double *partial;
+};
+
+static void plan_pass(const KLayout &L, int nq, int ngrp, int64_t nsp, int64_t nip, GridPass *gp)
+{
+    const int P = L.P;
+    gp->nq = nq;
+    gp->ngrp = ngrp;
+    gp->lds_strip = sizeof(double) * strip_lds_doubles(P, nq);
+    gp->strip = (L.Mx > P) && (L.My > P + kPad) && gp->lds_strip <= kLdsMax;
+    gp->lds_gen = sizeof(double) * ((size_t)2 * kBatch * (P + 1) + (size_t)kBatch * nq + (P + 1)) +
+                  sizeof(int) * 2 * kBatch;
+    gp->chunk = 4 * strip_batch(nq) * 4;   // 4 wavefronts x 4 batches
+    const int64_t tps_x = std::min<int64_t>(L.ntx, (kTile + P + kBin - 1) / kTile + 1);
+    const int64_t tps_y = std::min<int64_t>(L.nty, (kTile + P + kBin - 1) / kTile + 1);
+    gp->max_spread_chunks = L.ntiles + (tps_x * tps_y * nsp + gp->chunk - 1) / gp->chunk;
+    gp->max_interp_chunks = L.ntiles + (nip + 255) / 256;
+}
+
+static void carve_pass(Carve &cv, const KLayout &L, GridPass *gp)
+{
+    gp->scnt = cv.take<int>(L.ntiles + 1);
+    gp->soffc = cv.take<int>(L.ntiles + 1);
+    gp->icnt = cv.take<int>(L.ntiles + 1);
+    gp->ioffc = cv.take<int>(L.ntiles + 1);
+    gp->rng_start = cv.take<int>((size_t)L.ntiles * kTR);
+    gp->rng_len = cv.take<int>((size_t)L.ntiles * kTR);
+    gp->smap = cv.take<int>(gp->max_spread_chunks);
+    gp->imap = cv.take<int>(gp->max_interp_chunks);
+    gp->partial = gp->strip ? cv.take<double>((size_t)256 * gp->nq * gp->max_spread_chunks * gp->ngrp)
+                            : nullptr;
+}
+
+static SpreadArgs spread_args(const KLayout &L, const GridPass &gp, const int *boff, double *grids)
+{
+    SpreadArgs sa{};
+    sa.boff = boff;
+    sa.nbx = L.nbx;
+    sa.nby = L.nby;
+    sa.Mx = L.Mx;
+    sa.My = L.My;
+    sa.P = L.P;
+    sa.ch2 = L.ch2;
+    sa.grids = grids;
+    sa.gstride = L.G;
+    sa.chunk_off = gp.soffc;
+    sa.ntx = L.ntx;
+    sa.nty = L.nty;
+    sa.chunk = gp.chunk;
+    sa.partial = gp.partial;
+    sa.block_tile = gp.smap;
+    sa.rng_start = gp.rng_start;
+    sa.rng_len = gp.rng_len;
+    sa.max_chunks = gp.max_spread_chunks;
+    return sa;
+}
+
+// Sort n points by anchor bin.
+static int bin_points(se2p_ctx *ctx, const KLayout &L, const KGeom &g, const double *p, int64_t n,
+                      KeySort *ks)
+{
+    hipLaunchKernelGGL(k_bin_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p,
+                       (int)n, g.Lx, g.Ly, L.h, L.P, L.Mx, L.My, L.nby, ks->key);
+    return keysort_run(ctx, ks, n, L.nbins);
+}
+
+// Work split of a pass: candidate runs and chunk counts per tile for the
+// spreading (bins sa.boff) and for the interpolation at the points binned in
+// iboff.
+static void tile_maps(hipStream_t st, const KLayout &L, const GridPass &gp, const SpreadArgs &sa,
+                      const int *iboff)
+{
+    hipLaunchKernelGGL(k_tile_work, dim3(L.ntiles), dim3(64), 0, st, sa, iboff, gp.scnt, gp.icnt,
+                       gp.rng_start, gp.rng_len);
+    scan_ints(st, gp.scnt, L.ntiles, gp.soffc);
+    scan_ints(st, gp.icnt, L.ntiles, gp.ioffc);
+    expand_map(st, gp.soffc, L.ntiles, gp.smap);
+    expand_map(st, gp.ioffc, L.ntiles, gp.imap);
+}
+
+static int spread(se2p_ctx *ctx, const KLayout &L, const GridPass &gp, const SpreadArgs &sa)
 {
     hipStream_t st = ctx->stream;
-    if (g.nt == 0 || nrhs == 0) return 0;
-    const int64_t ns = g.ns, nt = g.nt;
-    const int Mx = g.Mx, My = g.My, P = g.P;
-    const double h = g.Lx / Mx;                  // also used for y (:45)
-    const double c = -2 * g.xi * g.xi / g.eta;
-    const int ncomp = kind == 0 ? 2 : 3;
-    // right-hand sides per spreading group (grids per group nq = ncomp*RB);
-    // the last group is padded with zero densities
-    const int RB = nrhs >= 2 ? 4 : 1;
-    const int ngrp = (nrhs + RB - 1) / RB;
-    const int nrp = ngrp * RB;                   // padded right-hand sides
-    const int nq = ncomp * RB;
-    const int64_t G = (int64_t)Mx * My;
-    const int Mh = My / 2 + 1;
-    const int64_t GH = (int64_t)Mx * Mh;
-    const int nbx = (Mx + kBin - 1) / kBin, nby = (My + kBin - 1) / kBin;
-    const int nbins = nbx * nby;
-    // targets are binned separately unless they are the source array itself
-    const bool same_points = (g.ptar == g.psrc) && (nt == ns);
-
-    // kernel variants
-    const size_t lds_max = 65536;
-    const bool strip = (Mx > P) && (My > P + kPad) && sizeof(double) * strip_lds_doubles(P, nq) <= lds_max;
-    const size_t lds_strip = sizeof(double) * strip_lds_doubles(P, nq);
-    const size_t lds_gen = sizeof(double) * ((size_t)2 * kBatch * (P + 1) + (size_t)kBatch * nq + (P + 1)) +
-                           sizeof(int) * 2 * kBatch;
-    const bool wrap = (Mx <= P) || (My <= P);
-    const size_t lds_itile = sizeof(double2) * (size_t)(kTile + P) * (kTile + P) + sizeof(double) * (P + 1);
-    const bool itile = lds_itile <= lds_max;
-
-    FftPlans *plans = nullptr;
-    if (ns > 0)
-        if (int rc = get_plans(ctx, Mx, My, ncomp * nrp, 2 * nrp, &plans)) return rc;
-
-    // work split of the tile kernels
-    const int ntx = (Mx + kTile - 1) / kTile, nty = (My + kTile - 1) / kTile;
-    const int ntiles = ntx * nty;
-    const int chunk = 4 * strip_batch(nq) * 4;   // 4 wavefronts x 4 batches
-    const int64_t tps_x = std::min<int64_t>(ntx, (kTile + P + kBin - 1) / kTile + 1);
-    const int64_t tps_y = std::min<int64_t>(nty, (kTile + P + kBin - 1) / kTile + 1);
-    const int64_t max_spread_chunks = ntiles + (tps_x * tps_y * ns + chunk - 1) / chunk;
-    const int64_t max_interp_chunks = ntiles + (nt + 255) / 256;
-
-    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t ks = keysort_bytes(ns, nbins);
-    const size_t kt = (itile && !same_points) ? keysort_bytes(nt, nbins) : 0;
-    const size_t nsz = (size_t)(ns > 0 ? ns : 1);
-    const size_t ngrid_all = (size_t)ncomp * nrp;
-    size_t need = ks + kt + up(sizeof(int2) * nsz) + up(sizeof(double) * 3 * nsz) +
-                  up(sizeof(double) * ngrid_all * nsz) + up(sizeof(double) * ngrid_all * (size_t)G) +
-                  up(sizeof(double2) * ngrid_all * (size_t)GH) +
-                  4 * up(sizeof(int) * (size_t)(ntiles + 1)) + 2 * up(sizeof(int) * (size_t)ntiles * kTR) +
-                  up(sizeof(int) * (size_t)max_spread_chunks) + up(sizeof(int) * (size_t)max_interp_chunks) +
-                  (strip ? up(sizeof(double) * 256 * nq * (size_t)max_spread_chunks * ngrp) : 0);
-    if (int rc = ensure_workspace(ctx, need)) return rc;
-    char *base = ctx->ws;
-    size_t o = 0;
-    KeySort kss, kst;
-    keysort_carve(base + o, ns, nbins, &kss); o += ks;
-    if (kt) { keysort_carve(base + o, nt, nbins, &kst); o += kt; }
-    int2 *anchor = (int2 *)(base + o); o += up(sizeof(int2) * nsz);
-    double *fgg = (double *)(base + o); o += up(sizeof(double) * 3 * nsz);
-    double *q = (double *)(base + o); o += up(sizeof(double) * ngrid_all * nsz);
-    double *grids = (double *)(base + o); o += up(sizeof(double) * ngrid_all * (size_t)G);
-    double2 *spec = (double2 *)(base + o); o += up(sizeof(double2) * ngrid_all * (size_t)GH);
-    int *scnt = (int *)(base + o); o += up(sizeof(int) * (size_t)(ntiles + 1));
-    int *soffc = (int *)(base + o); o += up(sizeof(int) * (size_t)(ntiles + 1));
-    int *icnt = (int *)(base + o); o += up(sizeof(int) * (size_t)(ntiles + 1));
-    int *ioffc = (int *)(base + o); o += up(sizeof(int) * (size_t)(ntiles + 1));
-    int *rng_start = (int *)(base + o); o += up(sizeof(int) * (size_t)ntiles * kTR);
-    int *rng_len = (int *)(base + o); o += up(sizeof(int) * (size_t)ntiles * kTR);
-    int *smap = (int *)(base + o); o += up(sizeof(int) * (size_t)max_spread_chunks);
-    int *imap = (int *)(base + o); o += up(sizeof(int) * (size_t)max_interp_chunks);
-    double *partial = strip ? (double *)(base + o) : nullptr;
-
-    const int threads = 256;
-    if (ns == 0) {
-        // no sources: every grid is zero, so is the result
-        for (int r = 0; r < nrhs; ++r)
-            hipLaunchKernelGGL(k_zero, dim3((unsigned)((2 * nt + threads - 1) / threads)),
-                               dim3(threads), 0, st, u + (int64_t)r * ustride, 2 * nt);
-        SE2P_HIP(hipGetLastError());
-        return 0;
+    const dim3 tiles2d(L.ntx, L.nty, gp.ngrp);
+#define SE2P_SPREAD(NQV)                                                                           \
+    if (gp.strip) {                                                                                \
+        hipLaunchKernelGGL((k_spread_strip<NQV>), dim3((unsigned)gp.max_spread_chunks, gp.ngrp),   \
+                           dim3(256), gp.lds_strip, st, sa);                                       \
+        hipLaunchKernelGGL((k_spread_combine<NQV>), dim3(L.ntiles, gp.ngrp), dim3(64), 0, st, sa); \
+    } else if (L.wrap) {                                                                           \
+        hipLaunchKernelGGL((k_spread<NQV, true>), tiles2d, dim3(256), gp.lds_gen, st, sa);        \
+    } else {                                                                                       \
+        hipLaunchKernelGGL((k_spread<NQV, false>), tiles2d, dim3(256), gp.lds_gen, st, sa);       \
     }
-
-    stage_begin(ctx, ST_SORT);
-    hipLaunchKernelGGL(k_bin_keys, dim3((unsigned)((ns + threads - 1) / threads)), dim3(threads), 0,
-                       st, g.psrc, (int)ns, g.Lx, g.Ly, h, P, Mx, My, nby, kss.key);
-    if (int rc = keysort_run(ctx, &kss, ns, nbins)) return rc;
-    if (kt) {
-        hipLaunchKernelGGL(k_bin_keys, dim3((unsigned)((nt + threads - 1) / threads)), dim3(threads),
-                           0, st, g.ptar, (int)nt, g.Lx, g.Ly, h, P, Mx, My, nby, kst.key);
-        if (int rc = keysort_run(ctx, &kst, nt, nbins)) return rc;
-    }
-    const KeySort &tbins = same_points ? kss : kst;
-    SpreadArgs sa{};
-    sa.boff = kss.off;
-    sa.nbx = nbx;
-    sa.nby = nby;
-    sa.Mx = Mx;
-    sa.My = My;
-    sa.P = P;
-    sa.gstride = G;
-    sa.chunk_off = soffc;
-    sa.ntx = ntx;
-    sa.nty = nty;
-    sa.chunk = chunk;
-    sa.partial = partial;
-    sa.block_tile = smap;
-    sa.rng_start = rng_start;
-    sa.rng_len = rng_len;
-    sa.max_chunks = max_spread_chunks;
-    if (strip || itile) {
-        hipLaunchKernelGGL(k_tile_work, dim3(ntiles), dim3(64), 0, st, sa, tbins.off, scnt, icnt,
-                           rng_start, rng_len);
-        scan_ints(st, scnt, ntiles, soffc);
-        scan_ints(st, icnt, ntiles, ioffc);
-        expand_map(st, soffc, ntiles, smap);
-        expand_map(st, ioffc, ntiles, imap);
-    }
-    stage_end(ctx, ST_SORT);
-
-    const double ch2 = c * h * h;
-    double s_int = 4 * g.xi * g.xi / g.eta;                  // :325-328
-    s_int = s_int * s_int * h * h / SE2P_PI;
-    const double interp_scale = s_int / (4 * SE2P_PI);
-
-    stage_begin(ctx, ST_SPREAD);
-    if (kind == 0)
-        hipLaunchKernelGGL(k_gather_grid_src<0>, dim3((unsigned)((ns + threads - 1) / threads)),
-                           dim3(threads), 0, st, g.psrc, f, nv, kss.perm, (int)ns, nrhs, nrp,
-                           dstride, g.Lx, g.Ly, h, P, Mx, My, c, g.w, anchor, fgg, q);
-    else
-        hipLaunchKernelGGL(k_gather_grid_src<1>, dim3((unsigned)((ns + threads - 1) / threads)),
-                           dim3(threads), 0, st, g.psrc, f, nv, kss.perm, (int)ns, nrhs, nrp,
-                           dstride, g.Lx, g.Ly, h, P, Mx, My, c, g.w, anchor, fgg, q);
-    sa.s.anchor = anchor;
-    sa.s.fgg = fgg;
-    sa.s.q = q;
-    sa.s.ns = ns;
-    sa.ch2 = ch2;
-    sa.grids = grids;
-    const dim3 tiles2d(ntx, nty, ngrp);
-#define SE2P_SPREAD(NQV)                                                                          \
-    if (strip) {                                                                                  \
-        hipLaunchKernelGGL((k_spread_strip<NQV>), dim3((unsigned)max_spread_chunks, ngrp),         \
-                           dim3(256), lds_strip, st, sa);                                         \
-        hipLaunchKernelGGL((k_spread_combine<NQV>), dim3(ntiles, ngrp), dim3(64), 0, st, sa);     \
-    } else if (wrap) hipLaunchKernelGGL((k_spread<NQV, true>), tiles2d, dim3(256), lds_gen, st, sa); \
-    else hipLaunchKernelGGL((k_spread<NQV, false>), tiles2d, dim3(256), lds_gen, st, sa);
-    switch (nq) {
+    switch (gp.nq) {
     case 2: SE2P_SPREAD(2); break;
     case 3: SE2P_SPREAD(3); break;
     case 8: SE2P_SPREAD(8); break;
     case 12: SE2P_SPREAD(12); break;
-    default: return fail(ctx, SE2P_ERR_ARG, "unsupported grid count %d", nq);
+    default: return fail(ctx, SE2P_ERR_ARG, "unsupported grid count %d", gp.nq);
     }
 #undef SE2P_SPREAD
     SE2P_HIP(hipGetLastError());
-    stage_end(ctx, ST_SPREAD);
+    return 0;
+}
 
+enum FilterKind { FILTER_SLP, FILTER_DLP, FILTER_DLP_ADJ };
+
+// Forward transforms, filter, inverse transforms; grids in, filtered real
+// grids out (spec is scratch).
+static int transform_filter(se2p_ctx *ctx, const KLayout &L, const KGeom &g, FftPlans *plans,
+                            FilterKind fk, int nrhs, double *grids, double2 *spec)
+{
+    hipStream_t st = ctx->stream;
     stage_begin(ctx, ST_FFT_FWD);
     rocfft_execution_info_set_stream(plans->fwd_info, st);
     {
@@ -1040,18 +1272,19 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     stage_begin(ctx, ST_FILTER);
     FilterArgs fa;
     fa.spec = spec;
-    fa.sstride = GH;
-    fa.Mx = Mx;
-    fa.My = My;
-    fa.nrhs = nrp;
+    fa.sstride = L.GH;
+    fa.Mx = L.Mx;
+    fa.My = L.My;
+    fa.nrhs = nrhs;
     fa.Lx = g.Lx;
     fa.Ly = g.Ly;
     fa.xi = g.xi;
     fa.eta = g.eta;
-    fa.scale = 1.0 / ((double)Mx * (double)My);
-    const unsigned fblocks = (unsigned)((GH + threads - 1) / threads);
-    if (kind == 0) hipLaunchKernelGGL(k_filter_slp, dim3(fblocks), dim3(threads), 0, st, fa);
-    else hipLaunchKernelGGL(k_filter_dlp, dim3(fblocks), dim3(threads), 0, st, fa);
+    fa.scale = 1.0 / ((double)L.Mx * (double)L.My);
+    const unsigned fblocks = (unsigned)((L.GH + 255) / 256);
+    if (fk == FILTER_SLP) hipLaunchKernelGGL(k_filter_slp, dim3(fblocks), dim3(256), 0, st, fa);
+    else if (fk == FILTER_DLP) hipLaunchKernelGGL(k_filter_dlp, dim3(fblocks), dim3(256), 0, st, fa);
+    else hipLaunchKernelGGL(k_filter_dlp_adj, dim3(fblocks), dim3(256), 0, st, fa);
     SE2P_HIP(hipGetLastError());
     stage_end(ctx, ST_FILTER);
 
@@ -1064,62 +1297,316 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         if (s != rocfft_status_success) return fail(ctx, SE2P_ERR_FFT, "rocfft_execute (inv) = %d", (int)s);
     }
     stage_end(ctx, ST_FFT_INV);
+    return 0;
+}
+
+// ---- forward -------------------------------------------------------------------
+int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *nv,
+               int nrhs, int64_t dstride, int64_t ustride, double *u)
+{
+    hipStream_t st = ctx->stream;
+    if (g.nt == 0 || nrhs == 0) return 0;
+    const int64_t ns = g.ns, nt = g.nt;
+    if (ns == 0) {
+        // no sources: every grid is zero, so is the result
+        for (int r = 0; r < nrhs; ++r) launch_zero(st, u + (int64_t)r * ustride, 2 * nt);
+        SE2P_HIP(hipGetLastError());
+        return 0;
+    }
+    const KLayout L = make_layout(g);
+    const int ncomp = kind == 0 ? 2 : 3;
+    // right-hand sides per spreading group (grids per group nq = ncomp*RB);
+    // the last group is padded with zero densities
+    const int RB = nrhs >= 2 ? 4 : 1;
+    const int ngrp = (nrhs + RB - 1) / RB;
+    const int nrp = ngrp * RB;   // padded right-hand sides
+    // targets are binned separately unless they are the source array itself
+    const bool same_points = (g.ptar == g.psrc) && (nt == ns);
+    GridPass gp;
+    plan_pass(L, ncomp * RB, ngrp, ns, nt, &gp);
+    const bool maps = gp.strip || L.itile;
+    FftPlans *plans = nullptr;
+    if (int rc = get_plans(ctx, L.Mx, L.My, ncomp * nrp, 2 * nrp, &plans)) return rc;
+
+    KeySort kss, kst;
+    int2 *anchor;
+    double *fgg, *q, *grids;
+    double2 *spec;
+    auto layout = [&](Carve &cv) {
+        char *p = cv.take<char>(keysort_bytes(ns, L.nbins));
+        if (p) keysort_carve(p, ns, L.nbins, &kss);
+        if (!same_points) {
+            p = cv.take<char>(keysort_bytes(nt, L.nbins));
+            if (p) keysort_carve(p, nt, L.nbins, &kst);
+        }
+        anchor = cv.take<int2>(ns);
+        fgg = cv.take<double>(3 * (size_t)ns);
+        q = cv.take<double>((size_t)ncomp * nrp * ns);
+        grids = cv.take<double>((size_t)ncomp * nrp * L.G);
+        spec = cv.take<double2>((size_t)ncomp * nrp * L.GH);
+        carve_pass(cv, L, &gp);
+    };
+    Carve sizing;
+    layout(sizing);
+    if (int rc = ensure_workspace(ctx, sizing.off)) return rc;
+    Carve cv;
+    cv.base = ctx->ws;
+    layout(cv);
+
+    stage_begin(ctx, ST_SORT);
+    if (int rc = bin_points(ctx, L, g, g.psrc, ns, &kss)) return rc;
+    if (!same_points)
+        if (int rc = bin_points(ctx, L, g, g.ptar, nt, &kst)) return rc;
+    const KeySort &tbins = same_points ? kss : kst;
+    SpreadArgs sa = spread_args(L, gp, kss.off, grids);
+    if (maps) tile_maps(st, L, gp, sa, tbins.off);
+    stage_end(ctx, ST_SORT);
+
+    stage_begin(ctx, ST_SPREAD);
+    const unsigned sblocks = (unsigned)((ns + 255) / 256);
+    if (kind == 0)
+        hipLaunchKernelGGL(k_gather_grid_src<0>, dim3(sblocks), dim3(256), 0, st, g.psrc, f, nv,
+                           kss.perm, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly, L.h, L.P, L.Mx, L.My,
+                           L.c, g.w, anchor, fgg, q);
+    else
+        hipLaunchKernelGGL(k_gather_grid_src<1>, dim3(sblocks), dim3(256), 0, st, g.psrc, f, nv,
+                           kss.perm, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly, L.h, L.P, L.Mx, L.My,
+                           L.c, g.w, anchor, fgg, q);
+    sa.s.anchor = anchor;
+    sa.s.fgg = fgg;
+    sa.s.q = q;
+    sa.s.ns = ns;
+    if (int rc = spread(ctx, L, gp, sa)) return rc;
+    stage_end(ctx, ST_SPREAD);
+
+    if (int rc = transform_filter(ctx, L, g, plans, kind == 0 ? FILTER_SLP : FILTER_DLP, nrp, grids, spec))
+        return rc;
 
     stage_begin(ctx, ST_INTERP);
-    if (itile) {
+    if (L.itile) {
         InterpTileArgs ia;
         ia.ptar = g.ptar;
         ia.perm = tbins.perm;
         ia.boff = tbins.off;
-        ia.nbx = nbx;
-        ia.nby = nby;
-        ia.chunk_off = ioffc;
-        ia.block_tile = imap;
-        ia.ntx = ntx;
-        ia.nty = nty;
+        ia.nbx = L.nbx;
+        ia.nby = L.nby;
+        ia.chunk_off = gp.ioffc;
+        ia.block_tile = gp.imap;
+        ia.ntx = L.ntx;
+        ia.nty = L.nty;
         ia.grids = grids;
-        ia.gstride = G;
-        ia.Mx = Mx;
-        ia.My = My;
-        ia.P = P;
+        ia.gstride = L.G;
+        ia.Mx = L.Mx;
+        ia.My = L.My;
+        ia.P = L.P;
         ia.nr = nrhs;
         ia.Lx = g.Lx;
         ia.Ly = g.Ly;
-        ia.h = h;
-        ia.c = c;
+        ia.h = L.h;
+        ia.c = L.c;
         ia.w = g.w;
-        ia.ch2 = ch2;
-        ia.scale = interp_scale;
+        ia.ch2 = L.ch2;
+        ia.scale = L.interp_scale;
         ia.u = u;
         ia.ustride = ustride;
-        hipLaunchKernelGGL(k_interp_tile, dim3((unsigned)max_interp_chunks, nrhs), dim3(256),
-                           lds_itile, st, ia);
+        hipLaunchKernelGGL(k_interp_tile, dim3((unsigned)gp.max_interp_chunks, nrhs), dim3(256),
+                           L.lds_itile, st, ia);
     } else {
         for (int r = 0; r < nrhs; ++r) {
             InterpArgs ia;
             ia.ptar = g.ptar;
             ia.nt = nt;
-            ia.grids = grids + (int64_t)(2 * r) * G;
-            ia.gstride = G;
-            ia.Mx = Mx;
-            ia.My = My;
-            ia.P = P;
+            ia.grids = grids + (int64_t)(2 * r) * L.G;
+            ia.gstride = L.G;
+            ia.Mx = L.Mx;
+            ia.My = L.My;
+            ia.P = L.P;
             ia.nr = 1;
             ia.Lx = g.Lx;
             ia.Ly = g.Ly;
-            ia.h = h;
-            ia.c = c;
+            ia.h = L.h;
+            ia.c = L.c;
             ia.w = g.w;
-            ia.ch2 = ch2;
-            ia.scale = interp_scale;
+            ia.ch2 = L.ch2;
+            ia.scale = L.interp_scale;
             ia.u = u + (int64_t)r * ustride;
             ia.ustride = ustride;
-            const unsigned iblocks = (unsigned)((nt + threads - 1) / threads);
-            hipLaunchKernelGGL(k_interp<1>, dim3(iblocks), dim3(threads), 0, st, ia);
+            hipLaunchKernelGGL(k_interp<1>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, ia);
         }
     }
     SE2P_HIP(hipGetLastError());
     stage_end(ctx, ST_INTERP);
+    return 0;
+}
+
+// ---- adjoint -------------------------------------------------------------------
+// Two passes, each one spreading + transform/filter + interpolation:
+//   target pass: the forward grids (sources spread, filtered), interpolated at
+//     the targets with derivative weights, weighted by du: dL/dptar;
+//   source pass: du spread at the targets, transposed filter (the Stokeslet
+//     filter is its own transpose), interpolated at the sources with value and
+//     derivative weights: dL/df, dL/dn, dL/dpsrc.
+template <int NR, int MODE>
+static void launch_interp_adj(hipStream_t st, const KLayout &L, const GridPass &gp, bool tile,
+                              const InterpAdjArgs &ia)
+{
+    if (tile)
+        hipLaunchKernelGGL((k_interp_adj_tile<NR, MODE>), dim3((unsigned)gp.max_interp_chunks),
+                           dim3(256), interp_adj_lds(L.P, NR), st, ia);
+    else
+        hipLaunchKernelGGL((k_interp_adj_direct<NR, MODE>), dim3((unsigned)((ia.n + 255) / 256)),
+                           dim3(256), 0, st, ia);
+}
+
+int kspace_adjoint(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *nv,
+                   const double *du, double *dpsrc, double *dptar, double *df, double *dn)
+{
+    hipStream_t st = ctx->stream;
+    const int64_t ns = g.ns, nt = g.nt;
+    const bool want_src = dpsrc || df || dn;
+    if (ns == 0 || nt == 0) {
+        // an empty side makes every grid zero
+        launch_zero(st, dptar, 2 * nt);
+        launch_zero(st, dpsrc, 2 * ns);
+        launch_zero(st, df, 2 * ns);
+        launch_zero(st, dn, 2 * ns);
+        SE2P_HIP(hipGetLastError());
+        return 0;
+    }
+    const KLayout L = make_layout(g);
+    const int ncomp = kind == 0 ? 2 : 3;
+    const bool same_points = (g.ptar == g.psrc) && (nt == ns);
+    GridPass gt, gs;   // target pass, source pass
+    plan_pass(L, ncomp, 1, ns, nt, &gt);
+    plan_pass(L, 2, 1, nt, ns, &gs);
+    const bool tile_t = interp_adj_lds(L.P, 2) <= kLdsMax;
+    const bool tile_s = interp_adj_lds(L.P, ncomp) <= kLdsMax;
+    FftPlans *pt = nullptr, *ps = nullptr;
+    if (dptar)
+        if (int rc = get_plans(ctx, L.Mx, L.My, ncomp, 2, &pt)) return rc;
+    if (want_src)
+        if (int rc = get_plans(ctx, L.Mx, L.My, 2, ncomp, &ps)) return rc;
+
+    const int64_t nmax = std::max(ns, nt);
+    KeySort kss, kst;
+    int2 *anchor;
+    double *fgg, *q, *grids;
+    double2 *spec;
+    auto layout = [&](Carve &cv) {
+        char *p = cv.take<char>(keysort_bytes(ns, L.nbins));
+        if (p) keysort_carve(p, ns, L.nbins, &kss);
+        if (!same_points) {
+            p = cv.take<char>(keysort_bytes(nt, L.nbins));
+            if (p) keysort_carve(p, nt, L.nbins, &kst);
+        }
+        anchor = cv.take<int2>(nmax);
+        fgg = cv.take<double>(3 * (size_t)nmax);
+        q = cv.take<double>((size_t)ncomp * nmax);
+        grids = cv.take<double>((size_t)ncomp * L.G);
+        spec = cv.take<double2>((size_t)ncomp * L.GH);
+        carve_pass(cv, L, &gt);
+        carve_pass(cv, L, &gs);
+    };
+    Carve sizing;
+    layout(sizing);
+    if (int rc = ensure_workspace(ctx, sizing.off)) return rc;
+    Carve cv;
+    cv.base = ctx->ws;
+    layout(cv);
+
+    stage_begin(ctx, ST_SORT);
+    if (int rc = bin_points(ctx, L, g, g.psrc, ns, &kss)) return rc;
+    if (!same_points)
+        if (int rc = bin_points(ctx, L, g, g.ptar, nt, &kst)) return rc;
+    const KeySort &tbins = same_points ? kss : kst;
+    SpreadArgs sat = spread_args(L, gt, kss.off, grids);     // sources -> grids
+    SpreadArgs sas = spread_args(L, gs, tbins.off, grids);   // du at the targets -> grids
+    if (dptar && (gt.strip || tile_t)) tile_maps(st, L, gt, sat, tbins.off);
+    if (want_src && (gs.strip || tile_s)) tile_maps(st, L, gs, sas, kss.off);
+    stage_end(ctx, ST_SORT);
+
+    InterpAdjArgs ia{};
+    ia.nbx = L.nbx;
+    ia.nby = L.nby;
+    ia.ntx = L.ntx;
+    ia.nty = L.nty;
+    ia.grids = grids;
+    ia.gstride = L.G;
+    ia.Mx = L.Mx;
+    ia.My = L.My;
+    ia.P = L.P;
+    ia.Lx = g.Lx;
+    ia.Ly = g.Ly;
+    ia.h = L.h;
+    ia.c = L.c;
+    ia.w = g.w;
+    ia.ch2 = L.ch2;
+    ia.scale = L.interp_scale;
+
+    if (dptar) {
+        stage_begin(ctx, ST_SPREAD);
+        const unsigned blocks = (unsigned)((ns + 255) / 256);
+        if (kind == 0)
+            hipLaunchKernelGGL(k_gather_grid_src<0>, dim3(blocks), dim3(256), 0, st, g.psrc, f, nv,
+                               kss.perm, (int)ns, 1, 1, (int64_t)0, g.Lx, g.Ly, L.h, L.P, L.Mx,
+                               L.My, L.c, g.w, anchor, fgg, q);
+        else
+            hipLaunchKernelGGL(k_gather_grid_src<1>, dim3(blocks), dim3(256), 0, st, g.psrc, f, nv,
+                               kss.perm, (int)ns, 1, 1, (int64_t)0, g.Lx, g.Ly, L.h, L.P, L.Mx,
+                               L.My, L.c, g.w, anchor, fgg, q);
+        sat.s.anchor = anchor;
+        sat.s.fgg = fgg;
+        sat.s.q = q;
+        sat.s.ns = ns;
+        if (int rc = spread(ctx, L, gt, sat)) return rc;
+        stage_end(ctx, ST_SPREAD);
+        if (int rc = transform_filter(ctx, L, g, pt, kind == 0 ? FILTER_SLP : FILTER_DLP, 1, grids, spec))
+            return rc;
+        stage_begin(ctx, ST_INTERP);
+        InterpAdjArgs a = ia;
+        a.pts = g.ptar;
+        a.perm = tbins.perm;
+        a.boff = tbins.off;
+        a.chunk_off = gt.ioffc;
+        a.block_tile = gt.imap;
+        a.n = nt;
+        a.du = du;
+        a.dpos = dptar;
+        launch_interp_adj<2, ADJ_TARGET>(st, L, gt, tile_t, a);
+        SE2P_HIP(hipGetLastError());
+        stage_end(ctx, ST_INTERP);
+    }
+    if (want_src) {
+        stage_begin(ctx, ST_SPREAD);
+        hipLaunchKernelGGL(k_gather_grid_src<0>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st,
+                           g.ptar, du, nullptr, tbins.perm, (int)nt, 1, 1, (int64_t)0, g.Lx, g.Ly,
+                           L.h, L.P, L.Mx, L.My, L.c, g.w, anchor, fgg, q);
+        sas.s.anchor = anchor;
+        sas.s.fgg = fgg;
+        sas.s.q = q;
+        sas.s.ns = nt;
+        if (int rc = spread(ctx, L, gs, sas)) return rc;
+        stage_end(ctx, ST_SPREAD);
+        if (int rc = transform_filter(ctx, L, g, ps, kind == 0 ? FILTER_SLP : FILTER_DLP_ADJ, 1, grids, spec))
+            return rc;
+        stage_begin(ctx, ST_INTERP);
+        InterpAdjArgs a = ia;
+        a.pts = g.psrc;
+        a.perm = kss.perm;
+        a.boff = kss.off;
+        a.chunk_off = gs.ioffc;
+        a.block_tile = gs.imap;
+        a.n = ns;
+        a.f = f;
+        a.nv = nv;
+        a.dpos = dpsrc;
+        a.df = df;
+        a.dn = dn;
+        if (kind == 0) launch_interp_adj<2, ADJ_SOURCE_SLP>(st, L, gs, tile_s, a);
+        else launch_interp_adj<3, ADJ_SOURCE_DLP>(st, L, gs, tile_s, a);
+        SE2P_HIP(hipGetLastError());
+        stage_end(ctx, ST_INTERP);
+    }
     return 0;
 }
 

@@ -287,6 +287,256 @@ __global__ __launch_bounds__(64) void k_real_count(RealArgs a)
     atomicAdd(&a.pair_counts[1], n_out);
 }
 
+// ---- adjoint ---------------------------------------------------------------
+// With u the output and L = sum_t du_t . u_t, a pair (target t, source s, image
+// offset z) contributes phi(d) / (4 pi) to L, d = x_s - x_t + z, phi = du_t . K(d) q_s.
+// Then dL/dx_s += grad phi / (4 pi), dL/dx_t -= grad phi / (4 pi), and the
+// density gradient is the transposed pair sum.  The cell and cutoff
+// decisions are those of the forward sum (taken as fixed).
+//
+// Stokeslet: phi = a . G(d) b, a = du_t, b = f_s, G = E1/2 I + e (d d^T/r^2 - I),
+//   grad phi = e { [-(s + 2 p q/r^2)/r^2 - 2 xi^2 (p q/r^2 - s)] d + (q a + p b)/r^2 }
+//   with p = d.a, q = d.b, s = a.b, e = exp(-xi^2 r^2); the E1 term gives
+//   -e d/r^2 (dE1/dx = -e^{-x}/x).
+__device__ __forceinline__ void slp_grad(double d1, double d2, double xi2, double e, double ir2,
+                                         double a1, double a2, double b1, double b2, double &g1,
+                                         double &g2)
+{
+    const double p = d1 * a1 + d2 * a2, q = d1 * b1 + d2 * b2, s = a1 * b1 + a2 * b2;
+    const double pq = p * q * ir2;
+    const double ei = e * ir2;
+    const double cd = -ei * (s + 2.0 * pq) - 2.0 * xi2 * e * (pq - s);
+    g1 += cd * d1 + ei * (q * a1 + p * b1);
+    g2 += cd * d2 + ei * (q * a2 + p * b2);
+}
+
+// Stresslet: phi = e2 [A p c + B (2 a.S d + p tr S)] with S the symmetric
+// density matrix [[q1, q2/2], [q2/2, q3]], A = -4 (1 + x)/r^4, B = 2 xi^2,
+// x = xi^2 r^2, p = d.a, c = d.S d;
+//   grad phi = [-2 xi^2 phi + e2 A' p c] d + e2 [A (c a + 2 p S d) + B (2 S a + tr S a)],
+//   A' d = grad A, A' = 8 (2 + x)/r^6.
+__device__ __forceinline__ void dlp_grad(double d1, double d2, double r2, double xi2, double e2,
+                                         double ir2, double a1, double a2, double q1, double q2,
+                                         double q3, double &g1, double &g2)
+{
+    const double x = xi2 * r2;
+    const double ir4 = ir2 * ir2;
+    const double A = -4.0 * (1.0 + x) * ir4;
+    const double Ap = 8.0 * (2.0 + x) * ir4 * ir2;
+    const double B = 2.0 * xi2;
+    const double hq = 0.5 * q2;
+    const double sd1 = q1 * d1 + hq * d2, sd2 = hq * d1 + q3 * d2;
+    const double sa1 = q1 * a1 + hq * a2, sa2 = hq * a1 + q3 * a2;
+    const double p = d1 * a1 + d2 * a2;
+    const double c = d1 * sd1 + d2 * sd2;
+    const double tr = q1 + q3;
+    const double phi = e2 * (A * p * c + B * (2.0 * (a1 * sd1 + a2 * sd2) + p * tr));
+    const double cd = -2.0 * xi2 * phi + e2 * Ap * p * c;
+    g1 += cd * d1 + e2 * (A * (c * a1 + 2.0 * p * sd1) + B * (2.0 * sa1 + tr * a1));
+    g2 += cd * d2 + e2 * (A * (c * a2 + 2.0 * p * sd2) + B * (2.0 * sa2 + tr * a2));
+}
+
+// Stresslet density gradient: m_j += sum_i a_i T_i(j), for the three merged
+// densities (S11, S12 + S21, S22).
+__device__ __forceinline__ void dlp_dens(double d1, double d2, double r2, double xi2, double e2,
+                                         double ir2, double a1, double a2, double &m1, double &m2,
+                                         double &m3)
+{
+    const double fb = -4.0 * (1.0 + xi2 * r2) * ir2 * ir2;
+    const double pre = 2.0 * xi2;
+    const double d11 = d1 * d1, d12 = d1 * d2, d22 = d2 * d2;
+    const double t111 = e2 * (d11 * d1 * fb + pre * 3.0 * d1);
+    const double t112 = e2 * (d11 * d2 * fb + pre * d2);
+    const double t122 = e2 * (d12 * d2 * fb + pre * d1);
+    const double t222 = e2 * (d22 * d2 * fb + pre * 3.0 * d2);
+    m1 += a1 * t111 + a2 * t112;
+    m2 += a1 * t112 + a2 * t122;
+    m3 += a1 * t122 + a2 * t222;
+}
+
+struct RealAdjArgs {
+    const double2 *lpos;   // lane points in (cell, sub-cell) order
+    const int *lperm;      // that order -> caller's index
+    const int *loff;       // [ncell*16+1] run offsets of the lane points
+    const int *chunk_off;  // [ncell+1] exclusive scan of ceil(lane points/64)
+    const int *block_cell; // workgroup -> cell
+    const double *ldata;   // lane data in lane order, [ncomp][nl]
+    int64_t nl, nu;        // lane / uniform point counts (data strides)
+    int nsx, nsy, ncell;
+    double Lx, Ly, xi, xi2, cut2, self;
+    const double *f, *nv;  // caller-order densities and normals (stresslet sources)
+    double *dpos, *df, *dn;   // caller order; may be null
+};
+
+// SIDE 0: lanes are targets (lane data du), uniform points are sources
+//   (uniform data: the densities); writes dL/dptar.
+// SIDE 1: lanes are sources (lane data: the densities), uniform points are
+//   targets (uniform data du); writes dL/dpsrc, dL/df, dL/dn.  The image offset
+//   from a source cell to a target cell is the negated offset of the forward
+//   visit, so d = x_s - (x_t + z') (SLP) or (x_s - x_t) - z' (DLP) repeats the
+//   forward's rounding bit for bit and every cutoff test agrees.
+template <int KIND, int SIDE>
+__global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *__restrict__ upos,
+                                                  const double *__restrict__ udata,
+                                                  const int *__restrict__ uoff)
+{
+    constexpr int NV = SIDE == 0 ? 2 : (KIND == 0 ? 4 : 5);
+    __shared__ double e1tab[KIND == 0 && SIDE == 1 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
+    __shared__ double red[kWaves - 1][NV][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (KIND == 0 && SIDE == 1)
+        for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
+    __syncthreads();
+
+    const int b = blockIdx.x;
+    if (b >= a.chunk_off[a.ncell]) return;   // uniform over the workgroup
+    const int cell = a.block_cell[b];
+    const int sub = 1 << (2 * kSubLog);
+    const int l0 = a.loff[cell * sub] + 64 * (b - a.chunk_off[cell]);
+    const int l1 = a.loff[(cell + 1) * sub];
+    const int l = l0 + lane;
+    const bool active = l < l1;
+    double lx, ly;
+    double ld[3] = {0.0, 0.0, 0.0};
+    constexpr int LC = SIDE == 0 ? 2 : (KIND == 0 ? 2 : 3);
+    if (active) {
+        const double2 p = a.lpos[l];
+        lx = p.x;
+        ly = p.y;
+#pragma unroll
+        for (int c = 0; c < LC; ++c) ld[c] = a.ldata[c * a.nl + l];
+    } else {
+        lx = ly = __builtin_nan("");   // inactive lanes compute garbage, never stored
+    }
+    const int cx = cell % a.nsx, cy = cell / a.nsx;
+    const int64_t us = a.nu;
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+
+    // one pair: d with the forward's rounding, then the side's contributions
+    auto pair = [&](double ux, double uy, double zx, double zy, int u, bool own) {
+        double d1, d2;
+        if (SIDE == 0) {
+            diff<KIND>(ux, uy, lx, ly, zx, zy, d1, d2);
+        } else if (KIND == 0) {
+            d1 = lx - (ux + zx);
+            d2 = ly - (uy + zy);
+        } else {
+            d1 = (lx - ux) - zx;
+            d2 = (ly - uy) - zy;
+        }
+        const double r2 = se2p_sqsum(d1, d2);
+        if (own ? (r2 == 0.0) : !(r2 < a.cut2)) {
+            if (KIND == 0 && SIDE == 1 && own) {   // coincident: self term, no gradient
+                acc[2] += a.self * udata[u];
+                acc[3] += a.self * udata[us + u];
+            }
+            return;
+        }
+        const double x = r2 * a.xi2;
+        const double e = exp(-x);
+        const double ir2 = rcp_nr(r2);
+        if (SIDE == 0) {
+            // lane: du_t; uniform: the source densities
+            if (KIND == 0)
+                slp_grad(d1, d2, a.xi2, e, ir2, ld[0], ld[1], udata[u], udata[us + u], acc[0], acc[1]);
+            else
+                dlp_grad(d1, d2, r2, a.xi2, e, ir2, ld[0], ld[1], udata[u], udata[us + u],
+                         udata[2 * us + u], acc[0], acc[1]);
+        } else {
+            // lane: the source densities; uniform: du_t
+            const double a1 = udata[u], a2 = udata[us + u];
+            if (KIND == 0) {
+                slp_grad(d1, d2, a.xi2, e, ir2, a1, a2, ld[0], ld[1], acc[0], acc[1]);
+                // G(d) du_t, arranged as the forward pair
+                const double E = se2p_e1(x, e, e1tab);
+                const double hE = 0.5 * E, g = e * ir2;
+                const double c = d1 * a2 - d2 * a1;
+                acc[2] += hE * a1 + g * d2 * c;
+                acc[3] += hE * a2 - g * d1 * c;
+            } else {
+                dlp_grad(d1, d2, r2, a.xi2, e, ir2, a1, a2, ld[0], ld[1], ld[2], acc[0], acc[1]);
+                dlp_dens(d1, d2, r2, a.xi2, e, ir2, a1, a2, acc[2], acc[3], acc[4]);
+            }
+        }
+    };
+
+    {   // own cell: every pair, no cutoff
+        const int u1 = uoff[(cell + 1) * sub];
+#pragma unroll 2
+        for (int u = uoff[cell * sub] + wave; u < u1; u += kWaves) {
+            const double2 up = upos[u];
+            pair(up.x, up.y, 0.0, 0.0, u, true);
+        }
+    }
+#pragma unroll 1
+    for (int j = 0; j < 8; ++j) {
+        const int qx = cx + nb_dx(j), qy = cy + nb_dy(j);
+        const int wx = (qx + a.nsx) % a.nsx, wy = (qy + a.nsy) % a.nsy;
+        const int ub = wy * a.nsx + wx;
+        const int u1 = uoff[(ub + 1) * sub];
+        const double zx = (a.Lx * (qx - wx)) / a.nsx;
+        const double zy = (a.Ly * (qy - wy)) / a.nsy;
+#pragma unroll 2
+        for (int u = uoff[ub * sub] + wave; u < u1; u += kWaves) {
+            const double2 up = upos[u];
+            pair(up.x, up.y, zx, zy, u, false);
+        }
+    }
+    if (wave > 0)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) red[wave - 1][v][lane] = acc[v];
+    __syncthreads();
+    if (wave != 0 || !active) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int w = 0; w < kWaves - 1; ++w) acc[v] += red[w][v][lane];
+    const int o = a.lperm[l];
+    const double s4 = 1.0 / (4 * SE2P_PI);
+    if (SIDE == 0) {
+        a.dpos[2 * o] = -acc[0] * s4;
+        a.dpos[2 * o + 1] = -acc[1] * s4;
+        return;
+    }
+    if (a.dpos) {
+        a.dpos[2 * o] = acc[0] * s4;
+        a.dpos[2 * o + 1] = acc[1] * s4;
+    }
+    if (KIND == 0) {
+        if (a.df) {
+            a.df[2 * o] = acc[2] * s4;
+            a.df[2 * o + 1] = acc[3] * s4;
+        }
+    } else {
+        // m = dL/d(S11, S12 + S21, S22), S = f (x) n
+        const double m1 = acc[2] * s4, m2 = acc[3] * s4, m3 = acc[4] * s4;
+        const double f1 = a.f[2 * o], f2 = a.f[2 * o + 1];
+        const double n1 = a.nv[2 * o], n2 = a.nv[2 * o + 1];
+        if (a.df) {
+            a.df[2 * o] = m1 * n1 + m2 * n2;
+            a.df[2 * o + 1] = m2 * n1 + m3 * n2;
+        }
+        if (a.dn) {
+            a.dn[2 * o] = m1 * f1 + m2 * f2;
+            a.dn[2 * o + 1] = m2 * f1 + m3 * f2;
+        }
+    }
+}
+
+// du in target (cell) order, [2][n]
+__global__ void k_gather_pairs(const double *__restrict__ v, const int *__restrict__ perm, int n,
+                               double *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int o = perm[i];
+    out[i] = v[2 * o];
+    out[n + i] = v[2 * o + 1];
+}
+
 // ---- gathers ------------------------------------------------------------
 __global__ void k_gather_targets(const double *__restrict__ p, const int *__restrict__ perm, int n,
                                  double2 *__restrict__ out)
@@ -480,6 +730,140 @@ int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const 
         pair_counts[0] = (int64_t)h[0];
         pair_counts[1] = (int64_t)h[1];
     }
+    return 0;
+}
+
+int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
+                 const double *du, double *dpsrc, double *dptar, double *df, double *dn)
+{
+    hipStream_t st = ctx->stream;
+    const int ncell = g.nsx * g.nsy;
+    const int sub = 1 << (2 * kSubLog);
+    const int nkeys = ncell * sub;
+    const int64_t ns = g.ns, nt = g.nt;
+    const bool want_src = dpsrc || df || dn;
+    // no partners: every gradient is zero
+    if (ns == 0 || nt == 0) {
+        launch_zero(st, dptar, 2 * nt);
+        launch_zero(st, dpsrc, 2 * ns);
+        launch_zero(st, df, 2 * ns);
+        launch_zero(st, dn, 2 * ns);
+        SE2P_HIP(hipGetLastError());
+        return 0;
+    }
+    if (kind == 0 && want_src && upload_table(ctx)) return SE2P_ERR_HIP;
+    const bool same_points = (g.ptar == g.psrc) && (nt == ns);
+    const int ncomp = kind == 0 ? 2 : 3;
+    const int64_t max_t = (nt + 63) / 64 + ncell, max_s = (ns + 63) / 64 + ncell;
+
+    KeySort kss, kst;
+    double2 *spos, *tpos;
+    double *q, *dut;
+    int *ccnt, *coff_t, *coff_s, *bmap_t, *bmap_s;
+    auto layout = [&](Carve &cv) {
+        char *p = cv.take<char>(keysort_bytes(ns, nkeys));
+        if (p) keysort_carve(p, ns, nkeys, &kss);
+        if (!same_points) {
+            p = cv.take<char>(keysort_bytes(nt, nkeys));
+            if (p) keysort_carve(p, nt, nkeys, &kst);
+        }
+        spos = cv.take<double2>(ns);
+        tpos = same_points ? spos : cv.take<double2>(nt);
+        q = cv.take<double>((size_t)ncomp * ns);
+        dut = cv.take<double>(2 * (size_t)nt);
+        ccnt = cv.take<int>(ncell + 1);
+        coff_t = cv.take<int>(ncell + 1);
+        coff_s = cv.take<int>(ncell + 1);
+        bmap_t = cv.take<int>(max_t);
+        bmap_s = cv.take<int>(max_s);
+    };
+    Carve sizing;
+    layout(sizing);
+    if (int rc = ensure_workspace(ctx, sizing.off)) return rc;
+    Carve cv;
+    cv.base = ctx->ws;
+    layout(cv);
+
+    stage_begin(ctx, ST_SORT);
+    if (int rc = cell_sort(ctx, g.psrc, ns, g.Lx, g.Ly, g.nsx, g.nsy, kSubLog, &kss)) return rc;
+    if (!same_points)
+        if (int rc = cell_sort(ctx, g.ptar, nt, g.Lx, g.Ly, g.nsx, g.nsy, kSubLog, &kst)) return rc;
+    const KeySort &tks = same_points ? kss : kst;
+    const int threads = 256;
+    const unsigned sblocks = (unsigned)((ns + threads - 1) / threads);
+    const unsigned tblocks = (unsigned)((nt + threads - 1) / threads);
+    if (kind == 0)
+        hipLaunchKernelGGL(k_gather_sources<0>, dim3(sblocks), dim3(threads), 0, st, g.psrc, f, n,
+                           kss.perm, (int)ns, 1, (int64_t)0, spos, q);
+    else
+        hipLaunchKernelGGL(k_gather_sources<1>, dim3(sblocks), dim3(threads), 0, st, g.psrc, f, n,
+                           kss.perm, (int)ns, 1, (int64_t)0, spos, q);
+    if (!same_points)
+        hipLaunchKernelGGL(k_gather_targets, dim3(tblocks), dim3(threads), 0, st, g.ptar, tks.perm,
+                           (int)nt, tpos);
+    hipLaunchKernelGGL(k_gather_pairs, dim3(tblocks), dim3(threads), 0, st, du, tks.perm, (int)nt, dut);
+    const unsigned cblocks = (unsigned)((ncell + threads - 1) / threads);
+    if (dptar) {
+        hipLaunchKernelGGL(k_chunk_counts, dim3(cblocks), dim3(threads), 0, st, tks.off, ncell, sub, ccnt);
+        scan_ints(st, ccnt, ncell, coff_t);
+        expand_map(st, coff_t, ncell, bmap_t);
+    }
+    if (want_src) {
+        hipLaunchKernelGGL(k_chunk_counts, dim3(cblocks), dim3(threads), 0, st, kss.off, ncell, sub, ccnt);
+        scan_ints(st, ccnt, ncell, coff_s);
+        expand_map(st, coff_s, ncell, bmap_s);
+    }
+    stage_end(ctx, ST_SORT);
+
+    RealAdjArgs a{};
+    a.nsx = g.nsx;
+    a.nsy = g.nsy;
+    a.ncell = ncell;
+    a.Lx = g.Lx;
+    a.Ly = g.Ly;
+    a.xi = g.xi;
+    a.xi2 = g.xi * g.xi;
+    a.cut2 = g.Lx * g.Ly / g.nsx / g.nsy;   // mex_stokes_slp_real.cpp:127
+    a.self = -(0.5 * SE2P_EULER_GAMMA + 1.0) - log(g.xi);
+    a.f = f;
+    a.nv = n;
+    stage_begin(ctx, ST_REAL);
+    if (dptar) {
+        RealAdjArgs t = a;
+        t.lpos = tpos;
+        t.lperm = tks.perm;
+        t.loff = tks.off;
+        t.chunk_off = coff_t;
+        t.block_cell = bmap_t;
+        t.ldata = dut;
+        t.nl = nt;
+        t.nu = ns;
+        t.dpos = dptar;
+        if (kind == 0)
+            hipLaunchKernelGGL((k_real_adj<0, 0>), dim3((unsigned)max_t), dim3(256), 0, st, t, spos, q, kss.off);
+        else
+            hipLaunchKernelGGL((k_real_adj<1, 0>), dim3((unsigned)max_t), dim3(256), 0, st, t, spos, q, kss.off);
+    }
+    if (want_src) {
+        RealAdjArgs s = a;
+        s.lpos = spos;
+        s.lperm = kss.perm;
+        s.loff = kss.off;
+        s.chunk_off = coff_s;
+        s.block_cell = bmap_s;
+        s.ldata = q;
+        s.nl = ns;
+        s.nu = nt;
+        s.dpos = dpsrc;
+        s.df = df;
+        s.dn = dn;
+        if (kind == 0)
+            hipLaunchKernelGGL((k_real_adj<0, 1>), dim3((unsigned)max_s), dim3(256), 0, st, s, tpos, dut, tks.off);
+        else
+            hipLaunchKernelGGL((k_real_adj<1, 1>), dim3((unsigned)max_s), dim3(256), 0, st, s, tpos, dut, tks.off);
+    }
+    stage_end(ctx, ST_REAL);
+    SE2P_HIP(hipGetLastError());
     return 0;
 }
 

@@ -288,6 +288,96 @@ def stokes_dlp_kspace_batched(psrc, ptar, xi, eta, F, n, Mx, My, Lx, Ly, w, P, *
     return u
 
 
+# -- adjoints (backward entry points) --------------------------------------------
+_WRT_ALL = ("psrc", "ptar", "f", "n")
+
+
+def _grad_outputs(ns, nt, dev, wrt, dlp):
+    out = {}
+    for name, rows in (("psrc", ns), ("ptar", nt), ("f", ns), ("n", ns)):
+        if name == "n" and not dlp:
+            continue
+        if name in wrt:
+            out[name] = torch.empty((rows, 2), dtype=torch.float64, device=dev)
+    return out
+
+
+def _optr(out, name):
+    t = out.get(name)
+    return _ptr(t) if t is not None else ctypes.c_void_p(0)
+
+
+def _adjoint_inputs(psrc, ptar, f, n, du, dlp):
+    dev = _device_of(psrc, ptar, f, n, du)
+    ps, pt = _as_points(psrc, "psrc", dev), _as_points(ptar, "ptar", dev)
+    ff = _as_points(f, "f", dev)
+    if ff.shape[0] != ps.shape[0]:
+        raise ValueError("psrc and f must be the same size.")
+    nn = None
+    if dlp:
+        nn = _as_points(n, "n", dev)
+        if nn.shape[0] != ps.shape[0]:
+            raise ValueError("psrc and n must be the same size.")
+    dd = _as_points(du, "du", dev)
+    if dd.shape[0] != pt.shape[0]:
+        raise ValueError("ptar and du must be the same size.")
+    return dev, ps, pt, ff, nn, dd
+
+
+def mex_stokes_slp_real_backward(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly, du, *,
+                                 wrt=_WRT_ALL, ctx=None):
+    """Gradients of sum(du * mex_stokes_slp_real(...)): dict with keys among
+    'psrc', 'ptar', 'f' (those named in wrt)."""
+    dev, ps, pt, ff, _, dd = _adjoint_inputs(psrc, ptar, f, None, du, False)
+    ctx = _prepare(ctx, dev)
+    g = _grad_outputs(ps.shape[0], pt.shape[0], dev, wrt, False)
+    ctx._check(ctx._lib.se2p_slp_real_backward(
+        ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0], _ptr(ff), float(xi),
+        int(nside_x), int(nside_y), float(Lx), float(Ly), _ptr(dd), _optr(g, "psrc"),
+        _optr(g, "ptar"), _optr(g, "f")))
+    return g
+
+
+def mex_stokes_slp_kspace_backward(psrc, ptar, xi, eta, f, Mx, My, Lx, Ly, w, P, du, *,
+                                   wrt=_WRT_ALL, ctx=None):
+    """Gradients of sum(du * mex_stokes_slp_kspace(...)) (dict, as above)."""
+    dev, ps, pt, ff, _, dd = _adjoint_inputs(psrc, ptar, f, None, du, False)
+    ctx = _prepare(ctx, dev)
+    g = _grad_outputs(ps.shape[0], pt.shape[0], dev, wrt, False)
+    ctx._check(ctx._lib.se2p_slp_kspace_backward(
+        ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0], float(xi), float(eta),
+        _ptr(ff), int(Mx), int(My), float(Lx), float(Ly), float(w), int(P), _ptr(dd),
+        _optr(g, "psrc"), _optr(g, "ptar"), _optr(g, "f")))
+    return g
+
+
+def mex_stokes_dlp_real_backward(psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly, du, *,
+                                 wrt=_WRT_ALL, ctx=None):
+    """Gradients of sum(du * mex_stokes_dlp_real(...)): keys among 'psrc',
+    'ptar', 'f', 'n'."""
+    dev, ps, pt, ff, nn, dd = _adjoint_inputs(psrc, ptar, f, n, du, True)
+    ctx = _prepare(ctx, dev)
+    g = _grad_outputs(ps.shape[0], pt.shape[0], dev, wrt, True)
+    ctx._check(ctx._lib.se2p_dlp_real_backward(
+        ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0], _ptr(ff), _ptr(nn), float(xi),
+        int(nside_x), int(nside_y), float(Lx), float(Ly), _ptr(dd), _optr(g, "psrc"),
+        _optr(g, "ptar"), _optr(g, "f"), _optr(g, "n")))
+    return g
+
+
+def mex_stokes_dlp_kspace_backward(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w, P, du, *,
+                                   wrt=_WRT_ALL, ctx=None):
+    """Gradients of sum(du * mex_stokes_dlp_kspace(...)) (dict, as above)."""
+    dev, ps, pt, ff, nn, dd = _adjoint_inputs(psrc, ptar, f, n, du, True)
+    ctx = _prepare(ctx, dev)
+    g = _grad_outputs(ps.shape[0], pt.shape[0], dev, wrt, True)
+    ctx._check(ctx._lib.se2p_dlp_kspace_backward(
+        ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0], float(xi), float(eta),
+        _ptr(ff), _ptr(nn), int(Mx), int(My), float(Lx), float(Ly), float(w), int(P), _ptr(dd),
+        _optr(g, "psrc"), _optr(g, "ptar"), _optr(g, "f"), _optr(g, "n")))
+    return g
+
+
 # -- the reference drivers as single calls --------------------------------------
 def _driver(kind, psrc, ptar, f, n, Lx, Ly, P, Nb, tol, M, ctx, same):
     dev = _device_of(psrc, ptar, f, n)

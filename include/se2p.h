@@ -115,6 +115,37 @@ int se2p_dlp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const
                             int nrhs, int64_t f_stride, int Mx, int My, double Lx, double Ly,
                             double w, int P, double *u, int64_t u_stride);
 
+/* ---- adjoints (backward entry points) ------------------------------------ */
+/* Gradients of L = sum_t du_t . u_t, u the output of the matching forward
+ * entry point on the same arguments (one density): dL/dpsrc and dL/df (Ns x 2),
+ * dL/dptar (Nt x 2) and, for the stresslet, dL/dn (Ns x 2).  du is Nt x 2.
+ * Any output pointer may be NULL; that gradient is then not computed.
+ * Treated as fixed (piecewise constant in the positions): the cell of every
+ * point, the cutoff decision of every pair and the grid anchor of every point.
+ * A coincident pair (r^2 == 0) enters the forward sum as a constant (the
+ * Stokeslet self term) or not at all (stresslet) and contributes no position
+ * gradient; dL/dpsrc + dL/dptar is then the gradient for sources and targets
+ * moved together (targets == sources).  The reference has no adjoint: these
+ * are the transposes of mex_stokes_slp_real.cpp, mex_stokes_slp_kspace.cpp,
+ * mex_stokes_dlp_real.cpp and mex_stokes_dlp_kspace.cpp as restated above. */
+int se2p_slp_real_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                           int64_t nt, const double *f, double xi, int nside_x, int nside_y,
+                           double Lx, double Ly, const double *du, double *dpsrc, double *dptar,
+                           double *df);
+int se2p_slp_kspace_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                             int64_t nt, double xi, double eta, const double *f, int Mx, int My,
+                             double Lx, double Ly, double w, int P, const double *du,
+                             double *dpsrc, double *dptar, double *df);
+int se2p_dlp_real_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                           int64_t nt, const double *f, const double *n, double xi, int nside_x,
+                           int nside_y, double Lx, double Ly, const double *du, double *dpsrc,
+                           double *dptar, double *df, double *dn);
+int se2p_dlp_kspace_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                             int64_t nt, double xi, double eta, const double *f, const double *n,
+                             int Mx, int My, double Lx, double Ly, double w, int P,
+                             const double *du, double *dpsrc, double *dptar, double *df,
+                             double *dn);
+
 /* ---- the reference drivers as single calls --------------------------------- */
 /* Parameters of the spectral Ewald split.  Inputs left at 0 take the
  * reference driver's defaults: P = 24, Nb = min(P*round(log2(Ns+Nt)), (Ns+Nt)/4),

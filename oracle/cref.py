@@ -48,6 +48,17 @@ def lib():
         L.or_slp_kspace.argtypes = [_P, _I64, _P, _I64, _D, _D, _P, _I, _I, _D, _D, _D, _I, _P]
         L.or_dlp_kspace.restype = _I
         L.or_dlp_kspace.argtypes = [_P, _I64, _P, _I64, _D, _D, _P, _P, _I, _I, _D, _D, _D, _I, _P]
+        L.or_slp_real_adj.restype = _I
+        L.or_slp_real_adj.argtypes = [_P, _I64, _P, _I64, _P, _D, _I, _I, _D, _D, _P, _P, _P, _P]
+        L.or_dlp_real_adj.restype = _I
+        L.or_dlp_real_adj.argtypes = [_P, _I64, _P, _I64, _P, _P, _D, _I, _I, _D, _D,
+                                      _P, _P, _P, _P, _P]
+        L.or_slp_kspace_adj.restype = _I
+        L.or_slp_kspace_adj.argtypes = [_P, _I64, _P, _I64, _D, _D, _P, _I, _I, _D, _D, _D, _I,
+                                        _P, _P, _P, _P]
+        L.or_dlp_kspace_adj.restype = _I
+        L.or_dlp_kspace_adj.argtypes = [_P, _I64, _P, _I64, _D, _D, _P, _P, _I, _I, _D, _D, _D,
+                                        _I, _P, _P, _P, _P, _P]
         L.or_fft2.restype = None
         L.or_fft2.argtypes = [_P, _P, _I, _I, _I]
         L.or_set_threads.restype = None
@@ -127,6 +138,54 @@ def dlp_kspace(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w, P):
                         _ptr(f), _ptr(n), int(Mx), int(My), float(Lx), float(Ly), float(w),
                         int(P), _ptr(u))
     return u
+
+
+def _grads(ns, nt, dlp):
+    out = {"dpsrc": np.zeros((ns, 2)), "dptar": np.zeros((nt, 2)), "df": np.zeros((ns, 2))}
+    if dlp:
+        out["dn"] = np.zeros((ns, 2))
+    return out
+
+
+def slp_real_adj(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly, du):
+    """Gradients of L = sum du . slp_real(...) w.r.t. psrc, ptar and f (dict)."""
+    psrc, ptar, f, du = _pts(psrc), _pts(ptar), _pts(f), _pts(du)
+    o = _grads(len(psrc), len(ptar), False)
+    lib().or_slp_real_adj(_ptr(psrc), len(psrc), _ptr(ptar), len(ptar), _ptr(f), float(xi),
+                          int(nside_x), int(nside_y), float(Lx), float(Ly), _ptr(du),
+                          _ptr(o["dpsrc"]), _ptr(o["dptar"]), _ptr(o["df"]))
+    return o
+
+
+def dlp_real_adj(psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly, du):
+    """Gradients of L = sum du . dlp_real(...) w.r.t. psrc, ptar, f and n (dict)."""
+    psrc, ptar, f, n, du = _pts(psrc), _pts(ptar), _pts(f), _pts(n), _pts(du)
+    o = _grads(len(psrc), len(ptar), True)
+    lib().or_dlp_real_adj(_ptr(psrc), len(psrc), _ptr(ptar), len(ptar), _ptr(f), _ptr(n),
+                          float(xi), int(nside_x), int(nside_y), float(Lx), float(Ly), _ptr(du),
+                          _ptr(o["dpsrc"]), _ptr(o["dptar"]), _ptr(o["df"]), _ptr(o["dn"]))
+    return o
+
+
+def slp_kspace_adj(psrc, ptar, xi, eta, f, Mx, My, Lx, Ly, w, P, du):
+    """Gradients of L = sum du . slp_kspace(...) w.r.t. psrc, ptar and f (dict)."""
+    psrc, ptar, f, du = _pts(psrc), _pts(ptar), _pts(f), _pts(du)
+    o = _grads(len(psrc), len(ptar), False)
+    lib().or_slp_kspace_adj(_ptr(psrc), len(psrc), _ptr(ptar), len(ptar), float(xi), float(eta),
+                            _ptr(f), int(Mx), int(My), float(Lx), float(Ly), float(w), int(P),
+                            _ptr(du), _ptr(o["dpsrc"]), _ptr(o["dptar"]), _ptr(o["df"]))
+    return o
+
+
+def dlp_kspace_adj(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w, P, du):
+    """Gradients of L = sum du . dlp_kspace(...) w.r.t. psrc, ptar, f and n (dict)."""
+    psrc, ptar, f, n, du = _pts(psrc), _pts(ptar), _pts(f), _pts(n), _pts(du)
+    o = _grads(len(psrc), len(ptar), True)
+    lib().or_dlp_kspace_adj(_ptr(psrc), len(psrc), _ptr(ptar), len(ptar), float(xi), float(eta),
+                            _ptr(f), _ptr(n), int(Mx), int(My), float(Lx), float(Ly), float(w),
+                            int(P), _ptr(du), _ptr(o["dpsrc"]), _ptr(o["dptar"]), _ptr(o["df"]),
+                            _ptr(o["dn"]))
+    return o
 
 
 def fft2(z: np.ndarray, inverse: bool = False) -> np.ndarray:

@@ -378,8 +378,8 @@ static void or_fft_any(double *re, double *im, int n, int sign)
     double *ai = (double *)calloc((size_t)m, sizeof(double));
     double *br = (double *)calloc((size_t)m, sizeof(double));
     double *bi = (double *)calloc((size_t)m, sizeof(double));
-    double *cr = (double *)malloc(sizeof(double) * (size_t)n);
-    double *ci = (double *)malloc(sizeof(double) * (size_t)n);
+    double *cr = (double *)calloc((size_t)n, sizeof(double));
+    double *ci = (double *)calloc((size_t)n, sizeof(double));
     for (int k = 0; k < n; ++k) {
         /* chirp w_k = e^{sign i pi k^2 / n}; k^2 mod 2n keeps the angle small */
         long long k2 = ((long long)k * k) % (2LL * n);
@@ -561,19 +561,19 @@ static double *or_e1_table(double xi, double eta, double h, int P)
     return e1;
 }
 
-int or_slp_kspace(const double *psrc, int64_t ns, const double *ptar, int64_t nt,
-                  double xi, double eta, const double *f, int Mx, int My,
-                  double Lx, double Ly, double w, int P, double *u)
+/* Filtered Stokeslet grids: spread f at the points, forward DFT, Stokeslet
+ * filter, k = 0 mode zeroed, inverse DFT; G1, G2 (Mx*My each, caller-owned,
+ * zeroed on entry) receive the real parts (mex_stokes_slp_kspace.cpp:59-240). */
+static void or_slp_kspace_grids(const double *p, int64_t n, double xi, double eta,
+                                const double *f, int Mx, int My, double Lx, double Ly,
+                                double w, int P, const double *e1, double *G1, double *G2)
 {
     size_t G = (size_t)Mx * My;
-    double h = Lx / Mx;
-    double *e1 = or_e1_table(xi, eta, h, P);
-    double *H1r = (double *)calloc(G, sizeof(double));
-    double *H2r = (double *)calloc(G, sizeof(double));
     double *H1i = (double *)calloc(G, sizeof(double));
     double *H2i = (double *)calloc(G, sizeof(double));
+    double *H1r = G1, *H2r = G2;
     double *H[2] = {H1r, H2r};
-    or_spread(psrc, ns, f, 2, xi, eta, Mx, My, Lx, Ly, w, P, e1, H);
+    or_spread(p, n, f, 2, xi, eta, Mx, My, Lx, Ly, w, P, e1, H);
     or_fft2(H1r, H1i, Mx, My, -1);
     or_fft2(H2r, H2i, Mx, My, -1);
     /* Stokeslet filter (mex_stokes_slp_kspace.cpp:189-223) */
@@ -597,10 +597,97 @@ int or_slp_kspace(const double *psrc, int64_t ns, const double *ptar, int64_t nt
     H1r[0] = H2r[0] = H1i[0] = H2i[0] = 0.0;   /* :226-229 */
     or_fft2(H1r, H1i, Mx, My, +1);
     or_fft2(H2r, H2i, Mx, My, +1);
+    free(H1i);
+    free(H2i);
+}
+
+int or_slp_kspace(const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                  double xi, double eta, const double *f, int Mx, int My,
+                  double Lx, double Ly, double w, int P, double *u)
+{
+    size_t G = (size_t)Mx * My;
+    double h = Lx / Mx;
+    double *e1 = or_e1_table(xi, eta, h, P);
+    double *G1 = (double *)calloc(G, sizeof(double));
+    double *G2 = (double *)calloc(G, sizeof(double));
+    or_slp_kspace_grids(psrc, ns, xi, eta, f, Mx, My, Lx, Ly, w, P, e1, G1, G2);
     /* real part (:240) */
-    or_interp(ptar, nt, xi, eta, Mx, My, Lx, Ly, w, P, e1, H1r, H2r, u);
-    free(e1); free(H1r); free(H2r); free(H1i); free(H2i);
+    or_interp(ptar, nt, xi, eta, Mx, My, Lx, Ly, w, P, e1, G1, G2, u);
+    free(e1); free(G1); free(G2);
     return 0;
+}
+
+/* Stresslet filter coefficients at wavenumber (k1, k2) (mex_stokes_dlp_kspace.cpp:
+ * 232-247): et = e*t and b111, b112, b122, b222. */
+static void or_dlp_filter_coef(double k1, double k2, double xi, double eta, double *et,
+                               double *b111, double *b112, double *b122, double *b222)
+{
+    double K2 = k1 * k1 + k2 * k2;
+    double e = exp(-0.25 * (1 - eta) / (xi * xi) * K2) / K2 / K2;
+    double t = 1 + 0.25 * K2 / xi / xi;
+    *et = e * t;
+    *b111 = 3.0 * k1 * K2 - 2.0 * k1 * k1 * k1;
+    *b112 = k2 * K2 - 2.0 * k1 * k1 * k2;
+    *b122 = k1 * K2 - 2.0 * k1 * k2 * k2;
+    *b222 = 3.0 * k2 * K2 - 2.0 * k2 * k2 * k2;
+}
+
+/* The four stresslet densities n_l f_j in the reference's grid order
+ * (mex_stokes_dlp_kspace.cpp:129-132): (n1 f1, n2 f1, n1 f2, n2 f2). */
+static double *or_dlp_q4(const double *f, const double *nv, int64_t n)
+{
+    double *q = (double *)malloc(sizeof(double) * (size_t)(4 * n + 4));
+    for (int64_t k = 0; k < n; ++k) {
+        q[4 * k] = nv[2 * k] * f[2 * k];
+        q[4 * k + 1] = nv[2 * k + 1] * f[2 * k];
+        q[4 * k + 2] = nv[2 * k] * f[2 * k + 1];
+        q[4 * k + 3] = nv[2 * k + 1] * f[2 * k + 1];
+    }
+    return q;
+}
+
+/* Filtered stresslet grids: imaginary parts of the two inverse transforms
+ * (mex_stokes_dlp_kspace.cpp:129-305) into G1, G2 (caller-owned). */
+static void or_dlp_kspace_grids(const double *psrc, int64_t ns, double xi, double eta,
+                                const double *f, const double *nv, int Mx, int My,
+                                double Lx, double Ly, double w, int P, const double *e1,
+                                double *G1, double *G2)
+{
+    size_t G = (size_t)Mx * My;
+    double *Hr[4], *Hi[4];
+    for (int c = 0; c < 4; ++c) {
+        Hr[c] = (double *)calloc(G, sizeof(double));
+        Hi[c] = (double *)calloc(G, sizeof(double));
+    }
+    double *q = or_dlp_q4(f, nv, ns);
+    or_spread(psrc, ns, q, 4, xi, eta, Mx, My, Lx, Ly, w, P, e1, Hr);
+    for (int c = 0; c < 4; ++c) or_fft2(Hr[c], Hi[c], Mx, My, -1);
+    /* stresslet filter (:219-279) */
+#pragma omp parallel for
+    for (int j = 0; j < Mx; ++j) {
+        double k1 = or_wavenumber(j, Mx, Lx);
+        for (int k = 0; k < My; ++k) {
+            double k2 = or_wavenumber(k, My, Ly);
+            double et, b111, b112, b122, b222;
+            or_dlp_filter_coef(k1, k2, xi, eta, &et, &b111, &b112, &b122, &b222);
+            size_t p = (size_t)j * My + k;
+            double m1r = Hr[1][p] + Hr[2][p], m1i = Hi[1][p] + Hi[2][p];
+            double o1r = et * (b111 * Hr[0][p] + b112 * m1r + b122 * Hr[3][p]);
+            double o1i = et * (b111 * Hi[0][p] + b112 * m1i + b122 * Hi[3][p]);
+            double o2r = et * (b112 * Hr[0][p] + b122 * m1r + b222 * Hr[3][p]);
+            double o2i = et * (b112 * Hi[0][p] + b122 * m1i + b222 * Hi[3][p]);
+            Hr[0][p] = o1r; Hi[0][p] = o1i;
+            Hr[1][p] = o2r; Hi[1][p] = o2i;
+        }
+    }
+    Hr[0][0] = Hr[1][0] = Hi[0][0] = Hi[1][0] = 0.0;   /* :282-285 */
+    or_fft2(Hr[0], Hi[0], Mx, My, +1);
+    or_fft2(Hr[1], Hi[1], Mx, My, +1);
+    /* imaginary part (:304-305) */
+    memcpy(G1, Hi[0], sizeof(double) * G);
+    memcpy(G2, Hi[1], sizeof(double) * G);
+    for (int c = 0; c < 4; ++c) { free(Hr[c]); free(Hi[c]); }
+    free(q);
 }
 
 int or_dlp_kspace(const double *psrc, int64_t ns, const double *ptar, int64_t nt,
@@ -611,51 +698,414 @@ int or_dlp_kspace(const double *psrc, int64_t ns, const double *ptar, int64_t nt
     size_t G = (size_t)Mx * My;
     double h = Lx / Mx;
     double *e1 = or_e1_table(xi, eta, h, P);
-    double *Hr[4], *Hi[4];
-    for (int c = 0; c < 4; ++c) {
-        Hr[c] = (double *)calloc(G, sizeof(double));
-        Hi[c] = (double *)calloc(G, sizeof(double));
-    }
-    /* four grids n_l f_j (mex_stokes_dlp_kspace.cpp:129-132) */
-    double *q = (double *)malloc(sizeof(double) * (size_t)(4 * ns + 4));
-    for (int64_t k = 0; k < ns; ++k) {
-        q[4 * k] = nv[2 * k] * f[2 * k];
-        q[4 * k + 1] = nv[2 * k + 1] * f[2 * k];
-        q[4 * k + 2] = nv[2 * k] * f[2 * k + 1];
-        q[4 * k + 3] = nv[2 * k + 1] * f[2 * k + 1];
-    }
-    or_spread(psrc, ns, q, 4, xi, eta, Mx, My, Lx, Ly, w, P, e1, Hr);
-    for (int c = 0; c < 4; ++c) or_fft2(Hr[c], Hi[c], Mx, My, -1);
-    /* stresslet filter (:219-279) */
-#pragma omp parallel for
-    for (int j = 0; j < Mx; ++j) {
-        double k1 = or_wavenumber(j, Mx, Lx);
-        for (int k = 0; k < My; ++k) {
-            double k2 = or_wavenumber(k, My, Ly);
-            double K2 = k1 * k1 + k2 * k2;
-            double e = exp(-0.25 * (1 - eta) / (xi * xi) * K2) / K2 / K2;
-            double t = 1 + 0.25 * K2 / xi / xi;
-            double b111 = 3.0 * k1 * K2 - 2.0 * k1 * k1 * k1;
-            double b112 = k2 * K2 - 2.0 * k1 * k1 * k2;
-            double b122 = k1 * K2 - 2.0 * k1 * k2 * k2;
-            double b222 = 3.0 * k2 * K2 - 2.0 * k2 * k2 * k2;
-            size_t p = (size_t)j * My + k;
-            double m1r = Hr[1][p] + Hr[2][p], m1i = Hi[1][p] + Hi[2][p];
-            double o1r = e * t * (b111 * Hr[0][p] + b112 * m1r + b122 * Hr[3][p]);
-            double o1i = e * t * (b111 * Hi[0][p] + b112 * m1i + b122 * Hi[3][p]);
-            double o2r = e * t * (b112 * Hr[0][p] + b122 * m1r + b222 * Hr[3][p]);
-            double o2i = e * t * (b112 * Hi[0][p] + b122 * m1i + b222 * Hi[3][p]);
-            Hr[0][p] = o1r; Hi[0][p] = o1i;
-            Hr[1][p] = o2r; Hi[1][p] = o2i;
+    double *G1 = (double *)malloc(sizeof(double) * G);
+    double *G2 = (double *)malloc(sizeof(double) * G);
+    or_dlp_kspace_grids(psrc, ns, xi, eta, f, nv, Mx, My, Lx, Ly, w, P, e1, G1, G2);
+    or_interp(ptar, nt, xi, eta, Mx, My, Lx, Ly, w, P, e1, G1, G2, u);
+    free(G1); free(G2);
+    free(e1);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoints (gradients) of the four sums.                                    */
+/* With u the output of a sum and L = sum_t du_t . u_t, each *_adj function  */
+/* returns dL/dpsrc, dL/dptar, dL/df (and dL/dn for the stresslet); NULL     */
+/* outputs are skipped.  The reference has no adjoint: these are derived     */
+/* here from the forward restatements above.  Pair inclusion, image offsets  */
+/* and the coincident-point rules are those of the forward sums; gradients   */
+/* treat the cell and anchor assignment as fixed (piecewise constant), and   */
+/* a coincident pair (r^2 == 0, a constant term) contributes no position     */
+/* gradient.  Written with full tensor derivatives (index loops), not the    */
+/* contracted forms of the product, so that the two are independent.        */
+/* ------------------------------------------------------------------------ */
+
+/* Stokeslet pair G(d) = 0.5 E1(xi^2 r^2) I + e^{-xi^2 r^2} (d d^T / r^2 - I)
+ * and its derivative dG_ij/dd_l.  v += G a;  g_l += sum_ij a_i b_j dG_ij/dd_l. */
+static void or_slp_pair_adj(const double d[2], double r2, double xi, const double a[2],
+                            const double b[2], double v[2], double g[2])
+{
+    double xi2 = xi * xi;
+    double e = exp(-xi2 * r2);
+    double E = or_e1(xi2 * r2);
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            double dij = (i == j) ? 1.0 : 0.0;
+            double Gij = 0.5 * E * dij + e * (d[i] * d[j] / r2 - dij);
+            v[i] += Gij * a[j];
+            for (int l = 0; l < 2; ++l) {
+                double dil = (i == l) ? 1.0 : 0.0, djl = (j == l) ? 1.0 : 0.0;
+                /* d/dd_l of 0.5 E1(xi^2 r^2) = -e d_l / r^2 (dE1/dx = -e^{-x}/x) */
+                double dG = -e * d[l] / r2 * dij
+                            - 2.0 * xi2 * e * d[l] * (d[i] * d[j] / r2 - dij)
+                            + e * ((dil * d[j] + djl * d[i]) / r2 -
+                                   2.0 * d[i] * d[j] * d[l] / (r2 * r2));
+                g[l] += a[i] * b[j] * dG;
+            }
+        }
+}
+
+/* Stresslet pair T_ijk(d) = e2 (fb d_i d_j d_k + 2 xi^2 (δij dk + δik dj + δjk di)),
+ * fb = -4 (1 + xi^2 r^2) / r^4, e2 = e^{-xi^2 r^2} (mex_stokes_dlp_real.cpp:125-138),
+ * contracted with a (the output weight) and S = f (x) n:
+ *   M_jk += sum_i a_i T_ijk;   g_l += sum_ijk a_i S_jk dT_ijk/dd_l. */
+static void or_dlp_pair_adj(const double d[2], double r2, double xi, const double a[2],
+                            const double S[2][2], double M[2][2], double g[2])
+{
+    double xi2 = xi * xi;
+    double e2 = exp(-xi2 * r2);
+    double fb = -4.0 * (1.0 + xi2 * r2) / (r2 * r2);
+    double fbp = 4.0 * (2.0 + xi2 * r2) / (r2 * r2 * r2);   /* d fb / d(r^2) */
+    double pre = 2.0 * xi2;
+#define OR_KD(p, q) ((p) == (q) ? 1.0 : 0.0)
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 2; ++k) {
+                double sym = OR_KD(i, j) * d[k] + OR_KD(i, k) * d[j] + OR_KD(j, k) * d[i];
+                double T = e2 * (fb * d[i] * d[j] * d[k] + pre * sym);
+                M[j][k] += a[i] * T;
+                for (int l = 0; l < 2; ++l) {
+                    double dsym = OR_KD(i, j) * OR_KD(k, l) + OR_KD(i, k) * OR_KD(j, l) +
+                                  OR_KD(j, k) * OR_KD(i, l);
+                    double dT = -2.0 * xi2 * d[l] * T +
+                                e2 * (2.0 * fbp * d[l] * d[i] * d[j] * d[k] +
+                                      fb * (OR_KD(i, l) * d[j] * d[k] + OR_KD(j, l) * d[i] * d[k] +
+                                            OR_KD(k, l) * d[i] * d[j]) +
+                                      pre * dsym);
+                    g[l] += a[i] * S[j][k] * dT;
+                }
+            }
+#undef OR_KD
+}
+
+/* Real-space adjoint, both kernels.  Targets are visited as in the forward
+ * sums (cells in parallel); contributions to sources go to per-thread buffers
+ * that are added at the end.  Targets with du == 0 are skipped (they
+ * contribute nothing). */
+static int or_real_adj(int dlp, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                       const double *f, const double *nv, double xi, int nsx, int nsy,
+                       double Lx, double Ly, const double *du, double *dpsrc,
+                       double *dptar, double *df, double *dn)
+{
+    int nbox = nsx * nsy;
+    int64_t *ps = (int64_t *)malloc(sizeof(int64_t) * (size_t)(ns + 1));
+    int64_t *pt = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nt + 1));
+    int64_t *os = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nbox + 1));
+    int64_t *ot = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nbox + 1));
+    or_cell_lists(psrc, ns, Lx, Ly, nsx, nsy, ps, os);
+    or_cell_lists(ptar, nt, Lx, Ly, nsx, nsy, pt, ot);
+    const double self = -(0.5 * OR_EULER_GAMMA + 1.0) - log(xi);   /* slp :124 */
+    const double cut2 = Lx * Ly / nsx / nsy;
+    /* per source: (dL/dx, dL/dy, then 2 (SLP: G du) or 4 (DLP: M) entries) */
+    const int W = dlp ? 6 : 4;
+    int nthr = 1;
+#ifdef _OPENMP
+    nthr = omp_get_max_threads();
+#endif
+    double *sbuf = (double *)calloc((size_t)nthr * (size_t)(W * ns + 1), sizeof(double));
+    double *tacc = (double *)calloc((size_t)(2 * nt + 2), sizeof(double));
+
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nbox; ++b) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *mine = sbuf + (size_t)tid * (size_t)(W * ns + 1);
+        int bx = b % nsx, by = b / nsx;
+        for (int64_t tk = ot[b]; tk < ot[b + 1]; ++tk) {
+            int64_t t = pt[tk];
+            double a[2] = {du[2 * t], du[2 * t + 1]};
+            if (a[0] == 0.0 && a[1] == 0.0) continue;
+            double tx = ptar[2 * t], ty = ptar[2 * t + 1];
+            double gt[2] = {0.0, 0.0};
+            for (int j = -1; j < 8; ++j) {   /* -1: own box */
+                int sb, own = (j < 0);
+                double zx = 0.0, zy = 0.0;
+                if (own) {
+                    sb = b;
+                } else {
+                    int qx = bx + OR_NBX[j], qy = by + OR_NBY[j];
+                    int wx = (qx + nsx) % nsx, wy = (qy + nsy) % nsy;
+                    sb = wy * nsx + wx;
+                    zx = (Lx * (qx - wx)) / nsx;
+                    zy = (Ly * (qy - wy)) / nsy;
+                }
+                for (int64_t sk = os[sb]; sk < os[sb + 1]; ++sk) {
+                    int64_t s = ps[sk];
+                    double xs = psrc[2 * s], ys = psrc[2 * s + 1];
+                    /* d = x_s - x_t + z with the forward sums' rounding */
+                    double d[2];
+                    if (dlp) {
+                        d[0] = xs - tx + zx;
+                        d[1] = ys - ty + zy;
+                    } else if (own) {
+                        d[0] = -(tx - xs);   /* forward: tx - xs; negation is exact */
+                        d[1] = -(ty - ys);
+                    } else {
+                        d[0] = xs - (tx - zx);
+                        d[1] = ys - (ty - zy);
+                    }
+                    double r2 = d[0] * d[0] + d[1] * d[1];
+                    double *ms = mine + (size_t)W * s;
+                    if (own && r2 == 0.0) {
+                        if (!dlp) {   /* self term: constant in the positions */
+                            ms[2] += self * a[0];
+                            ms[3] += self * a[1];
+                        }
+                        continue;
+                    }
+                    if (!own && !(r2 < cut2)) continue;
+                    double g[2] = {0.0, 0.0};
+                    if (!dlp) {
+                        double bvec[2] = {f[2 * s], f[2 * s + 1]};
+                        double v[2] = {0.0, 0.0};
+                        or_slp_pair_adj(d, r2, xi, a, bvec, v, g);
+                        ms[2] += v[0];
+                        ms[3] += v[1];
+                    } else {
+                        double S[2][2] = {{f[2 * s] * nv[2 * s], f[2 * s] * nv[2 * s + 1]},
+                                          {f[2 * s + 1] * nv[2 * s], f[2 * s + 1] * nv[2 * s + 1]}};
+                        double M[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+                        or_dlp_pair_adj(d, r2, xi, a, S, M, g);
+                        ms[2] += M[0][0];
+                        ms[3] += M[0][1];
+                        ms[4] += M[1][0];
+                        ms[5] += M[1][1];
+                    }
+                    ms[0] += g[0];
+                    ms[1] += g[1];
+                    gt[0] -= g[0];
+                    gt[1] -= g[1];
+                }
+            }
+            tacc[2 * t] = gt[0];
+            tacc[2 * t + 1] = gt[1];
         }
     }
-    Hr[0][0] = Hr[1][0] = Hi[0][0] = Hi[1][0] = 0.0;   /* :282-285 */
-    or_fft2(Hr[0], Hi[0], Mx, My, +1);
-    or_fft2(Hr[1], Hi[1], Mx, My, +1);
-    /* imaginary part (:304-305) */
-    or_interp(ptar, nt, xi, eta, Mx, My, Lx, Ly, w, P, e1, Hi[0], Hi[1], u);
-    for (int c = 0; c < 4; ++c) { free(Hr[c]); free(Hi[c]); }
-    free(q);
+    const double s4 = 1.0 / (4 * OR_PI);
+    if (dptar)
+        for (int64_t t = 0; t < nt; ++t) {
+            dptar[2 * t] = tacc[2 * t] * s4;
+            dptar[2 * t + 1] = tacc[2 * t + 1] * s4;
+        }
+    for (int64_t s = 0; s < ns; ++s) {
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int th = 0; th < nthr; ++th)
+            for (int c = 0; c < W; ++c) acc[c] += sbuf[(size_t)th * (size_t)(W * ns + 1) + (size_t)W * s + c];
+        for (int c = 0; c < W; ++c) acc[c] *= s4;
+        if (dpsrc) {
+            dpsrc[2 * s] = acc[0];
+            dpsrc[2 * s + 1] = acc[1];
+        }
+        if (!dlp) {
+            if (df) {
+                df[2 * s] = acc[2];
+                df[2 * s + 1] = acc[3];
+            }
+        } else {
+            /* M_jk = dL/dS_jk with S_jk = f_j n_k */
+            double n1 = nv[2 * s], n2 = nv[2 * s + 1], f1 = f[2 * s], f2 = f[2 * s + 1];
+            if (df) {
+                df[2 * s] = acc[2] * n1 + acc[3] * n2;
+                df[2 * s + 1] = acc[4] * n1 + acc[5] * n2;
+            }
+            if (dn) {
+                dn[2 * s] = acc[2] * f1 + acc[4] * f2;
+                dn[2 * s + 1] = acc[3] * f1 + acc[5] * f2;
+            }
+        }
+    }
+    free(ps); free(pt); free(os); free(ot); free(sbuf); free(tacc);
+    return 0;
+}
+
+int or_slp_real_adj(const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                    const double *f, double xi, int nsx, int nsy, double Lx, double Ly,
+                    const double *du, double *dpsrc, double *dptar, double *df)
+{
+    return or_real_adj(0, psrc, ns, ptar, nt, f, NULL, xi, nsx, nsy, Lx, Ly, du, dpsrc,
+                       dptar, df, NULL);
+}
+
+int or_dlp_real_adj(const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                    const double *f, const double *nv, double xi, int nsx, int nsy,
+                    double Lx, double Ly, const double *du, double *dpsrc, double *dptar,
+                    double *df, double *dn)
+{
+    return or_real_adj(1, psrc, ns, ptar, nt, f, nv, xi, nsx, nsy, Lx, Ly, du, dpsrc, dptar,
+                       df, dn);
+}
+
+/* Interpolation of ng grids at n points with the forward's weights and
+ * scaling, plus the derivative with respect to the point:
+ *   val[k*ng + c]  = s * sum_nodes W G_c                 (if val)
+ *   grad[2k + l]  += s * sum_c wt[k*ng + c] * dW/dx_l G_c (if grad)
+ * W = exp(c |node - x|^2) is evaluated as the forward does (FGG factors);
+ * dW/dx = -2c (h i - w - px) W for node index i of the (P+1)^2 support,
+ * the derivative of the factored exponent c (px^2 + 2 w px - 2 h i px + ...). */
+static void or_interp_vg(const double *pts, int64_t n, int ng, double *const *grids,
+                         double xi, double eta, int Mx, int My, double Lx, double Ly, double w,
+                         int P, const double *e1, double *val, const double *wt, double *grad)
+{
+    double h = Lx / Mx;
+    double s = 4 * xi * xi / eta;
+    s = s * s * h * h / OR_PI;
+    s = s / (4 * OR_PI);
+    double c = -2 * xi * xi / eta;
+#pragma omp parallel for
+    for (int64_t k = 0; k < n; ++k) {
+        or_fgg_point g;
+        double px, py;
+        int mx, my;
+        or_fgg_setup(pts[2 * k], pts[2 * k + 1], Lx, Ly, h, P, xi, eta, w, &g);
+        or_find_closest_node(pts[2 * k], pts[2 * k + 1], Lx, Ly, h, P, &mx, &my, &px, &py);
+        double v[8] = {0}, gx[8] = {0}, gy[8] = {0};
+        double ex = g.ex;
+        for (int i = 0; i <= P; ++i) {
+            int col = ((i + g.mx) % Mx + Mx) % Mx;
+            double ey = ex * g.e4y * e1[i];
+            double dx = h * i - w - px;
+            for (int j = 0; j <= P; ++j) {
+                int row = ((j + g.my) % My + My) % My;
+                double wgt = ey * e1[j];
+                double dy = h * j - w - py;
+                size_t idx = (size_t)col * My + row;
+                for (int q = 0; q < ng; ++q) {
+                    double gv = grids[q][idx];
+                    v[q] += wgt * gv;
+                    gx[q] += -2 * c * dx * wgt * gv;
+                    gy[q] += -2 * c * dy * wgt * gv;
+                }
+                ey *= g.e3y;
+            }
+            ex *= g.e3x;
+        }
+        if (val)
+            for (int q = 0; q < ng; ++q) val[k * ng + q] = s * v[q];
+        if (grad) {
+            double a0 = 0.0, a1 = 0.0;
+            for (int q = 0; q < ng; ++q) {
+                a0 += wt[k * ng + q] * gx[q];
+                a1 += wt[k * ng + q] * gy[q];
+            }
+            grad[2 * k] = s * a0;
+            grad[2 * k + 1] = s * a1;
+        }
+    }
+}
+
+/* Stokeslet Fourier-space adjoint.  The forward is u = s I_t Re(F^-1 B F S_s f)
+ * with B the real symmetric 2x2 filter per wavenumber (F unnormalised DFT,
+ * S spreading, I interpolation).  For a real-linear map X -> Re(Y X) the
+ * adjoint is Z -> Re(Y^H Z), and Y^H = F^-1 B^H F = F^-1 B F, so
+ * df = s S_s^T Re(F^-1 B F I_t^T du): the forward pipeline with the roles of
+ * sources and targets exchanged and du as the density. */
+int or_slp_kspace_adj(const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                      double xi, double eta, const double *f, int Mx, int My, double Lx,
+                      double Ly, double w, int P, const double *du, double *dpsrc,
+                      double *dptar, double *df)
+{
+    size_t G = (size_t)Mx * My;
+    double h = Lx / Mx;
+    double *e1 = or_e1_table(xi, eta, h, P);
+    double *A[2], *B[2];
+    for (int c = 0; c < 2; ++c) {
+        A[c] = (double *)calloc(G, sizeof(double));
+        B[c] = (double *)calloc(G, sizeof(double));
+    }
+    if (dptar) {
+        or_slp_kspace_grids(psrc, ns, xi, eta, f, Mx, My, Lx, Ly, w, P, e1, A[0], A[1]);
+        or_interp_vg(ptar, nt, 2, A, xi, eta, Mx, My, Lx, Ly, w, P, e1, NULL, du, dptar);
+    }
+    if (dpsrc || df) {
+        or_slp_kspace_grids(ptar, nt, xi, eta, du, Mx, My, Lx, Ly, w, P, e1, B[0], B[1]);
+        double *val = (double *)malloc(sizeof(double) * (size_t)(2 * ns + 2));
+        or_interp_vg(psrc, ns, 2, B, xi, eta, Mx, My, Lx, Ly, w, P, e1, val, f, dpsrc);
+        if (df) memcpy(df, val, sizeof(double) * (size_t)(2 * ns));
+        free(val);
+    }
+    for (int c = 0; c < 2; ++c) { free(A[c]); free(B[c]); }
+    free(e1);
+    return 0;
+}
+
+/* Stresslet Fourier-space adjoint.  The forward is u = s I_t Im(F^-1 B F S_s q)
+ * with q the four densities n_l f_j and B the real 2x4 filter.  For
+ * X -> Im(Y X) = Re(-i Y X) the adjoint is Z -> Re(i Y^H Z) = -Im(Y^H Z), with
+ * Y^H = F^-1 B^T F: spread du at the targets, apply the transposed filter
+ * (2 -> 4 grids), inverse transform, keep -Im, interpolate at the sources. */
+int or_dlp_kspace_adj(const double *psrc, int64_t ns, const double *ptar, int64_t nt,
+                      double xi, double eta, const double *f, const double *nv, int Mx,
+                      int My, double Lx, double Ly, double w, int P, const double *du,
+                      double *dpsrc, double *dptar, double *df, double *dn)
+{
+    size_t G = (size_t)Mx * My;
+    double h = Lx / Mx;
+    double *e1 = or_e1_table(xi, eta, h, P);
+    if (dptar) {
+        double *A[2];
+        for (int c = 0; c < 2; ++c) A[c] = (double *)calloc(G, sizeof(double));
+        or_dlp_kspace_grids(psrc, ns, xi, eta, f, nv, Mx, My, Lx, Ly, w, P, e1, A[0], A[1]);
+        or_interp_vg(ptar, nt, 2, A, xi, eta, Mx, My, Lx, Ly, w, P, e1, NULL, du, dptar);
+        for (int c = 0; c < 2; ++c) free(A[c]);
+    }
+    if (dpsrc || df || dn) {
+        double *Zr[2], *Zi[2], *Or[4], *Oi[4];
+        for (int c = 0; c < 2; ++c) {
+            Zr[c] = (double *)calloc(G, sizeof(double));
+            Zi[c] = (double *)calloc(G, sizeof(double));
+        }
+        for (int c = 0; c < 4; ++c) {
+            Or[c] = (double *)calloc(G, sizeof(double));
+            Oi[c] = (double *)calloc(G, sizeof(double));
+        }
+        or_spread(ptar, nt, du, 2, xi, eta, Mx, My, Lx, Ly, w, P, e1, Zr);
+        or_fft2(Zr[0], Zi[0], Mx, My, -1);
+        or_fft2(Zr[1], Zi[1], Mx, My, -1);
+#pragma omp parallel for
+        for (int j = 0; j < Mx; ++j) {
+            double k1 = or_wavenumber(j, Mx, Lx);
+            for (int k = 0; k < My; ++k) {
+                double k2 = or_wavenumber(k, My, Ly);
+                double et, b111, b112, b122, b222;
+                or_dlp_filter_coef(k1, k2, xi, eta, &et, &b111, &b112, &b122, &b222);
+                size_t p = (size_t)j * My + k;
+                /* B^T: column c of the forward's 2x4 matrix
+                 * [b111 b112 b112 b122; b112 b122 b122 b222] */
+                double col[4][2] = {{b111, b112}, {b112, b122}, {b112, b122}, {b122, b222}};
+                for (int c = 0; c < 4; ++c) {
+                    Or[c][p] = et * (col[c][0] * Zr[0][p] + col[c][1] * Zr[1][p]);
+                    Oi[c][p] = et * (col[c][0] * Zi[0][p] + col[c][1] * Zi[1][p]);
+                }
+            }
+        }
+        for (int c = 0; c < 4; ++c) {
+            Or[c][0] = Oi[c][0] = 0.0;   /* k = 0 mode, as in the forward */
+            or_fft2(Or[c], Oi[c], Mx, My, +1);
+            for (size_t i = 0; i < G; ++i) Oi[c][i] = -Oi[c][i];
+        }
+        double *q = or_dlp_q4(f, nv, ns);
+        double *val = (double *)malloc(sizeof(double) * (size_t)(4 * ns + 4));
+        or_interp_vg(psrc, ns, 4, Oi, xi, eta, Mx, My, Lx, Ly, w, P, e1, val, q, dpsrc);
+        /* val = dL/d(n1 f1, n2 f1, n1 f2, n2 f2) */
+        for (int64_t s = 0; s < ns; ++s) {
+            const double *D = val + 4 * s;
+            double n1 = nv[2 * s], n2 = nv[2 * s + 1], f1 = f[2 * s], f2 = f[2 * s + 1];
+            if (df) {
+                df[2 * s] = D[0] * n1 + D[1] * n2;
+                df[2 * s + 1] = D[2] * n1 + D[3] * n2;
+            }
+            if (dn) {
+                dn[2 * s] = D[0] * f1 + D[2] * f2;
+                dn[2 * s + 1] = D[1] * f1 + D[3] * f2;
+            }
+        }
+        free(q);
+        free(val);
+        for (int c = 0; c < 2; ++c) { free(Zr[c]); free(Zi[c]); }
+        for (int c = 0; c < 4; ++c) { free(Or[c]); free(Oi[c]); }
+    }
     free(e1);
     return 0;
 }

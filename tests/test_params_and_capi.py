@@ -191,3 +191,20 @@ def test_c_parameter_selection_matches_host_restatement(kind):
             (ref.nside_x, ref.nside_y, ref.Mx, ref.My, ref.kinf)
         assert abs(prm.xi - ref.xi) <= 1e-12 * ref.xi
         assert abs(prm.eta - ref.eta) <= 1e-12 * ref.eta and prm.converged == 1
+
+
+def test_backward_validation_codes(hctx):
+    lib, h = hctx
+    x = _fake(1)
+    P, M, L = 24, 64, 1.0
+    w = P * L / M / 2
+    # du is required; outputs may be null
+    assert lib.se2p_slp_real_backward(h, x, 4, x, 4, x, 10.0, 3, 3, L, L, None, x, x, x) == _capi.SE2P_ERR_ARG
+    assert lib.se2p_dlp_real_backward(h, x, 4, x, 4, x, None, 10.0, 3, 3, L, L, x, x, x, x, x) == _capi.SE2P_ERR_ARG
+    assert lib.se2p_slp_real_backward(h, x, 4, x, 4, x, 10.0, 0, 3, L, L, x, None, None, None) == _capi.SE2P_ERR_NSIDE
+    assert lib.se2p_slp_kspace_backward(h, x, 4, x, 4, 10.0, 0.5, x, 63, M, L, L, w, P, x, x, x, x) == _capi.SE2P_ERR_GRID
+    assert lib.se2p_dlp_kspace_backward(h, x, 4, x, 4, 10.0, 0.5, x, x, M, M, L, L, w * 1.01, P, x, x, x, x, x) == _capi.SE2P_ERR_W
+    assert lib.se2p_dlp_kspace_backward(h, x, -2, x, 4, 10.0, 0.5, x, x, M, M, L, L, w, P, x, x, x, x, x) == _capi.SE2P_ERR_ARG
+    # valid arguments, null outputs allowed: reach the device step
+    assert lib.se2p_slp_real_backward(h, x, 4, x, 4, x, 10.0, 3, 3, L, L, x, None, x, None) == _capi.SE2P_ERR_HIP
+    assert lib.se2p_dlp_kspace_backward(h, x, 4, x, 4, 10.0, 0.5, x, x, M, M, L, L, w, P, x, x, None, None, x) == _capi.SE2P_ERR_HIP
