@@ -105,6 +105,66 @@ def _batched_variant(args, pe, W, np, torch, dev, nrhs=64):
     return res
 
 
+def _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, fwd_ms):
+    """Forward + adjoint on the headline workload: one step is the two forward
+    sums plus the two backward entry points with every gradient (dL/dpsrc,
+    dL/dptar, dL/df, dL/dn) for a seeded du."""
+    N = ps.shape[0]
+    du = torch.from_numpy(np.random.default_rng(77).standard_normal((N, 2))).to(ps.device)
+    ur = torch.empty_like(du)
+    uk = torch.empty_like(du)
+
+    def backward():
+        pe.mex_stokes_dlp_real_backward(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly, du,
+                                        ctx=ctx)
+        pe.mex_stokes_dlp_kspace_backward(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, w.Lx, w.Ly, p.w,
+                                          p.P, du, ctx=ctx)
+
+    def step():
+        pe.mex_stokes_dlp_real(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly, ctx=ctx, out=ur)
+        pe.mex_stokes_dlp_kspace(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, w.Lx, w.Ly, p.w, p.P,
+                                 ctx=ctx, out=uk)
+        backward()
+
+    with torch.cuda.stream(stream):
+        ctx.bind_torch_stream(stream)
+        ctx.set_async(True)
+        for _ in range(max(1, args.warmup)):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = max(3, args.steps // 2)
+        e0.record(stream)
+        for _ in range(reps):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        # stage split of the two backward calls
+        ctx.set_timing(True)
+        stages = {}
+        for _ in range(3):
+            pe.mex_stokes_dlp_real_backward(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly,
+                                            du, ctx=ctx)
+            for k, v in ctx.stage_times().items():
+                if v >= 0:
+                    stages.setdefault("real_backward." + k, []).append(v)
+            pe.mex_stokes_dlp_kspace_backward(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, w.Lx, w.Ly,
+                                              p.w, p.P, du, ctx=ctx)
+            for k, v in ctx.stage_times().items():
+                if v >= 0:
+                    stages.setdefault("kspace_backward." + k, []).append(v)
+        ctx.set_timing(False)
+        ctx.set_async(False)
+    torch.cuda.synchronize()
+    print(f"# forward+adjoint: {ms:.3f} ms per step", file=sys.stderr, flush=True)
+    return {"ms_per_step": ms, "target_evals_per_s": N / (ms * 1e-3),
+            "adjoint_over_forward_time": (ms - fwd_ms) / fwd_ms,
+            "stage_ms": {k: statistics.mean(v) for k, v in stages.items()},
+            "workload": "same as the headline; step = forward real + Fourier sums, then both "
+                        "backward entry points with all four gradients"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +174,8 @@ def main() -> int:
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the plain-vs-batched right-hand-side comparison")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the forward+adjoint variant")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (scales HBM counters)")
     args = ap.parse_args()
@@ -240,6 +302,9 @@ def main() -> int:
                "seconds_per_step": med}
 
     variants = None if args.no_variants else _batched_variant(args, pe, W, np, torch, dev)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, ms_per_step)
 
     traffic = _pmc_traffic("k_real_dlp")
     out = {
@@ -259,7 +324,7 @@ def main() -> int:
                                "P=24, tol 1e-10",
                    "N": N, "Mx": p.Mx, "My": p.My, "P": p.P, "xi": p.xi,
                    "nside": [p.nside_x, p.nside_y], "tol": p.tol},
-        "roofline": {"bound": "valu", "kernel": "k_real<1,1,false> (real-space stresslet pair sum)",
+        "roofline": {"bound": "valu", "kernel": "k_real<1,1> (real-space stresslet pair sum)",
                      "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "flops_per_launch": flops_real, "pairs_in": pairs_in, "pairs_out": pairs_out,
@@ -268,6 +333,7 @@ def main() -> int:
         "parity_vs_oracle": parity,
         "cpu_baseline": cpu,
         "variants": variants,
+        "forward_plus_adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
     return 0
