@@ -87,6 +87,24 @@ SIGNATURES.update({
                             _P, _P, _P]),
 })
 
+class Se2pBoxDesc(ctypes.Structure):
+    """se2p_box_desc of include/se2p.h."""
+    _fields_ = [("psrc", ctypes.c_void_p), ("ns", ctypes.c_int64),
+                ("ptar", ctypes.c_void_p), ("nt", ctypes.c_int64),
+                ("f", ctypes.c_void_p), ("n", ctypes.c_void_p),
+                ("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("xi", ctypes.c_double),
+                ("nside_x", ctypes.c_int), ("nside_y", ctypes.c_int),
+                ("eta", ctypes.c_double), ("w", ctypes.c_double),
+                ("Mx", ctypes.c_int), ("My", ctypes.c_int), ("P", ctypes.c_int),
+                ("u", ctypes.c_void_p)]
+
+
+SIGNATURES.update({
+    name: (_I, [_P, ctypes.POINTER(Se2pBoxDesc), _I])
+    for name in ("se2p_slp_real_multibox", "se2p_slp_kspace_multibox",
+                 "se2p_dlp_real_multibox", "se2p_dlp_kspace_multibox")
+})
+
 STAGES = ("sort", "real", "spread", "fft_fwd", "filter", "fft_inv", "interp")
 
 _lib = None

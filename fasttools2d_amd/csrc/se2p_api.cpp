@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <string>
 #include <mutex>
 
 #include "se2p_device.h"
@@ -368,6 +369,60 @@ int se2p_dlp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const
     if (int rc = set_device(ctx)) return rc;
     if (int rc = kspace_sum(ctx, 1, g, f, n, nrhs, f_stride, u_stride, u)) return rc;
     return finish(ctx);
+}
+
+static int check_box(se2p_ctx *ctx, const se2p_box_desc &b, int i, bool dlp, bool kspace)
+{
+    int rc = check_common(ctx, b.psrc, b.ns, b.ptar, b.nt, b.u);
+    if (!rc && b.ns > 0 && (!b.f || (dlp && !b.n))) rc = fail(ctx, SE2P_ERR_ARG, "null density pointer");
+    if (!rc && !kspace) rc = check_real(ctx, b.xi, b.nside_x, b.nside_y, b.Lx, b.Ly);
+    if (!rc && kspace) {
+        KGeom g{b.psrc, b.ptar, b.ns, b.nt, b.xi, b.eta, b.Lx, b.Ly, b.w, b.Mx, b.My, b.P};
+        rc = check_kspace_params(ctx, g);
+    }
+    if (rc) ctx->err = "box " + std::to_string(i) + ": " + ctx->err;
+    return rc;
+}
+
+static int multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox, int kind, bool kspace)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (nbox < 0 || (nbox > 0 && !boxes)) return fail(ctx, SE2P_ERR_ARG, "bad box list");
+    for (int i = 0; i < nbox; ++i)
+        if (int rc = check_box(ctx, boxes[i], i, kind == 1, kspace)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    if (!kspace) {
+        if (nbox > 0)
+            if (int rc = real_sum_multibox(ctx, kind, boxes, nbox)) return rc;
+    } else {
+        for (int i = 0; i < nbox; ++i) {
+            const se2p_box_desc &b = boxes[i];
+            KGeom g{b.psrc, b.ptar, b.ns, b.nt, b.xi, b.eta, b.Lx, b.Ly, b.w, b.Mx, b.My, b.P};
+            if (int rc = kspace_sum(ctx, kind, g, b.f, b.n, 1, 2 * b.ns, 2 * b.nt, b.u)) return rc;
+        }
+    }
+    return finish(ctx);
+}
+
+int se2p_slp_real_multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox)
+{
+    return multibox(ctx, boxes, nbox, 0, false);
+}
+
+int se2p_slp_kspace_multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox)
+{
+    return multibox(ctx, boxes, nbox, 0, true);
+}
+
+int se2p_dlp_real_multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox)
+{
+    return multibox(ctx, boxes, nbox, 1, false);
+}
+
+int se2p_dlp_kspace_multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox)
+{
+    return multibox(ctx, boxes, nbox, 1, true);
 }
 
 int se2p_slp_real_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,

@@ -29,6 +29,48 @@ __host__ __device__ inline double se2p_sqsum(double dx, double dy)
     return a + b;
 }
 
+// 2^(-j/16), j = 0..15 (correctly rounded; values from mpmath at 200 bits).
+#define SE2P_EXP2_TABLE                                                          \
+    {1.0, 9.5760328069857365e-1, 9.1700404320467123e-1, 8.7812608018664974e-1, \
+     8.4089641525371454e-1, 8.0524516597462715e-1, 7.7110541270397041e-1,      \
+     7.3841307296974966e-1, 7.0710678118654752e-1, 6.7712777346844636e-1,      \
+     6.4841977732550483e-1, 6.2092890603674202e-1, 5.9460355750136053e-1,      \
+     5.6939431737834583e-1, 5.4525386633262883e-1, 5.2213689121370692e-1}
+
+// exp(-x) for finite x >= 0, table driven (Tang's method with 16 table
+// entries): k = rint(16 x / ln 2), r = x - k ln2/16 with ln2/16 split into a
+// double and its remainder (Cody-Waite), so |r| <= ln2/32; then
+// exp(-x) = 2^-(k>>4) * 2^-((k&15)/16) * exp(-r), exp(-r) by its degree-7
+// Taylor polynomial (truncation |r|^8/8! < 1.3e-18).  About 2 ulp; the sums
+// that use it are compared with libm's exp at 1e-12.  tab16 = SE2P_EXP2_TABLE.
+__host__ __device__ inline double se2p_exp_neg(double x, const double *tab16)
+{
+    const double k = rint(x * 23.083120654223414);   // 16 / ln 2
+    double r = fma(-k, 0.04332169878499658, x);        // ln2/16, leading part
+    r = fma(-k, 1.4494042586539372e-18, r);            // and the remainder
+    double p = -1.0 / 5040.0;
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, -1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, -1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, -1.0);
+    p = fma(p, r, 1.0);
+    const int ki = (int)k;
+    return ldexp(tab16[ki & 15] * p, -(ki >> 4));
+}
+
+// se2p_sqsum that also returns the two squares (same rounding).
+__host__ __device__ inline double se2p_sqsum2(double dx, double dy, double *dx2, double *dy2)
+{
+#pragma clang fp contract(off)
+    double a = dx * dx;
+    double b = dy * dy;
+    *dx2 = a;
+    *dy2 = b;
+    return a + b;
+}
+
 // Piece index of x > 0 in the E1 table (see gen_e1_table.py).
 __host__ __device__ inline int se2p_e1_piece(double x)
 {
@@ -87,4 +129,25 @@ __host__ __device__ inline int se2p_cell_of(double x, double y, double Lx, doubl
     bx = bx < 0 ? 0 : (bx >= nsx ? nsx - 1 : bx);
     by = by < 0 ? 0 : (by >= nsy ? nsy - 1 : by);
     return by * nsx + bx;
+}
+
+// Cell-list key of a point: cell index (above) followed by the Morton index
+// of its 2^sublog x 2^sublog sub-cell inside the cell (sublog = 0: the cell).
+__host__ __device__ inline int se2p_cell_key(double x, double y, double Lx, double Ly, int nsx,
+                                             int nsy, int sublog)
+{
+    int k = se2p_cell_of(x, y, Lx, Ly, nsx, nsy);
+    if (sublog > 0) {
+        const int sub = 1 << sublog;
+        const int bx = k % nsx, by = k / nsx;
+        int sx = (int)floor((nsx * (x / Lx + 0.5) - bx) * sub);
+        int sy = (int)floor((nsy * (y / Ly + 0.5) - by) * sub);
+        sx = sx < 0 ? 0 : (sx >= sub ? sub - 1 : sx);
+        sy = sy < 0 ? 0 : (sy >= sub ? sub - 1 : sy);
+        int m = 0;
+        for (int b = 0; b < sublog; ++b)
+            m |= (((sx >> b) & 1) << (2 * b)) | (((sy >> b) & 1) << (2 * b + 1));
+        k = (k << (2 * sublog)) | m;
+    }
+    return k;
 }

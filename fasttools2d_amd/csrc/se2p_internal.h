@@ -57,6 +57,7 @@ struct se2p_ctx {
     int64_t pairs_in = -1, pairs_out = -1;   // counted by se2p_count_pairs
     char *drv = nullptr;      // driver-level buffers (kept apart from ws)
     size_t drv_bytes = 0;
+    std::vector<char> host_stage;   // host copy of per-box tables (multi-box calls)
 };
 
 namespace se2p {
@@ -123,6 +124,12 @@ struct RealGeom {
 };
 int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
              int nrhs, int64_t dstride, int64_t ustride, double *u, int64_t *pair_counts);
+// Real-space sum over several independent boxes (host records of the boxes;
+// se2p_real.hip fills in their global bases).
+struct RealBox;
+int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, int64_t dstride,
+                   int64_t ustride, int64_t *pair_counts);
+int real_sum_multibox(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int nbox);
 // Adjoint of real_sum (one right-hand side): with L = sum_t du_t . u_t, the
 // gradients dL/dpsrc, dL/dptar, dL/df and (stresslet) dL/dn; null outputs are
 // skipped.

@@ -21,8 +21,21 @@
 // wave-uniform and arrives through the scalar cache.  The four partial sums
 // are added in a fixed order through LDS, so results are deterministic (no
 // atomics).  E1 for the Stokeslet comes from a piecewise polynomial table in
-// LDS.
+// LDS, exp(-x) from a 16-entry table (se2p_exp_neg).
+//
+// Neighbour cells are pruned by sub-cell: the bounding box of every
+// (cell, sub-cell) run of the sorted sources is computed once per call, and a
+// workgroup skips every run whose box (at the image offset) is farther than
+// the cutoff from the bounding box of its 64 targets.  Skipped pairs all fail
+// the cutoff test, so the sum is unchanged; the test has a relative margin of
+// 1e-9 on the squared distance, far above the rounding of either side.
+//
+// Workgroups are numbered so that each of the 8 XCDs (which receive
+// workgroups round-robin) takes a contiguous band of cells: its L2 then holds
+// only the sources of its band and a one-cell halo.
+#include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "se2p_device.h"
 #include "se2p_internal.h"
@@ -49,21 +62,36 @@ __device__ __forceinline__ double rcp_nr(double x)
     return fma(y, e, y);
 }
 
+// Per-box parameters of the real-space sum, read by the kernels from device
+// memory: one record per independent periodic box (a single-box call has
+// one).  Points of all boxes are numbered globally: box b owns sources
+// [s_base, s_base + ns), targets [t_base, t_base + nt) and cells
+// [cell_base, cell_base + nsx*nsy).
+struct RealBox {
+    const double *psrc, *ptar, *f, *nv;   // caller arrays (nv: stresslet)
+    double *u;                            // result r at u + r*ustride, caller's order
+    int ns, nt;
+    int s_base, t_base, cell_base;
+    int nsx, nsy;
+    double Lx, Ly, xi, xi2, cut2, self;
+};
+
 struct RealArgs {
     const double2 *tpos;   // targets in (cell, sub-cell) order
-    const int *tperm;      // that order -> caller's index
+    const int *tperm;      // that order -> global target index
     const int *toff;       // [ncell*16+1] run offsets of (cell, sub-cell) keys
     const int *chunk_off;  // [ncell+1] exclusive scan of ceil(targets/64)
-    const int *block_cell; // workgroup -> cell
+    const int *block_cell; // chunk -> cell
     const double2 *spos;   // sources in cell order
     const double *q;       // densities in cell order, [nrhs][ncomp][ns]
     const int *soff;       // [ncell*16+1] run offsets of the sources' (cell, sub-cell) keys
-    int64_t ns;
-    int nsx, nsy, ncell, nrhs;
+    int64_t ns;            // all sources (density stride)
+    int ncell;             // all cells
+    int nrhs;
     int r_begin, r_end;    // right-hand sides handled by this launch
-    double Lx, Ly, xi, xi2, cut2, self;
-    double *u;             // rhs r at u + r*ustride, interleaved, caller's order
     int64_t ustride;
+    const RealBox *boxes;  // [nbox]
+    const int *cell_box;   // cell -> box (null: one box)
     unsigned long long *pair_counts;
 };
 
@@ -73,11 +101,11 @@ struct RealArgs {
 // u1 += E/2 f1 + (e/r2) d2 c,  u2 += E/2 f2 - (e/r2) d1 c.
 template <int RB>
 __device__ __forceinline__ void slp_pair(double d1, double d2, double r2, double xi,
-                                         const double *e1tab, const double *q, int64_t qs,
-                                         double (&acc)[RB][2])
+                                         const double *e1tab, const double *etab, const double *q,
+                                         int64_t qs, double (&acc)[RB][2])
 {
     const double x = r2 * xi * xi;
-    const double e = exp(-x);
+    const double e = se2p_exp_neg(x, etab);
     const double E = se2p_e1(x, e, e1tab);
     const double g = e * rcp_nr(r2);
     const double hE = 0.5 * E;
@@ -92,26 +120,30 @@ __device__ __forceinline__ void slp_pair(double d1, double d2, double r2, double
 }
 
 // Stresslet pair, before 1/4pi (mex_stokes_dlp_real.cpp:125-138).  q holds
-// (S11, S12+S21, S22) with S = f (x) n.
+// (S11, S12+S21, S22) with S = f (x) n.  The tensor contraction
+//   u_i += e2 [fb d_i d_j d_k + 2 xi^2 (δij dk + δik dj + δjk di)] S_jk
+// is evaluated as u_i += g d_i (d.S d) + p l_i with g = e2 fb, p = 2 xi^2 e2,
+// l_1 = d1 (3 S11 + S22) + d2 (S12+S21), l_2 = d1 (S12+S21) + d2 (S11 + 3 S22);
+// d1^2 and d2^2 come from the r^2 computation.
 template <int RB>
-__device__ __forceinline__ void dlp_pair(double d1, double d2, double r2, double xi2,
-                                         const double *q, int64_t qs, double (&acc)[RB][2])
+__device__ __forceinline__ void dlp_pair(double d1, double d2, double d11, double d22, double r2,
+                                         double xi2, const double *etab, const double *q,
+                                         int64_t qs, double (&acc)[RB][2])
 {
     const double x = xi2 * r2;
-    const double e2 = exp(-x);
+    const double e2 = se2p_exp_neg(x, etab);
     const double ir2 = rcp_nr(r2);
-    const double fb = -4.0 * (1.0 + x) * ir2 * ir2;
-    const double pre = 2.0 * xi2;
-    const double d11 = d1 * d1, d12 = d1 * d2, d22 = d2 * d2;
-    const double t111 = e2 * (d11 * d1 * fb + pre * 3.0 * d1);
-    const double t112 = e2 * (d11 * d2 * fb + pre * d2);
-    const double t122 = e2 * (d12 * d2 * fb + pre * d1);
-    const double t222 = e2 * (d22 * d2 * fb + pre * 3.0 * d2);
+    const double g = -4.0 * e2 * (1.0 + x) * (ir2 * ir2);
+    const double pe = 2.0 * xi2 * e2;
+    const double d12 = d1 * d2;
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
         const double s0 = q[(3 * r) * qs], s1 = q[(3 * r + 1) * qs], s2 = q[(3 * r + 2) * qs];
-        acc[r][0] += t111 * s0 + t112 * s1 + t122 * s2;
-        acc[r][1] += t112 * s0 + t122 * s1 + t222 * s2;
+        const double gq = g * (d11 * s0 + d12 * s1 + d22 * s2);
+        const double l1 = d1 * (3.0 * s0 + s2) + d2 * s1;
+        const double l2 = d1 * s1 + d2 * (s0 + 3.0 * s2);
+        acc[r][0] += gq * d1 + pe * l1;
+        acc[r][1] += gq * d2 + pe * l2;
     }
 }
 
@@ -133,25 +165,112 @@ __device__ __forceinline__ void diff(double sx, double sy, double tx, double ty,
     }
 }
 
+// ---- work order and pruning helpers ---------------------------------------
+// Workgroup -> chunk such that XCD x (workgroups x, x+8, ...) takes chunks
+// [x*per, (x+1)*per).  The launch grid is a multiple of 8 and at least the
+// chunk count.  Returns -1 for workgroups without a chunk.
+__device__ __forceinline__ int xcd_chunk(int nvalid)
+{
+    const int per = (nvalid + 7) >> 3;
+    const int slot = blockIdx.x >> 3;
+    const int b = (blockIdx.x & 7) * per + slot;
+    return (slot < per && b < nvalid) ? b : -1;
+}
+
+// Bounding box of the wavefront's active lanes.
+__device__ __forceinline__ void wave_bbox(bool active, double x, double y, double &x0, double &y0,
+                                          double &x1, double &y1)
+{
+    const double inf = __builtin_inf();
+    x0 = active ? x : inf;
+    y0 = active ? y : inf;
+    x1 = active ? x : -inf;
+    y1 = active ? y : -inf;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        x0 = fmin(x0, __shfl_xor(x0, d));
+        y0 = fmin(y0, __shfl_xor(y0, d));
+        x1 = fmax(x1, __shfl_xor(x1, d));
+        y1 = fmax(y1, __shfl_xor(y1, d));
+    }
+}
+
+// Runs (sub-cells) of neighbour cells 4*rd .. 4*rd+3 that can hold a point
+// within the cutoff of the box (x0, y0, x1, y1): bit 16*(j-4*rd) + sub.
+// Lane l tests neighbour 4*rd + l/16, sub-cell l%16, from the run's bounding
+// box shifted by the image offset.
+static_assert(kSubLog == 2, "one lane per (neighbour, sub-cell) of 4 neighbours");
+__device__ __forceinline__ unsigned long long keep_mask(const double4 *__restrict__ box, int rd,
+                                                        int lane, int cbase, int cx, int cy, int nsx,
+                                                        int nsy, double Lx, double Ly, double lim,
+                                                        double x0, double y0, double x1, double y1)
+{
+    const int j = 4 * rd + (lane >> 4), sub = lane & 15;
+    const int qx = cx + nb_dx(j), qy = cy + nb_dy(j);
+    const int wx = (qx + nsx) % nsx, wy = (qy + nsy) % nsy;
+    const double zx = (Lx * (qx - wx)) / nsx, zy = (Ly * (qy - wy)) / nsy;
+    const double4 r = box[((cbase + wy * nsx + wx) << 4) + sub];
+    const double gx = fmax(fmax(r.x + zx - x1, x0 - (r.z + zx)), 0.0);
+    const double gy = fmax(fmax(r.y + zy - y1, y0 - (r.w + zy)), 0.0);
+    return __ballot(gx * gx + gy * gy < lim);
+}
+
+// Bounding box (xmin, ymin, xmax, ymax) of every key run of sorted points;
+// empty runs get (+inf, +inf, -inf, -inf), which no box is close to.  One
+// wavefront per run (runs are short but clustered points make some long).
+__global__ __launch_bounds__(256) void k_run_bbox(const double2 *__restrict__ pos,
+                                                  const int *__restrict__ off, int nkeys,
+                                                  double4 *__restrict__ box)
+{
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= nkeys) return;   // uniform over the wavefront
+    const double inf = __builtin_inf();
+    double x0 = inf, y0 = inf, x1 = -inf, y1 = -inf;
+    for (int i = off[k] + lane; i < off[k + 1]; i += 64) {
+        const double2 p = pos[i];
+        x0 = fmin(x0, p.x);
+        y0 = fmin(y0, p.y);
+        x1 = fmax(x1, p.x);
+        y1 = fmax(y1, p.y);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        x0 = fmin(x0, __shfl_xor(x0, d));
+        y0 = fmin(y0, __shfl_xor(y0, d));
+        x1 = fmax(x1, __shfl_xor(x1, d));
+        y1 = fmax(y1, __shfl_xor(y1, d));
+    }
+    if (lane == 0) box[k] = make_double4(x0, y0, x1, y1);
+}
+
 // ---- the cell-list kernel ------------------------------------------------
 template <int KIND, int RB>
 __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restrict__ spos,
                                               const double *__restrict__ qsrc,
                                               const int *__restrict__ soff,
-                                              double *__restrict__ uout)
+                                              const double4 *__restrict__ sbox)
 {
     __shared__ double e1tab[KIND == 0 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
+    __shared__ double etab[16];
     __shared__ double red[kWaves - 1][RB * 2][64];
     const int lane = threadIdx.x & 63;
     // wave index as a scalar so that the source loops and loads stay uniform
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (KIND == 0)
         for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
+    if (threadIdx.x < 16) {
+        const double t16[16] = SE2P_EXP2_TABLE;
+        etab[threadIdx.x] = t16[threadIdx.x];
+    }
     __syncthreads();
 
-    const int b = blockIdx.x;
-    if (b >= a.chunk_off[a.ncell]) return;   // uniform over the workgroup
+    const int b = xcd_chunk(a.chunk_off[a.ncell]);
+    if (b < 0) return;   // uniform over the workgroup
     const int cell = a.block_cell[b];
+    const RealBox &B = a.boxes[a.cell_box ? a.cell_box[cell] : 0];
+    const int nsx = B.nsx, nsy = B.nsy, cbase = B.cell_base;
+    const double Lx = B.Lx, Ly = B.Ly, cut2 = B.cut2;
     const int sub = 1 << (2 * kSubLog);
     const int t0 = a.toff[cell * sub] + 64 * (b - a.chunk_off[cell]);
     const int t1 = a.toff[(cell + 1) * sub];
@@ -167,7 +286,17 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
     }
     const int ncomp = KIND == 0 ? 2 : 3;
     const int64_t qs = a.ns;
-    const int cx = cell % a.nsx, cy = cell / a.nsx;
+    const int lcell = cell - cbase;
+    const int cx = lcell % nsx, cy = lcell / nsx;
+
+    // neighbour sub-cells within reach of this chunk's targets
+    double bx0, by0, bx1, by1;
+    wave_bbox(active, tx, ty, bx0, by0, bx1, by1);
+    const double lim = cut2 * (1.0 + 1e-9);
+    const unsigned long long keep0 =
+        keep_mask(sbox, 0, lane, cbase, cx, cy, nsx, nsy, Lx, Ly, lim, bx0, by0, bx1, by1);
+    const unsigned long long keep1 =
+        keep_mask(sbox, 1, lane, cbase, cx, cy, nsx, nsy, Lx, Ly, lim, bx0, by0, bx1, by1);
 
     // blockIdx.y selects this workgroup's group of RB right-hand sides
     for (int r0 = a.r_begin + blockIdx.y * RB; r0 < a.r_begin + (blockIdx.y + 1) * RB &&
@@ -183,40 +312,51 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
 #pragma unroll 2
             for (int s = soff[cell * sub] + wave; s < s1; s += kWaves) {
                 const double2 sp = spos[s];
-                double d1, d2;
+                double d1, d2, d11, d22;
                 diff<KIND>(sp.x, sp.y, tx, ty, 0.0, 0.0, d1, d2);
-                const double r2 = se2p_sqsum(d1, d2);
+                const double r2 = se2p_sqsum2(d1, d2, &d11, &d22);
                 if (r2 != 0.0) {
-                    if (KIND == 0) slp_pair<RB>(d1, d2, r2, a.xi, e1tab, qb + s, qs, acc);
-                    else dlp_pair<RB>(d1, d2, r2, a.xi2, qb + s, qs, acc);
+                    if (KIND == 0) slp_pair<RB>(d1, d2, r2, B.xi, e1tab, etab, qb + s, qs, acc);
+                    else dlp_pair<RB>(d1, d2, d11, d22, r2, B.xi2, etab, qb + s, qs, acc);
                 } else if (KIND == 0) {
 #pragma unroll
                     for (int r = 0; r < RB; ++r) {
-                        acc[r][0] += a.self * qb[(2 * r) * qs + s];
-                        acc[r][1] += a.self * qb[(2 * r + 1) * qs + s];
+                        acc[r][0] += B.self * qb[(2 * r) * qs + s];
+                        acc[r][1] += B.self * qb[(2 * r + 1) * qs + s];
                     }
                 }
             }
         }
-        // eight neighbour cells, periodic image offsets, r^2 < cut2
+        // eight neighbour cells, periodic image offsets, r^2 < cut2; the kept
+        // runs of a cell are visited in order, wave w taking every 4th source
+        // of their concatenation
+        int cnt = 0;
 #pragma unroll 1
         for (int j = 0; j < 8; ++j) {
+            unsigned bits = (unsigned)(((j < 4) ? keep0 : keep1) >> (16 * (j & 3))) & 0xffffu;
+            if (bits == 0) continue;
             const int qx = cx + nb_dx(j), qy = cy + nb_dy(j);
-            const int wx = (qx + a.nsx) % a.nsx, wy = (qy + a.nsy) % a.nsy;
-            const int sb = wy * a.nsx + wx;
-            const int s1 = soff[(sb + 1) * sub];
-            const double zx = (a.Lx * (qx - wx)) / a.nsx;   // :248-249
-            const double zy = (a.Ly * (qy - wy)) / a.nsy;
+            const int wx = (qx + nsx) % nsx, wy = (qy + nsy) % nsy;
+            const int sb = cbase + wy * nsx + wx;
+            const double zx = (Lx * (qx - wx)) / nsx;   // :248-249
+            const double zy = (Ly * (qy - wy)) / nsy;
+            while (bits) {
+                const int ra = __builtin_ctz(bits);
+                const int rb = ra + __builtin_ctz(~(bits >> ra));
+                bits &= ~((1u << rb) - 1u);
+                const int s0 = soff[sb * sub + ra], s1 = soff[sb * sub + rb];
 #pragma unroll 2
-            for (int s = soff[sb * sub] + wave; s < s1; s += kWaves) {
-                const double2 sp = spos[s];
-                double d1, d2;
-                diff<KIND>(sp.x, sp.y, tx, ty, zx, zy, d1, d2);
-                const double r2 = se2p_sqsum(d1, d2);
-                if (r2 < a.cut2) {
-                    if (KIND == 0) slp_pair<RB>(d1, d2, r2, a.xi, e1tab, qb + s, qs, acc);
-                    else dlp_pair<RB>(d1, d2, r2, a.xi2, qb + s, qs, acc);
+                for (int s = s0 + ((wave - cnt) & (kWaves - 1)); s < s1; s += kWaves) {
+                    const double2 sp = spos[s];
+                    double d1, d2, d11, d22;
+                    diff<KIND>(sp.x, sp.y, tx, ty, zx, zy, d1, d2);
+                    const double r2 = se2p_sqsum2(d1, d2, &d11, &d22);
+                    if (r2 < cut2) {
+                        if (KIND == 0) slp_pair<RB>(d1, d2, r2, B.xi, e1tab, etab, qb + s, qs, acc);
+                        else dlp_pair<RB>(d1, d2, d11, d22, r2, B.xi2, etab, qb + s, qs, acc);
+                    }
                 }
+                cnt += s1 - s0;
             }
         }
         // fixed-order reduction of the four partial sums
@@ -229,7 +369,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
         }
         __syncthreads();
         if (wave == 0 && active) {
-            const int o = a.tperm[t];
+            const int o = a.tperm[t] - B.t_base;
 #pragma unroll
             for (int r = 0; r < RB; ++r) {
                 double v0 = acc[r][0], v1 = acc[r][1];
@@ -238,7 +378,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
                     v0 += red[w][2 * r][lane];
                     v1 += red[w][2 * r + 1][lane];
                 }
-                double *ur = uout + (int64_t)(r0 + r) * a.ustride;
+                double *ur = B.u + (int64_t)(r0 + r) * a.ustride;
                 if (KIND == 0) {   // :344-345
                     ur[2 * o] = v0 / (4 * SE2P_PI);
                     ur[2 * o + 1] = v1 / (4 * SE2P_PI);
@@ -252,19 +392,22 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
     }
 }
 
-// Pair statistics: one wave per chunk, counts pairs evaluated / rejected.
+// Pair statistics: one wave per chunk, counts pairs evaluated / rejected
+// (every neighbour-cell pair the reference examines).
 __global__ __launch_bounds__(64) void k_real_count(RealArgs a)
 {
     const int lane = threadIdx.x;
     const int b = blockIdx.x;
     if (b >= a.chunk_off[a.ncell]) return;
     const int cell = a.block_cell[b];
+    const RealBox &B = a.boxes[a.cell_box ? a.cell_box[cell] : 0];
     const int sub = 1 << (2 * kSubLog);
     const int t = a.toff[cell * sub] + 64 * (b - a.chunk_off[cell]) + lane;
     if (t >= a.toff[(cell + 1) * sub]) return;
     const double2 p = a.tpos[t];
     unsigned long long n_in = 0, n_out = 0;
-    const int cx = cell % a.nsx, cy = cell / a.nsx;
+    const int lcell = cell - B.cell_base;
+    const int cx = lcell % B.nsx, cy = lcell / B.nsx;
     for (int s = a.soff[cell * sub]; s < a.soff[(cell + 1) * sub]; ++s) {
         double d1, d2;
         diff<1>(a.spos[s].x, a.spos[s].y, p.x, p.y, 0.0, 0.0, d1, d2);
@@ -272,14 +415,14 @@ __global__ __launch_bounds__(64) void k_real_count(RealArgs a)
     }
     for (int j = 0; j < 8; ++j) {
         const int qx = cx + nb_dx(j), qy = cy + nb_dy(j);
-        const int wx = (qx + a.nsx) % a.nsx, wy = (qy + a.nsy) % a.nsy;
-        const int sb = wy * a.nsx + wx;
-        const double zx = (a.Lx * (qx - wx)) / a.nsx;
-        const double zy = (a.Ly * (qy - wy)) / a.nsy;
+        const int wx = (qx + B.nsx) % B.nsx, wy = (qy + B.nsy) % B.nsy;
+        const int sb = B.cell_base + wy * B.nsx + wx;
+        const double zx = (B.Lx * (qx - wx)) / B.nsx;
+        const double zy = (B.Ly * (qy - wy)) / B.nsy;
         for (int s = a.soff[sb * sub]; s < a.soff[(sb + 1) * sub]; ++s) {
             double d1, d2;
             diff<1>(a.spos[s].x, a.spos[s].y, p.x, p.y, zx, zy, d1, d2);
-            if (se2p_sqsum(d1, d2) < a.cut2) ++n_in;
+            if (se2p_sqsum(d1, d2) < B.cut2) ++n_in;
             else ++n_out;
         }
     }
@@ -378,19 +521,25 @@ struct RealAdjArgs {
 template <int KIND, int SIDE>
 __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *__restrict__ upos,
                                                   const double *__restrict__ udata,
-                                                  const int *__restrict__ uoff)
+                                                  const int *__restrict__ uoff,
+                                                  const double4 *__restrict__ ubox)
 {
     constexpr int NV = SIDE == 0 ? 2 : (KIND == 0 ? 4 : 5);
     __shared__ double e1tab[KIND == 0 && SIDE == 1 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
+    __shared__ double etab[16];
     __shared__ double red[kWaves - 1][NV][64];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (KIND == 0 && SIDE == 1)
         for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
+    if (threadIdx.x < 16) {
+        const double t16[16] = SE2P_EXP2_TABLE;
+        etab[threadIdx.x] = t16[threadIdx.x];
+    }
     __syncthreads();
 
-    const int b = blockIdx.x;
-    if (b >= a.chunk_off[a.ncell]) return;   // uniform over the workgroup
+    const int b = xcd_chunk(a.chunk_off[a.ncell]);
+    if (b < 0) return;   // uniform over the workgroup
     const int cell = a.block_cell[b];
     const int sub = 1 << (2 * kSubLog);
     const int l0 = a.loff[cell * sub] + 64 * (b - a.chunk_off[cell]);
@@ -436,7 +585,7 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
             return;
         }
         const double x = r2 * a.xi2;
-        const double e = exp(-x);
+        const double e = se2p_exp_neg(x, etab);
         const double ir2 = rcp_nr(r2);
         if (SIDE == 0) {
             // lane: du_t; uniform: the source densities
@@ -471,18 +620,35 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
             pair(up.x, up.y, 0.0, 0.0, u, true);
         }
     }
+    // neighbour cells: only the uniform points' sub-cells within reach
+    double bx0, by0, bx1, by1;
+    wave_bbox(active, lx, ly, bx0, by0, bx1, by1);
+    const double lim = a.cut2 * (1.0 + 1e-9);
+    const unsigned long long keep0 =
+        keep_mask(ubox, 0, lane, 0, cx, cy, a.nsx, a.nsy, a.Lx, a.Ly, lim, bx0, by0, bx1, by1);
+    const unsigned long long keep1 =
+        keep_mask(ubox, 1, lane, 0, cx, cy, a.nsx, a.nsy, a.Lx, a.Ly, lim, bx0, by0, bx1, by1);
+    int cnt = 0;
 #pragma unroll 1
     for (int j = 0; j < 8; ++j) {
+        unsigned bits = (unsigned)(((j < 4) ? keep0 : keep1) >> (16 * (j & 3))) & 0xffffu;
+        if (bits == 0) continue;
         const int qx = cx + nb_dx(j), qy = cy + nb_dy(j);
         const int wx = (qx + a.nsx) % a.nsx, wy = (qy + a.nsy) % a.nsy;
         const int ub = wy * a.nsx + wx;
-        const int u1 = uoff[(ub + 1) * sub];
         const double zx = (a.Lx * (qx - wx)) / a.nsx;
         const double zy = (a.Ly * (qy - wy)) / a.nsy;
+        while (bits) {
+            const int ra = __builtin_ctz(bits);
+            const int rb = ra + __builtin_ctz(~(bits >> ra));
+            bits &= ~((1u << rb) - 1u);
+            const int u0 = uoff[ub * sub + ra], u1 = uoff[ub * sub + rb];
 #pragma unroll 2
-        for (int u = uoff[ub * sub] + wave; u < u1; u += kWaves) {
-            const double2 up = upos[u];
-            pair(up.x, up.y, zx, zy, u, false);
+            for (int u = u0 + ((wave - cnt) & (kWaves - 1)); u < u1; u += kWaves) {
+                const double2 up = upos[u];
+                pair(up.x, up.y, zx, zy, u, false);
+            }
+            cnt += u1 - u0;
         }
     }
     if (wave > 0)
@@ -536,6 +702,70 @@ __global__ void k_gather_pairs(const double *__restrict__ v, const int *__restri
     out[i] = v[2 * o];
     out[n + i] = v[2 * o + 1];
 }
+
+// ---- box-table gathers -----------------------------------------------------
+// Global point i belongs to box pt_box[i] (all to box 0 when pt_box is null).
+__device__ __forceinline__ int box_of(const int *pt_box, int i) { return pt_box ? pt_box[i] : 0; }
+
+// (global cell, Morton sub-cell) keys of sources (which = 0) or targets (1).
+__global__ void k_box_cell_keys(const RealBox *__restrict__ boxes, const int *__restrict__ pt_box,
+                                int n, int which, int *__restrict__ key)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const RealBox &B = boxes[box_of(pt_box, i)];
+    const int l = i - (which ? B.t_base : B.s_base);
+    const double *p = which ? B.ptar : B.psrc;
+    key[i] = (B.cell_base << (2 * kSubLog)) +
+             se2p_cell_key(p[2 * l], p[2 * l + 1], B.Lx, B.Ly, B.nsx, B.nsy, kSubLog);
+}
+
+// sources: positions and per-RHS densities in cell order
+template <int KIND>
+__global__ void k_box_gather_sources(const RealBox *__restrict__ boxes,
+                                     const int *__restrict__ pt_box, const int *__restrict__ perm,
+                                     int n, int nrhs, int64_t dstride, double2 *__restrict__ pos,
+                                     double *__restrict__ q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = perm[i];
+    const RealBox &B = boxes[box_of(pt_box, g)];
+    const int o = g - B.s_base;
+    pos[i] = make_double2(B.psrc[2 * o], B.psrc[2 * o + 1]);
+    if (KIND == 0) {
+        for (int r = 0; r < nrhs; ++r) {
+            const double *fr = B.f + (int64_t)r * dstride;
+            q[(int64_t)(2 * r) * n + i] = fr[2 * o];
+            q[(int64_t)(2 * r + 1) * n + i] = fr[2 * o + 1];
+        }
+    } else {
+        const double n1 = B.nv[2 * o], n2 = B.nv[2 * o + 1];
+        for (int r = 0; r < nrhs; ++r) {
+            const double *fr = B.f + (int64_t)r * dstride;
+            const double f1 = fr[2 * o], f2 = fr[2 * o + 1];
+            // S = f (x) n (mex_stokes_dlp_real.cpp:86-89); S12 + S21 merged
+            q[(int64_t)(3 * r) * n + i] = f1 * n1;
+            q[(int64_t)(3 * r + 1) * n + i] = f1 * n2 + f2 * n1;
+            q[(int64_t)(3 * r + 2) * n + i] = f2 * n2;
+        }
+    }
+}
+
+__global__ void k_box_gather_targets(const RealBox *__restrict__ boxes,
+                                     const int *__restrict__ pt_box, const int *__restrict__ perm,
+                                     int n, double2 *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = perm[i];
+    const RealBox &B = boxes[box_of(pt_box, g)];
+    const int o = g - B.t_base;
+    out[i] = make_double2(B.ptar[2 * o], B.ptar[2 * o + 1]);
+}
+
+// The one-box table written on the device (no host copy, so capturable).
+__global__ void k_set_box(RealBox b, RealBox *__restrict__ out) { *out = b; }
 
 // ---- gathers ------------------------------------------------------------
 __global__ void k_gather_targets(const double *__restrict__ p, const int *__restrict__ perm, int n,
@@ -606,73 +836,133 @@ static int upload_table(se2p_ctx *ctx)
 }
 
 // ---- host driver ----------------------------------------------------------
-int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
-             int nrhs, int64_t dstride, int64_t ustride, double *u, int64_t *pair_counts)
+// Real-space sum over nbox independent boxes in one pass: one sort of all
+// points by (global cell, sub-cell), one gather, one launch of the pair
+// kernel.  boxes[] are host records; their bases and derived parameters are
+// filled in here.
+int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, int64_t dstride,
+                   int64_t ustride, int64_t *pair_counts)
 {
     hipStream_t st = ctx->stream;
-    const int ncell = g.nsx * g.nsy;
     const int sub = 1 << (2 * kSubLog);
-    const int nkeys = ncell * sub;
-    const int64_t ns = g.ns, nt = g.nt;
+    int64_t ns = 0, nt = 0, ncell = 0;
+    for (int b = 0; b < nbox; ++b) {
+        RealBox &B = boxes[b];
+        B.s_base = (int)ns;
+        B.t_base = (int)nt;
+        B.cell_base = (int)ncell;
+        B.xi2 = B.xi * B.xi;
+        B.cut2 = B.Lx * B.Ly / B.nsx / B.nsy;                 // mex_stokes_slp_real.cpp:127
+        B.self = -(0.5 * SE2P_EULER_GAMMA + 1.0) - log(B.xi);  // :124
+        ns += B.ns;
+        nt += B.nt;
+        ncell += (int64_t)B.nsx * B.nsy;
+    }
+    if (ns > 0x7fffffffLL || nt > 0x7fffffffLL || ncell * sub > 0x7fffffffLL)
+        return fail(ctx, SE2P_ERR_ARG, "too many points or cells in one call");
     const bool counting = pair_counts != nullptr;
     if (nt == 0) return 0;
     if (kind == 0 && upload_table(ctx)) return SE2P_ERR_HIP;
     // targets that are the source array itself share the sources' cell lists
-    const bool same_points = (g.ptar == g.psrc) && (nt == ns);
-
+    const bool same_points = nbox == 1 && boxes[0].ptar == boxes[0].psrc && boxes[0].nt == boxes[0].ns;
+    const int nkeys = (int)(ncell * sub);
     const int ncomp = kind == 0 ? 2 : 3;
-    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t nsz = (size_t)(ns > 0 ? ns : 1);
-    const size_t ks_s = keysort_bytes(ns, nkeys);
-    const size_t ks_t = same_points ? 0 : keysort_bytes(nt, nkeys);
-    size_t need = ks_s + ks_t + up(sizeof(double2) * nsz) + up(sizeof(double2) * (size_t)nt) +
-                  up(sizeof(double) * (size_t)ncomp * nrhs * nsz) +
-                  2 * up(sizeof(int) * (size_t)(ncell + 1)) + up(2 * sizeof(unsigned long long)) +
-                  up(sizeof(int) * (size_t)((nt + 63) / 64 + ncell));
-    if (int rc = ensure_workspace(ctx, need)) return rc;
-    char *base = ctx->ws;
-    size_t o = 0;
-    KeySort kss, kst;
-    keysort_carve(base + o, ns, nkeys, &kss); o += ks_s;
-    if (!same_points) { keysort_carve(base + o, nt, nkeys, &kst); o += ks_t; }
-    double2 *spos = (double2 *)(base + o); o += up(sizeof(double2) * nsz);
-    double2 *tpos = (double2 *)(base + o); o += up(sizeof(double2) * (size_t)nt);
-    double *q = (double *)(base + o); o += up(sizeof(double) * (size_t)ncomp * nrhs * nsz);
-    int *ccnt = (int *)(base + o); o += up(sizeof(int) * (size_t)(ncell + 1));
-    int *coff = (int *)(base + o); o += up(sizeof(int) * (size_t)(ncell + 1));
-    unsigned long long *cnt = (unsigned long long *)(base + o); o += up(2 * sizeof(unsigned long long));
-    int *bmap = (int *)(base + o);
+    // upper bound on chunks (every non-empty cell wastes < 1 chunk); the
+    // cell-list kernels want a multiple of 8 workgroups (XCD order)
+    const int64_t max_chunks = (nt + 63) / 64 + ncell;
+    const unsigned grid8 = (unsigned)((max_chunks + 7) & ~int64_t(7));
 
+    KeySort kss, kst;
+    double2 *spos, *tpos;
+    double *q;
+    int *ccnt, *coff, *bmap, *offs = nullptr, *s_box = nullptr, *t_box = nullptr, *cell_box = nullptr;
+    unsigned long long *cnt;
+    double4 *sbox;
+    RealBox *dbox;
+    auto layout = [&](Carve &cv) {
+        char *p = cv.take<char>(keysort_bytes(ns, nkeys));
+        if (p) keysort_carve(p, ns, nkeys, &kss);
+        if (!same_points) {
+            p = cv.take<char>(keysort_bytes(nt, nkeys));
+            if (p) keysort_carve(p, nt, nkeys, &kst);
+        }
+        spos = cv.take<double2>(ns);
+        tpos = same_points ? spos : cv.take<double2>(nt);
+        q = cv.take<double>((size_t)ncomp * nrhs * ns);
+        ccnt = cv.take<int>(ncell + 1);
+        coff = cv.take<int>(ncell + 1);
+        cnt = cv.take<unsigned long long>(2);
+        bmap = cv.take<int>(max_chunks);
+        sbox = cv.take<double4>(nkeys);
+        dbox = cv.take<RealBox>(nbox);
+        if (nbox > 1) {
+            offs = cv.take<int>(3 * (size_t)(nbox + 1));
+            s_box = cv.take<int>(ns);
+            t_box = cv.take<int>(nt);
+            cell_box = cv.take<int>(ncell);
+        }
+    };
+    Carve sizing;
+    layout(sizing);
+    if (int rc = ensure_workspace(ctx, sizing.off)) return rc;
+    Carve cv;
+    cv.base = ctx->ws;
+    layout(cv);
+
+    stage_begin(ctx, ST_SORT);
+    const int threads = 256;
+    if (nbox == 1) {
+        hipLaunchKernelGGL(k_set_box, dim3(1), dim3(1), 0, st, boxes[0], dbox);
+    } else {
+        // box table and the source / target / cell offsets of every box
+        std::vector<char> &h = ctx->host_stage;
+        const size_t tb = sizeof(RealBox) * nbox, ob = sizeof(int) * 3 * (size_t)(nbox + 1);
+        h.resize(tb + ob);
+        std::memcpy(h.data(), boxes, tb);
+        int *ho = (int *)(h.data() + tb);
+        for (int b = 0; b <= nbox; ++b) {
+            ho[b] = b < nbox ? boxes[b].s_base : (int)ns;
+            ho[nbox + 1 + b] = b < nbox ? boxes[b].t_base : (int)nt;
+            ho[2 * (nbox + 1) + b] = b < nbox ? boxes[b].cell_base : (int)ncell;
+        }
+        SE2P_HIP(hipMemcpyAsync(dbox, h.data(), tb, hipMemcpyHostToDevice, st));
+        SE2P_HIP(hipMemcpyAsync(offs, h.data() + tb, ob, hipMemcpyHostToDevice, st));
+        expand_map(st, offs, nbox, s_box);
+        expand_map(st, offs + nbox + 1, nbox, t_box);
+        expand_map(st, offs + 2 * (nbox + 1), nbox, cell_box);
+    }
     // Sources and targets in (cell, Morton sub-cell) order, stable: a cell's
     // sources are summed in a fixed order on every run.
-    stage_begin(ctx, ST_SORT);
-    if (int rc = cell_sort(ctx, g.psrc, ns, g.Lx, g.Ly, g.nsx, g.nsy, kSubLog, &kss)) return rc;
-    if (!same_points)
-        if (int rc = cell_sort(ctx, g.ptar, nt, g.Lx, g.Ly, g.nsx, g.nsy, kSubLog, &kst)) return rc;
+    if (ns > 0)
+        hipLaunchKernelGGL(k_box_cell_keys, dim3((unsigned)((ns + threads - 1) / threads)),
+                           dim3(threads), 0, st, dbox, s_box, (int)ns, 0, kss.key);
+    if (int rc = keysort_run(ctx, &kss, ns, nkeys)) return rc;
+    if (!same_points) {
+        hipLaunchKernelGGL(k_box_cell_keys, dim3((unsigned)((nt + threads - 1) / threads)),
+                           dim3(threads), 0, st, dbox, t_box, (int)nt, 1, kst.key);
+        if (int rc = keysort_run(ctx, &kst, nt, nkeys)) return rc;
+    }
     const KeySort &tks = same_points ? kss : kst;
-    const int threads = 256;
     if (ns > 0) {
         const unsigned blocks = (unsigned)((ns + threads - 1) / threads);
         if (kind == 0)
-            hipLaunchKernelGGL(k_gather_sources<0>, dim3(blocks), dim3(threads), 0, st, g.psrc, f,
-                               n, kss.perm, (int)ns, nrhs, dstride, spos, q);
+            hipLaunchKernelGGL(k_box_gather_sources<0>, dim3(blocks), dim3(threads), 0, st, dbox,
+                               s_box, kss.perm, (int)ns, nrhs, dstride, spos, q);
         else
-            hipLaunchKernelGGL(k_gather_sources<1>, dim3(blocks), dim3(threads), 0, st, g.psrc, f,
-                               n, kss.perm, (int)ns, nrhs, dstride, spos, q);
+            hipLaunchKernelGGL(k_box_gather_sources<1>, dim3(blocks), dim3(threads), 0, st, dbox,
+                               s_box, kss.perm, (int)ns, nrhs, dstride, spos, q);
     }
-    if (same_points) {
-        tpos = spos;
-    } else {
-        hipLaunchKernelGGL(k_gather_targets, dim3((unsigned)((nt + threads - 1) / threads)),
-                           dim3(threads), 0, st, g.ptar, tks.perm, (int)nt, tpos);
-    }
-    hipLaunchKernelGGL(k_chunk_counts, dim3((ncell + threads - 1) / threads), dim3(threads), 0, st,
-                       tks.off, ncell, sub, ccnt);
-    scan_ints(st, ccnt, ncell, coff);
-    expand_map(st, coff, ncell, bmap);
+    if (!same_points)
+        hipLaunchKernelGGL(k_box_gather_targets, dim3((unsigned)((nt + threads - 1) / threads)),
+                           dim3(threads), 0, st, dbox, t_box, tks.perm, (int)nt, tpos);
+    hipLaunchKernelGGL(k_chunk_counts, dim3((unsigned)((ncell + threads - 1) / threads)),
+                       dim3(threads), 0, st, tks.off, (int)ncell, sub, ccnt);
+    scan_ints(st, ccnt, (int)ncell, coff);
+    expand_map(st, coff, (int)ncell, bmap);
+    hipLaunchKernelGGL(k_run_bbox, dim3((nkeys + 3) / 4), dim3(256), 0, st, spos, kss.off, nkeys, sbox);
     stage_end(ctx, ST_SORT);
 
-    RealArgs a;
+    RealArgs a{};
     a.tpos = tpos;
     a.tperm = tks.perm;
     a.toff = tks.off;
@@ -682,26 +972,17 @@ int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const 
     a.q = q;
     a.soff = kss.off;
     a.ns = ns;
-    a.nsx = g.nsx;
-    a.nsy = g.nsy;
-    a.ncell = ncell;
+    a.ncell = (int)ncell;
     a.nrhs = nrhs;
-    a.Lx = g.Lx;
-    a.Ly = g.Ly;
-    a.xi = g.xi;
-    a.xi2 = g.xi * g.xi;
-    a.cut2 = g.Lx * g.Ly / g.nsx / g.nsy;   // mex_stokes_slp_real.cpp:127
-    a.self = -(0.5 * SE2P_EULER_GAMMA + 1.0) - log(g.xi);   // :124
-    a.u = u;
     a.ustride = ustride;
+    a.boxes = dbox;
+    a.cell_box = cell_box;
     a.pair_counts = cnt;
 
-    // upper bound on chunks: every non-empty cell wastes < 1 chunk
-    const unsigned max_chunks = (unsigned)((nt + 63) / 64 + ncell);
     stage_begin(ctx, ST_REAL);
     if (counting) {
         SE2P_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(k_real_count, dim3(max_chunks), dim3(64), 0, st, a);
+        hipLaunchKernelGGL(k_real_count, dim3((unsigned)max_chunks), dim3(64), 0, st, a);
     } else {
         // groups of four right-hand sides share every pair evaluation; the
         // remainder goes one at a time
@@ -709,16 +990,16 @@ int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const 
         if (n4 > 0) {
             a.r_begin = 0;
             a.r_end = n4;
-            const dim3 grid(max_chunks, n4 / 4);
-            if (kind == 0) hipLaunchKernelGGL((k_real<0, 4>), grid, dim3(256), 0, st, a, spos, q, kss.off, u);
-            else hipLaunchKernelGGL((k_real<1, 4>), grid, dim3(256), 0, st, a, spos, q, kss.off, u);
+            const dim3 grid(grid8, n4 / 4);
+            if (kind == 0) hipLaunchKernelGGL((k_real<0, 4>), grid, dim3(256), 0, st, a, spos, q, kss.off, sbox);
+            else hipLaunchKernelGGL((k_real<1, 4>), grid, dim3(256), 0, st, a, spos, q, kss.off, sbox);
         }
         if (n4 < nrhs) {
             a.r_begin = n4;
             a.r_end = nrhs;
-            const dim3 grid(max_chunks, nrhs - n4);
-            if (kind == 0) hipLaunchKernelGGL((k_real<0, 1>), grid, dim3(256), 0, st, a, spos, q, kss.off, u);
-            else hipLaunchKernelGGL((k_real<1, 1>), grid, dim3(256), 0, st, a, spos, q, kss.off, u);
+            const dim3 grid(grid8, nrhs - n4);
+            if (kind == 0) hipLaunchKernelGGL((k_real<0, 1>), grid, dim3(256), 0, st, a, spos, q, kss.off, sbox);
+            else hipLaunchKernelGGL((k_real<1, 1>), grid, dim3(256), 0, st, a, spos, q, kss.off, sbox);
         }
     }
     stage_end(ctx, ST_REAL);
@@ -731,6 +1012,48 @@ int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const 
         pair_counts[1] = (int64_t)h[1];
     }
     return 0;
+}
+
+int real_sum_multibox(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int nbox)
+{
+    std::vector<RealBox> rb((size_t)nbox);
+    for (int i = 0; i < nbox; ++i) {
+        const se2p_box_desc &d = boxes[i];
+        RealBox &b = rb[(size_t)i];
+        b = RealBox{};
+        b.psrc = d.psrc;
+        b.ptar = d.ptar;
+        b.f = d.f;
+        b.nv = d.n;
+        b.u = d.u;
+        b.ns = (int)d.ns;
+        b.nt = (int)d.nt;
+        b.nsx = d.nside_x;
+        b.nsy = d.nside_y;
+        b.Lx = d.Lx;
+        b.Ly = d.Ly;
+        b.xi = d.xi;
+    }
+    return real_sum_boxes(ctx, kind, rb.data(), nbox, 1, 0, 0, nullptr);
+}
+
+int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
+             int nrhs, int64_t dstride, int64_t ustride, double *u, int64_t *pair_counts)
+{
+    RealBox b{};
+    b.psrc = g.psrc;
+    b.ptar = g.ptar;
+    b.f = f;
+    b.nv = n;
+    b.u = u;
+    b.ns = (int)g.ns;
+    b.nt = (int)g.nt;
+    b.nsx = g.nsx;
+    b.nsy = g.nsy;
+    b.Lx = g.Lx;
+    b.Ly = g.Ly;
+    b.xi = g.xi;
+    return real_sum_boxes(ctx, kind, &b, 1, nrhs, dstride, ustride, pair_counts);
 }
 
 int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
@@ -755,11 +1078,14 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
     const bool same_points = (g.ptar == g.psrc) && (nt == ns);
     const int ncomp = kind == 0 ? 2 : 3;
     const int64_t max_t = (nt + 63) / 64 + ncell, max_s = (ns + 63) / 64 + ncell;
+    const unsigned grid_t = (unsigned)((max_t + 7) & ~int64_t(7));   // XCD order: multiple of 8
+    const unsigned grid_s = (unsigned)((max_s + 7) & ~int64_t(7));
 
     KeySort kss, kst;
     double2 *spos, *tpos;
     double *q, *dut;
     int *ccnt, *coff_t, *coff_s, *bmap_t, *bmap_s;
+    double4 *sbox, *tbox;
     auto layout = [&](Carve &cv) {
         char *p = cv.take<char>(keysort_bytes(ns, nkeys));
         if (p) keysort_carve(p, ns, nkeys, &kss);
@@ -776,6 +1102,8 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         coff_s = cv.take<int>(ncell + 1);
         bmap_t = cv.take<int>(max_t);
         bmap_s = cv.take<int>(max_s);
+        sbox = cv.take<double4>(nkeys);
+        tbox = same_points ? sbox : cv.take<double4>(nkeys);
     };
     Carve sizing;
     layout(sizing);
@@ -802,6 +1130,13 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         hipLaunchKernelGGL(k_gather_targets, dim3(tblocks), dim3(threads), 0, st, g.ptar, tks.perm,
                            (int)nt, tpos);
     hipLaunchKernelGGL(k_gather_pairs, dim3(tblocks), dim3(threads), 0, st, du, tks.perm, (int)nt, dut);
+    const unsigned kblocks = (unsigned)((nkeys + 3) / 4);   // k_run_bbox: one wavefront per run
+    // run boxes of the uniform side: sources for the target pass, targets
+    // (the same array when targets are the sources) for the source pass
+    if (dptar || (want_src && same_points))
+        hipLaunchKernelGGL(k_run_bbox, dim3(kblocks), dim3(threads), 0, st, spos, kss.off, nkeys, sbox);
+    if (want_src && !same_points)
+        hipLaunchKernelGGL(k_run_bbox, dim3(kblocks), dim3(threads), 0, st, tpos, tks.off, nkeys, tbox);
     const unsigned cblocks = (unsigned)((ncell + threads - 1) / threads);
     if (dptar) {
         hipLaunchKernelGGL(k_chunk_counts, dim3(cblocks), dim3(threads), 0, st, tks.off, ncell, sub, ccnt);
@@ -840,9 +1175,9 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         t.nu = ns;
         t.dpos = dptar;
         if (kind == 0)
-            hipLaunchKernelGGL((k_real_adj<0, 0>), dim3((unsigned)max_t), dim3(256), 0, st, t, spos, q, kss.off);
+            hipLaunchKernelGGL((k_real_adj<0, 0>), dim3(grid_t), dim3(256), 0, st, t, spos, q, kss.off, sbox);
         else
-            hipLaunchKernelGGL((k_real_adj<1, 0>), dim3((unsigned)max_t), dim3(256), 0, st, t, spos, q, kss.off);
+            hipLaunchKernelGGL((k_real_adj<1, 0>), dim3(grid_t), dim3(256), 0, st, t, spos, q, kss.off, sbox);
     }
     if (want_src) {
         RealAdjArgs s = a;
@@ -858,9 +1193,9 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         s.df = df;
         s.dn = dn;
         if (kind == 0)
-            hipLaunchKernelGGL((k_real_adj<0, 1>), dim3((unsigned)max_s), dim3(256), 0, st, s, tpos, dut, tks.off);
+            hipLaunchKernelGGL((k_real_adj<0, 1>), dim3(grid_s), dim3(256), 0, st, s, tpos, dut, tks.off, tbox);
         else
-            hipLaunchKernelGGL((k_real_adj<1, 1>), dim3((unsigned)max_s), dim3(256), 0, st, s, tpos, dut, tks.off);
+            hipLaunchKernelGGL((k_real_adj<1, 1>), dim3(grid_s), dim3(256), 0, st, s, tpos, dut, tks.off, tbox);
     }
     stage_end(ctx, ST_REAL);
     SE2P_HIP(hipGetLastError());

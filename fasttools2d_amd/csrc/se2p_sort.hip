@@ -252,21 +252,7 @@ __global__ void k_cell_keys(const double *__restrict__ p, int n, double Lx, doub
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double x = p[2 * i], y = p[2 * i + 1];
-    int k = se2p_cell_of(x, y, Lx, Ly, nsx, nsy);
-    if (sublog > 0) {
-        const int sub = 1 << sublog;
-        const int bx = k % nsx, by = k / nsx;
-        int sx = (int)floor((nsx * (x / Lx + 0.5) - bx) * sub);
-        int sy = (int)floor((nsy * (y / Ly + 0.5) - by) * sub);
-        sx = sx < 0 ? 0 : (sx >= sub ? sub - 1 : sx);
-        sy = sy < 0 ? 0 : (sy >= sub ? sub - 1 : sy);
-        int m = 0;
-        for (int b = 0; b < sublog; ++b)
-            m |= (((sx >> b) & 1) << (2 * b)) | (((sy >> b) & 1) << (2 * b + 1));
-        k = (k << (2 * sublog)) | m;
-    }
-    key[i] = k;
+    key[i] = se2p_cell_key(p[2 * i], p[2 * i + 1], Lx, Ly, nsx, nsy, sublog);
 }
 
 int cell_sort(se2p_ctx *ctx, const double *p, int64_t n, double Lx, double Ly, int nsx, int nsy,

@@ -288,6 +288,71 @@ def stokes_dlp_kspace_batched(psrc, ptar, xi, eta, F, n, Mx, My, Lx, Ly, w, P, *
     return u
 
 
+# -- many independent boxes in one call -------------------------------------------
+_BOX_KEYS = ("psrc", "ptar", "f", "n", "Lx", "Ly", "xi", "nside_x", "nside_y", "eta", "w", "Mx",
+             "My", "P")
+
+
+def _multibox(name, boxes, dlp, ctx):
+    """boxes: sequence of dicts with keys among psrc, ptar, f, n (stresslet),
+    Lx, Ly, xi, nside_x, nside_y (real space), eta, w, Mx, My, P (Fourier
+    space).  Returns the list of (Nt, 2) results, one per box."""
+    arrays = [b[k] for b in boxes for k in ("psrc", "ptar", "f", "n") if b.get(k) is not None]
+    dev = _device_of(*arrays)
+    keep, outs = [], []
+    descs = (_capi.Se2pBoxDesc * max(1, len(boxes)))()
+    for i, b in enumerate(boxes):
+        unknown = set(b) - set(_BOX_KEYS)
+        if unknown:
+            raise ValueError(f"box {i}: unknown keys {sorted(unknown)}")
+        ps, pt = _as_points(b["psrc"], "psrc", dev), _as_points(b["ptar"], "ptar", dev)
+        ff = _as_points(b["f"], "f", dev)
+        if ff.shape[0] != ps.shape[0]:
+            raise ValueError("psrc and f must be the same size.")
+        nn = None
+        if dlp:
+            nn = _as_points(b["n"], "n", dev)
+            if nn.shape[0] != ps.shape[0]:
+                raise ValueError("psrc and n must be the same size.")
+        u = torch.empty((pt.shape[0], 2), dtype=torch.float64, device=dev)
+        keep += [ps, pt, ff, nn]
+        outs.append(u)
+        d = descs[i]
+        d.psrc, d.ns = _ptr(ps), ps.shape[0]
+        d.ptar, d.nt = _ptr(pt), pt.shape[0]
+        d.f = _ptr(ff)
+        d.n = _ptr(nn) if nn is not None else None
+        d.Lx, d.Ly = float(b["Lx"]), float(b["Ly"])
+        d.xi = float(b["xi"])
+        d.nside_x, d.nside_y = int(b.get("nside_x", 1)), int(b.get("nside_y", 1))
+        d.eta, d.w = float(b.get("eta", 0.0)), float(b.get("w", 0.0))
+        d.Mx, d.My, d.P = int(b.get("Mx", 0)), int(b.get("My", 0)), int(b.get("P", 0))
+        d.u = _ptr(u)
+    ctx = _prepare(ctx, dev)
+    ctx._check(getattr(ctx._lib, name)(ctx.handle, descs, len(boxes)))
+    return outs
+
+
+def stokes_slp_real_multibox(boxes, *, ctx=None):
+    """mex_stokes_slp_real on every box (list of box dicts), one fused launch."""
+    return _multibox("se2p_slp_real_multibox", boxes, False, ctx)
+
+
+def stokes_slp_kspace_multibox(boxes, *, ctx=None):
+    """mex_stokes_slp_kspace on every box."""
+    return _multibox("se2p_slp_kspace_multibox", boxes, False, ctx)
+
+
+def stokes_dlp_real_multibox(boxes, *, ctx=None):
+    """mex_stokes_dlp_real on every box, one fused launch."""
+    return _multibox("se2p_dlp_real_multibox", boxes, True, ctx)
+
+
+def stokes_dlp_kspace_multibox(boxes, *, ctx=None):
+    """mex_stokes_dlp_kspace on every box."""
+    return _multibox("se2p_dlp_kspace_multibox", boxes, True, ctx)
+
+
 # -- adjoints (backward entry points) --------------------------------------------
 _WRT_ALL = ("psrc", "ptar", "f", "n")
 

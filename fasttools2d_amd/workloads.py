@@ -152,3 +152,26 @@ def dlp_ellipses_config(seed: int = 2, n_ell: int = 200, n_nodes: int = 500,
     N = len(w.psrc)
     p = _params.select_params("dlp", N, N, w.f, w.Lx, w.Ly, tol=tol, M=M)
     return w, p
+
+
+def multibox_config(seed: int = 4, nbox: int = 32, L_range=(0.5, 4.0), N_range=(1e3, 2e5),
+                    tol: float = 1e-10, kind: str = "slp", P: int = 24):
+    """Many independent periodic boxes: square boxes with side L ~ U[0.5, 4],
+    N = round(exp(U[ln 1e3, ln 2e5])) uniform points each (targets = sources),
+    N(0,1) densities, unit normals at uniform angles, every box's Ewald
+    parameters from the driver formulas at tol.  Returns a list of box dicts
+    for the multi-box entry points."""
+    rng = np.random.default_rng(seed)
+    boxes = []
+    for _ in range(nbox):
+        L = float(rng.uniform(*L_range))
+        N = int(round(math.exp(rng.uniform(math.log(N_range[0]), math.log(N_range[1])))))
+        x = _params.wrap(rng.uniform(0.0, L, size=(N, 2)), L)
+        f = rng.standard_normal((N, 2))
+        th = rng.uniform(0.0, 2 * math.pi, N)
+        nv = np.stack([np.cos(th), np.sin(th)], axis=1)
+        p = _params.select_params(kind, N, N, f, L, L, P=P, tol=tol)
+        boxes.append({"psrc": x, "ptar": x, "f": f, "n": nv, "Lx": L, "Ly": L, "xi": p.xi,
+                      "nside_x": p.nside_x, "nside_y": p.nside_y, "eta": p.eta, "w": p.w,
+                      "Mx": p.Mx, "My": p.My, "P": p.P})
+    return boxes

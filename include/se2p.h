@@ -115,6 +115,32 @@ int se2p_dlp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const
                             int nrhs, int64_t f_stride, int Mx, int My, double Lx, double Ly,
                             double w, int P, double *u, int64_t u_stride);
 
+/* ---- many independent boxes in one call ------------------------------------ */
+/* One periodic problem per box: its own sources, targets, densities, box size
+ * and Ewald parameters (for instance many periodic cells of a simulation).
+ * Result b equals the single-box entry point's result on box b (the same
+ * kernels; to rounding).  All pointers are device arrays.  The real-space
+ * sums of all boxes run as one sort and one pair-kernel launch; the
+ * Fourier-space sums run box after box on the context stream.  The reference
+ * has no multi-box form; its callers loop over boxes. */
+typedef struct se2p_box_desc {
+    const double *psrc; int64_t ns;   /* Ns x 2 sources */
+    const double *ptar; int64_t nt;   /* Nt x 2 targets */
+    const double *f;                  /* Ns x 2 densities */
+    const double *n;                  /* Ns x 2 normals (stresslet; NULL for the Stokeslet) */
+    double Lx, Ly;
+    double xi;                        /* Ewald parameter */
+    int nside_x, nside_y;             /* real-space cells */
+    double eta, w;                    /* Fourier space: eta, Gaussian support half-width */
+    int Mx, My, P;                    /* Fourier space: grid and support */
+    double *u;                        /* Nt x 2 result (overwritten) */
+} se2p_box_desc;
+
+int se2p_slp_real_multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox);
+int se2p_slp_kspace_multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox);
+int se2p_dlp_real_multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox);
+int se2p_dlp_kspace_multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox);
+
 /* ---- adjoints (backward entry points) ------------------------------------ */
 /* Gradients of L = sum_t du_t . u_t, u the output of the matching forward
  * entry point on the same arguments (one density): dL/dpsrc and dL/df (Ns x 2),

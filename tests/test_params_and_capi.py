@@ -208,3 +208,23 @@ def test_backward_validation_codes(hctx):
     # valid arguments, null outputs allowed: reach the device step
     assert lib.se2p_slp_real_backward(h, x, 4, x, 4, x, 10.0, 3, 3, L, L, x, None, x, None) == _capi.SE2P_ERR_HIP
     assert lib.se2p_dlp_kspace_backward(h, x, 4, x, 4, 10.0, 0.5, x, x, M, M, L, L, w, P, x, x, None, None, x) == _capi.SE2P_ERR_HIP
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of se2p_params and se2p_box_desc have the C layout."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "se2p.h"\n'
+        'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(se2p_params), sizeof(se2p_box_desc),'
+        ' offsetof(se2p_box_desc, Lx), offsetof(se2p_box_desc, Mx), offsetof(se2p_box_desc, u));return 0;}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [ctypes.sizeof(_capi.Se2pParams), ctypes.sizeof(_capi.Se2pBoxDesc),
+            _capi.Se2pBoxDesc.Lx.offset, _capi.Se2pBoxDesc.Mx.offset, _capi.Se2pBoxDesc.u.offset]
+    assert got == want, (got, want)
