@@ -306,8 +306,10 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a)
 // a fixed order at the end (deterministic).
 constexpr int kPad = 3;
 
-// sources per wavefront batch: fewer when many grids share the LDS
-__host__ __device__ constexpr int strip_batch(int NQ) { return NQ <= 4 ? 32 : 16; }
+// sources per wavefront batch.  The four batch areas set the workgroup's LDS
+// (about 30 KB at P = 24), which keeps five workgroups (five waves per SIMD)
+// resident per CU; more grids per pass leave room for fewer sources.
+__host__ __device__ constexpr int strip_batch(int NQ) { return NQ <= 4 ? 16 : 8; }
 
 __host__ __device__ inline size_t strip_lds_doubles(int P, int NQ)
 {
