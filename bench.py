@@ -7,7 +7,7 @@ smooth densities, Ewald parameters from the reference driver's formulas at
 tol 1e-10 with the grid fixed at 512^2 (P = 24).  One step = the real-space sum
 plus the Fourier-space sum (the two kernels the reference driver calls,
 mex_stokes_dlp_real + mex_stokes_dlp_kspace) over all targets, inputs resident
-in HBM.
+in HBM.  The targets are the sources and are passed as the same array.
 
 Prints ONE JSON line: target evaluations per second (one evaluation = one
 2-vector u(x_t) for one density), the roofline of the dominant kernel (the
@@ -197,7 +197,12 @@ def main() -> int:
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    ps, pt, f, nv = t(w.psrc), t(w.ptar), t(w.f), t(w.n)
+    ps, f, nv = t(w.psrc), t(w.f), t(w.n)
+    # the targets are the sources: one array, as a caller evaluating the
+    # potential on its own points passes it (the library then builds one set
+    # of cell lists and bins for both)
+    assert np.array_equal(w.ptar, w.psrc)
+    pt = ps
     ur = torch.empty((N, 2), dtype=torch.float64, device=dev)
     uk = torch.empty((N, 2), dtype=torch.float64, device=dev)
 

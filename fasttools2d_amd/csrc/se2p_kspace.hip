@@ -297,6 +297,14 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a)
 }
 
 
+// LDS writes of a wavefront visible to all its lanes (no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Strip variant (grids with Mx > P and My > P + kPad): a lane owns a column
 // strip of 4 nodes (X, Ys..Ys+3) of the 16x16 tile, so the x factor and the
 // window test are shared by 4 nodes, and the y factors are zero-padded by
@@ -381,15 +389,19 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
 #pragma unroll
         for (int c = 0; c < NQ; ++c) acc[k][c] = 0.0;
 
+    // Wavefront w takes batches w, w+4, ... of the chunk.  Each wave fills and
+    // reads only its own batch area, so the waves run independently, with
+    // wavefront-scope fences around the fill instead of workgroup barriers.
     const int nbatch = (c1 - c0 + kSB - 1) / kSB;
-    const int rounds = (nbatch + 3) / 4;
-    for (int round = 0; round < rounds; ++round) {
-        const int base = c0 + (round * 4 + wave) * kSB;
-        if (base < c1 && lane < 2 * kSB) {
+    int rcur = 0;   // candidate run holding this wave's next batch start (runs ascend)
+    for (int bt = wave; bt < nbatch; bt += 4) {
+        const int base = c0 + bt * kSB;
+        while (rpref[rcur + 1] <= base) ++rcur;
+        if (lane < 2 * kSB) {
             const int sl = lane & (kSB - 1);
             const int v = base + sl;
             if (v < c1) {
-                int r = 0;
+                int r = rcur;
                 while (rpref[r + 1] <= v) ++r;
                 const int k = rstart[r] + (v - rpref[r]);
                 if (lane < kSB) {
@@ -417,29 +429,28 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
                 }
             }
         }
-        __syncthreads();
-        if (base < c1) {
-            const int nb = min(kSB, c1 - base);
+        wave_lds_sync();
+        const int nb = min(kSB, c1 - base);
 #pragma unroll 2
-            for (int s = 0; s < nb; ++s) {
-                int i = X - AXY[2 * s];
-                i += (i < 0) ? a.Mx : 0;
-                int j0 = Ys - AXY[2 * s + 1];
-                j0 += (j0 < -kPad) ? a.My : 0;
-                if (i <= a.P && j0 <= a.P) {
-                    const double wx = WX[s * P1 + i];
-                    const double *wy = WY + s * PW + kPad + j0;
+        for (int s = 0; s < nb; ++s) {
+            int i = X - AXY[2 * s];
+            i += (i < 0) ? a.Mx : 0;
+            int j0 = Ys - AXY[2 * s + 1];
+            j0 += (j0 < -kPad) ? a.My : 0;
+            if (i <= a.P && j0 <= a.P) {
+                const double wx = WX[s * P1 + i];
+                const double *wy = WY + s * PW + kPad + j0;
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const double wgt = wx * wy[k];
+                for (int k = 0; k < 4; ++k) {
+                    const double wgt = wx * wy[k];
 #pragma unroll
-                        for (int c = 0; c < NQ; ++c) acc[k][c] += wgt * Q[s * NQ + c];
-                    }
+                    for (int c = 0; c < NQ; ++c) acc[k][c] += wgt * Q[s * NQ + c];
                 }
             }
         }
-        __syncthreads();
+        wave_lds_sync();   // reads of this batch done before the next fill
     }
+    __syncthreads();   // every wave is done with its batch area (reused below)
     // fixed-order reduction of the four wavefronts' tiles, one wave at a time
     double *red = lds;   // [64][4*NQ]; the batch areas are dead now
     for (int w = 1; w < 4; ++w) {

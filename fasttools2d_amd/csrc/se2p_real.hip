@@ -99,51 +99,83 @@ struct RealArgs {
 // Stokeslet pair, before 1/4pi: 0.5 E1(xi^2 r^2) f + e^{-xi^2 r^2}((d.f) d/r^2 - f),
 // arranged as in mex_stokes_slp_real.cpp:207-225: c = d1 f2 - d2 f1,
 // u1 += E/2 f1 + (e/r2) d2 c,  u2 += E/2 f2 - (e/r2) d1 c.
+// Stokeslet pair geometry factors: hE = E1(xi^2 r^2)/2, g = e^{-xi^2 r^2}/r^2.
+__device__ __forceinline__ void slp_factors(double r2, double xi, const double *e1tab,
+                                            const double *etab, double &hE, double &g)
+{
+    const double x = r2 * xi * xi;
+    const double e = se2p_exp_neg(x, etab);
+    hE = 0.5 * se2p_e1(x, e, e1tab);
+    g = e * rcp_nr(r2);
+}
+
+// one density (f1, f2): u1 += E/2 f1 + (e/r2) d2 c, u2 += E/2 f2 - (e/r2) d1 c
+__device__ __forceinline__ void slp_add(double d1, double d2, double hE, double g, double f1,
+                                        double f2, double &a1, double &a2)
+{
+    const double c = d1 * f2 - d2 * f1;
+    a1 += hE * f1 + (g * d2) * c;
+    a2 += hE * f2 - (g * d1) * c;
+}
+
 template <int RB>
 __device__ __forceinline__ void slp_pair(double d1, double d2, double r2, double xi,
                                          const double *e1tab, const double *etab, const double *q,
                                          int64_t qs, double (&acc)[RB][2])
 {
-    const double x = r2 * xi * xi;
-    const double e = se2p_exp_neg(x, etab);
-    const double E = se2p_e1(x, e, e1tab);
-    const double g = e * rcp_nr(r2);
-    const double hE = 0.5 * E;
-    const double g1 = g * d1, g2 = g * d2;
+    double hE, g;
+    slp_factors(r2, xi, e1tab, etab, hE, g);
 #pragma unroll
-    for (int r = 0; r < RB; ++r) {
-        const double f1 = q[(2 * r) * qs], f2 = q[(2 * r + 1) * qs];
-        const double c = d1 * f2 - d2 * f1;
-        acc[r][0] += hE * f1 + g2 * c;
-        acc[r][1] += hE * f2 - g1 * c;
-    }
+    for (int r = 0; r < RB; ++r)
+        slp_add(d1, d2, hE, g, q[(2 * r) * qs], q[(2 * r + 1) * qs], acc[r][0], acc[r][1]);
 }
 
-// Stresslet pair, before 1/4pi (mex_stokes_dlp_real.cpp:125-138).  q holds
-// (S11, S12+S21, S22) with S = f (x) n.  The tensor contraction
-//   u_i += e2 [fb d_i d_j d_k + 2 xi^2 (δij dk + δik dj + δjk di)] S_jk
-// is evaluated as u_i += g d_i (d.S d) + p l_i with g = e2 fb, p = 2 xi^2 e2,
-// l_1 = d1 (3 S11 + S22) + d2 (S12+S21), l_2 = d1 (S12+S21) + d2 (S11 + 3 S22);
-// d1^2 and d2^2 come from the r^2 computation.
+// One right-hand side: each source is one record of SrcRec<KIND>::n doubles
+// (x, y and its densities: (f1, f2), or the five stresslet values and a pad),
+// fetched by one scalar load.
+template <int KIND>
+struct SrcRec {
+    static constexpr int n = KIND == 0 ? 4 : 8;
+};
+
+// q holds, per right-hand side, the five source values
+// (S11, S12+S21, S22, 3 S11 + S22, S11 + 3 S22) (kDlpComp).
+constexpr int kDlpComp = 5;
+
+// Stresslet pair geometry factors: g = e2 fb, pe = 2 xi^2 e2.
+__device__ __forceinline__ void dlp_factors(double r2, double xi2, const double *etab, double &g,
+                                            double &pe)
+{
+    const double x = xi2 * r2;
+    const double e2 = se2p_exp_neg(x, etab);
+    const double ir2 = rcp_nr(r2);
+    g = -4.0 * e2 * (1.0 + x) * (ir2 * ir2);
+    pe = 2.0 * xi2 * e2;
+}
+
+// one density: u_i += g d_i (d.S d) + pe l_i
+__device__ __forceinline__ void dlp_add(double d1, double d2, double d11, double d12, double d22,
+                                        double g, double pe, double s0, double s1, double s2,
+                                        double c1, double c2, double &a1, double &a2)
+{
+    const double gq = g * (d11 * s0 + d12 * s1 + d22 * s2);
+    a1 += gq * d1 + pe * (d1 * c1 + d2 * s1);
+    a2 += gq * d2 + pe * (d1 * s1 + d2 * c2);
+}
+
 template <int RB>
 __device__ __forceinline__ void dlp_pair(double d1, double d2, double d11, double d22, double r2,
                                          double xi2, const double *etab, const double *q,
                                          int64_t qs, double (&acc)[RB][2])
 {
-    const double x = xi2 * r2;
-    const double e2 = se2p_exp_neg(x, etab);
-    const double ir2 = rcp_nr(r2);
-    const double g = -4.0 * e2 * (1.0 + x) * (ir2 * ir2);
-    const double pe = 2.0 * xi2 * e2;
+    double g, pe;
+    dlp_factors(r2, xi2, etab, g, pe);
     const double d12 = d1 * d2;
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
-        const double s0 = q[(3 * r) * qs], s1 = q[(3 * r + 1) * qs], s2 = q[(3 * r + 2) * qs];
-        const double gq = g * (d11 * s0 + d12 * s1 + d22 * s2);
-        const double l1 = d1 * (3.0 * s0 + s2) + d2 * s1;
-        const double l2 = d1 * s1 + d2 * (s0 + 3.0 * s2);
-        acc[r][0] += gq * d1 + pe * l1;
-        acc[r][1] += gq * d2 + pe * l2;
+        const double *qr = q + (kDlpComp * r) * qs;
+        dlp_add(d1, d2, d11, d12, d22, g, pe, qr[0], qr[qs], qr[2 * qs], qr[3 * qs], qr[4 * qs],
+                acc[r][0], acc[r][1]);
     }
 }
 
@@ -245,9 +277,10 @@ __global__ __launch_bounds__(256) void k_run_bbox(const double2 *__restrict__ po
 }
 
 // ---- the cell-list kernel ------------------------------------------------
-template <int KIND, int RB>
+template <int KIND, int RB, bool REC>
 __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restrict__ spos,
                                               const double *__restrict__ qsrc,
+                                              const double *__restrict__ srec,
                                               const int *__restrict__ soff,
                                               const double4 *__restrict__ sbox)
 {
@@ -271,6 +304,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
     const RealBox &B = a.boxes[a.cell_box ? a.cell_box[cell] : 0];
     const int nsx = B.nsx, nsy = B.nsy, cbase = B.cell_base;
     const double Lx = B.Lx, Ly = B.Ly, cut2 = B.cut2;
+    const double xi = B.xi, xi2 = B.xi2, self = B.self;
     const int sub = 1 << (2 * kSubLog);
     const int t0 = a.toff[cell * sub] + 64 * (b - a.chunk_off[cell]);
     const int t1 = a.toff[(cell + 1) * sub];
@@ -284,7 +318,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
     } else {
         tx = ty = __builtin_nan("");   // inactive lanes compute garbage, never stored
     }
-    const int ncomp = KIND == 0 ? 2 : 3;
+    const int ncomp = KIND == 0 ? 2 : kDlpComp;
     const int64_t qs = a.ns;
     const int lcell = cell - cbase;
     const int cx = lcell % nsx, cy = lcell / nsx;
@@ -305,27 +339,75 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
 #pragma unroll
         for (int r = 0; r < RB; ++r) acc[r][0] = acc[r][1] = 0.0;
         const double *qb = qsrc + (int64_t)r0 * ncomp * qs;
+        // one source: its record (single density) or position + density array
+        auto pair = [&](int s, double zx, double zy, bool own) {
+            double sx, sy, q0[kDlpComp];
+            if (REC) {
+                const double *rp = srec + (int64_t)s * SrcRec<KIND>::n;
+                const double4 v0 = *(const double4 *)rp;
+                sx = v0.x;
+                sy = v0.y;
+                q0[0] = v0.z;
+                q0[1] = v0.w;
+                if (KIND == 1) {
+                    const double4 v1 = *(const double4 *)(rp + 4);
+                    q0[2] = v1.x;
+                    q0[3] = v1.y;
+                    q0[4] = v1.z;
+                }
+            } else {
+                const double2 sp = spos[s];
+                sx = sp.x;
+                sy = sp.y;
+            }
+            double d1, d2;
+            if (own) {
+                // own cell, no image: both kernels' difference is src - tgt
+                // (up to the sign of a zero, which only a skipped r^2 == 0 has)
+                d1 = sx - tx;
+                d2 = sy - ty;
+            } else {
+                diff<KIND>(sx, sy, tx, ty, zx, zy, d1, d2);
+            }
+            double d11, d22;
+            const double r2 = se2p_sqsum2(d1, d2, &d11, &d22);
+            if (own ? (r2 == 0.0) : !(r2 < cut2)) {
+                if (KIND == 0 && own) {   // coincident points: the self term
+                    if (REC) {
+                        acc[0][0] += self * q0[0];
+                        acc[0][1] += self * q0[1];
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < RB; ++r) {
+                            acc[r][0] += self * qb[(2 * r) * qs + s];
+                            acc[r][1] += self * qb[(2 * r + 1) * qs + s];
+                        }
+                    }
+                }
+                return;
+            }
+            if (REC) {
+                if (KIND == 0) {
+                    double hE, g;
+                    slp_factors(r2, xi, e1tab, etab, hE, g);
+                    slp_add(d1, d2, hE, g, q0[0], q0[1], acc[0][0], acc[0][1]);
+                } else {
+                    double g, pe;
+                    dlp_factors(r2, xi2, etab, g, pe);
+                    dlp_add(d1, d2, d11, d1 * d2, d22, g, pe, q0[0], q0[1], q0[2], q0[3], q0[4],
+                            acc[0][0], acc[0][1]);
+                }
+            } else {
+                if (KIND == 0) slp_pair<RB>(d1, d2, r2, xi, e1tab, etab, qb + s, qs, acc);
+                else dlp_pair<RB>(d1, d2, d11, d22, r2, xi2, etab, qb + s, qs, acc);
+            }
+        };
 
         // own cell: every pair, no cutoff
         {
             const int s1 = soff[(cell + 1) * sub];
 #pragma unroll 2
-            for (int s = soff[cell * sub] + wave; s < s1; s += kWaves) {
-                const double2 sp = spos[s];
-                double d1, d2, d11, d22;
-                diff<KIND>(sp.x, sp.y, tx, ty, 0.0, 0.0, d1, d2);
-                const double r2 = se2p_sqsum2(d1, d2, &d11, &d22);
-                if (r2 != 0.0) {
-                    if (KIND == 0) slp_pair<RB>(d1, d2, r2, B.xi, e1tab, etab, qb + s, qs, acc);
-                    else dlp_pair<RB>(d1, d2, d11, d22, r2, B.xi2, etab, qb + s, qs, acc);
-                } else if (KIND == 0) {
-#pragma unroll
-                    for (int r = 0; r < RB; ++r) {
-                        acc[r][0] += B.self * qb[(2 * r) * qs + s];
-                        acc[r][1] += B.self * qb[(2 * r + 1) * qs + s];
-                    }
-                }
-            }
+            for (int s = soff[cell * sub] + wave; s < s1; s += kWaves) pair(s, 0.0, 0.0, true);
         }
         // eight neighbour cells, periodic image offsets, r^2 < cut2; the kept
         // runs of a cell are visited in order, wave w taking every 4th source
@@ -346,16 +428,8 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
                 bits &= ~((1u << rb) - 1u);
                 const int s0 = soff[sb * sub + ra], s1 = soff[sb * sub + rb];
 #pragma unroll 2
-                for (int s = s0 + ((wave - cnt) & (kWaves - 1)); s < s1; s += kWaves) {
-                    const double2 sp = spos[s];
-                    double d1, d2, d11, d22;
-                    diff<KIND>(sp.x, sp.y, tx, ty, zx, zy, d1, d2);
-                    const double r2 = se2p_sqsum2(d1, d2, &d11, &d22);
-                    if (r2 < cut2) {
-                        if (KIND == 0) slp_pair<RB>(d1, d2, r2, B.xi, e1tab, etab, qb + s, qs, acc);
-                        else dlp_pair<RB>(d1, d2, d11, d22, r2, B.xi2, etab, qb + s, qs, acc);
-                    }
-                }
+                for (int s = s0 + ((wave - cnt) & (kWaves - 1)); s < s1; s += kWaves)
+                    pair(s, zx, zy, false);
                 cnt += s1 - s0;
             }
         }
@@ -721,18 +795,34 @@ __global__ void k_box_cell_keys(const RealBox *__restrict__ boxes, const int *__
 }
 
 // sources: positions and per-RHS densities in cell order
+// With one right-hand side (rec != null) the sources are written as records
+// (SrcRec) instead of the [nrhs][ncomp][n] density array.
 template <int KIND>
 __global__ void k_box_gather_sources(const RealBox *__restrict__ boxes,
                                      const int *__restrict__ pt_box, const int *__restrict__ perm,
                                      int n, int nrhs, int64_t dstride, double2 *__restrict__ pos,
-                                     double *__restrict__ q)
+                                     double *__restrict__ q, double *__restrict__ rec)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int g = perm[i];
     const RealBox &B = boxes[box_of(pt_box, g)];
     const int o = g - B.s_base;
-    pos[i] = make_double2(B.psrc[2 * o], B.psrc[2 * o + 1]);
+    const double x = B.psrc[2 * o], y = B.psrc[2 * o + 1];
+    pos[i] = make_double2(x, y);
+    if (rec) {
+        double *rp = rec + (int64_t)i * SrcRec<KIND>::n;
+        const double f1 = B.f[2 * o], f2 = B.f[2 * o + 1];
+        if (KIND == 0) {
+            *(double4 *)rp = make_double4(x, y, f1, f2);
+        } else {
+            const double n1 = B.nv[2 * o], n2 = B.nv[2 * o + 1];
+            const double s0 = f1 * n1, s1 = f1 * n2 + f2 * n1, s2 = f2 * n2;
+            *(double4 *)rp = make_double4(x, y, s0, s1);
+            *(double4 *)(rp + 4) = make_double4(s2, 3.0 * s0 + s2, s0 + 3.0 * s2, 0.0);
+        }
+        return;
+    }
     if (KIND == 0) {
         for (int r = 0; r < nrhs; ++r) {
             const double *fr = B.f + (int64_t)r * dstride;
@@ -744,10 +834,15 @@ __global__ void k_box_gather_sources(const RealBox *__restrict__ boxes,
         for (int r = 0; r < nrhs; ++r) {
             const double *fr = B.f + (int64_t)r * dstride;
             const double f1 = fr[2 * o], f2 = fr[2 * o + 1];
-            // S = f (x) n (mex_stokes_dlp_real.cpp:86-89); S12 + S21 merged
-            q[(int64_t)(3 * r) * n + i] = f1 * n1;
-            q[(int64_t)(3 * r + 1) * n + i] = f1 * n2 + f2 * n1;
-            q[(int64_t)(3 * r + 2) * n + i] = f2 * n2;
+            // S = f (x) n (mex_stokes_dlp_real.cpp:86-89); S12 + S21 merged,
+            // then the two combinations the pair kernel uses
+            const double s0 = f1 * n1, s1 = f1 * n2 + f2 * n1, s2 = f2 * n2;
+            double *qr = q + (int64_t)(kDlpComp * r) * n + i;
+            qr[0] = s0;
+            qr[(int64_t)n] = s1;
+            qr[2 * (int64_t)n] = s2;
+            qr[3 * (int64_t)n] = 3.0 * s0 + s2;
+            qr[4 * (int64_t)n] = s0 + 3.0 * s2;
         }
     }
 }
@@ -866,7 +961,7 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
     // targets that are the source array itself share the sources' cell lists
     const bool same_points = nbox == 1 && boxes[0].ptar == boxes[0].psrc && boxes[0].nt == boxes[0].ns;
     const int nkeys = (int)(ncell * sub);
-    const int ncomp = kind == 0 ? 2 : 3;
+    const int ncomp = kind == 0 ? 2 : kDlpComp;
     // upper bound on chunks (every non-empty cell wastes < 1 chunk); the
     // cell-list kernels want a multiple of 8 workgroups (XCD order)
     const int64_t max_chunks = (nt + 63) / 64 + ncell;
@@ -874,7 +969,7 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
 
     KeySort kss, kst;
     double2 *spos, *tpos;
-    double *q;
+    double *q = nullptr, *rec = nullptr;
     int *ccnt, *coff, *bmap, *offs = nullptr, *s_box = nullptr, *t_box = nullptr, *cell_box = nullptr;
     unsigned long long *cnt;
     double4 *sbox;
@@ -888,7 +983,8 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
         }
         spos = cv.take<double2>(ns);
         tpos = same_points ? spos : cv.take<double2>(nt);
-        q = cv.take<double>((size_t)ncomp * nrhs * ns);
+        if (nrhs == 1) rec = cv.take<double>((size_t)(kind == 0 ? SrcRec<0>::n : SrcRec<1>::n) * ns);
+        else q = cv.take<double>((size_t)ncomp * nrhs * ns);
         ccnt = cv.take<int>(ncell + 1);
         coff = cv.take<int>(ncell + 1);
         cnt = cv.take<unsigned long long>(2);
@@ -947,10 +1043,10 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
         const unsigned blocks = (unsigned)((ns + threads - 1) / threads);
         if (kind == 0)
             hipLaunchKernelGGL(k_box_gather_sources<0>, dim3(blocks), dim3(threads), 0, st, dbox,
-                               s_box, kss.perm, (int)ns, nrhs, dstride, spos, q);
+                               s_box, kss.perm, (int)ns, nrhs, dstride, spos, q, rec);
         else
             hipLaunchKernelGGL(k_box_gather_sources<1>, dim3(blocks), dim3(threads), 0, st, dbox,
-                               s_box, kss.perm, (int)ns, nrhs, dstride, spos, q);
+                               s_box, kss.perm, (int)ns, nrhs, dstride, spos, q, rec);
     }
     if (!same_points)
         hipLaunchKernelGGL(k_box_gather_targets, dim3((unsigned)((nt + threads - 1) / threads)),
@@ -991,15 +1087,20 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
             a.r_begin = 0;
             a.r_end = n4;
             const dim3 grid(grid8, n4 / 4);
-            if (kind == 0) hipLaunchKernelGGL((k_real<0, 4>), grid, dim3(256), 0, st, a, spos, q, kss.off, sbox);
-            else hipLaunchKernelGGL((k_real<1, 4>), grid, dim3(256), 0, st, a, spos, q, kss.off, sbox);
+            if (kind == 0) hipLaunchKernelGGL((k_real<0, 4, false>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
+            else hipLaunchKernelGGL((k_real<1, 4, false>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
         }
         if (n4 < nrhs) {
             a.r_begin = n4;
             a.r_end = nrhs;
             const dim3 grid(grid8, nrhs - n4);
-            if (kind == 0) hipLaunchKernelGGL((k_real<0, 1>), grid, dim3(256), 0, st, a, spos, q, kss.off, sbox);
-            else hipLaunchKernelGGL((k_real<1, 1>), grid, dim3(256), 0, st, a, spos, q, kss.off, sbox);
+            if (rec) {   // single density: source records
+                if (kind == 0) hipLaunchKernelGGL((k_real<0, 1, true>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
+                else hipLaunchKernelGGL((k_real<1, 1, true>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
+            } else {
+                if (kind == 0) hipLaunchKernelGGL((k_real<0, 1, false>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
+                else hipLaunchKernelGGL((k_real<1, 1, false>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
+            }
         }
     }
     stage_end(ctx, ST_REAL);
