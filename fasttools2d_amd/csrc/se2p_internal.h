@@ -114,6 +114,14 @@ int cell_sort(se2p_ctx *ctx, const double *p, int64_t n, double Lx, double Ly, i
 void scan_ints(hipStream_t st, const int *a, int n, int *out);
 // map[k] = c for every k in [off[c], off[c+1]), c < n.
 void expand_map(hipStream_t st, const int *off, int n, int *map);
+// One launch: out[0..n] = exclusive scan of the counts, map[k] = owner c of
+// chunk k.  Counts are cnt[c], or (cnt null) ceil(run length / per) of the
+// runs roff[c*sub] .. roff[(c+1)*sub].
+void plan_chunks(hipStream_t st, const int *cnt, const int *roff, int sub, int per, int n, int *out,
+                 int *map);
+// Two independent count arrays planned in one launch.
+void plan_chunks2(hipStream_t st, const int *cnt0, int n0, int *out0, int *map0, const int *cnt1,
+                  int n1, int *out1, int *map1);
 
 // ---- real space (se2p_real.hip) -----------------------------------------
 struct RealGeom {

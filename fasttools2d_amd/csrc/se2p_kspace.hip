@@ -1232,10 +1232,7 @@ static void tile_maps(hipStream_t st, const KLayout &L, const GridPass &gp, cons
 {
     hipLaunchKernelGGL(k_tile_work, dim3(L.ntiles), dim3(64), 0, st, sa, iboff, gp.scnt, gp.icnt,
                        gp.rng_start, gp.rng_len);
-    scan_ints(st, gp.scnt, L.ntiles, gp.soffc);
-    scan_ints(st, gp.icnt, L.ntiles, gp.ioffc);
-    expand_map(st, gp.soffc, L.ntiles, gp.smap);
-    expand_map(st, gp.ioffc, L.ntiles, gp.imap);
+    plan_chunks2(st, gp.scnt, L.ntiles, gp.soffc, gp.smap, gp.icnt, L.ntiles, gp.ioffc, gp.imap);
 }
 
 static int spread(se2p_ctx *ctx, const KLayout &L, const GridPass &gp, const SpreadArgs &sa)

@@ -76,6 +76,15 @@ struct RealBox {
     double Lx, Ly, xi, xi2, cut2, self;
 };
 
+// The box records as kernels see them: a device table for several boxes, or
+// the one record by value (no device copy, so a single-box call stays
+// capturable and needs no extra launch).
+struct BoxTable {
+    const RealBox *tab;   // [nbox] on the device, or null: one box, `one`
+    RealBox one;
+    __device__ __forceinline__ const RealBox &get(int i) const { return tab ? tab[i] : one; }
+};
+
 struct RealArgs {
     const double2 *tpos;   // targets in (cell, sub-cell) order
     const int *tperm;      // that order -> global target index
@@ -90,7 +99,7 @@ struct RealArgs {
     int nrhs;
     int r_begin, r_end;    // right-hand sides handled by this launch
     int64_t ustride;
-    const RealBox *boxes;  // [nbox]
+    BoxTable boxes;
     const int *cell_box;   // cell -> box (null: one box)
     unsigned long long *pair_counts;
 };
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
     const int b = xcd_chunk(a.chunk_off[a.ncell]);
     if (b < 0) return;   // uniform over the workgroup
     const int cell = a.block_cell[b];
-    const RealBox &B = a.boxes[a.cell_box ? a.cell_box[cell] : 0];
+    const RealBox &B = a.boxes.get(a.cell_box ? a.cell_box[cell] : 0);
     const int nsx = B.nsx, nsy = B.nsy, cbase = B.cell_base;
     const double Lx = B.Lx, Ly = B.Ly, cut2 = B.cut2;
     const double xi = B.xi, xi2 = B.xi2, self = B.self;
@@ -474,7 +483,7 @@ __global__ __launch_bounds__(64) void k_real_count(RealArgs a)
     const int b = blockIdx.x;
     if (b >= a.chunk_off[a.ncell]) return;
     const int cell = a.block_cell[b];
-    const RealBox &B = a.boxes[a.cell_box ? a.cell_box[cell] : 0];
+    const RealBox &B = a.boxes.get(a.cell_box ? a.cell_box[cell] : 0);
     const int sub = 1 << (2 * kSubLog);
     const int t = a.toff[cell * sub] + 64 * (b - a.chunk_off[cell]) + lane;
     if (t >= a.toff[(cell + 1) * sub]) return;
@@ -782,12 +791,12 @@ __global__ void k_gather_pairs(const double *__restrict__ v, const int *__restri
 __device__ __forceinline__ int box_of(const int *pt_box, int i) { return pt_box ? pt_box[i] : 0; }
 
 // (global cell, Morton sub-cell) keys of sources (which = 0) or targets (1).
-__global__ void k_box_cell_keys(const RealBox *__restrict__ boxes, const int *__restrict__ pt_box,
+__global__ void k_box_cell_keys(const BoxTable boxes, const int *__restrict__ pt_box,
                                 int n, int which, int *__restrict__ key)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const RealBox &B = boxes[box_of(pt_box, i)];
+    const RealBox &B = boxes.get(box_of(pt_box, i));
     const int l = i - (which ? B.t_base : B.s_base);
     const double *p = which ? B.ptar : B.psrc;
     key[i] = (B.cell_base << (2 * kSubLog)) +
@@ -798,7 +807,7 @@ __global__ void k_box_cell_keys(const RealBox *__restrict__ boxes, const int *__
 // With one right-hand side (rec != null) the sources are written as records
 // (SrcRec) instead of the [nrhs][ncomp][n] density array.
 template <int KIND>
-__global__ void k_box_gather_sources(const RealBox *__restrict__ boxes,
+__global__ void k_box_gather_sources(const BoxTable boxes,
                                      const int *__restrict__ pt_box, const int *__restrict__ perm,
                                      int n, int nrhs, int64_t dstride, double2 *__restrict__ pos,
                                      double *__restrict__ q, double *__restrict__ rec)
@@ -806,7 +815,7 @@ __global__ void k_box_gather_sources(const RealBox *__restrict__ boxes,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int g = perm[i];
-    const RealBox &B = boxes[box_of(pt_box, g)];
+    const RealBox &B = boxes.get(box_of(pt_box, g));
     const int o = g - B.s_base;
     const double x = B.psrc[2 * o], y = B.psrc[2 * o + 1];
     pos[i] = make_double2(x, y);
@@ -847,20 +856,18 @@ __global__ void k_box_gather_sources(const RealBox *__restrict__ boxes,
     }
 }
 
-__global__ void k_box_gather_targets(const RealBox *__restrict__ boxes,
+__global__ void k_box_gather_targets(const BoxTable boxes,
                                      const int *__restrict__ pt_box, const int *__restrict__ perm,
                                      int n, double2 *__restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int g = perm[i];
-    const RealBox &B = boxes[box_of(pt_box, g)];
+    const RealBox &B = boxes.get(box_of(pt_box, g));
     const int o = g - B.t_base;
     out[i] = make_double2(B.ptar[2 * o], B.ptar[2 * o + 1]);
 }
 
-// The one-box table written on the device (no host copy, so capturable).
-__global__ void k_set_box(RealBox b, RealBox *__restrict__ out) { *out = b; }
 
 // ---- gathers ------------------------------------------------------------
 __global__ void k_gather_targets(const double *__restrict__ p, const int *__restrict__ perm, int n,
@@ -900,15 +907,6 @@ __global__ void k_gather_sources(const double *__restrict__ p, const double *__r
             q[(int64_t)(3 * r + 2) * n + i] = f2 * n2;
         }
     }
-}
-
-// chunks of 64 targets per cell
-__global__ void k_chunk_counts(const int *__restrict__ toff, int ncell, int sub,
-                               int *__restrict__ cnt)
-{
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    cnt[c] = (toff[(c + 1) * sub] - toff[c * sub] + 63) / 64;
 }
 
 static int upload_table(se2p_ctx *ctx)
@@ -970,10 +968,10 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
     KeySort kss, kst;
     double2 *spos, *tpos;
     double *q = nullptr, *rec = nullptr;
-    int *ccnt, *coff, *bmap, *offs = nullptr, *s_box = nullptr, *t_box = nullptr, *cell_box = nullptr;
+    int *coff, *bmap, *offs = nullptr, *s_box = nullptr, *t_box = nullptr, *cell_box = nullptr;
     unsigned long long *cnt;
     double4 *sbox;
-    RealBox *dbox;
+    RealBox *dbox = nullptr;
     auto layout = [&](Carve &cv) {
         char *p = cv.take<char>(keysort_bytes(ns, nkeys));
         if (p) keysort_carve(p, ns, nkeys, &kss);
@@ -985,13 +983,12 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
         tpos = same_points ? spos : cv.take<double2>(nt);
         if (nrhs == 1) rec = cv.take<double>((size_t)(kind == 0 ? SrcRec<0>::n : SrcRec<1>::n) * ns);
         else q = cv.take<double>((size_t)ncomp * nrhs * ns);
-        ccnt = cv.take<int>(ncell + 1);
         coff = cv.take<int>(ncell + 1);
         cnt = cv.take<unsigned long long>(2);
         bmap = cv.take<int>(max_chunks);
         sbox = cv.take<double4>(nkeys);
-        dbox = cv.take<RealBox>(nbox);
         if (nbox > 1) {
+            dbox = cv.take<RealBox>(nbox);
             offs = cv.take<int>(3 * (size_t)(nbox + 1));
             s_box = cv.take<int>(ns);
             t_box = cv.take<int>(nt);
@@ -1007,9 +1004,11 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
 
     stage_begin(ctx, ST_SORT);
     const int threads = 256;
+    BoxTable btab{};
     if (nbox == 1) {
-        hipLaunchKernelGGL(k_set_box, dim3(1), dim3(1), 0, st, boxes[0], dbox);
+        btab.one = boxes[0];
     } else {
+        btab.tab = dbox;
         // box table and the source / target / cell offsets of every box
         std::vector<char> &h = ctx->host_stage;
         const size_t tb = sizeof(RealBox) * nbox, ob = sizeof(int) * 3 * (size_t)(nbox + 1);
@@ -1031,30 +1030,27 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
     // sources are summed in a fixed order on every run.
     if (ns > 0)
         hipLaunchKernelGGL(k_box_cell_keys, dim3((unsigned)((ns + threads - 1) / threads)),
-                           dim3(threads), 0, st, dbox, s_box, (int)ns, 0, kss.key);
+                           dim3(threads), 0, st, btab, s_box, (int)ns, 0, kss.key);
     if (int rc = keysort_run(ctx, &kss, ns, nkeys)) return rc;
     if (!same_points) {
         hipLaunchKernelGGL(k_box_cell_keys, dim3((unsigned)((nt + threads - 1) / threads)),
-                           dim3(threads), 0, st, dbox, t_box, (int)nt, 1, kst.key);
+                           dim3(threads), 0, st, btab, t_box, (int)nt, 1, kst.key);
         if (int rc = keysort_run(ctx, &kst, nt, nkeys)) return rc;
     }
     const KeySort &tks = same_points ? kss : kst;
     if (ns > 0) {
         const unsigned blocks = (unsigned)((ns + threads - 1) / threads);
         if (kind == 0)
-            hipLaunchKernelGGL(k_box_gather_sources<0>, dim3(blocks), dim3(threads), 0, st, dbox,
+            hipLaunchKernelGGL(k_box_gather_sources<0>, dim3(blocks), dim3(threads), 0, st, btab,
                                s_box, kss.perm, (int)ns, nrhs, dstride, spos, q, rec);
         else
-            hipLaunchKernelGGL(k_box_gather_sources<1>, dim3(blocks), dim3(threads), 0, st, dbox,
+            hipLaunchKernelGGL(k_box_gather_sources<1>, dim3(blocks), dim3(threads), 0, st, btab,
                                s_box, kss.perm, (int)ns, nrhs, dstride, spos, q, rec);
     }
     if (!same_points)
         hipLaunchKernelGGL(k_box_gather_targets, dim3((unsigned)((nt + threads - 1) / threads)),
-                           dim3(threads), 0, st, dbox, t_box, tks.perm, (int)nt, tpos);
-    hipLaunchKernelGGL(k_chunk_counts, dim3((unsigned)((ncell + threads - 1) / threads)),
-                       dim3(threads), 0, st, tks.off, (int)ncell, sub, ccnt);
-    scan_ints(st, ccnt, (int)ncell, coff);
-    expand_map(st, coff, (int)ncell, bmap);
+                           dim3(threads), 0, st, btab, t_box, tks.perm, (int)nt, tpos);
+    plan_chunks(st, nullptr, tks.off, sub, 64, (int)ncell, coff, bmap);   // chunks of 64 targets
     hipLaunchKernelGGL(k_run_bbox, dim3((nkeys + 3) / 4), dim3(256), 0, st, spos, kss.off, nkeys, sbox);
     stage_end(ctx, ST_SORT);
 
@@ -1071,7 +1067,7 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
     a.ncell = (int)ncell;
     a.nrhs = nrhs;
     a.ustride = ustride;
-    a.boxes = dbox;
+    a.boxes = btab;
     a.cell_box = cell_box;
     a.pair_counts = cnt;
 
@@ -1185,7 +1181,7 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
     KeySort kss, kst;
     double2 *spos, *tpos;
     double *q, *dut;
-    int *ccnt, *coff_t, *coff_s, *bmap_t, *bmap_s;
+    int *coff_t, *coff_s, *bmap_t, *bmap_s;
     double4 *sbox, *tbox;
     auto layout = [&](Carve &cv) {
         char *p = cv.take<char>(keysort_bytes(ns, nkeys));
@@ -1198,7 +1194,6 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         tpos = same_points ? spos : cv.take<double2>(nt);
         q = cv.take<double>((size_t)ncomp * ns);
         dut = cv.take<double>(2 * (size_t)nt);
-        ccnt = cv.take<int>(ncell + 1);
         coff_t = cv.take<int>(ncell + 1);
         coff_s = cv.take<int>(ncell + 1);
         bmap_t = cv.take<int>(max_t);
@@ -1238,17 +1233,8 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         hipLaunchKernelGGL(k_run_bbox, dim3(kblocks), dim3(threads), 0, st, spos, kss.off, nkeys, sbox);
     if (want_src && !same_points)
         hipLaunchKernelGGL(k_run_bbox, dim3(kblocks), dim3(threads), 0, st, tpos, tks.off, nkeys, tbox);
-    const unsigned cblocks = (unsigned)((ncell + threads - 1) / threads);
-    if (dptar) {
-        hipLaunchKernelGGL(k_chunk_counts, dim3(cblocks), dim3(threads), 0, st, tks.off, ncell, sub, ccnt);
-        scan_ints(st, ccnt, ncell, coff_t);
-        expand_map(st, coff_t, ncell, bmap_t);
-    }
-    if (want_src) {
-        hipLaunchKernelGGL(k_chunk_counts, dim3(cblocks), dim3(threads), 0, st, kss.off, ncell, sub, ccnt);
-        scan_ints(st, ccnt, ncell, coff_s);
-        expand_map(st, coff_s, ncell, bmap_s);
-    }
+    if (dptar) plan_chunks(st, nullptr, tks.off, sub, 64, ncell, coff_t, bmap_t);
+    if (want_src) plan_chunks(st, nullptr, kss.off, sub, 64, ncell, coff_s, bmap_s);
     stage_end(ctx, ST_SORT);
 
     RealAdjArgs a{};
