@@ -329,7 +329,7 @@ def main() -> int:
                                "P=24, tol 1e-10",
                    "N": N, "Mx": p.Mx, "My": p.My, "P": p.P, "xi": p.xi,
                    "nside": [p.nside_x, p.nside_y], "tol": p.tol},
-        "roofline": {"bound": "valu", "kernel": "k_real<1,1> (real-space stresslet pair sum)",
+        "roofline": {"bound": "valu", "kernel": "k_real<1,1,true> (real-space stresslet pair sum)",
                      "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "flops_per_launch": flops_real, "pairs_in": pairs_in, "pairs_out": pairs_out,
