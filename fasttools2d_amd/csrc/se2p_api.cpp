@@ -4,6 +4,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <string>
+#include <vector>
 #include <mutex>
 
 #include "se2p_device.h"
@@ -124,6 +125,98 @@ static int check_adjoint(se2p_ctx *ctx, const double *psrc, int64_t ns, const do
     return 0;
 }
 
+// Host-pointer mode (se2p_set_host_pointers): the call's arrays are host
+// memory.  Inputs are copied into a device staging area before the
+// computation and outputs back after it; such a call is always synchronous.
+// A host array passed twice (targets that are the sources) maps to one device
+// copy, so the shared cell lists still apply.  In device mode every method
+// returns its argument unchanged.
+class HostIo {
+public:
+    explicit HostIo(se2p_ctx *ctx) : ctx_(ctx), on_(ctx->host_ptrs) {}
+
+    // declare an array (input or output) and its size in bytes
+    void want(const void *p, size_t bytes)
+    {
+        if (!on_ || !p || bytes == 0 || find(p)) return;
+        items_.push_back({p, nullptr, bytes, false});
+        need_ += (bytes + 255) & ~size_t(255);
+    }
+    // allocate the staging area for everything declared
+    int map()
+    {
+        if (!on_) return 0;
+        if (need_ > ctx_->io_bytes) {
+            if (ctx_->io) {
+                SE2P_HIP(hipStreamSynchronize(ctx_->stream));
+                SE2P_HIP(hipFree(ctx_->io));
+                ctx_->io = nullptr;
+                ctx_->io_bytes = 0;
+            }
+            SE2P_HIP(hipMalloc((void **)&ctx_->io, need_));
+            ctx_->io_bytes = need_;
+        }
+        size_t o = 0;
+        for (Item &it : items_) {
+            it.dev = ctx_->io + o;
+            o += (it.bytes + 255) & ~size_t(255);
+        }
+        return 0;
+    }
+    template <class T>
+    const T *in(const T *p)
+    {
+        Item *it = on_ && p ? find(p) : nullptr;
+        if (!it) return p;
+        if (!it->copied) {
+            hipError_t e = hipMemcpyAsync(it->dev, p, it->bytes, hipMemcpyHostToDevice, ctx_->stream);
+            if (e != hipSuccess) err_ = e;
+            it->copied = true;
+        }
+        return (const T *)it->dev;
+    }
+    template <class T>
+    T *out(T *p)
+    {
+        Item *it = on_ && p ? find(p) : nullptr;
+        if (!it) return p;
+        outs_.push_back(it);
+        return (T *)it->dev;
+    }
+    // after the computation: copy the outputs back and wait for them
+    int done(int rc)
+    {
+        if (!on_ || rc != SE2P_OK) return rc;
+        SE2P_HIP(err_);
+        for (Item *it : outs_)
+            SE2P_HIP(hipMemcpyAsync((void *)it->host, it->dev, it->bytes, hipMemcpyDeviceToHost,
+                                    ctx_->stream));
+        SE2P_HIP(hipStreamSynchronize(ctx_->stream));
+        return SE2P_OK;
+    }
+
+private:
+    struct Item {
+        const void *host;
+        char *dev;
+        size_t bytes;
+        bool copied;
+    };
+    Item *find(const void *p)
+    {
+        for (Item &it : items_)
+            if (it.host == p) return &it;
+        return nullptr;
+    }
+    se2p_ctx *ctx_;
+    bool on_;
+    size_t need_ = 0;
+    hipError_t err_ = hipSuccess;
+    std::vector<Item> items_;
+    std::vector<Item *> outs_;
+    se2p_ctx *ctx = ctx_;   // for SE2P_HIP
+};
+
 }  // namespace se2p
 
 using namespace se2p;
@@ -179,6 +272,7 @@ int se2p_destroy(se2p_ctx *ctx)
     }
     if (ctx->ws) hipFree(ctx->ws);
     if (ctx->drv) hipFree(ctx->drv);
+    if (ctx->io) hipFree(ctx->io);
     for (auto &stage : ctx->ev)
         for (auto &visit : stage)
             for (hipEvent_t &e : visit)
@@ -220,6 +314,13 @@ int se2p_set_async(se2p_ctx *ctx, int a)
     return SE2P_OK;
 }
 
+int se2p_set_host_pointers(se2p_ctx *ctx, int enable)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    ctx->host_ptrs = enable != 0;
+    return SE2P_OK;
+}
+
 int se2p_set_timing(se2p_ctx *ctx, int enable)
 {
     if (!ctx) return SE2P_ERR_ARG;
@@ -244,9 +345,19 @@ int se2p_slp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *p
     if (ns > 0 && !f) return fail(ctx, SE2P_ERR_ARG, "null density pointer");
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 16 * (size_t)ns);
+    io.want(u, 16 * (size_t)nt);
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    u = io.out(u);
     RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
     if (int rc = real_sum(ctx, 0, g, f, nullptr, 1, 2 * ns, 2 * nt, u, nullptr)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_dlp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
@@ -259,9 +370,21 @@ int se2p_dlp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *p
     if (ns > 0 && (!f || !n)) return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 16 * (size_t)ns);
+    io.want(n, 16 * (size_t)ns);
+    io.want(u, 16 * (size_t)nt);
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    n = io.in(n);
+    u = io.out(u);
     RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
     if (int rc = real_sum(ctx, 1, g, f, n, 1, 2 * ns, 2 * nt, u, nullptr)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_slp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
@@ -275,8 +398,20 @@ int se2p_slp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double 
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 16 * (size_t)ns);
+    io.want(u, 16 * (size_t)nt);
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    u = io.out(u);
+    g.psrc = psrc;
+    g.ptar = ptar;
     if (int rc = kspace_sum(ctx, 0, g, f, nullptr, 1, 2 * ns, 2 * nt, u)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_dlp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
@@ -290,8 +425,22 @@ int se2p_dlp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double 
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 16 * (size_t)ns);
+    io.want(n, 16 * (size_t)ns);
+    io.want(u, 16 * (size_t)nt);
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    n = io.in(n);
+    u = io.out(u);
+    g.psrc = psrc;
+    g.ptar = ptar;
     if (int rc = kspace_sum(ctx, 1, g, f, n, 1, 2 * ns, 2 * nt, u)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 static int check_batch(se2p_ctx *ctx, int nrhs, int64_t f_stride, int64_t ns, int64_t u_stride,
@@ -315,9 +464,19 @@ int se2p_slp_real_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const d
     if (int rc = check_batch(ctx, nrhs, f_stride, ns, u_stride, nt)) return rc;
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 8 * ((size_t)(nrhs - 1) * f_stride + 2 * ns));
+    io.want(u, 8 * ((size_t)(nrhs - 1) * u_stride + 2 * nt));
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    u = io.out(u);
     RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
     if (int rc = real_sum(ctx, 0, g, f, nullptr, nrhs, f_stride, u_stride, u, nullptr)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_dlp_real_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
@@ -332,9 +491,21 @@ int se2p_dlp_real_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const d
     if (int rc = check_batch(ctx, nrhs, f_stride, ns, u_stride, nt)) return rc;
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 8 * ((size_t)(nrhs - 1) * f_stride + 2 * ns));
+    io.want(n, 16 * (size_t)ns);
+    io.want(u, 8 * ((size_t)(nrhs - 1) * u_stride + 2 * nt));
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    n = io.in(n);
+    u = io.out(u);
     RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
     if (int rc = real_sum(ctx, 1, g, f, n, nrhs, f_stride, u_stride, u, nullptr)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_slp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
@@ -350,8 +521,20 @@ int se2p_slp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 8 * ((size_t)(nrhs - 1) * f_stride + 2 * ns));
+    io.want(u, 8 * ((size_t)(nrhs - 1) * u_stride + 2 * nt));
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    u = io.out(u);
+    g.psrc = psrc;
+    g.ptar = ptar;
     if (int rc = kspace_sum(ctx, 0, g, f, nullptr, nrhs, f_stride, u_stride, u)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_dlp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
@@ -367,8 +550,22 @@ int se2p_dlp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 8 * ((size_t)(nrhs - 1) * f_stride + 2 * ns));
+    io.want(n, 16 * (size_t)ns);
+    io.want(u, 8 * ((size_t)(nrhs - 1) * u_stride + 2 * nt));
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    n = io.in(n);
+    u = io.out(u);
+    g.psrc = psrc;
+    g.ptar = ptar;
     if (int rc = kspace_sum(ctx, 1, g, f, n, nrhs, f_stride, u_stride, u)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 static int check_box(se2p_ctx *ctx, const se2p_box_desc &b, int i, bool dlp, bool kspace)
@@ -392,6 +589,28 @@ static int multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox, int kin
     for (int i = 0; i < nbox; ++i)
         if (int rc = check_box(ctx, boxes[i], i, kind == 1, kspace)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    std::vector<se2p_box_desc> staged;
+    if (ctx->host_ptrs && nbox > 0) {
+        for (int i = 0; i < nbox; ++i) {
+            const se2p_box_desc &b = boxes[i];
+            io.want(b.psrc, 16 * (size_t)b.ns);
+            io.want(b.ptar, 16 * (size_t)b.nt);
+            io.want(b.f, 16 * (size_t)b.ns);
+            if (kind == 1) io.want(b.n, 16 * (size_t)b.ns);
+            io.want(b.u, 16 * (size_t)b.nt);
+        }
+        if (int rc = io.map()) return rc;
+        staged.assign(boxes, boxes + nbox);
+        for (se2p_box_desc &b : staged) {
+            b.psrc = io.in(b.psrc);
+            b.ptar = io.in(b.ptar);
+            b.f = io.in(b.f);
+            if (kind == 1) b.n = io.in(b.n);
+            b.u = io.out(b.u);
+        }
+        boxes = staged.data();
+    }
     if (!kspace) {
         if (nbox > 0)
             if (int rc = real_sum_multibox(ctx, kind, boxes, nbox)) return rc;
@@ -402,7 +621,7 @@ static int multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox, int kin
             if (int rc = kspace_sum(ctx, kind, g, b.f, b.n, 1, 2 * b.ns, 2 * b.nt, b.u)) return rc;
         }
     }
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_slp_real_multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox)
@@ -435,9 +654,25 @@ int se2p_slp_real_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const 
     if (int rc = check_adjoint(ctx, psrc, ns, ptar, nt, f, nullptr, false, du)) return rc;
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 16 * (size_t)ns);
+    io.want(du, 16 * (size_t)nt);
+    io.want(dpsrc, 16 * (size_t)ns);
+    io.want(dptar, 16 * (size_t)nt);
+    io.want(df, 16 * (size_t)ns);
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    du = io.in(du);
+    dpsrc = io.out(dpsrc);
+    dptar = io.out(dptar);
+    df = io.out(df);
     RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
     if (int rc = real_adjoint(ctx, 0, g, f, nullptr, du, dpsrc, dptar, df, nullptr)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_dlp_real_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
@@ -450,9 +685,29 @@ int se2p_dlp_real_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const 
     if (int rc = check_adjoint(ctx, psrc, ns, ptar, nt, f, n, true, du)) return rc;
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 16 * (size_t)ns);
+    io.want(n, 16 * (size_t)ns);
+    io.want(du, 16 * (size_t)nt);
+    io.want(dpsrc, 16 * (size_t)ns);
+    io.want(dptar, 16 * (size_t)nt);
+    io.want(df, 16 * (size_t)ns);
+    io.want(dn, 16 * (size_t)ns);
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    n = io.in(n);
+    du = io.in(du);
+    dpsrc = io.out(dpsrc);
+    dptar = io.out(dptar);
+    df = io.out(df);
+    dn = io.out(dn);
     RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
     if (int rc = real_adjoint(ctx, 1, g, f, n, du, dpsrc, dptar, df, dn)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_slp_kspace_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
@@ -466,8 +721,26 @@ int se2p_slp_kspace_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, cons
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 16 * (size_t)ns);
+    io.want(du, 16 * (size_t)nt);
+    io.want(dpsrc, 16 * (size_t)ns);
+    io.want(dptar, 16 * (size_t)nt);
+    io.want(df, 16 * (size_t)ns);
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    du = io.in(du);
+    dpsrc = io.out(dpsrc);
+    dptar = io.out(dptar);
+    df = io.out(df);
+    g.psrc = psrc;
+    g.ptar = ptar;
     if (int rc = kspace_adjoint(ctx, 0, g, f, nullptr, du, dpsrc, dptar, df, nullptr)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_dlp_kspace_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
@@ -482,8 +755,30 @@ int se2p_dlp_kspace_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, cons
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 16 * (size_t)ns);
+    io.want(n, 16 * (size_t)ns);
+    io.want(du, 16 * (size_t)nt);
+    io.want(dpsrc, 16 * (size_t)ns);
+    io.want(dptar, 16 * (size_t)nt);
+    io.want(df, 16 * (size_t)ns);
+    io.want(dn, 16 * (size_t)ns);
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    n = io.in(n);
+    du = io.in(du);
+    dpsrc = io.out(dpsrc);
+    dptar = io.out(dptar);
+    df = io.out(df);
+    dn = io.out(dn);
+    g.psrc = psrc;
+    g.ptar = ptar;
     if (int rc = kspace_adjoint(ctx, 1, g, f, n, du, dpsrc, dptar, df, dn)) return rc;
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_select_params(int kind, int64_t ns, int64_t nt, double Q, double Lx, double Ly,
@@ -522,6 +817,22 @@ static int ewald_full(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, c
         return fail(ctx, SE2P_ERR_PARAM, "Lx and Ly must be positive and finite");
     if (prm->P < 0 || (prm->P % 2) != 0) return fail(ctx, SE2P_ERR_P, "P must be even");
     if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 16 * (size_t)ns);
+    io.want(n, 16 * (size_t)ns);
+    io.want(u, 16 * (size_t)nt);
+    io.want(ur_out, 16 * (size_t)nt);
+    io.want(uk_out, 16 * (size_t)nt);
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    n = io.in(n);
+    u = io.out(u);
+    ur_out = io.out(ur_out);
+    uk_out = io.out(uk_out);
     hipStream_t st = ctx->stream;
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
     const bool same = (psrc == ptar) && (ns == nt);
@@ -555,7 +866,7 @@ static int ewald_full(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, c
     if (int rc = check_kspace_params(ctx, kg)) return rc;
     if (int rc = kspace_sum(ctx, kind, kg, f, n, 1, 2 * ns, 2 * nt, uk)) return rc;
     launch_compose(st, ur, uk, nt, kind, sums, Lx * Ly, u);
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_slp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
@@ -592,13 +903,19 @@ int se2p_count_pairs(se2p_ctx *ctx, const double *psrc, int64_t ns, const double
     if (int rc = set_device(ctx)) return rc;
     *pairs_in = *pairs_out = 0;
     if (nt == 0) return SE2P_OK;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
     RealGeom g{psrc, ptar, ns, nt, 1.0, Lx, Ly, nside_x, nside_y};
     int64_t counts[2] = {0, 0};
     // densities are not read when counting; pass the positions as a stand-in
     if (int rc = real_sum(ctx, 1, g, psrc, psrc, 1, 2 * ns, 2 * nt, nullptr, counts)) return rc;
     *pairs_in = counts[0];
     *pairs_out = counts[1];
-    return finish(ctx);
+    return io.done(finish(ctx));
 }
 
 int se2p_selftest_keysort(se2p_ctx *ctx, const int *keys, int64_t n, int nkeys, int *perm,

@@ -58,6 +58,9 @@ struct se2p_ctx {
     char *drv = nullptr;      // driver-level buffers (kept apart from ws)
     size_t drv_bytes = 0;
     std::vector<char> host_stage;   // host copy of per-box tables (multi-box calls)
+    bool host_ptrs = false;   // array arguments are host memory (se2p_set_host_pointers)
+    char *io = nullptr;       // device staging area of host-pointer calls
+    size_t io_bytes = 0;
 };
 
 namespace se2p {

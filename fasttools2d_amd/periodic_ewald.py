@@ -78,6 +78,12 @@ class Context:
     def set_timing(self, flag: bool) -> None:
         self._check(self._lib.se2p_set_timing(self.handle, 1 if flag else 0))
 
+    def set_host_pointers(self, flag: bool) -> None:
+        """Array arguments of raw C-ABI calls on this context are host memory
+        (the library stages them).  The Python wrappers of this module pass
+        device tensors and need the default (False)."""
+        self._check(self._lib.se2p_set_host_pointers(self.handle, 1 if flag else 0))
+
     def stage_times(self) -> dict:
         buf = (ctypes.c_float * len(_capi.STAGES))()
         self._check(self._lib.se2p_stage_times(self.handle, buf, len(_capi.STAGES)))

@@ -65,6 +65,15 @@ const char *se2p_last_error(const se2p_ctx *ctx);
 int se2p_set_stream(se2p_ctx *ctx, void *hip_stream);
 void *se2p_get_stream(se2p_ctx *ctx);
 int se2p_set_async(se2p_ctx *ctx, int async_flag);
+/* Host-pointer mode: with enable != 0, every array argument of the compute
+ * entry points (points, densities, normals, du, outputs; for the multi-box
+ * entries the arrays the descriptors point to) is host memory.  Inputs are
+ * copied into a device staging area owned by the context, the computation
+ * runs on the device, and outputs are copied back before the call returns
+ * (such calls are synchronous whatever se2p_set_async says).  A host array
+ * passed twice, such as targets that are the sources, is copied once.  The
+ * default (0) is device pointers. */
+int se2p_set_host_pointers(se2p_ctx *ctx, int enable);
 /* Per-stage device timing with HIP events on the context stream.  When
  * enabled, se2p_stage_times fills up to n floats (milliseconds) for the
  * most recent call: sort, real, spread, fft_fwd, filter, fft_inv, interp
