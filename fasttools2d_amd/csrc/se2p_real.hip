@@ -592,8 +592,15 @@ struct RealAdjArgs {
     double Lx, Ly, xi, xi2, cut2, self;
     const double *f, *nv;  // caller-order densities and normals (stresslet sources)
     double *dpos, *df, *dn;   // caller order; may be null
+    double *dpos_t;        // SIDE 2: dL/dptar (dpos is then dL/dpsrc)
+    const double *ldata2;  // SIDE 2: the lane points' densities, [ncomp][nl]
 };
 
+// SIDE 2 (targets are the sources, one sorted point set): both of the
+//   above in one pass.  For lane point i and uniform point j the target term
+//   (i as target, j as source) and the source term (i as source, j as target)
+//   use their own difference vectors, rounded as the forward sums round them,
+//   and their own inclusion tests; exp, 1/r^2 and E1 are evaluated once.
 // SIDE 0: lanes are targets (lane data du), uniform points are sources
 //   (uniform data: the densities); writes dL/dptar.
 // SIDE 1: lanes are sources (lane data: the densities), uniform points are
@@ -604,16 +611,21 @@ struct RealAdjArgs {
 template <int KIND, int SIDE>
 __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *__restrict__ upos,
                                                   const double *__restrict__ udata,
+                                                  const double *__restrict__ udata2,
                                                   const int *__restrict__ uoff,
                                                   const double4 *__restrict__ ubox)
 {
-    constexpr int NV = SIDE == 0 ? 2 : (KIND == 0 ? 4 : 5);
-    __shared__ double e1tab[KIND == 0 && SIDE == 1 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
+    // accumulators: SIDE 0 [grad]; SIDE 1 [grad, density terms];
+    // SIDE 2 [target grad, source grad, density terms]
+    constexpr int NS = KIND == 0 ? 4 : 5;
+    constexpr int NV = SIDE == 0 ? 2 : (SIDE == 1 ? NS : 2 + NS);
+    constexpr int SO = SIDE == 2 ? 2 : 0;   // offset of the source-side accumulators
+    __shared__ double e1tab[KIND == 0 && SIDE >= 1 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
     __shared__ double etab[16];
     __shared__ double red[kWaves - 1][NV][64];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (KIND == 0 && SIDE == 1)
+    if (KIND == 0 && SIDE >= 1)
         for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
     if (threadIdx.x < 16) {
         const double t16[16] = SE2P_EXP2_TABLE;
@@ -630,14 +642,17 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
     const int l = l0 + lane;
     const bool active = l < l1;
     double lx, ly;
-    double ld[3] = {0.0, 0.0, 0.0};
-    constexpr int LC = SIDE == 0 ? 2 : (KIND == 0 ? 2 : 3);
+    // lane data: SIDE 0 du; SIDE 1 densities; SIDE 2 du, then densities
+    double ld[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    constexpr int NQ = KIND == 0 ? 2 : 3;
+    constexpr int LC = SIDE == 0 ? 2 : (SIDE == 1 ? NQ : 2 + NQ);
     if (active) {
         const double2 p = a.lpos[l];
         lx = p.x;
         ly = p.y;
 #pragma unroll
-        for (int c = 0; c < LC; ++c) ld[c] = a.ldata[c * a.nl + l];
+        for (int c = 0; c < LC; ++c)
+            ld[c] = (SIDE == 2 && c >= 2) ? a.ldata2[(c - 2) * a.nl + l] : a.ldata[c * a.nl + l];
     } else {
         lx = ly = __builtin_nan("");   // inactive lanes compute garbage, never stored
     }
@@ -647,8 +662,60 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = 0.0;
 
+    // both sides of one pair (SIDE 2): udata = densities, udata2 = du
+    auto pair2 = [&](double ux, double uy, double zx, double zy, int u, bool own) {
+        if constexpr (SIDE == 2) {
+        double t1, t2, s1, s2;
+        diff<KIND>(ux, uy, lx, ly, zx, zy, t1, t2);   // lane as target
+        if (KIND == 0) {                               // lane as source
+            s1 = lx - (ux + zx);
+            s2 = ly - (uy + zy);
+        } else {
+            s1 = (lx - ux) - zx;
+            s2 = (ly - uy) - zy;
+        }
+        const double rt = se2p_sqsum(t1, t2), rs = se2p_sqsum(s1, s2);
+        const bool in_t = own ? rt != 0.0 : rt < a.cut2;
+        const bool in_s = own ? rs != 0.0 : rs < a.cut2;
+        const double a1 = udata2[u], a2 = udata2[us + u];   // du of the uniform point
+        if (KIND == 0 && own && rs == 0.0) {   // coincident: self term, no gradient
+            acc[4] += a.self * a1;
+            acc[5] += a.self * a2;
+        }
+        if (!(in_t || in_s)) return;
+        const double r2 = in_t ? rt : rs;
+        const double x = r2 * a.xi2;
+        const double e = se2p_exp_neg(x, etab);
+        const double ir2 = rcp_nr(r2);
+        if (in_t) {   // lane: du_i; uniform: the densities of j
+            if (KIND == 0)
+                slp_grad(t1, t2, a.xi2, e, ir2, ld[0], ld[1], udata[u], udata[us + u], acc[0], acc[1]);
+            else
+                dlp_grad(t1, t2, r2, a.xi2, e, ir2, ld[0], ld[1], udata[u], udata[us + u],
+                         udata[2 * us + u], acc[0], acc[1]);
+        }
+        if (in_s) {   // lane: the densities of i; uniform: du_j
+            if (KIND == 0) {
+                slp_grad(s1, s2, a.xi2, e, ir2, a1, a2, ld[2], ld[3], acc[2], acc[3]);
+                const double E = se2p_e1(x, e, e1tab);
+                const double hE = 0.5 * E, g = e * ir2;
+                const double c = s1 * a2 - s2 * a1;
+                acc[4] += hE * a1 + g * s2 * c;
+                acc[5] += hE * a2 - g * s1 * c;
+            } else {
+                dlp_grad(s1, s2, r2, a.xi2, e, ir2, a1, a2, ld[2], ld[3], ld[4], acc[2], acc[3]);
+                dlp_dens(s1, s2, r2, a.xi2, e, ir2, a1, a2, acc[4], acc[5], acc[6]);
+            }
+        }
+        }
+    };
+
     // one pair: d with the forward's rounding, then the side's contributions
     auto pair = [&](double ux, double uy, double zx, double zy, int u, bool own) {
+        if constexpr (SIDE == 2) {
+            pair2(ux, uy, zx, zy, u, own);
+            return;
+        }
         double d1, d2;
         if (SIDE == 0) {
             diff<KIND>(ux, uy, lx, ly, zx, zy, d1, d2);
@@ -750,18 +817,22 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
         a.dpos[2 * o + 1] = -acc[1] * s4;
         return;
     }
+    if (SIDE == 2 && a.dpos_t) {
+        a.dpos_t[2 * o] = -acc[0] * s4;
+        a.dpos_t[2 * o + 1] = -acc[1] * s4;
+    }
     if (a.dpos) {
-        a.dpos[2 * o] = acc[0] * s4;
-        a.dpos[2 * o + 1] = acc[1] * s4;
+        a.dpos[2 * o] = acc[SO] * s4;
+        a.dpos[2 * o + 1] = acc[SO + 1] * s4;
     }
     if (KIND == 0) {
         if (a.df) {
-            a.df[2 * o] = acc[2] * s4;
-            a.df[2 * o + 1] = acc[3] * s4;
+            a.df[2 * o] = acc[SO + 2] * s4;
+            a.df[2 * o + 1] = acc[SO + 3] * s4;
         }
     } else {
         // m = dL/d(S11, S12 + S21, S22), S = f (x) n
-        const double m1 = acc[2] * s4, m2 = acc[3] * s4, m3 = acc[4] * s4;
+        const double m1 = acc[SO + 2] * s4, m2 = acc[SO + 3] * s4, m3 = acc[SO + 4] * s4;
         const double f1 = a.f[2 * o], f2 = a.f[2 * o + 1];
         const double n1 = a.nv[2 * o], n2 = a.nv[2 * o + 1];
         if (a.df) {
@@ -1250,6 +1321,29 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
     a.f = f;
     a.nv = n;
     stage_begin(ctx, ST_REAL);
+    if (same_points && dptar && want_src) {
+        // one pass computes both sides
+        RealAdjArgs t = a;
+        t.lpos = spos;
+        t.lperm = kss.perm;
+        t.loff = kss.off;
+        t.chunk_off = coff_s;
+        t.block_cell = bmap_s;
+        t.ldata = dut;
+        t.ldata2 = q;
+        t.nl = ns;
+        t.nu = ns;
+        t.dpos = dpsrc;
+        t.dpos_t = dptar;
+        t.df = df;
+        t.dn = dn;
+        if (kind == 0)
+            hipLaunchKernelGGL((k_real_adj<0, 2>), dim3(grid_s), dim3(256), 0, st, t, spos, q, dut, kss.off, sbox);
+        else
+            hipLaunchKernelGGL((k_real_adj<1, 2>), dim3(grid_s), dim3(256), 0, st, t, spos, q, dut, kss.off, sbox);
+        dptar = nullptr;
+        dpsrc = df = dn = nullptr;   // done
+    }
     if (dptar) {
         RealAdjArgs t = a;
         t.lpos = tpos;
@@ -1262,11 +1356,11 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         t.nu = ns;
         t.dpos = dptar;
         if (kind == 0)
-            hipLaunchKernelGGL((k_real_adj<0, 0>), dim3(grid_t), dim3(256), 0, st, t, spos, q, kss.off, sbox);
+            hipLaunchKernelGGL((k_real_adj<0, 0>), dim3(grid_t), dim3(256), 0, st, t, spos, q, nullptr, kss.off, sbox);
         else
-            hipLaunchKernelGGL((k_real_adj<1, 0>), dim3(grid_t), dim3(256), 0, st, t, spos, q, kss.off, sbox);
+            hipLaunchKernelGGL((k_real_adj<1, 0>), dim3(grid_t), dim3(256), 0, st, t, spos, q, nullptr, kss.off, sbox);
     }
-    if (want_src) {
+    if (dpsrc || df || dn) {
         RealAdjArgs s = a;
         s.lpos = spos;
         s.lperm = kss.perm;
@@ -1280,9 +1374,9 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         s.df = df;
         s.dn = dn;
         if (kind == 0)
-            hipLaunchKernelGGL((k_real_adj<0, 1>), dim3(grid_s), dim3(256), 0, st, s, tpos, dut, tks.off, tbox);
+            hipLaunchKernelGGL((k_real_adj<0, 1>), dim3(grid_s), dim3(256), 0, st, s, tpos, dut, nullptr, tks.off, tbox);
         else
-            hipLaunchKernelGGL((k_real_adj<1, 1>), dim3(grid_s), dim3(256), 0, st, s, tpos, dut, tks.off, tbox);
+            hipLaunchKernelGGL((k_real_adj<1, 1>), dim3(grid_s), dim3(256), 0, st, s, tpos, dut, nullptr, tks.off, tbox);
     }
     stage_end(ctx, ST_REAL);
     SE2P_HIP(hipGetLastError());
