@@ -246,6 +246,16 @@ int se2p_create(se2p_ctx **out, int device)
         return SE2P_ERR_HIP;
     }
     ctx->own_stream = true;
+    // fused-sort key counts: zero once here, cleared again by every sort
+    const size_t fs_bytes = sizeof(int) * kFsMaxKeys;
+    if (hipMalloc((void **)&ctx->fs, fs_bytes) != hipSuccess ||
+        hipMemsetAsync(ctx->fs, 0, fs_bytes, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        if (ctx->fs) hipFree(ctx->fs);
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return SE2P_ERR_HIP;
+    }
     for (auto &stage : ctx->ev)
         for (auto &visit : stage)
             for (hipEvent_t &e : visit) hipEventCreate(&e);
@@ -273,6 +283,7 @@ int se2p_destroy(se2p_ctx *ctx)
     if (ctx->ws) hipFree(ctx->ws);
     if (ctx->drv) hipFree(ctx->drv);
     if (ctx->io) hipFree(ctx->io);
+    if (ctx->fs) hipFree(ctx->fs);
     for (auto &stage : ctx->ev)
         for (auto &visit : stage)
             for (hipEvent_t &e : visit)
@@ -306,6 +317,15 @@ int se2p_set_stream(se2p_ctx *ctx, void *s)
 }
 
 void *se2p_get_stream(se2p_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int se2p_set_sort_mode(se2p_ctx *ctx, int mode)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    if (mode & ~(SE2P_SORT_RADIX | SE2P_SORT_BOX_KERNEL))
+        return fail(ctx, SE2P_ERR_ARG, "unknown sort mode bits 0x%x", mode);
+    ctx->sort_mode = mode;
+    return SE2P_OK;
+}
 
 int se2p_set_async(se2p_ctx *ctx, int a)
 {

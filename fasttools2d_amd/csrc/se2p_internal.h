@@ -61,9 +61,14 @@ struct se2p_ctx {
     bool host_ptrs = false;   // array arguments are host memory (se2p_set_host_pointers)
     char *io = nullptr;       // device staging area of host-pointer calls
     size_t io_bytes = 0;
+    int *fs = nullptr;        // fused-sort state (se2p_fsort.h), zero between sorts
+    int sort_mode = 0;        // SE2P_SORT_* bits
 };
 
 namespace se2p {
+
+// Fused-sort state per context: the key counts of up to kFsMaxKeys keys.
+constexpr int kFsMaxKeys = 14336;
 
 int fail(se2p_ctx *ctx, int code, const char *fmt, ...);
 

@@ -33,6 +33,7 @@
 //     1/(Mx My) is folded into the filter.
 //   * Interpolation: one thread per target, (P+1)^2 gathers per grid.
 #include "se2p_device.h"
+#include "se2p_fsort.h"
 #include "se2p_internal.h"
 
 #include <algorithm>
@@ -52,16 +53,26 @@ struct SpreadSrc {
 };
 
 // ---- per-source gridding data -------------------------------------------
-__global__ void k_bin_keys(const double *__restrict__ p, int n, double Lx, double Ly, double h,
-                           int P, int Mx, int My, int nby, int *__restrict__ keys)
+// Anchor bin of a point: kBin x kBin blocks of grid nodes, x-major.
+struct BinKeyFn {
+    const double *p;
+    double Lx, Ly, h;
+    int P, Mx, My, nby;
+    __device__ int operator()(int i) const
+    {
+        int mx, my;
+        double px, py;
+        se2p_anchor(p[2 * i], p[2 * i + 1], Lx, Ly, h, P, &mx, &my, &px, &py);
+        const int ax = ((mx % Mx) + Mx) % Mx, ay = ((my % My) + My) % My;
+        return (ax / kBin) * nby + ay / kBin;
+    }
+};
+
+__global__ void k_bin_keys(BinKeyFn kf, int n, int *__restrict__ keys)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int mx, my;
-    double px, py;
-    se2p_anchor(p[2 * i], p[2 * i + 1], Lx, Ly, h, P, &mx, &my, &px, &py);
-    int ax = ((mx % Mx) + Mx) % Mx, ay = ((my % My) + My) % My;
-    keys[i] = (ax / kBin) * nby + ay / kBin;
+    keys[i] = kf(i);
 }
 
 // FGG factors of mex_stokes_slp_kspace.cpp:95-99 with c = -2 xi^2 / eta.
@@ -75,18 +86,20 @@ __device__ inline void fgg_factors(double px, double py, double c, double w, dou
     *e3y = exp(-2 * c * h * py);
 }
 
-// kind 0: q = f (2 comps); kind 1: q = (n1 f1, n2 f1 + n1 f2, n2 f2)
+// Gridding data of input point o at sorted slot i: wrapped anchor, FGG
+// factors and the densities of nr right-hand sides (kind 0: q = f, 2 comps;
+// kind 1: q = (n1 f1, n2 f1 + n1 f2, n2 f2)).
 template <int KIND>
-__global__ void k_gather_grid_src(const double *__restrict__ p, const double *__restrict__ f,
-                                  const double *__restrict__ nv, const int *__restrict__ perm,
-                                  int n, int nreal, int nr, int64_t dstride, double Lx, double Ly,
-                                  double h, int P, int Mx, int My, double c, double w,
-                                  int2 *__restrict__ anchor, double *__restrict__ fgg,
-                                  double *__restrict__ q)
+__device__ __forceinline__ void gather_grid_source(const double *__restrict__ p,
+                                                   const double *__restrict__ f,
+                                                   const double *__restrict__ nv, int o, int i,
+                                                   int n, int nreal, int nr, int64_t dstride,
+                                                   double Lx, double Ly, double h, int P, int Mx,
+                                                   int My, double c, double w,
+                                                   int2 *__restrict__ anchor,
+                                                   double *__restrict__ fgg,
+                                                   double *__restrict__ q)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int o = perm[i];
     int mx, my;
     double px, py;
     se2p_anchor(p[2 * o], p[2 * o + 1], Lx, Ly, h, P, &mx, &my, &px, &py);
@@ -114,6 +127,39 @@ __global__ void k_gather_grid_src(const double *__restrict__ p, const double *__
         }
     }
 }
+
+template <int KIND>
+__global__ void k_gather_grid_src(const double *__restrict__ p, const double *__restrict__ f,
+                                  const double *__restrict__ nv, const int *__restrict__ perm,
+                                  int n, int nreal, int nr, int64_t dstride, double Lx, double Ly,
+                                  double h, int P, int Mx, int My, double c, double w,
+                                  int2 *__restrict__ anchor, double *__restrict__ fgg,
+                                  double *__restrict__ q)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gather_grid_source<KIND>(p, f, nv, perm[i], i, n, nreal, nr, dstride, Lx, Ly, h, P, Mx, My, c,
+                             w, anchor, fgg, q);
+}
+
+// The same gather run by the final scatter of the fused sort.
+template <int KIND>
+struct GridSrcGather {
+    static constexpr bool kBox = false;
+    const double *p, *f, *nv;
+    int n, nreal, nr;
+    int64_t dstride;
+    double Lx, Ly, h;
+    int P, Mx, My;
+    double c, w;
+    int2 *anchor;
+    double *fgg, *q;
+    __device__ void operator()(int i, int o, int) const
+    {
+        gather_grid_source<KIND>(p, f, nv, o, i, n, nreal, nr, dstride, Lx, Ly, h, P, Mx, My, c, w,
+                                 anchor, fgg, q);
+    }
+};
 
 // e1[k] = exp(c h^2 (k - P/2)^2), k = 0..P  (mex_stokes_slp_kspace.cpp:64-67)
 __device__ inline void fill_e1(double *e1, int P, double ch2)
@@ -531,13 +577,11 @@ __global__ __launch_bounds__(64) void k_spread_combine(SpreadArgs a)
 // Work split per tile (one wavefront per tile): the candidate runs of the
 // bin-sorted sources, the number of spreading chunks (at least one per tile,
 // so every grid node is written) and of interpolation chunks of 256 targets.
-__global__ __launch_bounds__(64) void k_tile_work(SpreadArgs a, const int *__restrict__ tboff,
-                                                  int *__restrict__ scnt, int *__restrict__ icnt,
-                                                  int *__restrict__ rng_start,
-                                                  int *__restrict__ rng_len)
+__device__ __forceinline__ void tile_work(const SpreadArgs &a, const int *__restrict__ tboff,
+                                          int tile, int l, int *__restrict__ scnt,
+                                          int *__restrict__ icnt, int *__restrict__ rng_start,
+                                          int *__restrict__ rng_len)
 {
-    const int tile = blockIdx.x;
-    const int l = threadIdx.x;
     const int tx = tile / a.nty, ty = tile % a.nty;
     int xb0[2], xn[2], yb0[2], yn[2];
     const int nxr = bin_runs(tx * kTile, a.Mx, a.P, a.nbx, xb0, xn);
@@ -568,6 +612,29 @@ __global__ __launch_bounds__(64) void k_tile_work(SpreadArgs a, const int *__res
         icnt[tile] = (nt + 255) / 256;
     }
 }
+
+__global__ __launch_bounds__(64) void k_tile_work(SpreadArgs a, const int *__restrict__ tboff,
+                                                  int *__restrict__ scnt, int *__restrict__ icnt,
+                                                  int *__restrict__ rng_start,
+                                                  int *__restrict__ rng_len)
+{
+    tile_work(a, tboff, blockIdx.x, threadIdx.x, scnt, icnt, rng_start, rng_len);
+}
+
+// The tile work as extra workgroups of the fused sort: one tile per wavefront.
+struct TileWorkExtra {
+    SpreadArgs a;
+    const int *tboff;
+    int *scnt, *icnt, *rng_start, *rng_len;
+    int ntiles;
+    int blocks(int nw) const { return ntiles > 0 ? (ntiles + nw - 1) / nw : 0; }
+    template <int NW>
+    __device__ void run(int xb, int *) const
+    {
+        const int tile = xb * NW + (int)(threadIdx.x >> 6);
+        if (tile < ntiles) tile_work(a, tboff, tile, threadIdx.x & 63, scnt, icnt, rng_start, rng_len);
+    }
+};
 
 // ---- filter ----------------------------------------------------------------
 struct FilterArgs {
@@ -1215,12 +1282,18 @@ static SpreadArgs spread_args(const KLayout &L, const GridPass &gp, const int *b
     return sa;
 }
 
+static BinKeyFn bin_key_fn(const KLayout &L, const KGeom &g, const double *p)
+{
+    return BinKeyFn{p, g.Lx, g.Ly, L.h, L.P, L.Mx, L.My, L.nby};
+}
+
 // Sort n points by anchor bin.
 static int bin_points(se2p_ctx *ctx, const KLayout &L, const KGeom &g, const double *p, int64_t n,
                       KeySort *ks)
 {
-    hipLaunchKernelGGL(k_bin_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p,
-                       (int)n, g.Lx, g.Ly, L.h, L.P, L.Mx, L.My, L.nby, ks->key);
+    if (n > 0)
+        hipLaunchKernelGGL(k_bin_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                           bin_key_fn(L, g, p), (int)n, ks->key);
     return keysort_run(ctx, ks, n, L.nbins);
 }
 
@@ -1363,29 +1436,68 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     cv.base = ctx->ws;
     layout(cv);
 
+    // Bins of the sources (and of the targets unless they are the sources).
+    // The fused sort (se2p_fsort.h) also gathers the sources' gridding data
+    // and, beside the sort that completes the target bins, builds the tile
+    // work lists and chunk plans; otherwise separate launches do.
     stage_begin(ctx, ST_SORT);
-    if (int rc = bin_points(ctx, L, g, g.psrc, ns, &kss)) return rc;
-    if (!same_points)
-        if (int rc = bin_points(ctx, L, g, g.ptar, nt, &kst)) return rc;
     const KeySort &tbins = same_points ? kss : kst;
     SpreadArgs sa = spread_args(L, gp, kss.off, grids);
-    if (maps) tile_maps(st, L, gp, sa, tbins.off);
-    stage_end(ctx, ST_SORT);
-
-    stage_begin(ctx, ST_SPREAD);
-    const unsigned sblocks = (unsigned)((ns + 255) / 256);
-    if (kind == 0)
-        hipLaunchKernelGGL(k_gather_grid_src<0>, dim3(sblocks), dim3(256), 0, st, g.psrc, f, nv,
-                           kss.perm, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly, L.h, L.P, L.Mx, L.My,
-                           L.c, g.w, anchor, fgg, q);
-    else
-        hipLaunchKernelGGL(k_gather_grid_src<1>, dim3(sblocks), dim3(256), 0, st, g.psrc, f, nv,
-                           kss.perm, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly, L.h, L.P, L.Mx, L.My,
-                           L.c, g.w, anchor, fgg, q);
     sa.s.anchor = anchor;
     sa.s.fgg = fgg;
     sa.s.q = q;
     sa.s.ns = ns;
+    const TileWorkExtra tw{sa, tbins.off, gp.scnt, gp.icnt, gp.rng_start, gp.rng_len,
+                           maps ? L.ntiles : 0};
+    const FsPlanExtra pl{PlanArgs{gp.scnt, nullptr, 1, 1, L.ntiles, gp.soffc, gp.smap},
+                         PlanArgs{gp.icnt, nullptr, 1, 1, L.ntiles, gp.ioffc, gp.imap}, maps ? 2 : 0};
+    const TileWorkExtra no_tw{sa, tbins.off, nullptr, nullptr, nullptr, nullptr, 0};
+    const FsPlanExtra no_pl{pl.p0, pl.p1, 0};
+    bool gathered = false, mapped = false;
+    if (fsort_ok(ctx, ns, L.nbins)) {
+        const BinKeyFn kf = bin_key_fn(L, g, g.psrc);
+        int rc;
+        if (kind == 0)
+            rc = fsort_run(ctx, &kss, ns, L.nbins, kf,
+                           GridSrcGather<0>{g.psrc, f, nv, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly,
+                                            L.h, L.P, L.Mx, L.My, L.c, g.w, anchor, fgg, q},
+                           same_points ? tw : no_tw, same_points ? pl : no_pl);
+        else
+            rc = fsort_run(ctx, &kss, ns, L.nbins, kf,
+                           GridSrcGather<1>{g.psrc, f, nv, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly,
+                                            L.h, L.P, L.Mx, L.My, L.c, g.w, anchor, fgg, q},
+                           same_points ? tw : no_tw, same_points ? pl : no_pl);
+        if (rc) return rc;
+        gathered = true;
+        mapped = same_points;
+    } else if (int rc = bin_points(ctx, L, g, g.psrc, ns, &kss)) {
+        return rc;
+    }
+    if (!same_points) {
+        if (fsort_ok(ctx, nt, L.nbins)) {
+            if (int rc = fsort_run(ctx, &kst, nt, L.nbins, bin_key_fn(L, g, g.ptar), FsNoPayload{},
+                                   tw, pl))
+                return rc;
+            mapped = true;
+        } else if (int rc = bin_points(ctx, L, g, g.ptar, nt, &kst)) {
+            return rc;
+        }
+    }
+    if (maps && !mapped) tile_maps(st, L, gp, sa, tbins.off);
+    stage_end(ctx, ST_SORT);
+
+    stage_begin(ctx, ST_SPREAD);
+    const unsigned sblocks = (unsigned)((ns + 255) / 256);
+    if (gathered) {
+    } else if (kind == 0) {
+        hipLaunchKernelGGL(k_gather_grid_src<0>, dim3(sblocks), dim3(256), 0, st, g.psrc, f, nv,
+                           kss.perm, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly, L.h, L.P, L.Mx, L.My,
+                           L.c, g.w, anchor, fgg, q);
+    } else {
+        hipLaunchKernelGGL(k_gather_grid_src<1>, dim3(sblocks), dim3(256), 0, st, g.psrc, f, nv,
+                           kss.perm, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly, L.h, L.P, L.Mx, L.My,
+                           L.c, g.w, anchor, fgg, q);
+    }
     if (int rc = spread(ctx, L, gp, sa)) return rc;
     stage_end(ctx, ST_SPREAD);
 

@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "se2p_device.h"
+#include "se2p_fsort.h"
 #include "se2p_internal.h"
 
 namespace se2p {
@@ -861,31 +862,33 @@ __global__ void k_gather_pairs(const double *__restrict__ v, const int *__restri
 // Global point i belongs to box pt_box[i] (all to box 0 when pt_box is null).
 __device__ __forceinline__ int box_of(const int *pt_box, int i) { return pt_box ? pt_box[i] : 0; }
 
-// (global cell, Morton sub-cell) keys of sources (which = 0) or targets (1).
+// (global cell, Morton sub-cell) key of source (which = 0) or target (1) i.
+__device__ __forceinline__ int box_cell_key(const BoxTable &boxes, const int *pt_box, int which,
+                                            int i)
+{
+    const RealBox &B = boxes.get(box_of(pt_box, i));
+    const int l = i - (which ? B.t_base : B.s_base);
+    const double *p = which ? B.ptar : B.psrc;
+    return (B.cell_base << (2 * kSubLog)) +
+           se2p_cell_key(p[2 * l], p[2 * l + 1], B.Lx, B.Ly, B.nsx, B.nsy, kSubLog);
+}
+
 __global__ void k_box_cell_keys(const BoxTable boxes, const int *__restrict__ pt_box,
                                 int n, int which, int *__restrict__ key)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const RealBox &B = boxes.get(box_of(pt_box, i));
-    const int l = i - (which ? B.t_base : B.s_base);
-    const double *p = which ? B.ptar : B.psrc;
-    key[i] = (B.cell_base << (2 * kSubLog)) +
-             se2p_cell_key(p[2 * l], p[2 * l + 1], B.Lx, B.Ly, B.nsx, B.nsy, kSubLog);
+    key[i] = box_cell_key(boxes, pt_box, which, i);
 }
 
-// sources: positions and per-RHS densities in cell order
+// sources: position and per-RHS densities of input point g at sorted slot i.
 // With one right-hand side (rec != null) the sources are written as records
 // (SrcRec) instead of the [nrhs][ncomp][n] density array.
 template <int KIND>
-__global__ void k_box_gather_sources(const BoxTable boxes,
-                                     const int *__restrict__ pt_box, const int *__restrict__ perm,
-                                     int n, int nrhs, int64_t dstride, double2 *__restrict__ pos,
-                                     double *__restrict__ q, double *__restrict__ rec)
+__device__ __forceinline__ void gather_source(const BoxTable &boxes, const int *pt_box, int g, int i,
+                                              int n, int nrhs, int64_t dstride, double2 *pos,
+                                              double *q, double *rec)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int g = perm[i];
     const RealBox &B = boxes.get(box_of(pt_box, g));
     const int o = g - B.s_base;
     const double x = B.psrc[2 * o], y = B.psrc[2 * o + 1];
@@ -927,17 +930,70 @@ __global__ void k_box_gather_sources(const BoxTable boxes,
     }
 }
 
+template <int KIND>
+__global__ void k_box_gather_sources(const BoxTable boxes,
+                                     const int *__restrict__ pt_box, const int *__restrict__ perm,
+                                     int n, int nrhs, int64_t dstride, double2 *__restrict__ pos,
+                                     double *__restrict__ q, double *__restrict__ rec)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gather_source<KIND>(boxes, pt_box, perm[i], i, n, nrhs, dstride, pos, q, rec);
+}
+
+__device__ __forceinline__ void gather_target(const BoxTable &boxes, const int *pt_box, int g, int i,
+                                              double2 *out)
+{
+    const RealBox &B = boxes.get(box_of(pt_box, g));
+    const int o = g - B.t_base;
+    out[i] = make_double2(B.ptar[2 * o], B.ptar[2 * o + 1]);
+}
+
 __global__ void k_box_gather_targets(const BoxTable boxes,
                                      const int *__restrict__ pt_box, const int *__restrict__ perm,
                                      int n, double2 *__restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int g = perm[i];
-    const RealBox &B = boxes.get(box_of(pt_box, g));
-    const int o = g - B.t_base;
-    out[i] = make_double2(B.ptar[2 * o], B.ptar[2 * o + 1]);
+    gather_target(boxes, pt_box, perm[i], i, out);
 }
+
+// Fused-sort functors (se2p_fsort.h): cell keys, and the gathers run by the
+// final scatter.
+struct CellKeyFn {
+    BoxTable boxes;
+    const int *pt_box;
+    int which;
+    __device__ int operator()(int i) const { return box_cell_key(boxes, pt_box, which, i); }
+};
+
+// Sources: also the bounding box of every key run (as k_run_bbox).
+template <int KIND>
+struct SrcGather {
+    static constexpr bool kBox = true;
+    BoxTable boxes;
+    const int *pt_box;
+    int n, nrhs;
+    int64_t dstride;
+    double2 *pos;
+    double *q, *rec;
+    double4 *box;
+    __device__ double2 operator()(int i, int g, int) const
+    {
+        gather_source<KIND>(boxes, pt_box, g, i, n, nrhs, dstride, pos, q, rec);
+        const RealBox &B = boxes.get(box_of(pt_box, g));
+        const int o = g - B.s_base;
+        return make_double2(B.psrc[2 * o], B.psrc[2 * o + 1]);
+    }
+};
+
+struct TgtGather {
+    static constexpr bool kBox = false;
+    BoxTable boxes;
+    const int *pt_box;
+    double2 *pos;
+    __device__ void operator()(int i, int g, int) const { gather_target(boxes, pt_box, g, i, pos); }
+};
 
 
 // ---- gathers ------------------------------------------------------------
@@ -1098,31 +1154,61 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
         expand_map(st, offs + 2 * (nbox + 1), nbox, cell_box);
     }
     // Sources and targets in (cell, Morton sub-cell) order, stable: a cell's
-    // sources are summed in a fixed order on every run.
-    if (ns > 0)
-        hipLaunchKernelGGL(k_box_cell_keys, dim3((unsigned)((ns + threads - 1) / threads)),
-                           dim3(threads), 0, st, btab, s_box, (int)ns, 0, kss.key);
-    if (int rc = keysort_run(ctx, &kss, ns, nkeys)) return rc;
-    if (!same_points) {
-        hipLaunchKernelGGL(k_box_cell_keys, dim3((unsigned)((nt + threads - 1) / threads)),
-                           dim3(threads), 0, st, btab, t_box, (int)nt, 1, kst.key);
-        if (int rc = keysort_run(ctx, &kst, nt, nkeys)) return rc;
-    }
+    // sources are summed in a fixed order on every run.  The fused sort
+    // (se2p_fsort.h) also gathers the sorted data and plans the chunks of 64
+    // targets; otherwise separate launches do.
     const KeySort &tks = same_points ? kss : kst;
-    if (ns > 0) {
-        const unsigned blocks = (unsigned)((ns + threads - 1) / threads);
+    const PlanArgs plan{nullptr, tks.off, sub, 64, (int)ncell, coff, bmap};
+    bool planned = false, boxed = false;
+    const bool box_kernel = (ctx->sort_mode & SE2P_SORT_BOX_KERNEL) != 0;
+    if (fsort_ok(ctx, ns, nkeys)) {
+        FsPlanExtra x0{plan, plan, same_points ? 1 : 0};
+        const CellKeyFn kf{btab, s_box, 0};
+        int rc;
         if (kind == 0)
-            hipLaunchKernelGGL(k_box_gather_sources<0>, dim3(blocks), dim3(threads), 0, st, btab,
-                               s_box, kss.perm, (int)ns, nrhs, dstride, spos, q, rec);
+            rc = fsort_run(ctx, &kss, ns, nkeys, kf,
+                           SrcGather<0>{btab, s_box, (int)ns, nrhs, dstride, spos, q, rec, sbox}, x0,
+                           FsNoExtra{});
         else
-            hipLaunchKernelGGL(k_box_gather_sources<1>, dim3(blocks), dim3(threads), 0, st, btab,
-                               s_box, kss.perm, (int)ns, nrhs, dstride, spos, q, rec);
+            rc = fsort_run(ctx, &kss, ns, nkeys, kf,
+                           SrcGather<1>{btab, s_box, (int)ns, nrhs, dstride, spos, q, rec, sbox}, x0,
+                           FsNoExtra{});
+        if (rc) return rc;
+        planned = same_points;
+        boxed = !box_kernel;
+    } else {
+        if (ns > 0)
+            hipLaunchKernelGGL(k_box_cell_keys, dim3((unsigned)((ns + threads - 1) / threads)),
+                               dim3(threads), 0, st, btab, s_box, (int)ns, 0, kss.key);
+        if (int rc = keysort_run(ctx, &kss, ns, nkeys)) return rc;
+        if (ns > 0) {
+            const unsigned blocks = (unsigned)((ns + threads - 1) / threads);
+            if (kind == 0)
+                hipLaunchKernelGGL(k_box_gather_sources<0>, dim3(blocks), dim3(threads), 0, st, btab,
+                                   s_box, kss.perm, (int)ns, nrhs, dstride, spos, q, rec);
+            else
+                hipLaunchKernelGGL(k_box_gather_sources<1>, dim3(blocks), dim3(threads), 0, st, btab,
+                                   s_box, kss.perm, (int)ns, nrhs, dstride, spos, q, rec);
+        }
     }
-    if (!same_points)
-        hipLaunchKernelGGL(k_box_gather_targets, dim3((unsigned)((nt + threads - 1) / threads)),
-                           dim3(threads), 0, st, btab, t_box, tks.perm, (int)nt, tpos);
-    plan_chunks(st, nullptr, tks.off, sub, 64, (int)ncell, coff, bmap);   // chunks of 64 targets
-    hipLaunchKernelGGL(k_run_bbox, dim3((nkeys + 3) / 4), dim3(256), 0, st, spos, kss.off, nkeys, sbox);
+    if (!same_points) {
+        if (fsort_ok(ctx, nt, nkeys)) {
+            if (int rc = fsort_run(ctx, &kst, nt, nkeys, CellKeyFn{btab, t_box, 1},
+                                   TgtGather{btab, t_box, tpos}, FsPlanExtra{plan, plan, 1},
+                                   FsNoExtra{}))
+                return rc;
+            planned = true;
+        } else {
+            hipLaunchKernelGGL(k_box_cell_keys, dim3((unsigned)((nt + threads - 1) / threads)),
+                               dim3(threads), 0, st, btab, t_box, (int)nt, 1, kst.key);
+            if (int rc = keysort_run(ctx, &kst, nt, nkeys)) return rc;
+            hipLaunchKernelGGL(k_box_gather_targets, dim3((unsigned)((nt + threads - 1) / threads)),
+                               dim3(threads), 0, st, btab, t_box, tks.perm, (int)nt, tpos);
+        }
+    }
+    if (!planned) plan_chunks(st, nullptr, tks.off, sub, 64, (int)ncell, coff, bmap);   // chunks of 64 targets
+    if (!boxed)
+        hipLaunchKernelGGL(k_run_bbox, dim3((nkeys + 3) / 4), dim3(256), 0, st, spos, kss.off, nkeys, sbox);
     stage_end(ctx, ST_SORT);
 
     RealArgs a{};

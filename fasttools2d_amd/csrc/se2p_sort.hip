@@ -14,6 +14,7 @@
 //             block's four rounds) and writes them to their final slots.
 // A final launch turns the sorted keys into run offsets (lower bound per key).
 #include "se2p_device.h"
+#include "se2p_fsort.h"
 #include "se2p_internal.h"
 
 namespace se2p {
@@ -266,36 +267,12 @@ int cell_sort(se2p_ctx *ctx, const double *p, int64_t n, double Lx, double Ly, i
     return keysort_run(ctx, ks, n, nkeys);
 }
 
-// Chunk plans, one workgroup per plan (blockIdx.x selects plan 0 or 1):
-// out[0..n] = exclusive scan of the counts and map[k] = c for every k in
-// [out[c], out[c+1]).  The count of item c is cnt[c], or with cnt == null
-// ceil((roff[(c+1)*sub] - roff[c*sub]) / per), the chunks of `per` points of
-// a run.  Replaces a count + scan + expand sequence with one launch.
-struct PlanArgs {
-    const int *cnt, *roff;
-    int sub, per, n;
-    int *out, *map;
-};
-
+// Chunk plans (plan_body), one workgroup per plan (blockIdx.x selects plan 0
+// or 1).  Replaces a count + scan + expand sequence with one launch.
 __global__ __launch_bounds__(256) void k_plan(PlanArgs p0, PlanArgs p1)
 {
     __shared__ int wsum[4];
-    const PlanArgs &p = blockIdx.x == 0 ? p0 : p1;
-    int carry = 0;
-    for (int base = 0; base < p.n; base += 256) {
-        const int c = base + threadIdx.x;
-        int v = 0;
-        if (c < p.n)
-            v = p.cnt ? p.cnt[c] : (p.roff[(c + 1) * p.sub] - p.roff[c * p.sub] + p.per - 1) / p.per;
-        int tot;
-        const int ex = block_exclusive_scan_256(v, wsum, &tot);
-        if (c < p.n) {
-            p.out[c] = carry + ex;
-            for (int k = 0; k < v; ++k) p.map[carry + ex + k] = c;
-        }
-        carry += tot;
-    }
-    if (threadIdx.x == 0) p.out[p.n] = carry;
+    plan_body<4>(blockIdx.x == 0 ? p0 : p1, wsum);
 }
 
 void plan_chunks(hipStream_t st, const int *cnt, const int *roff, int sub, int per, int n, int *out,

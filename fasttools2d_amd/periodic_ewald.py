@@ -84,6 +84,15 @@ class Context:
         device tensors and need the default (False)."""
         self._check(self._lib.se2p_set_host_pointers(self.handle, 1 if flag else 0))
 
+    SORT_RADIX = 1       # force the multi-pass radix sort
+    SORT_BOX_KERNEL = 2  # separate run-bounding-box kernel
+
+    def set_sort_mode(self, mode: int) -> None:
+        """Sorting of cell lists and bins: 0 (default) uses the fused sort
+        where it fits; SORT_RADIX / SORT_BOX_KERNEL force the separate
+        kernels.  Every mode gives bitwise identical results."""
+        self._check(self._lib.se2p_set_sort_mode(self.handle, int(mode)))
+
     def stage_times(self) -> dict:
         buf = (ctypes.c_float * len(_capi.STAGES))()
         self._check(self._lib.se2p_stage_times(self.handle, buf, len(_capi.STAGES)))

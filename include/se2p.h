@@ -74,6 +74,15 @@ int se2p_set_async(se2p_ctx *ctx, int async_flag);
  * passed twice, such as targets that are the sources, is copied once.  The
  * default (0) is device pointers. */
 int se2p_set_host_pointers(se2p_ctx *ctx, int enable);
+/* Sorting of the cell lists and gridding bins.  Point sets of up to 229376
+ * points with up to 14336 keys are sorted by a fused kernel sequence that
+ * also gathers the sorted data (mode 0, the default); larger ones by a
+ * multi-pass radix sort.  SE2P_SORT_RADIX forces the radix sort and
+ * SE2P_SORT_BOX_KERNEL the separate run-bounding-box kernel, for validation:
+ * every mode gives the same, bitwise identical results. */
+#define SE2P_SORT_RADIX 1
+#define SE2P_SORT_BOX_KERNEL 2
+int se2p_set_sort_mode(se2p_ctx *ctx, int mode);
 /* Per-stage device timing with HIP events on the context stream.  When
  * enabled, se2p_stage_times fills up to n floats (milliseconds) for the
  * most recent call: sort, real, spread, fft_fwd, filter, fft_inv, interp

@@ -139,6 +139,15 @@ def test_kspace_validation_codes(hctx):
     assert lib.se2p_dlp_kspace(h, x, 4, x, 4, 10.0, 0.5, x, None, M, M, L, L, w, P, u) == _capi.SE2P_ERR_ARG
 
 
+def test_sort_mode_option(hctx):
+    lib, h = hctx
+    for mode in (0, 1, 2, 3):
+        assert lib.se2p_set_sort_mode(h, mode) == 0
+    assert lib.se2p_set_sort_mode(h, 4) == _capi.SE2P_ERR_ARG
+    assert b"sort mode" in lib.se2p_last_error(h)
+    assert lib.se2p_set_sort_mode(None, 0) == _capi.SE2P_ERR_ARG
+
+
 def test_real_validation_codes(hctx):
     lib, h = hctx
     x = _fake(1)
