@@ -49,6 +49,84 @@ def _pmc_traffic(kernel_tag: str):
         return None
 
 
+def _host_cpu():
+    """Logical CPUs, physical cores and threads per core of this host (from
+    /proc/cpuinfo), or None where it cannot be read."""
+    try:
+        phys, cores, logical = set(), set(), 0
+        cur = {}
+        with open("/proc/cpuinfo") as fh:
+            for line in list(fh) + [""]:
+                if not line.strip():
+                    if cur:
+                        logical += 1
+                        cores.add((cur.get("physical id"), cur.get("core id")))
+                        phys.add(cur.get("physical id"))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+        return {"logical_cpus": logical, "physical_cores": len(cores), "sockets": len(phys),
+                "smt": logical > len(cores)}
+    except OSError:
+        return None
+
+
+def _stage_model(N, Nt, G, P, pairs_in, pairs_out, stage_ms):
+    """Algorithmic flops and HBM bytes per stage of one stresslet step, each
+    stage's roofline time max(bytes / HBM peak, flops / FP64 peak) and its
+    fraction of the measured stage time.  K = (P+1)^2 nodes per point, c_g = 3
+    grids, c_in = 4 input components per source (f, n)."""
+    import math
+
+    K, cg, cin = (P + 1) ** 2, 3, 4
+    lg = math.log2(G)
+    model = {
+        "real": (pairs_in * F_PAIR_IN + pairs_out * F_PAIR_OUT, N * 8 * (2 + cin) + 32 * Nt,
+                 ["real_call.real"]),
+        "spread": (N * K * (2 + 2 * cg), N * 8 * (2 + cin) + 8 * G * cg, ["kspace_call.spread"]),
+        "fft_fwd": (cg * 2.5 * G * lg, cg * 24 * G, ["kspace_call.fft_fwd"]),
+        "filter": (40 * G, 16 * G * (cg + 2), ["kspace_call.filter"]),
+        "fft_inv": (2 * 2.5 * G * lg, 2 * 24 * G, ["kspace_call.fft_inv"]),
+        "interp": (Nt * K * 6, 16 * G + 32 * Nt, ["kspace_call.interp"]),
+        "sort": (0.0, 0.0, ["real_call.sort", "kspace_call.sort"]),
+    }
+    out, t_model = {}, 0.0
+    for name, (fl, by, keys) in model.items():
+        ms = sum(stage_ms.get(k, 0.0) for k in keys)
+        tm = max(by / (PEAK_HBM_GBS * 1e9), fl / (PEAK_FP64_TFLOPS * 1e12)) * 1e3
+        t_model += tm
+        out[name] = {"flops": fl, "hbm_bytes": by, "measured_ms": ms, "model_ms": tm,
+                     "frac": tm / ms if ms > 0 else None,
+                     "tflops": fl / (ms * 1e-3) / 1e12 if ms > 0 else None,
+                     "hbm_gbs": by / (ms * 1e-3) / 1e9 if ms > 0 else None}
+    return out, t_model
+
+
+def _direct_ewald_error(np, ur, uk):
+    """Relative max-norm error of the step's output against the committed FP64
+    direct Ewald sum on 256 targets (tests/golden/truth_dlp_ellipses_N1e5.json),
+    per output; None if the fixture is absent."""
+    path = os.path.join(ROOT, "tests", "golden", "truth_dlp_ellipses_N1e5.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    sub = np.array(d["targets"])
+    tr, tk = np.array(d["u_real"]), np.array(d["u_kspace"])
+    tt = tr + tk
+    u = ur[sub] + uk[sub]
+    scale = np.abs(tt).max()
+    return {"u": float(np.abs(u - tt).max() / scale),
+            "u1": float(np.abs(u[:, 0] - tt[:, 0]).max() / np.abs(tt[:, 0]).max()),
+            "u2": float(np.abs(u[:, 1] - tt[:, 1]).max() / np.abs(tt[:, 1]).max()),
+            "real_part": float(np.abs(ur[sub] - tr).max() / scale),
+            "fourier_part": float(np.abs(uk[sub] - tk).max() / scale),
+            "targets": len(sub),
+            "truth": "3x3-image real sum + direct Fourier sum, j in (-256,256]^2 (FP64)"}
+
+
 def _batched_variant(args, pe, W, np, torch, dev, nrhs=64):
     """Per-RHS throughput of 64 densities on the 1000-point unit-box geometry
     (grid 64^2, tol 1e-10), Stokeslet + stresslet, as 64 plain calls of the
@@ -171,7 +249,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the plain-vs-batched right-hand-side comparison")
     ap.add_argument("--no-adjoint", action="store_true",
@@ -273,38 +351,57 @@ def main() -> int:
     ms_per_step = gpu_ms / args.steps
     value = N * args.steps / (gpu_ms * 1e-3)
 
-    # parity of the measured result against the CPU oracle on a target subset
-    from oracle import cref
+    ur_h, uk_h = ur.cpu().numpy(), uk.cpu().numpy()
+    accuracy = _direct_ewald_error(np, ur_h, uk_h)
+    stages_model, t_model = _stage_model(N, N, p.Mx * p.My, p.P, pairs_in, pairs_out, stage_ms)
 
-    rng = np.random.default_rng(55)
-    sub = rng.choice(N, 256, replace=False)
-    u_gpu = (ur + uk).cpu().numpy()[sub]
-    c_r = cref.dlp_real(w.psrc, w.ptar[sub], w.f, w.n, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly)
-    c_k = cref.dlp_kspace(w.psrc, w.ptar[sub], p.xi, p.eta, w.f, w.n, p.Mx, p.My, w.Lx, w.Ly,
-                          p.w, p.P)
-    c = c_r + c_k
-    parity = float(np.abs(u_gpu - c).max() / np.abs(c).max())
-
+    # CPU baseline leg: the oracle (the CPU restatement of the reference) on
+    # this host, which also checks the measured result on a target subset
     cpu = None
+    parity = None
     if not args.no_cpu:
+        from oracle import cref
+
+        sub = np.random.default_rng(55).choice(N, 256, replace=False)
+        c = (cref.dlp_real(w.psrc, w.ptar[sub], w.f, w.n, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly) +
+             cref.dlp_kspace(w.psrc, w.ptar[sub], p.xi, p.eta, w.f, w.n, p.Mx, p.My, w.Lx, w.Ly,
+                             p.w, p.P))
+        parity = float(np.abs((ur_h + uk_h)[sub] - c).max() / np.abs(c).max())
         threads = cref.get_threads()
 
-        def cpu_step():
+        def cpu_real():
             cref.dlp_real(w.psrc, w.ptar, w.f, w.n, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly)
+
+        def cpu_kspace():
             cref.dlp_kspace(w.psrc, w.ptar, p.xi, p.eta, w.f, w.n, p.Mx, p.My, w.Lx, w.Ly, p.w, p.P)
 
-        cpu_step()   # warm-up
-        times = []
-        for _ in range(args.cpu_runs):
-            t0 = time.perf_counter()
-            cpu_step()
-            times.append(time.perf_counter() - t0)
-            print(f"# cpu run {len(times)}: {times[-1]:.3f} s", file=sys.stderr, flush=True)
-        med = statistics.median(times)
+        def timed(fn, runs, label):
+            ts = []
+            for _ in range(runs):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            print(f"# cpu {label}: {', '.join(f'{x:.3f}' for x in ts)} s", file=sys.stderr, flush=True)
+            return statistics.median(ts)
+
+        cpu_real()   # warm-up
+        cpu_kspace()
+        real_s = timed(cpu_real, args.cpu_runs, f"real, {threads} threads")
+        k_s = timed(cpu_kspace, args.cpu_runs, f"Fourier, {threads} threads")
+        med = real_s + k_s
+        # one thread, one run of each sum (a few seconds)
+        cref.set_threads(1)
+        real1 = timed(cpu_real, 1, "real, 1 thread")
+        k1 = timed(cpu_kspace, 1, "Fourier, 1 thread")
+        cref.set_threads(threads)
         cpu = {"value": N / med, "unit": "target evals/s", "cores": threads, "kind": "port",
                "sample": f"full workload (N={N} targets, real + Fourier sums), median of "
-                         f"{args.cpu_runs} runs after 1 warm-up, {threads} OpenMP threads",
-               "seconds_per_step": med}
+                         f"{args.cpu_runs} runs of each sum after 1 warm-up, {threads} OpenMP threads",
+               "seconds_per_step": med,
+               "stage_seconds": {"real": real_s, "fourier": k_s},
+               "one_thread": {"value": N / (real1 + k1), "seconds_per_step": real1 + k1,
+                              "stage_seconds": {"real": real1, "fourier": k1}},
+               "host_cpu": _host_cpu()}
 
     variants = None if args.no_variants else _batched_variant(args, pe, W, np, torch, dev)
     adjoint = None
@@ -335,7 +432,12 @@ def main() -> int:
                      "flops_per_launch": flops_real, "pairs_in": pairs_in, "pairs_out": pairs_out,
                      "avg_kernel_ms": real_ms},
         "stage_ms": stage_ms,
+        "stage_roofline": {"stages": stages_model, "model_ms": t_model,
+                           "frac": t_model / ms_per_step,
+                           "note": "model = sum over stages of max(HBM bytes / 8 TB/s, "
+                                   "flops / 78.6 TFLOP/s); sort stages have no model time"},
         "parity_vs_oracle": parity,
+        "error_vs_direct_ewald": accuracy,
         "cpu_baseline": cpu,
         "variants": variants,
         "forward_plus_adjoint": adjoint,
