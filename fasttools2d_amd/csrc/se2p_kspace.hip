@@ -375,7 +375,12 @@ __host__ __device__ inline size_t strip_lds_doubles(int P, int NQ)
     return batch > red ? batch : red;
 }
 
-template <int NQ>
+// WIDE (Mx, My >= P + 16): no node of a tile is reached through the periodic
+// wrap from the far side, so a source's reach over the tile is one interval
+// per axis and the test is one unsigned compare; the source loop has no
+// branch (a lane outside the support adds an exact zero), which lets the
+// LDS reads of the next sources run ahead.
+template <int NQ, bool WIDE>
 __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
 {
     constexpr int kSB = strip_batch(NQ);
@@ -404,6 +409,8 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
     const int nchunk = a.chunk_off[tile + 1] - a.chunk_off[tile];
     const int X0 = (tile / a.nty) * kTile, Y0 = (tile % a.nty) * kTile;
     fill_e1(E1, a.P, a.ch2);
+    if (WIDE)   // zero padding of the y rows, written once
+        for (size_t i = threadIdx.x; i < 4 * per_wave; i += 256) lds[i] = 0.0;
     if (threadIdx.x < 64) {
         // candidate runs of the tile (precomputed), prefix sums by wave 0
         const int l = threadIdx.x;
@@ -443,7 +450,49 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
     for (int bt = wave; bt < nbatch; bt += 4) {
         const int base = c0 + bt * kSB;
         while (rpref[rcur + 1] <= base) ++rcur;
-        if (lane < 2 * kSB) {
+        if (WIDE) {
+            // Fill with every lane: LPR lanes per weight row (rows 0..kSB-1:
+            // x rows, then the y rows), each lane one segment of the row,
+            // starting from e3^j0 by squaring; the first lane of an x row
+            // also writes the source's tile offsets and densities.  The y
+            // rows' zero padding was written once at the start.
+            constexpr int LPR = 64 / (2 * kSB);
+            const int row = lane / LPR, part = lane % LPR;
+            const bool isx = row < kSB;
+            const int sl = isx ? row : row - kSB;
+            const int v = base + sl;
+            if (v < c1) {
+                int r = rcur;
+                while (rpref[r + 1] <= v) ++r;
+                const int k = rstart[r] + (v - rpref[r]);
+                const int seg = (P1 + LPR - 1) / LPR;
+                const int j0 = part * seg, j1 = min(P1, j0 + seg);
+                const double e3 = a.s.fgg[(isx ? 1 : 2) * a.s.ns + k];
+                double w = isx ? a.s.fgg[k] : 1.0;
+                double pw = e3;
+                for (int e = j0; e; e >>= 1) {
+                    if (e & 1) w *= pw;
+                    pw *= pw;
+                }
+                double *dst = isx ? WX + sl * P1 : WY + sl * PW + kPad;
+                for (int j = j0; j < j1; ++j) {
+                    dst[j] = w * E1[j];
+                    w *= e3;
+                }
+                if (isx && part == 0) {
+                    // tile-relative offsets d = (X0 - ax) mod Mx taken in
+                    // (P - Mx, P]: column X0 + l is node l + d of the support
+                    const int2 an = a.s.anchor[k];
+                    int d = X0 - an.x, e = Y0 - an.y;
+                    d += d < 0 ? a.Mx : 0;
+                    e += e < 0 ? a.My : 0;
+                    AXY[2 * sl] = d <= a.P ? d : d - a.Mx;
+                    AXY[2 * sl + 1] = e <= a.P ? e : e - a.My;
+#pragma unroll
+                    for (int c = 0; c < NQ; ++c) Q[sl * NQ + c] = qg[(int64_t)c * a.s.ns + k];
+                }
+            }
+        } else if (lane < 2 * kSB) {
             const int sl = lane & (kSB - 1);
             const int v = base + sl;
             if (v < c1) {
@@ -477,6 +526,25 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
         }
         wave_lds_sync();
         const int nb = min(kSB, c1 - base);
+        if (WIDE) {
+            const int lx = lane & 15, ly4 = 4 * (lane >> 4);
+            const unsigned Pu = (unsigned)a.P, Pj = (unsigned)(a.P + kPad);
+#pragma unroll 4
+            for (int s = 0; s < nb; ++s) {
+                const unsigned iu = (unsigned)(lx + AXY[2 * s]);
+                const unsigned ju = (unsigned)(ly4 + AXY[2 * s + 1] + kPad);
+                const bool hit = iu <= Pu && ju <= Pj;
+                const double wx = WX[s * P1 + min(iu, Pu)];
+                const double wxe = hit ? wx : 0.0;
+                const double *wy = WY + s * PW + min(ju, Pj);   // zero-padded row
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double wgt = wxe * wy[kk];
+#pragma unroll
+                    for (int c = 0; c < NQ; ++c) acc[kk][c] += wgt * Q[s * NQ + c];
+                }
+            }
+        } else {
 #pragma unroll 2
         for (int s = 0; s < nb; ++s) {
             int i = X - AXY[2 * s];
@@ -493,6 +561,7 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
                     for (int c = 0; c < NQ; ++c) acc[k][c] += wgt * Q[s * NQ + c];
                 }
             }
+        }
         }
         wave_lds_sync();   // reads of this batch done before the next fill
     }
@@ -1312,10 +1381,15 @@ static int spread(se2p_ctx *ctx, const KLayout &L, const GridPass &gp, const Spr
 {
     hipStream_t st = ctx->stream;
     const dim3 tiles2d(L.ntx, L.nty, gp.ngrp);
+    const bool wide = L.Mx >= L.P + 16 && L.My >= L.P + 16;
 #define SE2P_SPREAD(NQV)                                                                           \
-    if (gp.strip) {                                                                                \
-        hipLaunchKernelGGL((k_spread_strip<NQV>), dim3((unsigned)gp.max_spread_chunks, gp.ngrp),   \
-                           dim3(256), gp.lds_strip, st, sa);                                       \
+    if (gp.strip && wide) {                                                                        \
+        hipLaunchKernelGGL((k_spread_strip<NQV, true>), dim3((unsigned)gp.max_spread_chunks,       \
+                           gp.ngrp), dim3(256), gp.lds_strip, st, sa);                             \
+        hipLaunchKernelGGL((k_spread_combine<NQV>), dim3(L.ntiles, gp.ngrp), dim3(64), 0, st, sa); \
+    } else if (gp.strip) {                                                                         \
+        hipLaunchKernelGGL((k_spread_strip<NQV, false>), dim3((unsigned)gp.max_spread_chunks,      \
+                           gp.ngrp), dim3(256), gp.lds_strip, st, sa);                             \
         hipLaunchKernelGGL((k_spread_combine<NQV>), dim3(L.ntiles, gp.ngrp), dim3(64), 0, st, sa); \
     } else if (L.wrap) {                                                                           \
         hipLaunchKernelGGL((k_spread<NQV, true>), tiles2d, dim3(256), gp.lds_gen, st, sa);        \
