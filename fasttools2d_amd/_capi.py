@@ -65,6 +65,7 @@ SIGNATURES = {
     "se2p_count_pairs": (_I, [_P, _P, _I64, _P, _I64, _I, _I, _D, _D,
                               ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "se2p_selftest_e1": (_D, [_D]),
+    "se2p_selftest_exp_neg": (_D, [_D]),
     "se2p_selftest_anchor": (None, [_D, _D, _D, _D, _D, _I, _P, _P, _P, _P]),
     "se2p_selftest_cell": (_I, [_D, _D, _D, _D, _I, _I]),
     "se2p_selftest_keysort": (_I, [_P, _P, _I64, _I, _P, _P]),

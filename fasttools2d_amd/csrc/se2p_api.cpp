@@ -977,6 +977,12 @@ double se2p_selftest_e1(double x)
     return se2p_e1(x, std::exp(-x), tab);
 }
 
+double se2p_selftest_exp_neg(double x)
+{
+    static const double tab[SE2P_EXP2_N] = SE2P_EXP2_TABLE;
+    return se2p_exp_neg(x, tab);
+}
+
 void se2p_selftest_anchor(double x, double y, double Lx, double Ly, double h, int P, int *mx,
                           int *my, double *px, double *py)
 {

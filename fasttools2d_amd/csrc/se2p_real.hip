@@ -295,16 +295,16 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
                                               const double4 *__restrict__ sbox)
 {
     __shared__ double e1tab[KIND == 0 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
-    __shared__ double etab[16];
+    __shared__ double etab[SE2P_EXP2_N];
     __shared__ double red[kWaves - 1][RB * 2][64];
     const int lane = threadIdx.x & 63;
     // wave index as a scalar so that the source loops and loads stay uniform
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (KIND == 0)
         for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
-    if (threadIdx.x < 16) {
-        const double t16[16] = SE2P_EXP2_TABLE;
-        etab[threadIdx.x] = t16[threadIdx.x];
+    if (threadIdx.x < SE2P_EXP2_N) {
+        const double t[SE2P_EXP2_N] = SE2P_EXP2_TABLE;
+        etab[threadIdx.x] = t[threadIdx.x];
     }
     __syncthreads();
 
@@ -350,26 +350,35 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
         for (int r = 0; r < RB; ++r) acc[r][0] = acc[r][1] = 0.0;
         const double *qb = qsrc + (int64_t)r0 * ncomp * qs;
         // one source: its record (single density) or position + density array
-        auto pair = [&](int s, double zx, double zy, bool own) {
-            double sx, sy, q0[kDlpComp];
+        // a source's position and (one right-hand side) its record values
+        struct Src {
+            double x, y, q[kDlpComp];
+        };
+        auto load = [&](int s) {
+            Src o;
             if (REC) {
                 const double *rp = srec + (int64_t)s * SrcRec<KIND>::n;
                 const double4 v0 = *(const double4 *)rp;
-                sx = v0.x;
-                sy = v0.y;
-                q0[0] = v0.z;
-                q0[1] = v0.w;
+                o.x = v0.x;
+                o.y = v0.y;
+                o.q[0] = v0.z;
+                o.q[1] = v0.w;
                 if (KIND == 1) {
                     const double4 v1 = *(const double4 *)(rp + 4);
-                    q0[2] = v1.x;
-                    q0[3] = v1.y;
-                    q0[4] = v1.z;
+                    o.q[2] = v1.x;
+                    o.q[3] = v1.y;
+                    o.q[4] = v1.z;
                 }
             } else {
                 const double2 sp = spos[s];
-                sx = sp.x;
-                sy = sp.y;
+                o.x = sp.x;
+                o.y = sp.y;
             }
+            return o;
+        };
+        auto pair = [&](const Src &src, int s, double zx, double zy, bool own) {
+            const double sx = src.x, sy = src.y;
+            const double *q0 = src.q;
             double d1, d2;
             if (own) {
                 // own cell, no image: both kernels' difference is src - tgt
@@ -413,12 +422,28 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
             }
         };
 
-        // own cell: every pair, no cutoff
-        {
-            const int s1 = soff[(cell + 1) * sub];
+        // sources s0 + w, s0 + w + kWaves, ... < s1; with records, the next
+        // source's values are loaded before the current pair is evaluated
+        // (the last load repeats the current source and is unused)
+        auto run = [&](int s, int s1, double zx, double zy, bool own) {
+            if constexpr (!REC) {   // several right-hand sides: plain loop
 #pragma unroll 2
-            for (int s = soff[cell * sub] + wave; s < s1; s += kWaves) pair(s, 0.0, 0.0, true);
-        }
+                for (; s < s1; s += kWaves) pair(load(s), s, zx, zy, own);
+                return;
+            }
+            if (s >= s1) return;
+            Src cur = load(s);
+            for (;;) {
+                const int sn = s + kWaves;
+                const Src nxt = load(sn < s1 ? sn : s);
+                pair(cur, s, zx, zy, own);
+                if (sn >= s1) break;
+                cur = nxt;
+                s = sn;
+            }
+        };
+        // own cell: every pair, no cutoff
+        run(soff[cell * sub] + wave, soff[(cell + 1) * sub], 0.0, 0.0, true);
         // eight neighbour cells, periodic image offsets, r^2 < cut2; the kept
         // runs of a cell are visited in order, wave w taking every 4th source
         // of their concatenation
@@ -437,9 +462,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
                 const int rb = ra + __builtin_ctz(~(bits >> ra));
                 bits &= ~((1u << rb) - 1u);
                 const int s0 = soff[sb * sub + ra], s1 = soff[sb * sub + rb];
-#pragma unroll 2
-                for (int s = s0 + ((wave - cnt) & (kWaves - 1)); s < s1; s += kWaves)
-                    pair(s, zx, zy, false);
+                run(s0 + ((wave - cnt) & (kWaves - 1)), s1, zx, zy, false);
                 cnt += s1 - s0;
             }
         }
@@ -622,15 +645,15 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
     constexpr int NV = SIDE == 0 ? 2 : (SIDE == 1 ? NS : 2 + NS);
     constexpr int SO = SIDE == 2 ? 2 : 0;   // offset of the source-side accumulators
     __shared__ double e1tab[KIND == 0 && SIDE >= 1 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
-    __shared__ double etab[16];
+    __shared__ double etab[SE2P_EXP2_N];
     __shared__ double red[kWaves - 1][NV][64];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (KIND == 0 && SIDE >= 1)
         for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
-    if (threadIdx.x < 16) {
-        const double t16[16] = SE2P_EXP2_TABLE;
-        etab[threadIdx.x] = t16[threadIdx.x];
+    if (threadIdx.x < SE2P_EXP2_N) {
+        const double t[SE2P_EXP2_N] = SE2P_EXP2_TABLE;
+        etab[threadIdx.x] = t[threadIdx.x];
     }
     __syncthreads();
 

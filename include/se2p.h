@@ -241,6 +241,8 @@ int se2p_count_pairs(se2p_ctx *ctx, const double *psrc, int64_t ns, const double
  * (same arithmetic as FindClosestNode, ewald_tools.cpp:86-114) and the cell
  * index (ewald_tools.cpp:28-35). */
 double se2p_selftest_e1(double x);
+/* exp(-x) as the real-space kernels evaluate it (x >= 0). */
+double se2p_selftest_exp_neg(double x);
 void se2p_selftest_anchor(double x, double y, double Lx, double Ly, double h, int P, int *mx,
                           int *my, double *px, double *py);
 int se2p_selftest_cell(double x, double y, double Lx, double Ly, int nside_x, int nside_y);

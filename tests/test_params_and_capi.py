@@ -170,6 +170,23 @@ def test_product_e1_against_mpmath():
     assert lib.se2p_selftest_e1(2000.0) == 0.0
 
 
+def test_product_exp_against_mpmath():
+    """The real-space kernels' table-driven exp(-x) (host build of the same
+    code) over the range the pair sums use (x = xi^2 r^2 below about 50), the
+    table seams (x = k ln2/64 and half-way) and large x."""
+    lib = _capi.load()
+    ln2_64 = float(mpmath.log(2) / 64)
+    xs = np.concatenate([[0.0, 1e-300, 1e-17, 1e-9], np.linspace(0, 60, 3001),
+                         ln2_64 * np.arange(0, 4000, 7), ln2_64 * (np.arange(0, 4000, 7) + 0.5),
+                         [100.0, 300.0, 700.0]])
+    worst = 0.0
+    for x in xs:
+        ref = mpmath.exp(-mpmath.mpf(float(x)))
+        got = lib.se2p_selftest_exp_neg(float(x))
+        worst = max(worst, float(abs(got - ref) / ref))
+    assert worst < 1.5e-15, worst
+
+
 def test_product_cell_index_matches_oracle_cells():
     from oracle import cref
     lib = _capi.load()
