@@ -120,8 +120,8 @@ int cell_sort(se2p_ctx *ctx, const double *p, int64_t n, double Lx, double Ly, i
               int sublog, KeySort *ks);
 // Exclusive scan of n ints into out[0..n] with one workgroup.
 void scan_ints(hipStream_t st, const int *a, int n, int *out);
-// map[k] = c for every k in [off[c], off[c+1]), c < n.
-void expand_map(hipStream_t st, const int *off, int n, int *map);
+// map[k] = c for every k in [off[c], off[c+1]), c < n; total = off[n].
+void expand_map(hipStream_t st, const int *off, int n, int total, int *map);
 // One launch: out[0..n] = exclusive scan of the counts, map[k] = owner c of
 // chunk k.  Counts are cnt[c], or (cnt null) ceil(run length / per) of the
 // runs roff[c*sub] .. roff[(c+1)*sub].

@@ -1172,9 +1172,9 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
         }
         SE2P_HIP(hipMemcpyAsync(dbox, h.data(), tb, hipMemcpyHostToDevice, st));
         SE2P_HIP(hipMemcpyAsync(offs, h.data() + tb, ob, hipMemcpyHostToDevice, st));
-        expand_map(st, offs, nbox, s_box);
-        expand_map(st, offs + nbox + 1, nbox, t_box);
-        expand_map(st, offs + 2 * (nbox + 1), nbox, cell_box);
+        expand_map(st, offs, nbox, (int)ns, s_box);
+        expand_map(st, offs + nbox + 1, nbox, (int)nt, t_box);
+        expand_map(st, offs + 2 * (nbox + 1), nbox, (int)ncell, cell_box);
     }
     // Sources and targets in (cell, Morton sub-cell) order, stable: a cell's
     // sources are summed in a fixed order on every run.  The fused sort

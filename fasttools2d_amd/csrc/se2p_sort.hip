@@ -233,16 +233,26 @@ int keysort_run(se2p_ctx *ctx, KeySort *ks, int64_t n, int nkeys)
 }
 
 // map[k] = c for k in [off[c], off[c+1]): workgroup index -> owning item.
-__global__ void k_expand_map(const int *__restrict__ off, int n, int *__restrict__ map)
+// map[k] = c for every k in [off[c], off[c+1]), k < total = off[n]: one thread
+// per entry, c by binary search over the n + 1 offsets (non-decreasing).
+__global__ void k_expand_map(const int *__restrict__ off, int n, int total, int *__restrict__ map)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    for (int k = off[c]; k < off[c + 1]; ++k) map[k] = c;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    int lo = 0, hi = n;   // off[lo] <= k < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    map[k] = lo;
 }
 
-void expand_map(hipStream_t st, const int *off, int n, int *map)
+void expand_map(hipStream_t st, const int *off, int n, int total, int *map)
 {
-    hipLaunchKernelGGL(k_expand_map, dim3((n + 255) / 256), dim3(256), 0, st, off, n, map);
+    if (total > 0)
+        hipLaunchKernelGGL(k_expand_map, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, off,
+                           n, total, map);
 }
 
 // cell key (ewald_tools.cpp:28-35); with sublog > 0 the key also carries a
