@@ -48,7 +48,7 @@ struct se2p_ctx {
     char *ws = nullptr;       // device workspace
     size_t ws_bytes = 0;
     std::string err;
-    std::map<std::tuple<int, int, int, int>, se2p::FftPlans> plans;  // (Mx, My, nfwd, ninv)
+    std::map<std::tuple<int, int, int, int, int>, se2p::FftPlans> plans;  // (Mx, My, nfwd, ninv, slot)
     // stage timing: up to kStageVisits (start, end) event pairs per stage and call
     static constexpr int kStageVisits = 4;
     hipEvent_t ev[se2p::ST_COUNT][kStageVisits][2] = {};
@@ -57,12 +57,25 @@ struct se2p_ctx {
     int64_t pairs_in = -1, pairs_out = -1;   // counted by se2p_count_pairs
     char *drv = nullptr;      // driver-level buffers (kept apart from ws)
     size_t drv_bytes = 0;
-    std::vector<char> host_stage;   // host copy of per-box tables (multi-box calls)
     bool host_ptrs = false;   // array arguments are host memory (se2p_set_host_pointers)
     char *io = nullptr;       // device staging area of host-pointer calls
     size_t io_bytes = 0;
     int *fs = nullptr;        // fused-sort state (se2p_fsort.h), zero between sorts
     int sort_mode = 0;        // SE2P_SORT_* bits
+    // Stream slots for independent work run side by side (multi-box Fourier
+    // sums): each has its own stream, workspace, sort state and FFT plans
+    // (plan_slot in the plan key).  Slot work forks from and joins into
+    // `stream` through events.
+    struct Slot {
+        hipStream_t st = nullptr;
+        char *ws = nullptr;
+        size_t ws_bytes = 0;
+        int *fs = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    std::vector<Slot> slots;
+    hipEvent_t fork_ev = nullptr;
+    int plan_slot = 0;        // 0: the context's own plans
 };
 
 namespace se2p {

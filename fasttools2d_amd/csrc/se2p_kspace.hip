@@ -1191,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_interp_adj_direct(InterpAdjArgs a)
 // ---- rocFFT plans ------------------------------------------------------------
 static int get_plans(se2p_ctx *ctx, int Mx, int My, int nfwd, int ninv, FftPlans **out)
 {
-    auto key = std::make_tuple(Mx, My, nfwd, ninv);
+    auto key = std::make_tuple(Mx, My, nfwd, ninv, ctx->plan_slot);
     auto it = ctx->plans.find(key);
     if (it != ctx->plans.end()) {
         *out = &it->second;

@@ -77,6 +77,34 @@ struct RealBox {
     double Lx, Ly, xi, xi2, cut2, self;
 };
 
+// Up to kBoxesPerLaunch box records by value (a kernel argument of about
+// 2 KB) and the kernel that writes them, with the box offset arrays
+// [s_base..., ns | t_base..., nt | cell_base..., ncell], to the device.
+constexpr int kBoxesPerLaunch = 16;
+struct BoxBatch {
+    RealBox b[kBoxesPerLaunch];
+    int first, count;
+};
+static_assert(sizeof(BoxBatch) < 3072, "kernel argument size");
+
+__global__ void k_write_boxes(BoxBatch bb, RealBox *__restrict__ tab, int *__restrict__ offs,
+                              int nbox, int ns, int nt, int ncell)
+{
+    const int i = threadIdx.x;
+    if (i < bb.count) {
+        const int g = bb.first + i;
+        tab[g] = bb.b[i];
+        offs[g] = bb.b[i].s_base;
+        offs[nbox + 1 + g] = bb.b[i].t_base;
+        offs[2 * (nbox + 1) + g] = bb.b[i].cell_base;
+    }
+    if (i == 0 && bb.first + bb.count == nbox) {
+        offs[nbox] = ns;
+        offs[2 * nbox + 1] = nt;
+        offs[3 * nbox + 2] = ncell;
+    }
+}
+
 // The box records as kernels see them: a device table for several boxes, or
 // the one record by value (no device copy, so a single-box call stays
 // capturable and needs no extra launch).
@@ -1159,19 +1187,17 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
         btab.one = boxes[0];
     } else {
         btab.tab = dbox;
-        // box table and the source / target / cell offsets of every box
-        std::vector<char> &h = ctx->host_stage;
-        const size_t tb = sizeof(RealBox) * nbox, ob = sizeof(int) * 3 * (size_t)(nbox + 1);
-        h.resize(tb + ob);
-        std::memcpy(h.data(), boxes, tb);
-        int *ho = (int *)(h.data() + tb);
-        for (int b = 0; b <= nbox; ++b) {
-            ho[b] = b < nbox ? boxes[b].s_base : (int)ns;
-            ho[nbox + 1 + b] = b < nbox ? boxes[b].t_base : (int)nt;
-            ho[2 * (nbox + 1) + b] = b < nbox ? boxes[b].cell_base : (int)ncell;
+        // box table and the source / target / cell offsets of every box,
+        // written by kernels that carry the records as arguments: no host
+        // copy enters the stream, so a captured sequence replays on its own
+        for (int b0 = 0; b0 < nbox; b0 += kBoxesPerLaunch) {
+            BoxBatch bb;
+            bb.first = b0;
+            bb.count = std::min(kBoxesPerLaunch, nbox - b0);
+            for (int i = 0; i < bb.count; ++i) bb.b[i] = boxes[b0 + i];
+            hipLaunchKernelGGL(k_write_boxes, dim3(1), dim3(kBoxesPerLaunch), 0, st, bb, dbox, offs,
+                               nbox, (int)ns, (int)nt, (int)ncell);
         }
-        SE2P_HIP(hipMemcpyAsync(dbox, h.data(), tb, hipMemcpyHostToDevice, st));
-        SE2P_HIP(hipMemcpyAsync(offs, h.data() + tb, ob, hipMemcpyHostToDevice, st));
         expand_map(st, offs, nbox, (int)ns, s_box);
         expand_map(st, offs + nbox + 1, nbox, (int)nt, t_box);
         expand_map(st, offs + 2 * (nbox + 1), nbox, (int)ncell, cell_box);
