@@ -183,6 +183,90 @@ def _batched_variant(args, pe, W, np, torch, dev, nrhs=64):
     return res
 
 
+def _multibox_variant(args, pe, W, np, torch, dev):
+    """The many-box workload: 32 independent periodic boxes, L ~ U[0.5, 4],
+    N ~ log-uniform in [1e3, 2e5] uniform points each (1.41e6 in all; seed 4),
+    per-box parameters from the driver formulas at tol 1e-10; one step is the
+    stresslet real-space sums of every box (one launch) and their Fourier
+    sums (box by box on four streams).  Timed as eager async calls and as a
+    replayed HIP graph of the same step."""
+    boxes = W.multibox_config(seed=4, nbox=32)
+    tb = []
+    for b in boxes:
+        d = dict(b)
+        for k in ("psrc", "f", "n"):
+            d[k] = torch.from_numpy(np.ascontiguousarray(b[k])).to(dev)
+        d["ptar"] = d["psrc"]
+        tb.append(d)
+    ntot = sum(len(b["psrc"]) for b in boxes)
+    ctx = pe.Context(0)
+    stream = torch.cuda.Stream(dev)
+
+    def step():
+        return (pe.stokes_dlp_real_multibox(tb, ctx=ctx) + pe.stokes_dlp_kspace_multibox(tb, ctx=ctx))
+
+    def timed(fn, reps=5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    with torch.cuda.stream(stream):
+        ctx.bind_torch_stream(stream)
+        ctx.set_async(True)
+        step()
+        step()
+        torch.cuda.synchronize()
+        eager_ms = timed(step)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            step()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        graph_ms = timed(g.replay)
+        ctx.set_async(False)
+    print(f"# multibox variant: eager {eager_ms:.3f} ms, graph {graph_ms:.3f} ms", file=sys.stderr,
+          flush=True)
+    return {"eager": {"ms_per_step": eager_ms, "target_evals_per_s": ntot / (eager_ms * 1e-3)},
+            "graph": {"ms_per_step": graph_ms, "target_evals_per_s": ntot / (graph_ms * 1e-3)},
+            "workload": f"32 boxes, {ntot} points in all (targets = sources), stresslet real + Fourier sums, "
+                        "per-box parameters at tol 1e-10"}
+
+
+def _graph_variant(pe, torch, step, ctx, stream, N, reps=10):
+    """The headline step captured once in a HIP graph and replayed, against
+    the same step as async calls."""
+    with torch.cuda.stream(stream):
+        ctx.bind_torch_stream(stream)
+        ctx.set_async(True)
+        step()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            step()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        res = {}
+        for name, fn in (("async", step), ("graph", g.replay)):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            res[name] = {"ms_per_step": ms, "target_evals_per_s": N / (ms * 1e-3)}
+        ctx.set_async(False)
+    print(f"# graph variant: async {res['async']['ms_per_step']:.3f} ms, graph "
+          f"{res['graph']['ms_per_step']:.3f} ms", file=sys.stderr, flush=True)
+    return res
+
+
 def _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, fwd_ms):
     """Forward + adjoint on the headline workload: one step is the two forward
     sums plus the two backward entry points with every gradient (dL/dpsrc,
@@ -403,7 +487,11 @@ def main() -> int:
                               "stage_seconds": {"real": real1, "fourier": k1}},
                "host_cpu": _host_cpu()}
 
-    variants = None if args.no_variants else _batched_variant(args, pe, W, np, torch, dev)
+    variants = None
+    if not args.no_variants:
+        variants = _batched_variant(args, pe, W, np, torch, dev)
+        variants["multibox"] = _multibox_variant(args, pe, W, np, torch, dev)
+        variants["graph"] = _graph_variant(pe, torch, step, ctx, stream, N)
     adjoint = None
     if not args.no_adjoint:
         adjoint = _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, ms_per_step)
