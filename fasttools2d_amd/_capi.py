@@ -90,6 +90,18 @@ SIGNATURES.update({
                             _P, _P, _P]),
 })
 
+class Se2pTuneInfo(ctypes.Structure):
+    """se2p_tune_info of include/se2p.h."""
+    _fields_ = [("ncand", ctypes.c_int), ("best", ctypes.c_int), ("cached", ctypes.c_int),
+                ("P", ctypes.c_int), ("Nb", ctypes.c_double * 16), ("nside_x", ctypes.c_int * 16),
+                ("Mx", ctypes.c_int * 16), ("ms", ctypes.c_double * 16)]
+
+
+SIGNATURES.update({
+    "se2p_autotune": (_I, [_P, _I, _P, _I64, _P, _I64, _P, _P, _D, _D, ctypes.POINTER(Se2pParams),
+                           ctypes.POINTER(Se2pTuneInfo)]),
+})
+
 class Se2pBoxDesc(ctypes.Structure):
     """se2p_box_desc of include/se2p.h."""
     _fields_ = [("psrc", ctypes.c_void_p), ("ns", ctypes.c_int64),

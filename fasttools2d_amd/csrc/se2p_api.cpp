@@ -923,16 +923,48 @@ static int ensure_driver_buffer(se2p_ctx *ctx, size_t bytes)
     return 0;
 }
 
-// Shared body of the two driver entry points.
-static int ewald_full(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const double *ptar,
-                      int64_t nt, const double *f, const double *n, double Lx, double Ly,
-                      se2p_params *prm, double *u, double *ur_out, double *uk_out)
+// Driver-level device buffers: the wrapped points, the real- and
+// Fourier-space outputs and the reduction scratch of launch_driver_sums.
+struct DrvBufs {
+    double *src, *tar, *ur, *uk, *partial, *sums;
+};
+
+static int driver_buffers(se2p_ctx *ctx, int64_t ns, int64_t nt, bool same, DrvBufs *b)
+{
+    auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const size_t s16 = up(16 * (size_t)(ns > 0 ? ns : 1)), t16 = up(16 * (size_t)(nt > 0 ? nt : 1));
+    const int64_t nb = (ns + 255) / 256;
+    const size_t need = s16 + (same ? 0 : t16) + 2 * t16 + up(24 * (size_t)(nb + 1)) + up(24);
+    if (int rc = ensure_driver_buffer(ctx, need)) return rc;
+    size_t o = 0;
+    b->src = (double *)(ctx->drv + o); o += s16;
+    b->tar = b->src;
+    if (!same) { b->tar = (double *)(ctx->drv + o); o += t16; }
+    b->ur = (double *)(ctx->drv + o); o += t16;
+    b->uk = (double *)(ctx->drv + o); o += t16;
+    b->partial = (double *)(ctx->drv + o); o += up(24 * (size_t)(nb + 1));
+    b->sums = (double *)(ctx->drv + o);
+    return 0;
+}
+
+// Shared checks of the driver-level entries.
+static int check_driver(se2p_ctx *ctx, int64_t ns, int64_t nt, double Lx, double Ly,
+                        const se2p_params *prm)
 {
     if (!prm) return fail(ctx, SE2P_ERR_ARG, "null parameter struct");
     if (ns + nt < 2) return fail(ctx, SE2P_ERR_ARG, "need at least two points in total");
     if (!(Lx > 0) || !(Ly > 0) || !std::isfinite(Lx) || !std::isfinite(Ly))
         return fail(ctx, SE2P_ERR_PARAM, "Lx and Ly must be positive and finite");
     if (prm->P < 0 || (prm->P % 2) != 0) return fail(ctx, SE2P_ERR_P, "P must be even");
+    return 0;
+}
+
+// Shared body of the two driver entry points.
+static int ewald_full(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const double *ptar,
+                      int64_t nt, const double *f, const double *n, double Lx, double Ly,
+                      se2p_params *prm, double *u, double *ur_out, double *uk_out)
+{
+    if (int rc = check_driver(ctx, ns, nt, Lx, Ly, prm)) return rc;
     if (int rc = set_device(ctx)) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
@@ -951,39 +983,181 @@ static int ewald_full(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, c
     ur_out = io.out(ur_out);
     uk_out = io.out(uk_out);
     hipStream_t st = ctx->stream;
-    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
     const bool same = (psrc == ptar) && (ns == nt);
-    const int64_t nb = (ns + 255) / 256;
-    const size_t need = up(16 * (size_t)(ns > 0 ? ns : 1)) + (same ? 0 : up(16 * (size_t)(nt > 0 ? nt : 1))) +
-                        2 * up(16 * (size_t)(nt > 0 ? nt : 1)) + up(24 * (size_t)(nb + 1)) + up(24);
-    if (int rc = ensure_driver_buffer(ctx, need)) return rc;
-    size_t o = 0;
-    double *ws_src = (double *)(ctx->drv + o); o += up(16 * (size_t)(ns > 0 ? ns : 1));
-    double *ws_tar = ws_src;
-    if (!same) { ws_tar = (double *)(ctx->drv + o); o += up(16 * (size_t)(nt > 0 ? nt : 1)); }
-    double *ur = (double *)(ctx->drv + o); o += up(16 * (size_t)(nt > 0 ? nt : 1));
-    double *uk = (double *)(ctx->drv + o); o += up(16 * (size_t)(nt > 0 ? nt : 1));
-    double *partial = (double *)(ctx->drv + o); o += up(24 * (size_t)(nb + 1));
-    double *sums = (double *)(ctx->drv + o);
-    if (ur_out) ur = ur_out;
-    if (uk_out) uk = uk_out;
+    DrvBufs b;
+    if (int rc = driver_buffers(ctx, ns, nt, same, &b)) return rc;
+    double *ur = ur_out ? ur_out : b.ur;
+    double *uk = uk_out ? uk_out : b.uk;
 
     // Q = sum(f.^2) + 1 and, for the DLP, sum((f.n) x) over the caller's points
-    launch_driver_sums(st, f, kind == 1 ? n : nullptr, psrc, ns, partial, sums);
+    launch_driver_sums(st, f, kind == 1 ? n : nullptr, psrc, ns, b.partial, b.sums);
     double hs[3];
-    SE2P_HIP(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, st));
+    SE2P_HIP(hipMemcpyAsync(hs, b.sums, sizeof(hs), hipMemcpyDeviceToHost, st));
     SE2P_HIP(hipStreamSynchronize(st));
     select_params(kind, ns, nt, hs[0] + 1.0, Lx, Ly, prm);
 
-    launch_wrap(st, psrc, ns, Lx, Ly, ws_src);
-    if (!same) launch_wrap(st, ptar, nt, Lx, Ly, ws_tar);
-    RealGeom rg{ws_src, ws_tar, ns, nt, prm->xi, Lx, Ly, prm->nside_x, prm->nside_y};
+    launch_wrap(st, psrc, ns, Lx, Ly, b.src);
+    if (!same) launch_wrap(st, ptar, nt, Lx, Ly, b.tar);
+    RealGeom rg{b.src, b.tar, ns, nt, prm->xi, Lx, Ly, prm->nside_x, prm->nside_y};
     if (int rc = real_sum(ctx, kind, rg, f, n, 1, 2 * ns, 2 * nt, ur, nullptr)) return rc;
-    KGeom kg{ws_src, ws_tar, ns, nt, prm->xi, prm->eta, Lx, Ly, prm->w, prm->Mx, prm->My, prm->P};
+    KGeom kg{b.src, b.tar, ns, nt, prm->xi, prm->eta, Lx, Ly, prm->w, prm->Mx, prm->My, prm->P};
     if (int rc = check_kspace_params(ctx, kg)) return rc;
     if (int rc = kspace_sum(ctx, kind, kg, f, n, 1, 2 * ns, 2 * nt, uk)) return rc;
-    launch_compose(st, ur, uk, nt, kind, sums, Lx * Ly, u);
+    launch_compose(st, ur, uk, nt, kind, b.sums, Lx * Ly, u);
     return io.done(finish(ctx));
+}
+
+// ---- autotuner -----------------------------------------------------------------
+namespace {
+struct TuneKey {
+    int dev, kind, P;
+    int64_t ns, nt;
+    int same;
+    double Lx, Ly, tol;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(dev, kind, P, ns, nt, same, Lx, Ly, tol) <
+               std::tie(o.dev, o.kind, o.P, o.ns, o.nt, o.same, o.Lx, o.Ly, o.tol);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, double> g_tune_nb;   // key -> chosen points per cell
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// Restores the context's timing flag when tuning ends.
+struct TimingOff {
+    se2p_ctx *ctx;
+    bool was;
+    explicit TimingOff(se2p_ctx *c) : ctx(c), was(c->timing) { c->timing = false; }
+    ~TimingOff() { ctx->timing = was; }
+};
+}  // namespace
+
+static int autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const double *ptar,
+                    int64_t nt, const double *f, const double *n, double Lx, double Ly,
+                    se2p_params *prm, se2p_tune_info *info)
+{
+    constexpr int kReps = 3, kBlocks = 3;
+    constexpr double kMaxGrid = 4096.0 * 4096.0;
+    if (int rc = check_driver(ctx, ns, nt, Lx, Ly, prm)) return rc;
+    if (!(prm->tol > 0) || !(prm->tol < 1))
+        return fail(ctx, SE2P_ERR_PARAM, "tol must be in (0, 1)");
+    if (prm->M != 0) return fail(ctx, SE2P_ERR_ARG, "M must be 0: the grid follows from the tuned cell size");
+    if (info) std::memset(info, 0, sizeof(*info));
+    const int P = prm->P > 0 ? prm->P : 24;   // the drivers' default support
+    if (info) info->P = P;
+    if (int rc = set_device(ctx)) return rc;
+    HostIo io(ctx);
+    io.want(psrc, 16 * (size_t)ns);
+    io.want(ptar, 16 * (size_t)nt);
+    io.want(f, 16 * (size_t)ns);
+    io.want(n, 16 * (size_t)ns);
+    if (int rc = io.map()) return rc;
+    psrc = io.in(psrc);
+    ptar = io.in(ptar);
+    f = io.in(f);
+    n = io.in(n);
+    hipStream_t st = ctx->stream;
+    const bool same = (psrc == ptar) && (ns == nt);
+    DrvBufs b;
+    if (int rc = driver_buffers(ctx, ns, nt, same, &b)) return rc;
+    launch_driver_sums(st, f, kind == 1 ? n : nullptr, psrc, ns, b.partial, b.sums);
+    double hs[3];
+    SE2P_HIP(hipMemcpyAsync(hs, b.sums, sizeof(hs), hipMemcpyDeviceToHost, st));
+    SE2P_HIP(hipStreamSynchronize(st));
+    const double Q = hs[0] + 1.0;
+    const double tol = prm->tol;
+    auto params_for = [&](double Nb, se2p_params *c) {
+        *c = se2p_params{};
+        c->P = P;
+        c->Nb = Nb;
+        c->tol = tol;
+        select_params(kind, ns, nt, Q, Lx, Ly, c);
+    };
+    const TuneKey key{ctx->device, kind, prm->P, ns, nt, same ? 1 : 0, Lx, Ly, tol};
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune_nb.find(key);
+        if (it != g_tune_nb.end()) {
+            params_for(it->second, prm);
+            if (info) info->cached = 1;
+            return io.done(SE2P_OK);
+        }
+    }
+
+    // candidates around the drivers' default points per cell, one per cell grid
+    const double npts = (double)(ns + nt);
+    const double Nb0 = std::min(P * std::round(std::log2(npts)), npts / 4);
+    std::vector<se2p_params> cand;
+    for (int k = -6; k <= 2 && (int)cand.size() < SE2P_TUNE_MAX; ++k) {
+        se2p_params c;
+        params_for(Nb0 * std::pow(2.0, 0.5 * k), &c);
+        if ((double)c.Mx * (double)c.My > kMaxGrid) continue;
+        bool dup = false;
+        for (const se2p_params &o : cand) dup = dup || (o.nside_x == c.nside_x && o.nside_y == c.nside_y);
+        if (!dup) cand.push_back(c);
+    }
+    if (cand.empty()) return fail(ctx, SE2P_ERR_PARAM, "no candidate grid within 4096^2 points");
+
+    launch_wrap(st, psrc, ns, Lx, Ly, b.src);
+    if (!same) launch_wrap(st, ptar, nt, Lx, Ly, b.tar);
+    TimingOff quiet(ctx);
+    EventPair ev;
+    SE2P_HIP(hipEventCreate(&ev.a));
+    SE2P_HIP(hipEventCreate(&ev.b));
+    int best = 0;
+    double best_ms = std::numeric_limits<double>::infinity();
+    for (size_t i = 0; i < cand.size(); ++i) {
+        const se2p_params &c = cand[i];
+        const RealGeom rg{b.src, b.tar, ns, nt, c.xi, Lx, Ly, c.nside_x, c.nside_y};
+        const KGeom kg{b.src, b.tar, ns, nt, c.xi, c.eta, Lx, Ly, c.w, c.Mx, c.My, c.P};
+        if (int rc = check_kspace_params(ctx, kg)) return rc;
+        auto step = [&]() -> int {
+            if (int rc = real_sum(ctx, kind, rg, f, n, 1, 2 * ns, 2 * nt, b.ur, nullptr)) return rc;
+            return kspace_sum(ctx, kind, kg, f, n, 1, 2 * ns, 2 * nt, b.uk);
+        };
+        if (int rc = step()) return rc;   // sizes the workspace, makes the FFT plans
+        SE2P_HIP(hipStreamSynchronize(st));
+        // the best of kBlocks blocks of kReps back-to-back steps
+        double t = std::numeric_limits<double>::infinity();
+        for (int blk = 0; blk < kBlocks; ++blk) {
+            SE2P_HIP(hipEventRecord(ev.a, st));
+            for (int r = 0; r < kReps; ++r)
+                if (int rc = step()) return rc;
+            SE2P_HIP(hipEventRecord(ev.b, st));
+            SE2P_HIP(hipEventSynchronize(ev.b));
+            float ms = 0.0f;
+            SE2P_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+            t = std::min(t, (double)ms / kReps);
+        }
+        if (info) {
+            info->Nb[i] = c.Nb;
+            info->nside_x[i] = c.nside_x;
+            info->Mx[i] = c.Mx;
+            info->ms[i] = t;
+        }
+        if (t < best_ms) {
+            best_ms = t;
+            best = (int)i;
+        }
+    }
+    SE2P_HIP(hipGetLastError());
+    *prm = cand[(size_t)best];
+    if (info) {
+        info->ncand = (int)cand.size();
+        info->best = best;
+    }
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune_nb[key] = prm->Nb;
+    return io.done(SE2P_OK);
 }
 
 int se2p_slp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
@@ -1006,6 +1180,19 @@ int se2p_dlp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *
     if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
     if (ns > 0 && (!f || !n)) return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
     return ewald_full(ctx, 1, psrc, ns, ptar, nt, f, n, Lx, Ly, prm, u, ur, uk);
+}
+
+int se2p_autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const double *ptar,
+                  int64_t nt, const double *f, const double *n, double Lx, double Ly,
+                  se2p_params *prm, se2p_tune_info *info)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (kind != 0 && kind != 1) return fail(ctx, SE2P_ERR_ARG, "kind must be 0 (SLP) or 1 (DLP)");
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, ptar)) return rc;
+    if (ns > 0 && (!f || (kind == 1 && !n)))
+        return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
+    return autotune(ctx, kind, psrc, ns, ptar, nt, f, n, Lx, Ly, prm, info);
 }
 
 int se2p_count_pairs(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,

@@ -491,6 +491,37 @@ def _driver(kind, psrc, ptar, f, n, Lx, Ly, P, Nb, tol, M, ctx, same):
     return u, ur, uk, prm
 
 
+def autotune(kind, psrc, ptar, f, n, Lx, Ly, *, tol, P=0, ctx=None):
+    """Driver parameters for one geometry chosen by timing (se2p_autotune).
+
+    ``kind`` is 'slp' or 'dlp'; points are the caller's (N, 2) arrays
+    (unwrapped, as for the drivers); ``ptar is psrc`` means targets are the
+    sources.  P = 0 takes the drivers' support, 24.  Returns
+    ``(params, info)``: the se2p_params to pass on (``Nb`` and ``P`` set, so
+    ``StokesSLP_ewald_2p(..., P=params.P, Nb=params.Nb, tol=tol)`` reproduces
+    them) and a dict of the candidates timed."""
+    k = {"slp": 0, "dlp": 1}[kind]
+    same = ptar is psrc
+    dev = _device_of(psrc, ptar, f, n) if n is not None else _device_of(psrc, ptar, f)
+    ps = _as_points(psrc, "psrc", dev)
+    pt = ps if same else _as_points(ptar, "ptar", dev)
+    ff = _as_points(f, "f", dev)
+    nn = _as_points(n, "n", dev) if k == 1 else None
+    ctx = _prepare(ctx, dev)
+    prm = _capi.Se2pParams()
+    prm.P = int(P)
+    prm.tol = float(tol)
+    info = _capi.Se2pTuneInfo()
+    ctx._check(ctx._lib.se2p_autotune(ctx.handle, k, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0],
+                                      _ptr(ff), _ptr(nn) if nn is not None else None, float(Lx),
+                                      float(Ly), ctypes.byref(prm), ctypes.byref(info)))
+    nc = info.ncand
+    out = {"P": info.P, "cached": bool(info.cached), "best": info.best,
+           "candidates": [{"Nb": info.Nb[i], "nside_x": info.nside_x[i], "Mx": info.Mx[i],
+                           "ms": info.ms[i]} for i in range(nc)]}
+    return prm, out
+
+
 def _cols(*arrays):
     return np.stack([np.asarray(a, dtype=np.float64).ravel() for a in arrays], axis=1)
 

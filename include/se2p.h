@@ -227,6 +227,40 @@ int se2p_dlp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *
                    const double *f, const double *n, double Lx, double Ly, se2p_params *prm,
                    double *u, double *ur, double *uk);
 
+/* ---- parameter autotuning (no reference counterpart) --------------------- */
+#define SE2P_TUNE_MAX 16
+typedef struct se2p_tune_info {
+    int ncand;                 /* candidates timed (0 on a cache hit) */
+    int best;                  /* index of the chosen candidate */
+    int cached;                /* 1: the choice came from the per-process cache */
+    int P;                     /* the Gaussian support used (prm->P, or the drivers' 24) */
+    double Nb[SE2P_TUNE_MAX];  /* candidate points per cell */
+    int nside_x[SE2P_TUNE_MAX];
+    int Mx[SE2P_TUNE_MAX];
+    double ms[SE2P_TUNE_MAX];  /* measured real + Fourier step of each candidate, ms */
+} se2p_tune_info;
+
+/* Chooses the drivers' free parameter for one geometry at prm->tol
+ * (required, 0 < tol < 1): the points per cell Nb, which sets the cell grid
+ * (nside), the splitting xi and the Fourier grid through the driver formulas
+ * (StokesSLP_ewald_2p.m:33-104, StokesDLP_ewald_2p.m:33-125).  The real- plus
+ * Fourier-space sums are timed on this context for Nb = Nb0 * 2^(k/2),
+ * k = -6..2, around the drivers' default Nb0 = min(P*round(log2(Ns+Nt)),
+ * (Ns+Nt)/4), each as the best of three blocks of three back-to-back
+ * steps after a warm-up step, and the fastest is kept.  Every candidate meets the drivers'
+ * truncation estimates at tol; candidates whose grid exceeds 4096^2 points
+ * are skipped.  P is prm->P, or the drivers' 24 when 0.  prm->M must be 0
+ * (the grid follows from the cell size).  On return *prm holds the chosen
+ * candidate's parameters as se2p_select_params fills them for its Nb and P:
+ * passing *prm on to se2p_slp_ewald / se2p_dlp_ewald reproduces them.
+ * Choices are cached per process by (device, kind, Ns, Nt,
+ * targets-are-sources, Lx, Ly, tol, P requested); a cache hit times nothing.
+ * Points are the caller's (unwrapped) points, as for the driver entries.
+ * Synchronous.  info may be NULL. */
+int se2p_autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const double *ptar,
+                  int64_t nt, const double *f, const double *n, double Lx, double Ly,
+                  se2p_params *prm, se2p_tune_info *info);
+
 /* ---- pair statistics (no reference counterpart) ------------------------- */
 /* Number of source-target pairs the real-space sum evaluates (pairs_in: same
  * cell, or neighbour cell with r^2 < cutoff^2) and rejects (pairs_out).  Used

@@ -1,0 +1,97 @@
+"""se2p_autotune on the GPU.
+
+* The stresslet benchmark (200 ellipses x 500 nodes, N = 1e5) at tol 1e-10:
+  the tuned parameters (points per cell by timing, the drivers' P = 24)
+  evaluated through the MEX-equivalent entries on the 256 targets of
+  tests/golden/truth_dlp_ellipses_N1e5.json meet 1e-10 against the FP64
+  direct Ewald sum (real + Fourier parts are xi-independent in total).
+* A Stokeslet box with separate targets and a 1 x 2 aspect: the driver entry
+  at the tuned parameters reproduces them and matches the oracle composition
+  at 1e-12.
+* A larger Stokeslet box at tol 1e-8: the tuned driver result agrees with a
+  run at tol 1e-13 to 1e-8 (the Ewald total does not depend on the split).
+* A second call with the same geometry is served from the cache.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from fasttools2d_amd import workloads as W
+from tests.conftest import ROOT, relerr
+from tests.test_gpu_driver import _oracle_full
+
+pytestmark = pytest.mark.gpu
+
+TRUTH = os.path.join(ROOT, "tests", "golden", "truth_dlp_ellipses_N1e5.json")
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def test_autotune_stresslet_benchmark_meets_tol(gpu):
+    pe = gpu
+    d = json.load(open(TRUTH))
+    w, _ = W.dlp_ellipses_config()
+    ps, f, n = _t(w.psrc), _t(w.f), _t(w.n)
+    print("[autotune dlp ellipses N=1e5 tol 1e-10]", flush=True)
+    prm, info = pe.autotune("dlp", ps, ps, f, n, 1.0, 1.0, tol=1e-10)
+    assert info["P"] == 24 and prm.P == 24 and not info["cached"]
+    ms = [c["ms"] for c in info["candidates"]]
+    assert len(ms) >= 4 and info["best"] == int(np.argmin(ms))
+    assert prm.nside_x == info["candidates"][info["best"]]["nside_x"]
+    print(f"  candidates {info['candidates']}", flush=True)
+    pt = _t(w.ptar[np.array(d["targets"])])
+    ur = pe.mex_stokes_dlp_real(ps, pt, f, n, prm.xi, prm.nside_x, prm.nside_y, 1.0, 1.0)
+    uk = pe.mex_stokes_dlp_kspace(ps, pt, prm.xi, prm.eta, f, n, prm.Mx, prm.My, 1.0, 1.0,
+                                  prm.w, prm.P)
+    truth = np.array(d["u_real"]) + np.array(d["u_kspace"])
+    err = relerr((ur + uk).cpu().numpy(), truth)
+    print(f"  tuned Nb {prm.Nb:.1f} nside {prm.nside_x} M {prm.Mx}: error vs direct Ewald "
+          f"{err:.3e}", flush=True)
+    assert err <= 1e-10
+    prm2, info2 = pe.autotune("dlp", ps, ps, f, n, 1.0, 1.0, tol=1e-10)
+    assert info2["cached"] and info2["candidates"] == []
+    assert (prm2.Nb, prm2.xi, prm2.Mx, prm2.nside_x) == (prm.Nb, prm.xi, prm.Mx, prm.nside_x)
+
+
+def test_autotune_separate_targets_driver_parity(gpu):
+    pe = gpu
+    rng = np.random.default_rng(41)
+    Lx, Ly = 1.0, 2.0
+    xs = rng.uniform(0.0, 1.0, (3000, 2)) * [Lx, Ly]
+    xt = rng.uniform(-0.5, 1.5, (1000, 2)) * [Lx, Ly]
+    f = rng.standard_normal((3000, 2))
+    print("[autotune slp 3000 sources / 1000 targets, 1x2 box]", flush=True)
+    prm, info = pe.autotune("slp", _t(xs), _t(xt), _t(f), None, Lx, Ly, tol=1e-10)
+    assert prm.My == 2 * prm.Mx and prm.nside_y == 2 * prm.nside_x
+    u1, u2, ur, uk, xi, prm2 = pe.StokesSLP_ewald_2p(xs[:, 0], xs[:, 1], xt[:, 0], xt[:, 1],
+                                                     f[:, 0], f[:, 1], Lx, Ly, P=prm.P,
+                                                     Nb=prm.Nb, tol=1e-10, return_params=True)
+    assert (prm2.xi, prm2.Mx, prm2.My, prm2.nside_x, prm2.P) == \
+        (prm.xi, prm.Mx, prm.My, prm.nside_x, prm.P)
+    u = np.stack([u1.cpu().numpy(), u2.cpu().numpy()], 1)
+    assert relerr(u, _oracle_full("slp", xs, xt, f, None, Lx, Ly, prm)) <= 1e-12
+
+
+def test_autotune_slp_matches_tighter_tolerance(gpu):
+    pe = gpu
+    w = W.uniform_box(20000, seed=43, L=1.3)
+    x = w.psrc_unwrapped
+    print("[autotune slp N=2e4 L=1.3 tol 1e-8]", flush=True)
+    xt = _t(x)
+    prm, info = pe.autotune("slp", xt, xt, _t(w.f), None, 1.3, 1.3, tol=1e-8)
+    assert info["P"] == 24
+    kw = dict(ctx=None, return_params=True)
+    a = pe.StokesSLP_ewald_2p(x[:, 0], x[:, 1], x[:, 0], x[:, 1], w.f[:, 0], w.f[:, 1], 1.3, 1.3,
+                              P=prm.P, Nb=prm.Nb, tol=1e-8, **kw)
+    b = pe.StokesSLP_ewald_2p(x[:, 0], x[:, 1], x[:, 0], x[:, 1], w.f[:, 0], w.f[:, 1], 1.3, 1.3,
+                              P=24, tol=1e-13, **kw)
+    ua = np.stack([a[0].cpu().numpy(), a[1].cpu().numpy()], 1)
+    ub = np.stack([b[0].cpu().numpy(), b[1].cpu().numpy()], 1)
+    err = relerr(ua, ub)
+    print(f"  tuned Nb {prm.Nb:.1f} M {prm.Mx}: vs tol 1e-13 run {err:.3e}", flush=True)
+    assert err <= 1e-8
