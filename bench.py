@@ -270,7 +270,10 @@ def _graph_variant(pe, torch, step, ctx, stream, N, reps=10):
 def _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, fwd_ms):
     """Forward + adjoint on the headline workload: one step is the two forward
     sums plus the two backward entry points with every gradient (dL/dpsrc,
-    dL/dptar, dL/df, dL/dn) for a seeded du."""
+    dL/dptar, dL/df, dL/dn) for a seeded du.  Measured twice: with the
+    forward calls keeping their sorted state for the backward calls
+    (se2p_set_keep, the training pattern; the reported step), and with the
+    backward calls rebuilding it."""
     N = ps.shape[0]
     du = torch.from_numpy(np.random.default_rng(77).standard_normal((N, 2))).to(ps.device)
     ur = torch.empty_like(du)
@@ -288,9 +291,7 @@ def _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, fwd_
                                  ctx=ctx, out=uk)
         backward()
 
-    with torch.cuda.stream(stream):
-        ctx.bind_torch_stream(stream)
-        ctx.set_async(True)
+    def timed_steps():
         for _ in range(max(1, args.warmup)):
             step()
         torch.cuda.synchronize()
@@ -301,11 +302,18 @@ def _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, fwd_
             step()
         e1.record(stream)
         torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        # stage split of the two backward calls
-        ctx.set_timing(True)
+        return e0.elapsed_time(e1) / reps
+
+    def stage_split():
+        # forward calls (keeping state when enabled), then the backward calls timed
+        ctx.set_async(False)
         stages = {}
         for _ in range(3):
+            pe.mex_stokes_dlp_real(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly, ctx=ctx,
+                                   out=ur)
+            pe.mex_stokes_dlp_kspace(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, w.Lx, w.Ly, p.w, p.P,
+                                     ctx=ctx, out=uk)
+            ctx.set_timing(True)
             pe.mex_stokes_dlp_real_backward(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly,
                                             du, ctx=ctx)
             for k, v in ctx.stage_times().items():
@@ -316,15 +324,32 @@ def _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, fwd_
             for k, v in ctx.stage_times().items():
                 if v >= 0:
                     stages.setdefault("kspace_backward." + k, []).append(v)
-        ctx.set_timing(False)
+            ctx.set_timing(False)
+        return {k: statistics.mean(v) for k, v in stages.items()}
+
+    with torch.cuda.stream(stream):
+        ctx.bind_torch_stream(stream)
+        ctx.set_async(True)
+        ms_rebuild = timed_steps()
+        st_rebuild = stage_split()
+        ctx.set_keep(True)
+        ctx.set_async(True)
+        ms = timed_steps()
+        st_keep = stage_split()
+        ctx.set_keep(False)
         ctx.set_async(False)
     torch.cuda.synchronize()
-    print(f"# forward+adjoint: {ms:.3f} ms per step", file=sys.stderr, flush=True)
+    print(f"# forward+adjoint: {ms:.3f} ms per step (kept forward state), {ms_rebuild:.3f} ms "
+          f"(rebuilt)", file=sys.stderr, flush=True)
     return {"ms_per_step": ms, "target_evals_per_s": N / (ms * 1e-3),
+            "over_forward": ms / fwd_ms,
             "adjoint_over_forward_time": (ms - fwd_ms) / fwd_ms,
-            "stage_ms": {k: statistics.mean(v) for k, v in stages.items()},
+            "stage_ms": st_keep,
+            "rebuilt": {"ms_per_step": ms_rebuild, "over_forward": ms_rebuild / fwd_ms,
+                        "stage_ms": st_rebuild},
             "workload": "same as the headline; step = forward real + Fourier sums, then both "
-                        "backward entry points with all four gradients"}
+                        "backward entry points with all four gradients; the forward calls keep "
+                        "their sorted state for the backward calls (se2p_set_keep)"}
 
 
 def _autotune_variant(args, pe, np, torch, w, ps, f, nv, ctx, stream, fwd_ms):

@@ -40,6 +40,7 @@ SIGNATURES = {
     "se2p_set_async": (_I, [_P, _I]),
     "se2p_set_host_pointers": (_I, [_P, _I]),
     "se2p_set_sort_mode": (_I, [_P, _I]),
+    "se2p_set_keep": (_I, [_P, _I]),
     "se2p_set_timing": (_I, [_P, _I]),
     "se2p_stage_times": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
     "se2p_slp_real": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _I, _I, _D, _D, _P]),

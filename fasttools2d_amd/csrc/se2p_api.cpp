@@ -281,6 +281,8 @@ int se2p_destroy(se2p_ctx *ctx)
         if (p.inv_info) rocfft_execution_info_destroy(p.inv_info);
         if (p.work) hipFree(p.work);
     }
+    free_kept_real(ctx);
+    free_kept_k(ctx);
     if (ctx->ws) hipFree(ctx->ws);
     if (ctx->drv) hipFree(ctx->drv);
     if (ctx->io) hipFree(ctx->io);
@@ -347,6 +349,18 @@ int se2p_set_host_pointers(se2p_ctx *ctx, int enable)
 {
     if (!ctx) return SE2P_ERR_ARG;
     ctx->host_ptrs = enable != 0;
+    return SE2P_OK;
+}
+
+int se2p_set_keep(se2p_ctx *ctx, int enable)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    ctx->keep = enable != 0;
+    if (!ctx->keep && ctx->device != SE2P_HOST_ONLY) {
+        if (ctx->stream) hipStreamSynchronize(ctx->stream);
+        free_kept_real(ctx);
+        free_kept_k(ctx);
+    }
     return SE2P_OK;
 }
 
@@ -666,8 +680,18 @@ static int ensure_slots(se2p_ctx *ctx, int n)
 // Every box is computed exactly as a single-box call computes it.
 constexpr int kKspaceSlots = 4;
 
+// Restores the context's keep flag: multi-box calls keep no forward state
+// (their boxes run on several streams).
+struct KeepOff {
+    se2p_ctx *ctx;
+    bool was;
+    explicit KeepOff(se2p_ctx *c) : ctx(c), was(c->keep) { c->keep = false; }
+    ~KeepOff() { ctx->keep = was; }
+};
+
 static int kspace_multibox(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int nbox)
 {
+    KeepOff no_keep(ctx);
     auto run = [&](const se2p_box_desc &b) {
         KGeom g{b.psrc, b.ptar, b.ns, b.nt, b.xi, b.eta, b.Lx, b.Ly, b.w, b.Mx, b.My, b.P};
         return kspace_sum(ctx, kind, g, b.f, b.n, 1, 2 * b.ns, 2 * b.nt, b.u);
@@ -1110,6 +1134,7 @@ static int autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, con
     launch_wrap(st, psrc, ns, Lx, Ly, b.src);
     if (!same) launch_wrap(st, ptar, nt, Lx, Ly, b.tar);
     TimingOff quiet(ctx);
+    KeepOff no_keep(ctx);
     EventPair ev;
     SE2P_HIP(hipEventCreate(&ev.a));
     SE2P_HIP(hipEventCreate(&ev.b));

@@ -37,6 +37,11 @@ struct FftPlans {
     void *work = nullptr;   // shared work buffer (hipMalloc, freed with the context)
 };
 
+// State a forward call keeps for its backward call (se2p_set_keep); defined
+// in se2p_real.hip and se2p_kspace.hip.
+struct KeptReal;
+struct KeptK;
+
 }  // namespace se2p
 
 struct se2p_ctx {
@@ -76,6 +81,9 @@ struct se2p_ctx {
     std::vector<Slot> slots;
     hipEvent_t fork_ev = nullptr;
     int plan_slot = 0;        // 0: the context's own plans
+    bool keep = false;        // forward calls keep their sorted state (se2p_set_keep)
+    se2p::KeptReal *kept_real = nullptr;
+    se2p::KeptK *kept_k = nullptr;
 };
 
 namespace se2p {
@@ -93,6 +101,10 @@ void stage_begin(se2p_ctx *ctx, Stage s);
 void stage_end(se2p_ctx *ctx, Stage s);
 
 int finish(se2p_ctx *ctx);   // error check + optional synchronisation
+
+// Release the kept forward state (se2p_real.hip, se2p_kspace.hip).
+void free_kept_real(se2p_ctx *ctx);
+void free_kept_k(se2p_ctx *ctx);
 
 // Bump allocator over the workspace, 256-byte aligned pieces.  With a null
 // base it only measures: run the same sequence of take() calls once to size

@@ -1457,6 +1457,37 @@ static int transform_filter(se2p_ctx *ctx, const KLayout &L, const KGeom &g, Fft
     return 0;
 }
 
+// State a forward call keeps for its backward call (se2p_set_keep): one
+// single-density, targets-are-sources call, identified by its arguments.
+// The bins, Gaussian factors, work plans and the filtered grids live in a
+// buffer of their own.
+struct KeptK {
+    char *buf = nullptr;
+    size_t bytes = 0;
+    bool valid = false;
+    int kind = 0;
+    KGeom g{};
+    const double *f = nullptr, *nv = nullptr;
+    KeySort kss{};
+    GridPass gp{};
+    double *grids = nullptr;   // the two filtered grids
+    bool match(int k, const KGeom &q, const double *ff, const double *nn) const
+    {
+        return valid && k == kind && q.psrc == g.psrc && q.ptar == g.psrc && q.ns == g.ns &&
+               q.nt == g.ns && ff == f && (k == 0 || nn == nv) && q.xi == g.xi && q.eta == g.eta &&
+               q.Lx == g.Lx && q.Ly == g.Ly && q.w == g.w && q.Mx == g.Mx && q.My == g.My &&
+               q.P == g.P;
+    }
+};
+
+void free_kept_k(se2p_ctx *ctx)
+{
+    if (!ctx->kept_k) return;
+    if (ctx->kept_k->buf) (void)hipFree(ctx->kept_k->buf);
+    delete ctx->kept_k;
+    ctx->kept_k = nullptr;
+}
+
 // ---- forward -------------------------------------------------------------------
 int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *nv,
                int nrhs, int64_t dstride, int64_t ustride, double *u)
@@ -1503,11 +1534,30 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         spec = cv.take<double2>((size_t)ncomp * nrp * L.GH);
         carve_pass(cv, L, &gp);
     };
+    // a call whose backward may follow keeps its state in a buffer of its own
+    const bool keep = ctx->keep && !ctx->host_ptrs && same_points && nrhs == 1;
     Carve sizing;
     layout(sizing);
-    if (int rc = ensure_workspace(ctx, sizing.off)) return rc;
     Carve cv;
-    cv.base = ctx->ws;
+    if (keep) {
+        if (!ctx->kept_k) ctx->kept_k = new KeptK;
+        KeptK &kk = *ctx->kept_k;
+        kk.valid = false;
+        if (sizing.off > kk.bytes) {
+            if (kk.buf) {
+                SE2P_HIP(hipStreamSynchronize(st));
+                SE2P_HIP(hipFree(kk.buf));
+                kk.buf = nullptr;
+                kk.bytes = 0;
+            }
+            SE2P_HIP(hipMalloc((void **)&kk.buf, sizing.off));
+            kk.bytes = sizing.off;
+        }
+        cv.base = kk.buf;
+    } else {
+        if (int rc = ensure_workspace(ctx, sizing.off)) return rc;
+        cv.base = ctx->ws;
+    }
     layout(cv);
 
     // Bins of the sources (and of the targets unless they are the sources).
@@ -1577,6 +1627,17 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
 
     if (int rc = transform_filter(ctx, L, g, plans, kind == 0 ? FILTER_SLP : FILTER_DLP, nrp, grids, spec))
         return rc;
+    if (keep) {
+        KeptK &kk = *ctx->kept_k;
+        kk.kind = kind;
+        kk.g = g;
+        kk.f = f;
+        kk.nv = nv;
+        kk.kss = kss;
+        kk.gp = gp;
+        kk.grids = grids;
+        kk.valid = true;
+    }
 
     stage_begin(ctx, ST_INTERP);
     if (L.itile) {
@@ -1672,13 +1733,23 @@ int kspace_adjoint(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, con
     const KLayout L = make_layout(g);
     const int ncomp = kind == 0 ? 2 : 3;
     const bool same_points = (g.ptar == g.psrc) && (nt == ns);
+    // the forward call's state, when it kept a matching one: bins, plans and
+    // the filtered grids the target pass interpolates
+    const KeptK *kept = same_points && ctx->kept_k && ctx->kept_k->match(kind, g, f, nv)
+                            ? ctx->kept_k : nullptr;
     GridPass gt, gs;   // target pass, source pass
     plan_pass(L, ncomp, 1, ns, nt, &gt);
     plan_pass(L, 2, 1, nt, ns, &gs);
+    // the source pass spreads over the same bins with the same chunking as
+    // the forward: its work plans are the forward's
+    const bool share_maps = kept && gs.strip == kept->gp.strip && gs.chunk == kept->gp.chunk &&
+                            gs.max_spread_chunks == kept->gp.max_spread_chunks &&
+                            gs.max_interp_chunks == kept->gp.max_interp_chunks;
+    if (kept) gt = kept->gp;
     const bool tile_t = interp_adj_lds(L.P, 2) <= kLdsMax;
     const bool tile_s = interp_adj_lds(L.P, ncomp) <= kLdsMax;
     FftPlans *pt = nullptr, *ps = nullptr;
-    if (dptar)
+    if (dptar && !kept)
         if (int rc = get_plans(ctx, L.Mx, L.My, ncomp, 2, &pt)) return rc;
     if (want_src)
         if (int rc = get_plans(ctx, L.Mx, L.My, 2, ncomp, &ps)) return rc;
@@ -1689,19 +1760,36 @@ int kspace_adjoint(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, con
     double *fgg, *q, *grids;
     double2 *spec;
     auto layout = [&](Carve &cv) {
-        char *p = cv.take<char>(keysort_bytes(ns, L.nbins));
-        if (p) keysort_carve(p, ns, L.nbins, &kss);
-        if (!same_points) {
-            p = cv.take<char>(keysort_bytes(nt, L.nbins));
-            if (p) keysort_carve(p, nt, L.nbins, &kst);
+        if (!kept) {
+            char *p = cv.take<char>(keysort_bytes(ns, L.nbins));
+            if (p) keysort_carve(p, ns, L.nbins, &kss);
+            if (!same_points) {
+                p = cv.take<char>(keysort_bytes(nt, L.nbins));
+                if (p) keysort_carve(p, nt, L.nbins, &kst);
+            }
         }
         anchor = cv.take<int2>(nmax);
         fgg = cv.take<double>(3 * (size_t)nmax);
         q = cv.take<double>((size_t)ncomp * nmax);
         grids = cv.take<double>((size_t)ncomp * L.G);
         spec = cv.take<double2>((size_t)ncomp * L.GH);
-        carve_pass(cv, L, &gt);
-        carve_pass(cv, L, &gs);
+        if (!kept) carve_pass(cv, L, &gt);
+        if (share_maps) {
+            GridPass keep_gs = gs;
+            keep_gs.scnt = kept->gp.scnt;
+            keep_gs.soffc = kept->gp.soffc;
+            keep_gs.icnt = kept->gp.icnt;
+            keep_gs.ioffc = kept->gp.ioffc;
+            keep_gs.rng_start = kept->gp.rng_start;
+            keep_gs.rng_len = kept->gp.rng_len;
+            keep_gs.smap = kept->gp.smap;
+            keep_gs.imap = kept->gp.imap;
+            keep_gs.partial = gs.strip ? cv.take<double>((size_t)256 * gs.nq * gs.max_spread_chunks)
+                                       : nullptr;
+            gs = keep_gs;
+        } else {
+            carve_pass(cv, L, &gs);
+        }
     };
     Carve sizing;
     layout(sizing);
@@ -1709,16 +1797,19 @@ int kspace_adjoint(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, con
     Carve cv;
     cv.base = ctx->ws;
     layout(cv);
+    if (kept) kss = kept->kss;
 
     stage_begin(ctx, ST_SORT);
-    if (int rc = bin_points(ctx, L, g, g.psrc, ns, &kss)) return rc;
-    if (!same_points)
-        if (int rc = bin_points(ctx, L, g, g.ptar, nt, &kst)) return rc;
+    if (!kept) {
+        if (int rc = bin_points(ctx, L, g, g.psrc, ns, &kss)) return rc;
+        if (!same_points)
+            if (int rc = bin_points(ctx, L, g, g.ptar, nt, &kst)) return rc;
+    }
     const KeySort &tbins = same_points ? kss : kst;
     SpreadArgs sat = spread_args(L, gt, kss.off, grids);     // sources -> grids
     SpreadArgs sas = spread_args(L, gs, tbins.off, grids);   // du at the targets -> grids
-    if (dptar && (gt.strip || tile_t)) tile_maps(st, L, gt, sat, tbins.off);
-    if (want_src && (gs.strip || tile_s)) tile_maps(st, L, gs, sas, kss.off);
+    if (dptar && !kept && (gt.strip || tile_t)) tile_maps(st, L, gt, sat, tbins.off);
+    if (want_src && !share_maps && (gs.strip || tile_s)) tile_maps(st, L, gs, sas, kss.off);
     stage_end(ctx, ST_SORT);
 
     InterpAdjArgs ia{};
@@ -1739,7 +1830,23 @@ int kspace_adjoint(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, con
     ia.ch2 = L.ch2;
     ia.scale = L.interp_scale;
 
-    if (dptar) {
+    if (dptar && kept) {
+        // the forward's filtered grids, interpolated with derivative weights
+        stage_begin(ctx, ST_INTERP);
+        InterpAdjArgs a = ia;
+        a.grids = kept->grids;
+        a.pts = g.ptar;
+        a.perm = tbins.perm;
+        a.boff = tbins.off;
+        a.chunk_off = gt.ioffc;
+        a.block_tile = gt.imap;
+        a.n = nt;
+        a.du = du;
+        a.dpos = dptar;
+        launch_interp_adj<2, ADJ_TARGET>(st, L, gt, tile_t, a);
+        SE2P_HIP(hipGetLastError());
+        stage_end(ctx, ST_INTERP);
+    } else if (dptar) {
         stage_begin(ctx, ST_SPREAD);
         const unsigned blocks = (unsigned)((ns + 255) / 256);
         if (kind == 0)

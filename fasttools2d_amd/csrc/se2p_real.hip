@@ -644,15 +644,8 @@ struct RealAdjArgs {
     double Lx, Ly, xi, xi2, cut2, self;
     const double *f, *nv;  // caller-order densities and normals (stresslet sources)
     double *dpos, *df, *dn;   // caller order; may be null
-    double *dpos_t;        // SIDE 2: dL/dptar (dpos is then dL/dpsrc)
-    const double *ldata2;  // SIDE 2: the lane points' densities, [ncomp][nl]
 };
 
-// SIDE 2 (targets are the sources, one sorted point set): both of the
-//   above in one pass.  For lane point i and uniform point j the target term
-//   (i as target, j as source) and the source term (i as source, j as target)
-//   use their own difference vectors, rounded as the forward sums round them,
-//   and their own inclusion tests; exp, 1/r^2 and E1 are evaluated once.
 // SIDE 0: lanes are targets (lane data du), uniform points are sources
 //   (uniform data: the densities); writes dL/dptar.
 // SIDE 1: lanes are sources (lane data: the densities), uniform points are
@@ -663,15 +656,13 @@ struct RealAdjArgs {
 template <int KIND, int SIDE>
 __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *__restrict__ upos,
                                                   const double *__restrict__ udata,
-                                                  const double *__restrict__ udata2,
                                                   const int *__restrict__ uoff,
                                                   const double4 *__restrict__ ubox)
 {
-    // accumulators: SIDE 0 [grad]; SIDE 1 [grad, density terms];
-    // SIDE 2 [target grad, source grad, density terms]
+    // accumulators: SIDE 0 [grad]; SIDE 1 [grad, density terms]
     constexpr int NS = KIND == 0 ? 4 : 5;
-    constexpr int NV = SIDE == 0 ? 2 : (SIDE == 1 ? NS : 2 + NS);
-    constexpr int SO = SIDE == 2 ? 2 : 0;   // offset of the source-side accumulators
+    constexpr int NV = SIDE == 0 ? 2 : NS;
+    constexpr int SO = 0;   // offset of the source-side accumulators
     __shared__ double e1tab[KIND == 0 && SIDE >= 1 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
     __shared__ double etab[SE2P_EXP2_N];
     __shared__ double red[kWaves - 1][NV][64];
@@ -694,17 +685,17 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
     const int l = l0 + lane;
     const bool active = l < l1;
     double lx, ly;
-    // lane data: SIDE 0 du; SIDE 1 densities; SIDE 2 du, then densities
-    double ld[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // lane data: SIDE 0 du; SIDE 1 densities
+    double ld[3] = {0.0, 0.0, 0.0};
     constexpr int NQ = KIND == 0 ? 2 : 3;
-    constexpr int LC = SIDE == 0 ? 2 : (SIDE == 1 ? NQ : 2 + NQ);
+    constexpr int LC = SIDE == 0 ? 2 : NQ;
     if (active) {
         const double2 p = a.lpos[l];
         lx = p.x;
         ly = p.y;
 #pragma unroll
         for (int c = 0; c < LC; ++c)
-            ld[c] = (SIDE == 2 && c >= 2) ? a.ldata2[(c - 2) * a.nl + l] : a.ldata[c * a.nl + l];
+            ld[c] = a.ldata[c * a.nl + l];
     } else {
         lx = ly = __builtin_nan("");   // inactive lanes compute garbage, never stored
     }
@@ -714,60 +705,8 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = 0.0;
 
-    // both sides of one pair (SIDE 2): udata = densities, udata2 = du
-    auto pair2 = [&](double ux, double uy, double zx, double zy, int u, bool own) {
-        if constexpr (SIDE == 2) {
-        double t1, t2, s1, s2;
-        diff<KIND>(ux, uy, lx, ly, zx, zy, t1, t2);   // lane as target
-        if (KIND == 0) {                               // lane as source
-            s1 = lx - (ux + zx);
-            s2 = ly - (uy + zy);
-        } else {
-            s1 = (lx - ux) - zx;
-            s2 = (ly - uy) - zy;
-        }
-        const double rt = se2p_sqsum(t1, t2), rs = se2p_sqsum(s1, s2);
-        const bool in_t = own ? rt != 0.0 : rt < a.cut2;
-        const bool in_s = own ? rs != 0.0 : rs < a.cut2;
-        const double a1 = udata2[u], a2 = udata2[us + u];   // du of the uniform point
-        if (KIND == 0 && own && rs == 0.0) {   // coincident: self term, no gradient
-            acc[4] += a.self * a1;
-            acc[5] += a.self * a2;
-        }
-        if (!(in_t || in_s)) return;
-        const double r2 = in_t ? rt : rs;
-        const double x = r2 * a.xi2;
-        const double e = se2p_exp_neg(x, etab);
-        const double ir2 = rcp_nr(r2);
-        if (in_t) {   // lane: du_i; uniform: the densities of j
-            if (KIND == 0)
-                slp_grad(t1, t2, a.xi2, e, ir2, ld[0], ld[1], udata[u], udata[us + u], acc[0], acc[1]);
-            else
-                dlp_grad(t1, t2, r2, a.xi2, e, ir2, ld[0], ld[1], udata[u], udata[us + u],
-                         udata[2 * us + u], acc[0], acc[1]);
-        }
-        if (in_s) {   // lane: the densities of i; uniform: du_j
-            if (KIND == 0) {
-                slp_grad(s1, s2, a.xi2, e, ir2, a1, a2, ld[2], ld[3], acc[2], acc[3]);
-                const double E = se2p_e1(x, e, e1tab);
-                const double hE = 0.5 * E, g = e * ir2;
-                const double c = s1 * a2 - s2 * a1;
-                acc[4] += hE * a1 + g * s2 * c;
-                acc[5] += hE * a2 - g * s1 * c;
-            } else {
-                dlp_grad(s1, s2, r2, a.xi2, e, ir2, a1, a2, ld[2], ld[3], ld[4], acc[2], acc[3]);
-                dlp_dens(s1, s2, r2, a.xi2, e, ir2, a1, a2, acc[4], acc[5], acc[6]);
-            }
-        }
-        }
-    };
-
     // one pair: d with the forward's rounding, then the side's contributions
     auto pair = [&](double ux, double uy, double zx, double zy, int u, bool own) {
-        if constexpr (SIDE == 2) {
-            pair2(ux, uy, zx, zy, u, own);
-            return;
-        }
         double d1, d2;
         if (SIDE == 0) {
             diff<KIND>(ux, uy, lx, ly, zx, zy, d1, d2);
@@ -869,10 +808,6 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
         a.dpos[2 * o + 1] = -acc[1] * s4;
         return;
     }
-    if (SIDE == 2 && a.dpos_t) {
-        a.dpos_t[2 * o] = -acc[0] * s4;
-        a.dpos_t[2 * o + 1] = -acc[1] * s4;
-    }
     if (a.dpos) {
         a.dpos[2 * o] = acc[SO] * s4;
         a.dpos[2 * o + 1] = acc[SO + 1] * s4;
@@ -899,6 +834,274 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
 }
 
 // du in target (cell) order, [2][n]
+// ---- adjoint, targets are the sources -------------------------------------
+// One sorted point set (the forward's plan: source records, cell lists, chunk
+// plans, run boxes).  Lane i and uniform point j give two ordered pairs: i as
+// target of source j (the target side, dL/dptar) and i as source of target j
+// (the source side, dL/dpsrc and the density gradient).  Each uses the
+// difference vector and inclusion test the forward sums use.
+//
+// Stresslet: the lane-as-source difference is exactly the negated
+// lane-as-target one (IEEE subtraction is antisymmetric and the image offsets
+// of the two visits are negatives), so one r^2 and one test serve both.  With
+// t = e2 T (the three-index stresslet tensor, fully symmetric; components
+// t111, t112, t122, t222) and the merged densities q = (S11, S12+S21, S22),
+//   phi(a, q) = sum_k w_k(a, q) t_k,
+//   w = (a1 q1, a1 q2 + a2 q1, a1 q3 + a2 q2, a2 q3),
+// so grad phi = sum_k w_k D_k with D_lk = d_l t_k: the 2 x 4 tensor D is
+// computed once per pair and contracted with the target side's weights
+// (lane du, uniform densities) and the source side's (uniform du, lane
+// densities); grad t is even in d, t is odd.  With t = alpha P3(d) + beta P1(d),
+// alpha = -4 e2 (1 + x)/r^4, beta = 2 xi^2 e2, x = xi^2 r^2:
+//   D_lk = d_l [g1 P3_k + g2 P1_k] + alpha dP3_k/dd_l + beta dP1_k/dd_l,
+//   g1 = 8 e2 (x^2 + 2 x + 2)/r^6, g2 = -2 xi^2 beta,
+//   P3 = (d1^3, d1^2 d2, d1 d2^2, d2^3), P1 = (3 d1, d2, d1, 3 d2).
+struct RealAdjSameArgs {
+    const double *rec;       // source records in (cell, sub-cell) order
+    const double2 *du;       // du in the same order
+    const int *off, *perm;   // run offsets, order -> caller index
+    const int *chunk_off, *block_cell;
+    const double4 *box;      // run bounding boxes
+    int ncell, nsx, nsy;
+    double Lx, Ly, xi2, cut2, self;
+    const double *f, *nv;    // caller-order densities and normals (stresslet df, dn)
+    double *dpsrc, *dptar, *df, *dn;   // caller order; may be null
+};
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
+{
+    constexpr int RN = SrcRec<KIND>::n;
+    constexpr int NV = KIND == 0 ? 6 : 7;   // target grad, source grad, density terms
+    __shared__ double e1tab[KIND == 0 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
+    __shared__ double etab[SE2P_EXP2_N];
+    __shared__ double red[kWaves - 1][NV][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (KIND == 0)
+        for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
+    if (threadIdx.x < SE2P_EXP2_N) {
+        const double t[SE2P_EXP2_N] = SE2P_EXP2_TABLE;
+        etab[threadIdx.x] = t[threadIdx.x];
+    }
+    __syncthreads();
+
+    const int b = xcd_chunk(a.chunk_off[a.ncell]);
+    if (b < 0) return;   // uniform over the workgroup
+    const int cell = a.block_cell[b];
+    const int sub = 1 << (2 * kSubLog);
+    const int l0 = a.off[cell * sub] + 64 * (b - a.chunk_off[cell]);
+    const int l1 = a.off[(cell + 1) * sub];
+    const int l = l0 + lane;
+    const bool active = l < l1;
+    // lane point: position, densities, du
+    double lx, ly, lq0 = 0.0, lq1 = 0.0, lq2 = 0.0, la1 = 0.0, la2 = 0.0;
+    if (active) {
+        const double *rp = a.rec + (int64_t)l * RN;
+        const double4 v0 = *(const double4 *)rp;
+        lx = v0.x;
+        ly = v0.y;
+        lq0 = v0.z;
+        lq1 = v0.w;
+        if (KIND == 1) lq2 = rp[4];
+        const double2 d = a.du[l];
+        la1 = d.x;
+        la2 = d.y;
+    } else {
+        lx = ly = __builtin_nan("");   // inactive lanes compute garbage, never stored
+    }
+    const int cx = cell % a.nsx, cy = cell / a.nsx;
+    const double xi2 = a.xi2, cut2 = a.cut2;
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+
+    struct U {
+        double x, y, q0, q1, q2, b1, b2;
+    };
+    auto load = [&](int u) {
+        U o;
+        const double *rp = a.rec + (int64_t)u * RN;
+        const double4 v0 = *(const double4 *)rp;
+        o.x = v0.x;
+        o.y = v0.y;
+        o.q0 = v0.z;
+        o.q1 = v0.w;
+        o.q2 = KIND == 1 ? rp[4] : 0.0;
+        const double2 d = a.du[u];
+        o.b1 = d.x;
+        o.b2 = d.y;
+        return o;
+    };
+    auto pair = [&](const U &u, double zx, double zy, bool own) {
+        if constexpr (KIND == 1) {
+            double t1, t2;
+            if (own) {   // own cell, no image
+                t1 = u.x - lx;
+                t2 = u.y - ly;
+            } else {
+                diff<1>(u.x, u.y, lx, ly, zx, zy, t1, t2);
+            }
+            double d11, d22;
+            const double r2 = se2p_sqsum2(t1, t2, &d11, &d22);
+            if (own ? (r2 == 0.0) : !(r2 < cut2)) return;
+            const double x = xi2 * r2;
+            const double e2 = se2p_exp_neg(x, etab);
+            const double ir2 = rcp_nr(r2);
+            const double ir4 = ir2 * ir2;
+            const double al = -4.0 * e2 * (1.0 + x) * ir4;
+            const double be = 2.0 * xi2 * e2;
+            const double g1 = 8.0 * e2 * (x * (x + 2.0) + 2.0) * (ir4 * ir2);
+            const double g2 = -2.0 * xi2 * be;
+            const double A11 = al * d11, A22 = al * d22, A12 = al * (t1 * t2);
+            const double b1 = A11 + be, b3 = A22 + be;
+            // t (lane as target); the source side's tensor is -t
+            const double tt0 = t1 * (b1 + 2.0 * be), tt1 = t2 * b1, tt2 = t1 * b3,
+                         tt3 = t2 * (b3 + 2.0 * be);
+            const double G11 = g1 * d11, G22 = g1 * d22;
+            const double h0 = t1 * (G11 + 3.0 * g2), h1 = t2 * (G11 + g2), h2 = t1 * (G22 + g2),
+                         h3 = t2 * (G22 + 3.0 * g2);
+            const double D10 = t1 * h0 + 3.0 * b1, D11 = t1 * h1 + 2.0 * A12, D12 = t1 * h2 + b3,
+                         D13 = t1 * h3;
+            const double D20 = t2 * h0, D21 = t2 * h1 + b1, D22 = t2 * h2 + 2.0 * A12,
+                         D23 = t2 * h3 + 3.0 * b3;
+            // target side: lane du, uniform densities
+            {
+                const double w0 = la1 * u.q0, w1 = la1 * u.q1 + la2 * u.q0,
+                             w2 = la1 * u.q2 + la2 * u.q1, w3 = la2 * u.q2;
+                acc[0] += D10 * w0 + D11 * w1 + D12 * w2 + D13 * w3;
+                acc[1] += D20 * w0 + D21 * w1 + D22 * w2 + D23 * w3;
+            }
+            // source side: uniform du, lane densities
+            {
+                const double w0 = u.b1 * lq0, w1 = u.b1 * lq1 + u.b2 * lq0,
+                             w2 = u.b1 * lq2 + u.b2 * lq1, w3 = u.b2 * lq2;
+                acc[2] += D10 * w0 + D11 * w1 + D12 * w2 + D13 * w3;
+                acc[3] += D20 * w0 + D21 * w1 + D22 * w2 + D23 * w3;
+            }
+            acc[4] -= u.b1 * tt0 + u.b2 * tt1;
+            acc[5] -= u.b1 * tt1 + u.b2 * tt2;
+            acc[6] -= u.b1 * tt2 + u.b2 * tt3;
+        } else {
+            double t1, t2, s1, s2;
+            diff<0>(u.x, u.y, lx, ly, zx, zy, t1, t2);   // lane as target
+            s1 = lx - (u.x + zx);                         // lane as source
+            s2 = ly - (u.y + zy);
+            const double rt = se2p_sqsum(t1, t2), rs = se2p_sqsum(s1, s2);
+            const bool in_t = own ? rt != 0.0 : rt < cut2;
+            const bool in_s = own ? rs != 0.0 : rs < cut2;
+            if (own && rs == 0.0) {   // coincident: self term, no gradient
+                acc[4] += a.self * u.b1;
+                acc[5] += a.self * u.b2;
+            }
+            if (!(in_t || in_s)) return;
+            const double r2 = in_t ? rt : rs;
+            const double x = r2 * xi2;
+            const double e = se2p_exp_neg(x, etab);
+            const double ir2 = rcp_nr(r2);
+            if (in_t) slp_grad(t1, t2, xi2, e, ir2, la1, la2, u.q0, u.q1, acc[0], acc[1]);
+            if (in_s) {
+                slp_grad(s1, s2, xi2, e, ir2, u.b1, u.b2, lq0, lq1, acc[2], acc[3]);
+                const double E = se2p_e1(x, e, e1tab);
+                const double hE = 0.5 * E, g = e * ir2;
+                const double c = s1 * u.b2 - s2 * u.b1;
+                acc[4] += hE * u.b1 + g * s2 * c;
+                acc[5] += hE * u.b2 - g * s1 * c;
+            }
+        }
+    };
+    // uniform points u, u + kWaves, ... < u1; the next record is loaded
+    // before the current pair is evaluated (the last load repeats the current
+    // point and is unused)
+    auto run = [&](int u, int u1, double zx, double zy, bool own) {
+        if (u >= u1) return;
+        U cur = load(u);
+        for (;;) {
+            const int un = u + kWaves;
+            const U nxt = load(un < u1 ? un : u);
+            pair(cur, zx, zy, own);
+            if (un >= u1) break;
+            cur = nxt;
+            u = un;
+        }
+    };
+    run(a.off[cell * sub] + wave, a.off[(cell + 1) * sub], 0.0, 0.0, true);
+    double bx0, by0, bx1, by1;
+    wave_bbox(active, lx, ly, bx0, by0, bx1, by1);
+    const double lim = cut2 * (1.0 + 1e-9);
+    const unsigned long long keep0 =
+        keep_mask(a.box, 0, lane, 0, cx, cy, a.nsx, a.nsy, a.Lx, a.Ly, lim, bx0, by0, bx1, by1);
+    const unsigned long long keep1 =
+        keep_mask(a.box, 1, lane, 0, cx, cy, a.nsx, a.nsy, a.Lx, a.Ly, lim, bx0, by0, bx1, by1);
+    int cnt = 0;
+#pragma unroll 1
+    for (int j = 0; j < 8; ++j) {
+        unsigned bits = (unsigned)(((j < 4) ? keep0 : keep1) >> (16 * (j & 3))) & 0xffffu;
+        if (bits == 0) continue;
+        const int qx = cx + nb_dx(j), qy = cy + nb_dy(j);
+        const int wx = (qx + a.nsx) % a.nsx, wy = (qy + a.nsy) % a.nsy;
+        const int ub = wy * a.nsx + wx;
+        const double zx = (a.Lx * (qx - wx)) / a.nsx;
+        const double zy = (a.Ly * (qy - wy)) / a.nsy;
+        while (bits) {
+            const int ra = __builtin_ctz(bits);
+            const int rb = ra + __builtin_ctz(~(bits >> ra));
+            bits &= ~((1u << rb) - 1u);
+            const int u0 = a.off[ub * sub + ra], u1 = a.off[ub * sub + rb];
+            run(u0 + ((wave - cnt) & (kWaves - 1)), u1, zx, zy, false);
+            cnt += u1 - u0;
+        }
+    }
+    if (wave > 0)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) red[wave - 1][v][lane] = acc[v];
+    __syncthreads();
+    if (wave != 0 || !active) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int w = 0; w < kWaves - 1; ++w) acc[v] += red[w][v][lane];
+    const int o = a.perm[l];
+    const double s4 = 1.0 / (4 * SE2P_PI);
+    if (a.dptar) {
+        a.dptar[2 * o] = -acc[0] * s4;
+        a.dptar[2 * o + 1] = -acc[1] * s4;
+    }
+    if (a.dpsrc) {
+        a.dpsrc[2 * o] = acc[2] * s4;
+        a.dpsrc[2 * o + 1] = acc[3] * s4;
+    }
+    if (KIND == 0) {
+        if (a.df) {
+            a.df[2 * o] = acc[4] * s4;
+            a.df[2 * o + 1] = acc[5] * s4;
+        }
+    } else {
+        // m = dL/d(S11, S12 + S21, S22), S = f (x) n
+        const double m1 = acc[4] * s4, m2 = acc[5] * s4, m3 = acc[6] * s4;
+        const double f1 = a.f[2 * o], f2 = a.f[2 * o + 1];
+        const double n1 = a.nv[2 * o], n2 = a.nv[2 * o + 1];
+        if (a.df) {
+            a.df[2 * o] = m1 * n1 + m2 * n2;
+            a.df[2 * o + 1] = m2 * n1 + m3 * n2;
+        }
+        if (a.dn) {
+            a.dn[2 * o] = m1 * f1 + m2 * f2;
+            a.dn[2 * o + 1] = m2 * f1 + m3 * f2;
+        }
+    }
+}
+
+// du in sorted order, one double2 per point.
+__global__ void k_gather_du(const double *__restrict__ v, const int *__restrict__ perm, int n,
+                            double2 *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = perm[i];
+    out[i] = make_double2(v[2 * o], v[2 * o + 1]);
+}
+
 __global__ void k_gather_pairs(const double *__restrict__ v, const int *__restrict__ perm, int n,
                                double *__restrict__ out)
 {
@@ -1107,14 +1310,33 @@ static int upload_table(se2p_ctx *ctx)
 }
 
 // ---- host driver ----------------------------------------------------------
-// Real-space sum over nbox independent boxes in one pass: one sort of all
-// points by (global cell, sub-cell), one gather, one launch of the pair
-// kernel.  boxes[] are host records; their bases and derived parameters are
-// filled in here.
-int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, int64_t dstride,
-                   int64_t ustride, int64_t *pair_counts)
+// Sorted state of one real-space call: the cell lists of the sources (and of
+// the targets unless they are the sources), the gathered source data, the
+// chunk plans of the targets and the run boxes of the sources.
+// real_layout carves it, real_prepare fills it (the sort stage), the pair
+// kernels read it.
+struct RealPlan {
+    int kind = 0, nbox = 0, nrhs = 1;
+    int64_t ns = 0, nt = 0, ncell = 0, dstride = 0;
+    int nkeys = 0;
+    bool same_points = false;
+    int64_t max_chunks = 0;
+    unsigned grid8 = 0;
+    KeySort kss{}, kst{};
+    double2 *spos = nullptr, *tpos = nullptr;
+    double *q = nullptr, *rec = nullptr;
+    int *coff = nullptr, *bmap = nullptr, *offs = nullptr, *s_box = nullptr, *t_box = nullptr,
+        *cell_box = nullptr;
+    unsigned long long *cnt = nullptr;
+    double4 *sbox = nullptr;
+    RealBox *dbox = nullptr;
+    BoxTable btab{};
+};
+
+// Box bases, derived parameters and the plan's sizes; 0 or an error code.
+static int real_plan(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, int64_t dstride,
+                     RealPlan *p)
 {
-    hipStream_t st = ctx->stream;
     const int sub = 1 << (2 * kSubLog);
     int64_t ns = 0, nt = 0, ncell = 0;
     for (int b = 0; b < nbox; ++b) {
@@ -1131,62 +1353,64 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
     }
     if (ns > 0x7fffffffLL || nt > 0x7fffffffLL || ncell * sub > 0x7fffffffLL)
         return fail(ctx, SE2P_ERR_ARG, "too many points or cells in one call");
-    const bool counting = pair_counts != nullptr;
-    if (nt == 0) return 0;
-    if (kind == 0 && upload_table(ctx)) return SE2P_ERR_HIP;
+    *p = RealPlan{};
+    p->kind = kind;
+    p->nbox = nbox;
+    p->nrhs = nrhs;
+    p->dstride = dstride;
+    p->ns = ns;
+    p->nt = nt;
+    p->ncell = ncell;
     // targets that are the source array itself share the sources' cell lists
-    const bool same_points = nbox == 1 && boxes[0].ptar == boxes[0].psrc && boxes[0].nt == boxes[0].ns;
-    const int nkeys = (int)(ncell * sub);
-    const int ncomp = kind == 0 ? 2 : kDlpComp;
+    p->same_points = nbox == 1 && boxes[0].ptar == boxes[0].psrc && boxes[0].nt == boxes[0].ns;
+    p->nkeys = (int)(ncell * sub);
     // upper bound on chunks (every non-empty cell wastes < 1 chunk); the
     // cell-list kernels want a multiple of 8 workgroups (XCD order)
-    const int64_t max_chunks = (nt + 63) / 64 + ncell;
-    const unsigned grid8 = (unsigned)((max_chunks + 7) & ~int64_t(7));
+    p->max_chunks = (nt + 63) / 64 + ncell;
+    p->grid8 = (unsigned)((p->max_chunks + 7) & ~int64_t(7));
+    return 0;
+}
 
-    KeySort kss, kst;
-    double2 *spos, *tpos;
-    double *q = nullptr, *rec = nullptr;
-    int *coff, *bmap, *offs = nullptr, *s_box = nullptr, *t_box = nullptr, *cell_box = nullptr;
-    unsigned long long *cnt;
-    double4 *sbox;
-    RealBox *dbox = nullptr;
-    auto layout = [&](Carve &cv) {
-        char *p = cv.take<char>(keysort_bytes(ns, nkeys));
-        if (p) keysort_carve(p, ns, nkeys, &kss);
-        if (!same_points) {
-            p = cv.take<char>(keysort_bytes(nt, nkeys));
-            if (p) keysort_carve(p, nt, nkeys, &kst);
-        }
-        spos = cv.take<double2>(ns);
-        tpos = same_points ? spos : cv.take<double2>(nt);
-        if (nrhs == 1) rec = cv.take<double>((size_t)(kind == 0 ? SrcRec<0>::n : SrcRec<1>::n) * ns);
-        else q = cv.take<double>((size_t)ncomp * nrhs * ns);
-        coff = cv.take<int>(ncell + 1);
-        cnt = cv.take<unsigned long long>(2);
-        bmap = cv.take<int>(max_chunks);
-        sbox = cv.take<double4>(nkeys);
-        if (nbox > 1) {
-            dbox = cv.take<RealBox>(nbox);
-            offs = cv.take<int>(3 * (size_t)(nbox + 1));
-            s_box = cv.take<int>(ns);
-            t_box = cv.take<int>(nt);
-            cell_box = cv.take<int>(ncell);
-        }
-    };
-    Carve sizing;
-    layout(sizing);
-    if (int rc = ensure_workspace(ctx, sizing.off)) return rc;
-    Carve cv;
-    cv.base = ctx->ws;
-    layout(cv);
+static void real_layout(Carve &cv, RealPlan &p)
+{
+    const int ncomp = p.kind == 0 ? 2 : kDlpComp;
+    char *c = cv.take<char>(keysort_bytes(p.ns, p.nkeys));
+    if (c) keysort_carve(c, p.ns, p.nkeys, &p.kss);
+    if (!p.same_points) {
+        c = cv.take<char>(keysort_bytes(p.nt, p.nkeys));
+        if (c) keysort_carve(c, p.nt, p.nkeys, &p.kst);
+    }
+    p.spos = cv.take<double2>(p.ns);
+    p.tpos = p.same_points ? p.spos : cv.take<double2>(p.nt);
+    if (p.nrhs == 1) p.rec = cv.take<double>((size_t)(p.kind == 0 ? SrcRec<0>::n : SrcRec<1>::n) * p.ns);
+    else p.q = cv.take<double>((size_t)ncomp * p.nrhs * p.ns);
+    p.coff = cv.take<int>(p.ncell + 1);
+    p.cnt = cv.take<unsigned long long>(2);
+    p.bmap = cv.take<int>(p.max_chunks);
+    p.sbox = cv.take<double4>(p.nkeys);
+    if (p.nbox > 1) {
+        p.dbox = cv.take<RealBox>(p.nbox);
+        p.offs = cv.take<int>(3 * (size_t)(p.nbox + 1));
+        p.s_box = cv.take<int>(p.ns);
+        p.t_box = cv.take<int>(p.nt);
+        p.cell_box = cv.take<int>(p.ncell);
+    }
+}
 
-    stage_begin(ctx, ST_SORT);
+// The sort stage: box tables, cell lists, gathers, chunk plans, run boxes.
+static int real_prepare(se2p_ctx *ctx, const RealBox *boxes, RealPlan &p)
+{
+    hipStream_t st = ctx->stream;
+    const int sub = 1 << (2 * kSubLog);
     const int threads = 256;
-    BoxTable btab{};
+    const int64_t ns = p.ns, nt = p.nt, ncell = p.ncell;
+    const int nkeys = p.nkeys, nbox = p.nbox, nrhs = p.nrhs, kind = p.kind;
+    const int64_t dstride = p.dstride;
     if (nbox == 1) {
-        btab.one = boxes[0];
+        p.btab.tab = nullptr;
+        p.btab.one = boxes[0];
     } else {
-        btab.tab = dbox;
+        p.btab.tab = p.dbox;
         // box table and the source / target / cell offsets of every box,
         // written by kernels that carry the records as arguments: no host
         // copy enters the stream, so a captured sequence replays on its own
@@ -1195,92 +1419,191 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
             bb.first = b0;
             bb.count = std::min(kBoxesPerLaunch, nbox - b0);
             for (int i = 0; i < bb.count; ++i) bb.b[i] = boxes[b0 + i];
-            hipLaunchKernelGGL(k_write_boxes, dim3(1), dim3(kBoxesPerLaunch), 0, st, bb, dbox, offs,
-                               nbox, (int)ns, (int)nt, (int)ncell);
+            hipLaunchKernelGGL(k_write_boxes, dim3(1), dim3(kBoxesPerLaunch), 0, st, bb, p.dbox,
+                               p.offs, nbox, (int)ns, (int)nt, (int)ncell);
         }
-        expand_map(st, offs, nbox, (int)ns, s_box);
-        expand_map(st, offs + nbox + 1, nbox, (int)nt, t_box);
-        expand_map(st, offs + 2 * (nbox + 1), nbox, (int)ncell, cell_box);
+        expand_map(st, p.offs, nbox, (int)ns, p.s_box);
+        expand_map(st, p.offs + nbox + 1, nbox, (int)nt, p.t_box);
+        expand_map(st, p.offs + 2 * (nbox + 1), nbox, (int)ncell, p.cell_box);
     }
+    const BoxTable &btab = p.btab;
     // Sources and targets in (cell, Morton sub-cell) order, stable: a cell's
     // sources are summed in a fixed order on every run.  The fused sort
     // (se2p_fsort.h) also gathers the sorted data and plans the chunks of 64
     // targets; otherwise separate launches do.
-    const KeySort &tks = same_points ? kss : kst;
-    const PlanArgs plan{nullptr, tks.off, sub, 64, (int)ncell, coff, bmap};
+    const KeySort &tks = p.same_points ? p.kss : p.kst;
+    const PlanArgs plan{nullptr, tks.off, sub, 64, (int)ncell, p.coff, p.bmap};
     bool planned = false, boxed = false;
     const bool box_kernel = (ctx->sort_mode & SE2P_SORT_BOX_KERNEL) != 0;
     if (fsort_ok(ctx, ns, nkeys)) {
-        FsPlanExtra x0{plan, plan, same_points ? 1 : 0};
-        const CellKeyFn kf{btab, s_box, 0};
+        FsPlanExtra x0{plan, plan, p.same_points ? 1 : 0};
+        const CellKeyFn kf{btab, p.s_box, 0};
         int rc;
         if (kind == 0)
-            rc = fsort_run(ctx, &kss, ns, nkeys, kf,
-                           SrcGather<0>{btab, s_box, (int)ns, nrhs, dstride, spos, q, rec, sbox}, x0,
-                           FsNoExtra{});
+            rc = fsort_run(ctx, &p.kss, ns, nkeys, kf,
+                           SrcGather<0>{btab, p.s_box, (int)ns, nrhs, dstride, p.spos, p.q, p.rec,
+                                        p.sbox}, x0, FsNoExtra{});
         else
-            rc = fsort_run(ctx, &kss, ns, nkeys, kf,
-                           SrcGather<1>{btab, s_box, (int)ns, nrhs, dstride, spos, q, rec, sbox}, x0,
-                           FsNoExtra{});
+            rc = fsort_run(ctx, &p.kss, ns, nkeys, kf,
+                           SrcGather<1>{btab, p.s_box, (int)ns, nrhs, dstride, p.spos, p.q, p.rec,
+                                        p.sbox}, x0, FsNoExtra{});
         if (rc) return rc;
-        planned = same_points;
+        planned = p.same_points;
         boxed = !box_kernel;
     } else {
         if (ns > 0)
             hipLaunchKernelGGL(k_box_cell_keys, dim3((unsigned)((ns + threads - 1) / threads)),
-                               dim3(threads), 0, st, btab, s_box, (int)ns, 0, kss.key);
-        if (int rc = keysort_run(ctx, &kss, ns, nkeys)) return rc;
+                               dim3(threads), 0, st, btab, p.s_box, (int)ns, 0, p.kss.key);
+        if (int rc = keysort_run(ctx, &p.kss, ns, nkeys)) return rc;
         if (ns > 0) {
             const unsigned blocks = (unsigned)((ns + threads - 1) / threads);
             if (kind == 0)
                 hipLaunchKernelGGL(k_box_gather_sources<0>, dim3(blocks), dim3(threads), 0, st, btab,
-                                   s_box, kss.perm, (int)ns, nrhs, dstride, spos, q, rec);
+                                   p.s_box, p.kss.perm, (int)ns, nrhs, dstride, p.spos, p.q, p.rec);
             else
                 hipLaunchKernelGGL(k_box_gather_sources<1>, dim3(blocks), dim3(threads), 0, st, btab,
-                                   s_box, kss.perm, (int)ns, nrhs, dstride, spos, q, rec);
+                                   p.s_box, p.kss.perm, (int)ns, nrhs, dstride, p.spos, p.q, p.rec);
         }
     }
-    if (!same_points) {
+    if (!p.same_points) {
         if (fsort_ok(ctx, nt, nkeys)) {
-            if (int rc = fsort_run(ctx, &kst, nt, nkeys, CellKeyFn{btab, t_box, 1},
-                                   TgtGather{btab, t_box, tpos}, FsPlanExtra{plan, plan, 1},
+            if (int rc = fsort_run(ctx, &p.kst, nt, nkeys, CellKeyFn{btab, p.t_box, 1},
+                                   TgtGather{btab, p.t_box, p.tpos}, FsPlanExtra{plan, plan, 1},
                                    FsNoExtra{}))
                 return rc;
             planned = true;
         } else {
             hipLaunchKernelGGL(k_box_cell_keys, dim3((unsigned)((nt + threads - 1) / threads)),
-                               dim3(threads), 0, st, btab, t_box, (int)nt, 1, kst.key);
-            if (int rc = keysort_run(ctx, &kst, nt, nkeys)) return rc;
+                               dim3(threads), 0, st, btab, p.t_box, (int)nt, 1, p.kst.key);
+            if (int rc = keysort_run(ctx, &p.kst, nt, nkeys)) return rc;
             hipLaunchKernelGGL(k_box_gather_targets, dim3((unsigned)((nt + threads - 1) / threads)),
-                               dim3(threads), 0, st, btab, t_box, tks.perm, (int)nt, tpos);
+                               dim3(threads), 0, st, btab, p.t_box, tks.perm, (int)nt, p.tpos);
         }
     }
-    if (!planned) plan_chunks(st, nullptr, tks.off, sub, 64, (int)ncell, coff, bmap);   // chunks of 64 targets
+    if (!planned) plan_chunks(st, nullptr, tks.off, sub, 64, (int)ncell, p.coff, p.bmap);   // chunks of 64 targets
     if (!boxed)
-        hipLaunchKernelGGL(k_run_bbox, dim3((nkeys + 3) / 4), dim3(256), 0, st, spos, kss.off, nkeys, sbox);
-    stage_end(ctx, ST_SORT);
+        hipLaunchKernelGGL(k_run_bbox, dim3((nkeys + 3) / 4), dim3(256), 0, st, p.spos, p.kss.off,
+                           nkeys, p.sbox);
+    SE2P_HIP(hipGetLastError());
+    return 0;
+}
 
+// State a forward call keeps for its backward call (se2p_set_keep): one
+// single-box, single-density, targets-are-sources call, identified by its
+// arguments.  The plan's arrays live in a buffer of their own.
+struct KeptReal {
+    char *buf = nullptr;
+    size_t bytes = 0;
+    bool valid = false;
+    int kind = 0;
+    const double *psrc = nullptr, *f = nullptr, *nv = nullptr;
+    int64_t n = 0;
+    double xi = 0, Lx = 0, Ly = 0;
+    int nsx = 0, nsy = 0;
+    RealPlan plan;
+    bool match(int k, const RealGeom &g, const double *ff, const double *nn) const
+    {
+        return valid && k == kind && g.psrc == psrc && g.ptar == psrc && g.ns == n && g.nt == n &&
+               ff == f && (k == 0 || nn == nv) && g.xi == xi && g.Lx == Lx && g.Ly == Ly &&
+               g.nsx == nsx && g.nsy == nsy;
+    }
+};
+
+void free_kept_real(se2p_ctx *ctx)
+{
+    if (!ctx->kept_real) return;
+    if (ctx->kept_real->buf) (void)hipFree(ctx->kept_real->buf);
+    delete ctx->kept_real;
+    ctx->kept_real = nullptr;
+}
+
+// Real-space sum over nbox independent boxes in one pass: one sort of all
+// points by (global cell, sub-cell), one gather, one launch of the pair
+// kernel.  boxes[] are host records; their bases and derived parameters are
+// filled in here.
+int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, int64_t dstride,
+                   int64_t ustride, int64_t *pair_counts)
+{
+    hipStream_t st = ctx->stream;
+    RealPlan p;
+    if (int rc = real_plan(ctx, kind, boxes, nbox, nrhs, dstride, &p)) return rc;
+    const bool counting = pair_counts != nullptr;
+    if (p.nt == 0) return 0;
+    if (kind == 0 && upload_table(ctx)) return SE2P_ERR_HIP;
+    // a call whose backward may follow keeps its plan in a buffer of its own
+    const bool keep = ctx->keep && !ctx->host_ptrs && nbox == 1 && nrhs == 1 && !counting &&
+                      p.same_points;
+    Carve sizing;
+    real_layout(sizing, p);
+    Carve cv;
+    if (keep) {
+        if (!ctx->kept_real) ctx->kept_real = new KeptReal;
+        KeptReal &kr = *ctx->kept_real;
+        kr.valid = false;
+        if (sizing.off > kr.bytes) {
+            if (kr.buf) {
+                SE2P_HIP(hipStreamSynchronize(st));
+                SE2P_HIP(hipFree(kr.buf));
+                kr.buf = nullptr;
+                kr.bytes = 0;
+            }
+            SE2P_HIP(hipMalloc((void **)&kr.buf, sizing.off));
+            kr.bytes = sizing.off;
+        }
+        cv.base = kr.buf;
+    } else {
+        if (int rc = ensure_workspace(ctx, sizing.off)) return rc;
+        cv.base = ctx->ws;
+    }
+    real_layout(cv, p);
+
+    stage_begin(ctx, ST_SORT);
+    if (int rc = real_prepare(ctx, boxes, p)) return rc;
+    stage_end(ctx, ST_SORT);
+    if (keep) {
+        KeptReal &kr = *ctx->kept_real;
+        const RealBox &B = boxes[0];
+        kr.kind = kind;
+        kr.psrc = B.psrc;
+        kr.f = B.f;
+        kr.nv = B.nv;
+        kr.n = B.ns;
+        kr.xi = B.xi;
+        kr.Lx = B.Lx;
+        kr.Ly = B.Ly;
+        kr.nsx = B.nsx;
+        kr.nsy = B.nsy;
+        kr.plan = p;
+        kr.valid = true;
+    }
+
+    const KeySort &tks = p.same_points ? p.kss : p.kst;
     RealArgs a{};
-    a.tpos = tpos;
+    a.tpos = p.tpos;
     a.tperm = tks.perm;
     a.toff = tks.off;
-    a.chunk_off = coff;
-    a.block_cell = bmap;
-    a.spos = spos;
-    a.q = q;
-    a.soff = kss.off;
-    a.ns = ns;
-    a.ncell = (int)ncell;
+    a.chunk_off = p.coff;
+    a.block_cell = p.bmap;
+    a.spos = p.spos;
+    a.q = p.q;
+    a.soff = p.kss.off;
+    a.ns = p.ns;
+    a.ncell = (int)p.ncell;
     a.nrhs = nrhs;
     a.ustride = ustride;
-    a.boxes = btab;
-    a.cell_box = cell_box;
-    a.pair_counts = cnt;
+    a.boxes = p.btab;
+    a.cell_box = p.cell_box;
+    a.pair_counts = p.cnt;
+    const double2 *spos = p.spos;
+    const double *q = p.q, *rec = p.rec;
+    const int *soff = p.kss.off;
+    const double4 *sbox = p.sbox;
+    const unsigned grid8 = p.grid8;
 
     stage_begin(ctx, ST_REAL);
     if (counting) {
-        SE2P_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(k_real_count, dim3((unsigned)max_chunks), dim3(64), 0, st, a);
+        SE2P_HIP(hipMemsetAsync(p.cnt, 0, 2 * sizeof(unsigned long long), st));
+        hipLaunchKernelGGL(k_real_count, dim3((unsigned)p.max_chunks), dim3(64), 0, st, a);
     } else {
         // groups of four right-hand sides share every pair evaluation; the
         // remainder goes one at a time
@@ -1289,19 +1612,19 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
             a.r_begin = 0;
             a.r_end = n4;
             const dim3 grid(grid8, n4 / 4);
-            if (kind == 0) hipLaunchKernelGGL((k_real<0, 4, false>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
-            else hipLaunchKernelGGL((k_real<1, 4, false>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
+            if (kind == 0) hipLaunchKernelGGL((k_real<0, 4, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
+            else hipLaunchKernelGGL((k_real<1, 4, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
         }
         if (n4 < nrhs) {
             a.r_begin = n4;
             a.r_end = nrhs;
             const dim3 grid(grid8, nrhs - n4);
             if (rec) {   // single density: source records
-                if (kind == 0) hipLaunchKernelGGL((k_real<0, 1, true>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
-                else hipLaunchKernelGGL((k_real<1, 1, true>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
+                if (kind == 0) hipLaunchKernelGGL((k_real<0, 1, true>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
+                else hipLaunchKernelGGL((k_real<1, 1, true>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
             } else {
-                if (kind == 0) hipLaunchKernelGGL((k_real<0, 1, false>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
-                else hipLaunchKernelGGL((k_real<1, 1, false>), grid, dim3(256), 0, st, a, spos, q, rec, kss.off, sbox);
+                if (kind == 0) hipLaunchKernelGGL((k_real<0, 1, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
+                else hipLaunchKernelGGL((k_real<1, 1, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
             }
         }
     }
@@ -1309,7 +1632,7 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
     SE2P_HIP(hipGetLastError());
     if (counting) {
         unsigned long long h[2];
-        SE2P_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+        SE2P_HIP(hipMemcpyAsync(h, p.cnt, sizeof(h), hipMemcpyDeviceToHost, st));
         SE2P_HIP(hipStreamSynchronize(st));
         pair_counts[0] = (int64_t)h[0];
         pair_counts[1] = (int64_t)h[1];
@@ -1359,6 +1682,81 @@ int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const 
     return real_sum_boxes(ctx, kind, &b, 1, nrhs, dstride, ustride, pair_counts);
 }
 
+// Adjoint when the targets are the sources: the forward's plan (kept by the
+// forward call when it matches, else rebuilt the same way), du gathered into
+// its order, one pass of k_real_adj_same.
+static int real_adjoint_same(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f,
+                             const double *n, const double *du, double *dpsrc, double *dptar,
+                             double *df, double *dn)
+{
+    hipStream_t st = ctx->stream;
+    if (kind == 0 && upload_table(ctx)) return SE2P_ERR_HIP;
+    const bool kept = ctx->kept_real && ctx->kept_real->match(kind, g, f, n);
+    RealBox box{};
+    box.psrc = g.psrc;
+    box.ptar = g.ptar;
+    box.f = f;
+    box.nv = n;
+    box.ns = (int)g.ns;
+    box.nt = (int)g.nt;
+    box.nsx = g.nsx;
+    box.nsy = g.nsy;
+    box.Lx = g.Lx;
+    box.Ly = g.Ly;
+    box.xi = g.xi;
+    RealPlan p;
+    if (kept) p = ctx->kept_real->plan;
+    else if (int rc = real_plan(ctx, kind, &box, 1, 1, 2 * g.ns, &p)) return rc;
+    double2 *dut = nullptr;
+    auto layout = [&](Carve &cv) {
+        if (!kept) real_layout(cv, p);
+        dut = cv.take<double2>(g.nt);
+    };
+    Carve sizing;
+    layout(sizing);
+    if (int rc = ensure_workspace(ctx, sizing.off)) return rc;
+    Carve cv;
+    cv.base = ctx->ws;
+    layout(cv);
+
+    stage_begin(ctx, ST_SORT);
+    if (!kept)
+        if (int rc = real_prepare(ctx, &box, p)) return rc;
+    hipLaunchKernelGGL(k_gather_du, dim3((unsigned)((g.nt + 255) / 256)), dim3(256), 0, st, du,
+                       p.kss.perm, (int)g.nt, dut);
+    stage_end(ctx, ST_SORT);
+
+    const RealBox &B = p.btab.one;
+    RealAdjSameArgs a{};
+    a.rec = p.rec;
+    a.du = dut;
+    a.off = p.kss.off;
+    a.perm = p.kss.perm;
+    a.chunk_off = p.coff;
+    a.block_cell = p.bmap;
+    a.box = p.sbox;
+    a.ncell = (int)p.ncell;
+    a.nsx = B.nsx;
+    a.nsy = B.nsy;
+    a.Lx = B.Lx;
+    a.Ly = B.Ly;
+    a.xi2 = B.xi2;
+    a.cut2 = B.cut2;
+    a.self = B.self;
+    a.f = f;
+    a.nv = n;
+    a.dpsrc = dpsrc;
+    a.dptar = dptar;
+    a.df = df;
+    a.dn = dn;
+    stage_begin(ctx, ST_REAL);
+    if (kind == 0) hipLaunchKernelGGL(k_real_adj_same<0>, dim3(p.grid8), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_real_adj_same<1>, dim3(p.grid8), dim3(256), 0, st, a);
+    stage_end(ctx, ST_REAL);
+    SE2P_HIP(hipGetLastError());
+    return 0;
+}
+
 int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
                  const double *du, double *dpsrc, double *dptar, double *df, double *dn)
 {
@@ -1377,8 +1775,9 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         SE2P_HIP(hipGetLastError());
         return 0;
     }
-    if (kind == 0 && want_src && upload_table(ctx)) return SE2P_ERR_HIP;
     const bool same_points = (g.ptar == g.psrc) && (nt == ns);
+    if (same_points) return real_adjoint_same(ctx, kind, g, f, n, du, dpsrc, dptar, df, dn);
+    if (kind == 0 && want_src && upload_table(ctx)) return SE2P_ERR_HIP;
     const int ncomp = kind == 0 ? 2 : 3;
     const int64_t max_t = (nt + 63) / 64 + ncell, max_s = (ns + 63) / 64 + ncell;
     const unsigned grid_t = (unsigned)((max_t + 7) & ~int64_t(7));   // XCD order: multiple of 8
@@ -1456,29 +1855,6 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
     a.f = f;
     a.nv = n;
     stage_begin(ctx, ST_REAL);
-    if (same_points && dptar && want_src) {
-        // one pass computes both sides
-        RealAdjArgs t = a;
-        t.lpos = spos;
-        t.lperm = kss.perm;
-        t.loff = kss.off;
-        t.chunk_off = coff_s;
-        t.block_cell = bmap_s;
-        t.ldata = dut;
-        t.ldata2 = q;
-        t.nl = ns;
-        t.nu = ns;
-        t.dpos = dpsrc;
-        t.dpos_t = dptar;
-        t.df = df;
-        t.dn = dn;
-        if (kind == 0)
-            hipLaunchKernelGGL((k_real_adj<0, 2>), dim3(grid_s), dim3(256), 0, st, t, spos, q, dut, kss.off, sbox);
-        else
-            hipLaunchKernelGGL((k_real_adj<1, 2>), dim3(grid_s), dim3(256), 0, st, t, spos, q, dut, kss.off, sbox);
-        dptar = nullptr;
-        dpsrc = df = dn = nullptr;   // done
-    }
     if (dptar) {
         RealAdjArgs t = a;
         t.lpos = tpos;
@@ -1491,9 +1867,9 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         t.nu = ns;
         t.dpos = dptar;
         if (kind == 0)
-            hipLaunchKernelGGL((k_real_adj<0, 0>), dim3(grid_t), dim3(256), 0, st, t, spos, q, nullptr, kss.off, sbox);
+            hipLaunchKernelGGL((k_real_adj<0, 0>), dim3(grid_t), dim3(256), 0, st, t, spos, q, kss.off, sbox);
         else
-            hipLaunchKernelGGL((k_real_adj<1, 0>), dim3(grid_t), dim3(256), 0, st, t, spos, q, nullptr, kss.off, sbox);
+            hipLaunchKernelGGL((k_real_adj<1, 0>), dim3(grid_t), dim3(256), 0, st, t, spos, q, kss.off, sbox);
     }
     if (dpsrc || df || dn) {
         RealAdjArgs s = a;
@@ -1509,9 +1885,9 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         s.df = df;
         s.dn = dn;
         if (kind == 0)
-            hipLaunchKernelGGL((k_real_adj<0, 1>), dim3(grid_s), dim3(256), 0, st, s, tpos, dut, nullptr, tks.off, tbox);
+            hipLaunchKernelGGL((k_real_adj<0, 1>), dim3(grid_s), dim3(256), 0, st, s, tpos, dut, tks.off, tbox);
         else
-            hipLaunchKernelGGL((k_real_adj<1, 1>), dim3(grid_s), dim3(256), 0, st, s, tpos, dut, nullptr, tks.off, tbox);
+            hipLaunchKernelGGL((k_real_adj<1, 1>), dim3(grid_s), dim3(256), 0, st, s, tpos, dut, tks.off, tbox);
     }
     stage_end(ctx, ST_REAL);
     SE2P_HIP(hipGetLastError());

@@ -87,6 +87,12 @@ class Context:
     SORT_RADIX = 1       # force the multi-pass radix sort
     SORT_BOX_KERNEL = 2  # separate run-bounding-box kernel
 
+    def set_keep(self, flag: bool) -> None:
+        """Keep the sorted state of single-density, targets-are-sources
+        forward calls for the matching backward call (se2p_set_keep).  The
+        forward's arrays must stay unchanged until the backward call."""
+        self._check(self._lib.se2p_set_keep(self.handle, int(bool(flag))))
+
     def set_sort_mode(self, mode: int) -> None:
         """Sorting of cell lists and bins: 0 (default) uses the fused sort
         where it fits; SORT_RADIX / SORT_BOX_KERNEL force the separate

@@ -83,6 +83,21 @@ int se2p_set_host_pointers(se2p_ctx *ctx, int enable);
 #define SE2P_SORT_RADIX 1
 #define SE2P_SORT_BOX_KERNEL 2
 int se2p_set_sort_mode(se2p_ctx *ctx, int mode);
+/* Forward state kept for the backward call.  With enable != 0, a
+ * single-density forward call whose targets are its sources (ptar == psrc,
+ * Nt == Ns; se2p_slp_real, se2p_dlp_real, se2p_slp_kspace, se2p_dlp_kspace)
+ * keeps its sorted state in context-owned device memory: the real-space
+ * cell lists, source records, chunk plans and run boxes; the Fourier-space
+ * gridding bins, Gaussian factors, work plans and filtered grids.  The
+ * matching backward call (same entry kind, same pointers, sizes and
+ * parameters) then reuses it instead of sorting and, in Fourier space,
+ * spreading and transforming again.  The caller must leave the arrays of
+ * the forward call unchanged until the backward call, as for any saved
+ * state; the library cannot detect an in-place change.  A later forward
+ * call of the same kind replaces the state.  Host-pointer mode keeps
+ * nothing.  Disabling frees the state.  Default 0.  No reference
+ * counterpart. */
+int se2p_set_keep(se2p_ctx *ctx, int enable);
 /* Per-stage device timing with HIP events on the context stream.  When
  * enabled, se2p_stage_times fills up to n floats (milliseconds) for the
  * most recent call: sort, real, spread, fft_fwd, filter, fft_inv, interp
