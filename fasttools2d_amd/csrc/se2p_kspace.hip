@@ -903,13 +903,22 @@ struct InterpTileArgs {
     int64_t ustride;
 };
 
+// PT > 0: the support P = PT is a compile-time constant; each row of the
+// support is then one unrolled block (its LDS loads issue together) and the
+// separable y weights e3y^j E1[j] sit in registers, computed once per target.
+// PT = 0: any P, weights formed node by node.
+template <int PT>
 __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
 {
     extern __shared__ double lds[];
-    const int P1 = a.P + 1;
-    const int R = kTile + a.P;            // region edge
-    double2 *G = (double2 *)lds;          // [R][R]
-    double *E1 = (double *)(G + R * R);   // [P1]
+    const int P = PT > 0 ? PT : a.P;
+    const int P1 = P + 1;
+    const int R = kTile + P;              // region edge
+    // row pitch R + 1: an odd number of 16-byte nodes, so that anchors a few
+    // rows apart (points along a curve) fall on different LDS banks
+    const int RP = R + 1;
+    double2 *G = (double2 *)lds;          // [R][RP]
+    double *E1 = (double *)(G + R * RP);  // [P1]
     const int ntiles = a.ntx * a.nty;
     const int b = blockIdx.x;
     if (b >= a.chunk_off[ntiles]) return;   // uniform over the workgroup
@@ -931,7 +940,7 @@ __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
     }
     const int n0 = r0e - r0s, total = n0 + (r1e - r1s);
     const int v0 = chunk * 256, v1 = min(total, v0 + 256);
-    fill_e1(E1, a.P, a.ch2);
+    fill_e1(E1, P, a.ch2);
     const int X0 = tx * kTile, Y0 = ty * kTile;
     // blockIdx.y selects the right-hand side
     for (int r = blockIdx.y; r < blockIdx.y + 1 && r < a.nr; ++r) {
@@ -944,7 +953,7 @@ __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
             X %= a.Mx;
             Y %= a.My;
             const int64_t idx = (int64_t)X * a.My + Y;
-            G[e] = make_double2(g1[idx], g2[idx]);
+            G[i * RP + j] = make_double2(g1[idx], g2[idx]);
         }
         __syncthreads();
         for (int v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
@@ -952,7 +961,7 @@ __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
             const int t = a.perm[k];
             int mx, my;
             double px, py;
-            se2p_anchor(a.ptar[2 * t], a.ptar[2 * t + 1], a.Lx, a.Ly, a.h, a.P, &mx, &my, &px,
+            se2p_anchor(a.ptar[2 * t], a.ptar[2 * t + 1], a.Lx, a.Ly, a.h, P, &mx, &my, &px,
                         &py);
             double bx, e3x, e3y;
             fgg_factors(px, py, a.c, a.w, a.h, &bx, &e3x, &e3y);
@@ -960,18 +969,47 @@ __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
             int lj = ((my % a.My) + a.My) % a.My - Y0;
             double s1 = 0.0, s2 = 0.0;
             double ex = bx;
-            const double2 *row = G + li * R + lj;
-            for (int i = 0; i < P1; ++i) {
-                double ey = ex * E1[i];
-                for (int j = 0; j < P1; ++j) {
-                    const double wgt = ey * E1[j];
-                    const double2 gv = row[j];
-                    s1 += wgt * gv.x;
-                    s2 += wgt * gv.y;
+            const double2 *row = G + li * RP + lj;
+            if constexpr (PT > 0) {
+                double wy[PT + 1];
+                double ey = 1.0;
+#pragma unroll
+                for (int j = 0; j <= PT; ++j) {
+                    wy[j] = ey * E1[j];
                     ey *= e3y;
                 }
-                ex *= e3x;
-                row += R;
+                for (int i = 0; i <= PT; ++i) {
+                    double r1a = 0.0, r2a = 0.0, r1b = 0.0, r2b = 0.0;
+#pragma unroll
+                    for (int j = 0; j <= PT; j += 2) {
+                        const double2 gv = row[j];
+                        r1a = fma(wy[j], gv.x, r1a);
+                        r2a = fma(wy[j], gv.y, r2a);
+                        if (j + 1 <= PT) {
+                            const double2 gw = row[j + 1];
+                            r1b = fma(wy[j + 1], gw.x, r1b);
+                            r2b = fma(wy[j + 1], gw.y, r2b);
+                        }
+                    }
+                    const double wx = ex * E1[i];
+                    s1 = fma(wx, r1a + r1b, s1);
+                    s2 = fma(wx, r2a + r2b, s2);
+                    ex *= e3x;
+                    row += RP;
+                }
+            } else {
+                for (int i = 0; i < P1; ++i) {
+                    double ey = ex * E1[i];
+                    for (int j = 0; j < P1; ++j) {
+                        const double wgt = ey * E1[j];
+                        const double2 gv = row[j];
+                        s1 += wgt * gv.x;
+                        s2 += wgt * gv.y;
+                        ey *= e3y;
+                    }
+                    ex *= e3x;
+                    row += RP;
+                }
             }
             double *ur = a.u + (int64_t)r * a.ustride;
             ur[2 * t] = s1 * a.scale;
@@ -1012,12 +1050,51 @@ struct InterpAdjArgs {
     double *dpos, *df, *dn;   // outputs, caller order; may be null
 };
 
-template <int NR, class GET>
+template <int NR, int PT, class GET>
 __device__ inline void interp_sums(const GET &get, int P, const double *E1, double bx, double e3x,
                                    double e3y, double (&V)[NR], double (&X)[NR], double (&Y)[NR])
 {
 #pragma unroll
     for (int m = 0; m < NR; ++m) V[m] = X[m] = Y[m] = 0.0;
+    if constexpr (PT > 0) {
+        // compile-time support: y weights and (j - P/2) y weights in registers,
+        // one unrolled block per row
+        constexpr double hq = 0.5 * PT;
+        double wy[PT + 1], wj[PT + 1];
+        double ey = 1.0;
+#pragma unroll
+        for (int j = 0; j <= PT; ++j) {
+            wy[j] = ey * E1[j];
+            wj[j] = (j - hq) * wy[j];
+            ey *= e3y;
+        }
+        double ex = bx;
+        for (int i = 0; i <= PT; ++i) {
+            double r0[NR], ry[NR];
+#pragma unroll
+            for (int m = 0; m < NR; ++m) r0[m] = ry[m] = 0.0;
+#pragma unroll
+            for (int j = 0; j <= PT; ++j) {
+#pragma unroll
+                for (int m = 0; m < NR; ++m) {
+                    const double g = get(i, j, m);
+                    r0[m] = fma(wy[j], g, r0[m]);
+                    ry[m] = fma(wj[j], g, ry[m]);
+                }
+            }
+            const double wx = ex * E1[i];
+            const double ii = i - hq;
+#pragma unroll
+            for (int m = 0; m < NR; ++m) {
+                const double v = wx * r0[m];
+                V[m] += v;
+                X[m] = fma(ii, v, X[m]);
+                Y[m] = fma(wx, ry[m], Y[m]);
+            }
+            ex *= e3x;
+        }
+        return;
+    }
     const double hp = 0.5 * P;
     double ex = bx;
     for (int i = 0; i <= P; ++i) {
@@ -1099,19 +1176,21 @@ __device__ inline void interp_adj_store(const InterpAdjArgs &a, int o, double px
 __host__ __device__ inline size_t interp_adj_lds(int P, int NR)
 {
     const size_t R = kTile + P;
-    return sizeof(double) * (R * R * NR + P + 1);
+    return sizeof(double) * (R * (R + 1) * NR + P + 1);
 }
 
 // Tile kernel: the (16+P)^2 region of the NR grids that a 16x16 block of
 // anchors can touch is staged in LDS, as in k_interp_tile.
-template <int NR, int MODE>
+template <int NR, int MODE, int PT>
 __global__ __launch_bounds__(256) void k_interp_adj_tile(InterpAdjArgs a)
 {
     extern __shared__ double lds[];
-    const int P1 = a.P + 1;
-    const int R = kTile + a.P;
-    double *G = lds;                           // [R*R][NR]
-    double *E1 = G + (size_t)R * R * NR;       // [P1]
+    const int P = PT > 0 ? PT : a.P;
+    const int P1 = P + 1;
+    const int R = kTile + P;
+    const int RP = R + 1;                      // row pitch (bank spread, as k_interp_tile)
+    double *G = lds;                           // [R][RP][NR]
+    double *E1 = G + (size_t)R * RP * NR;      // [P1]
     const int ntiles = a.ntx * a.nty;
     const int b = blockIdx.x;
     if (b >= a.chunk_off[ntiles]) return;   // uniform over the workgroup
@@ -1133,13 +1212,13 @@ __global__ __launch_bounds__(256) void k_interp_adj_tile(InterpAdjArgs a)
     }
     const int n0 = r0e - r0s, total = n0 + (r1e - r1s);
     const int v0 = chunk * 256, v1 = min(total, v0 + 256);
-    fill_e1(E1, a.P, a.ch2);
+    fill_e1(E1, P, a.ch2);
     const int X0 = tx * kTile, Y0 = ty * kTile;
     for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
         const int i = e / R, j = e % R;
         const int64_t idx = (int64_t)((X0 + i) % a.Mx) * a.My + (Y0 + j) % a.My;
 #pragma unroll
-        for (int m = 0; m < NR; ++m) G[e * NR + m] = a.grids[m * a.gstride + idx];
+        for (int m = 0; m < NR; ++m) G[(i * RP + j) * NR + m] = a.grids[m * a.gstride + idx];
     }
     __syncthreads();
     for (int v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
@@ -1147,15 +1226,15 @@ __global__ __launch_bounds__(256) void k_interp_adj_tile(InterpAdjArgs a)
         const int o = a.perm[k];
         int mx, my;
         double px, py;
-        se2p_anchor(a.pts[2 * o], a.pts[2 * o + 1], a.Lx, a.Ly, a.h, a.P, &mx, &my, &px, &py);
+        se2p_anchor(a.pts[2 * o], a.pts[2 * o + 1], a.Lx, a.Ly, a.h, P, &mx, &my, &px, &py);
         double bx, e3x, e3y;
         fgg_factors(px, py, a.c, a.w, a.h, &bx, &e3x, &e3y);
         const int li = ((mx % a.Mx) + a.Mx) % a.Mx - X0;
         const int lj = ((my % a.My) + a.My) % a.My - Y0;
-        const double *base = G + (size_t)(li * R + lj) * NR;
-        auto get = [&](int i, int j, int m) { return base[(i * R + j) * NR + m]; };
+        const double *base = G + (size_t)(li * RP + lj) * NR;
+        auto get = [&](int i, int j, int m) { return base[(i * RP + j) * NR + m]; };
         double V[NR], X[NR], Y[NR];
-        interp_sums<NR>(get, a.P, E1, bx, e3x, e3y, V, X, Y);
+        interp_sums<NR, PT>(get, P, E1, bx, e3x, e3y, V, X, Y);
         interp_adj_store<NR, MODE>(a, o, px, py, V, X, Y);
     }
     (void)P1;
@@ -1184,7 +1263,7 @@ __global__ __launch_bounds__(256) void k_interp_adj_direct(InterpAdjArgs a)
         return a.grids[m * a.gstride + idx];
     };
     double V[NR], X[NR], Y[NR];
-    interp_sums<NR>(get, a.P, E1, bx, e3x, e3y, V, X, Y);
+    interp_sums<NR, 0>(get, a.P, E1, bx, e3x, e3y, V, X, Y);
     interp_adj_store<NR, MODE>(a, o, px, py, V, X, Y);
 }
 
@@ -1279,7 +1358,7 @@ static KLayout make_layout(const KGeom &g)
     s = s * s * L.h * L.h / SE2P_PI;
     L.interp_scale = s / (4 * SE2P_PI);
     L.wrap = (L.Mx <= L.P) || (L.My <= L.P);
-    L.lds_itile = sizeof(double2) * (size_t)(kTile + L.P) * (kTile + L.P) + sizeof(double) * (L.P + 1);
+    L.lds_itile = sizeof(double2) * (size_t)(kTile + L.P) * (kTile + L.P + 1) + sizeof(double) * (L.P + 1);
     L.itile = L.lds_itile <= kLdsMax;
     return L;
 }
@@ -1666,8 +1745,12 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         ia.scale = L.interp_scale;
         ia.u = u;
         ia.ustride = ustride;
-        hipLaunchKernelGGL(k_interp_tile, dim3((unsigned)gp.max_interp_chunks, nrhs), dim3(256),
-                           L.lds_itile, st, ia);
+        if (L.P == 24)
+            hipLaunchKernelGGL(k_interp_tile<24>, dim3((unsigned)gp.max_interp_chunks, nrhs),
+                               dim3(256), L.lds_itile, st, ia);
+        else
+            hipLaunchKernelGGL(k_interp_tile<0>, dim3((unsigned)gp.max_interp_chunks, nrhs),
+                               dim3(256), L.lds_itile, st, ia);
     } else {
         for (int r = 0; r < nrhs; ++r) {
             InterpArgs ia;
@@ -1707,8 +1790,11 @@ template <int NR, int MODE>
 static void launch_interp_adj(hipStream_t st, const KLayout &L, const GridPass &gp, bool tile,
                               const InterpAdjArgs &ia)
 {
-    if (tile)
-        hipLaunchKernelGGL((k_interp_adj_tile<NR, MODE>), dim3((unsigned)gp.max_interp_chunks),
+    if (tile && L.P == 24)
+        hipLaunchKernelGGL((k_interp_adj_tile<NR, MODE, 24>), dim3((unsigned)gp.max_interp_chunks),
+                           dim3(256), interp_adj_lds(L.P, NR), st, ia);
+    else if (tile)
+        hipLaunchKernelGGL((k_interp_adj_tile<NR, MODE, 0>), dim3((unsigned)gp.max_interp_chunks),
                            dim3(256), interp_adj_lds(L.P, NR), st, ia);
     else
         hipLaunchKernelGGL((k_interp_adj_direct<NR, MODE>), dim3((unsigned)((ia.n + 255) / 256)),
