@@ -882,6 +882,128 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs a)
 }
 
 
+// ---- interpolation sums ----------------------------------------------------
+// For a point and each of NR grids g_m: V_m = sum W g_m and the sums
+// X_m = sum (i - P/2) W g_m, Y_m = sum (j - P/2) W g_m over the (P+1)^2 support
+// nodes (i, j), W = ex E1[i] E1[j] e3x^i e3y^j the fast-Gaussian-gridding
+// weight (mex_stokes_slp_kspace.cpp:264-329).  get(i, j, m) returns grid m at
+// support node (i, j).
+// DERIV false: V only (X, Y stay zero); V is computed the same way either way.
+template <int NR, int PT, class GET, bool DERIV = true>
+__device__ inline void interp_sums(const GET &get, int P, const double *E1, double bx, double e3x,
+                                   double e3y, double (&V)[NR], double (&X)[NR], double (&Y)[NR])
+{
+#pragma unroll
+    for (int m = 0; m < NR; ++m) V[m] = X[m] = Y[m] = 0.0;
+    if constexpr (PT > 0) {
+        // compile-time support: y weights and (j - P/2) y weights in registers,
+        // one unrolled block per row
+        constexpr double hq = 0.5 * PT;
+        double wy[PT + 1], wj[PT + 1];
+        double ey = 1.0;
+#pragma unroll
+        for (int j = 0; j <= PT; ++j) {
+            wy[j] = ey * E1[j];
+            wj[j] = (j - hq) * wy[j];
+            ey *= e3y;
+        }
+        double ex = bx;
+        for (int i = 0; i <= PT; ++i) {
+            // even and odd nodes of the row in separate chains
+            double ra[NR], rb[NR], ya[NR], yb[NR];
+#pragma unroll
+            for (int m = 0; m < NR; ++m) ra[m] = rb[m] = ya[m] = yb[m] = 0.0;
+#pragma unroll
+            for (int j = 0; j <= PT; j += 2) {
+#pragma unroll
+                for (int m = 0; m < NR; ++m) {
+                    const double g = get(i, j, m);
+                    ra[m] = fma(wy[j], g, ra[m]);
+                    if (DERIV) ya[m] = fma(wj[j], g, ya[m]);
+                }
+                if (j + 1 <= PT) {
+#pragma unroll
+                    for (int m = 0; m < NR; ++m) {
+                        const double g = get(i, j + 1, m);
+                        rb[m] = fma(wy[j + 1], g, rb[m]);
+                        if (DERIV) yb[m] = fma(wj[j + 1], g, yb[m]);
+                    }
+                }
+            }
+            double r0[NR], ry[NR];
+#pragma unroll
+            for (int m = 0; m < NR; ++m) {
+                r0[m] = ra[m] + rb[m];
+                ry[m] = ya[m] + yb[m];
+            }
+            const double wx = ex * E1[i];
+            const double ii = i - hq;
+#pragma unroll
+            for (int m = 0; m < NR; ++m) {
+                const double v = wx * r0[m];
+                V[m] += v;
+                if (DERIV) {
+                    X[m] = fma(ii, v, X[m]);
+                    Y[m] = fma(wx, ry[m], Y[m]);
+                }
+            }
+            ex *= e3x;
+        }
+        return;
+    }
+    const double hp = 0.5 * P;
+    double ex = bx;
+    for (int i = 0; i <= P; ++i) {
+        double ey = ex * E1[i];
+        double r0[NR], ry[NR];
+#pragma unroll
+        for (int m = 0; m < NR; ++m) r0[m] = ry[m] = 0.0;
+        double jj = -hp;
+        for (int j = 0; j <= P; ++j) {
+            const double wgt = ey * E1[j];
+#pragma unroll
+            for (int m = 0; m < NR; ++m) {
+                const double wg = wgt * get(i, j, m);
+                r0[m] += wg;
+                ry[m] = fma(jj, wg, ry[m]);
+            }
+            ey *= e3y;
+            jj += 1.0;
+        }
+        const double ii = i - hp;
+#pragma unroll
+        for (int m = 0; m < NR; ++m) {
+            V[m] += r0[m];
+            X[m] = fma(ii, r0[m], X[m]);
+            Y[m] += ry[m];
+        }
+        ex *= e3x;
+    }
+}
+
+// Gradients of the interpolated values, d(scale V_m)/dx and /dy, from the sums.
+template <int NR>
+__device__ inline void interp_grad(double c, double scale, double h, int P, double w, double px,
+                                   double py, const double (&V)[NR], const double (&X)[NR],
+                                   const double (&Y)[NR], double (&gx)[NR], double (&gy)[NR])
+{
+    const double k = -2.0 * c * scale;
+    const double ox = h * (0.5 * P) - w - px, oy = h * (0.5 * P) - w - py;
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+        gx[m] = k * (h * X[m] + ox * V[m]);
+        gy[m] = k * (h * Y[m] + oy * V[m]);
+    }
+}
+
+// dL/dptar from du and the gradients (gx1, gx2, gy1, gy2) of the two outputs.
+__device__ __forceinline__ void target_grad(double a1, double a2, double gx0, double gx1,
+                                            double gy0, double gy1, double *out)
+{
+    out[0] = a1 * gx0 + a2 * gx1;
+    out[1] = a1 * gy0 + a2 * gy1;
+}
+
 // Tile variant: one workgroup per 16x16 block of target anchors.  The grid
 // values the block's targets can touch, a (16+P)^2 region per grid, are staged
 // in LDS once (as (G1, G2) pairs), then each thread interpolates one target at
@@ -901,13 +1023,17 @@ struct InterpTileArgs {
     double Lx, Ly, h, c, w, ch2, scale;
     double *u;
     int64_t ustride;
+    double *jac;           // DERIV: [4 nt] output gradients, caller order
 };
 
 // PT > 0: the support P = PT is a compile-time constant; each row of the
 // support is then one unrolled block (its LDS loads issue together) and the
 // separable y weights e3y^j E1[j] sit in registers, computed once per target.
 // PT = 0: any P, weights formed node by node.
-template <int PT>
+// DERIV (keeping the forward state): also the gradients of both outputs at
+// every target, jac[4 t] = (du1/dx, du2/dx, du1/dy, du2/dy) in caller order,
+// from which the backward call forms dL/dptar.
+template <int PT, bool DERIV>
 __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
 {
     extern __shared__ double lds[];
@@ -971,31 +1097,16 @@ __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
             double ex = bx;
             const double2 *row = G + li * RP + lj;
             if constexpr (PT > 0) {
-                double wy[PT + 1];
-                double ey = 1.0;
-#pragma unroll
-                for (int j = 0; j <= PT; ++j) {
-                    wy[j] = ey * E1[j];
-                    ey *= e3y;
-                }
-                for (int i = 0; i <= PT; ++i) {
-                    double r1a = 0.0, r2a = 0.0, r1b = 0.0, r2b = 0.0;
-#pragma unroll
-                    for (int j = 0; j <= PT; j += 2) {
-                        const double2 gv = row[j];
-                        r1a = fma(wy[j], gv.x, r1a);
-                        r2a = fma(wy[j], gv.y, r2a);
-                        if (j + 1 <= PT) {
-                            const double2 gw = row[j + 1];
-                            r1b = fma(wy[j + 1], gw.x, r1b);
-                            r2b = fma(wy[j + 1], gw.y, r2b);
-                        }
-                    }
-                    const double wx = ex * E1[i];
-                    s1 = fma(wx, r1a + r1b, s1);
-                    s2 = fma(wx, r2a + r2b, s2);
-                    ex *= e3x;
-                    row += RP;
+                const double *base = (const double *)row;
+                auto get = [&](int i, int j, int m) { return base[(i * RP + j) * 2 + m]; };
+                double V[2], X[2], Y[2];
+                interp_sums<2, PT, decltype(get), DERIV>(get, PT, E1, bx, e3x, e3y, V, X, Y);
+                s1 = V[0];
+                s2 = V[1];
+                if (DERIV) {
+                    double gx[2], gy[2];
+                    interp_grad<2>(a.c, a.scale, a.h, PT, a.w, px, py, V, X, Y, gx, gy);
+                    *(double4 *)(a.jac + 4 * (int64_t)t) = make_double4(gx[0], gx[1], gy[0], gy[1]);
                 }
             } else {
                 for (int i = 0; i < P1; ++i) {
@@ -1050,98 +1161,16 @@ struct InterpAdjArgs {
     double *dpos, *df, *dn;   // outputs, caller order; may be null
 };
 
-template <int NR, int PT, class GET>
-__device__ inline void interp_sums(const GET &get, int P, const double *E1, double bx, double e3x,
-                                   double e3y, double (&V)[NR], double (&X)[NR], double (&Y)[NR])
-{
-#pragma unroll
-    for (int m = 0; m < NR; ++m) V[m] = X[m] = Y[m] = 0.0;
-    if constexpr (PT > 0) {
-        // compile-time support: y weights and (j - P/2) y weights in registers,
-        // one unrolled block per row
-        constexpr double hq = 0.5 * PT;
-        double wy[PT + 1], wj[PT + 1];
-        double ey = 1.0;
-#pragma unroll
-        for (int j = 0; j <= PT; ++j) {
-            wy[j] = ey * E1[j];
-            wj[j] = (j - hq) * wy[j];
-            ey *= e3y;
-        }
-        double ex = bx;
-        for (int i = 0; i <= PT; ++i) {
-            double r0[NR], ry[NR];
-#pragma unroll
-            for (int m = 0; m < NR; ++m) r0[m] = ry[m] = 0.0;
-#pragma unroll
-            for (int j = 0; j <= PT; ++j) {
-#pragma unroll
-                for (int m = 0; m < NR; ++m) {
-                    const double g = get(i, j, m);
-                    r0[m] = fma(wy[j], g, r0[m]);
-                    ry[m] = fma(wj[j], g, ry[m]);
-                }
-            }
-            const double wx = ex * E1[i];
-            const double ii = i - hq;
-#pragma unroll
-            for (int m = 0; m < NR; ++m) {
-                const double v = wx * r0[m];
-                V[m] += v;
-                X[m] = fma(ii, v, X[m]);
-                Y[m] = fma(wx, ry[m], Y[m]);
-            }
-            ex *= e3x;
-        }
-        return;
-    }
-    const double hp = 0.5 * P;
-    double ex = bx;
-    for (int i = 0; i <= P; ++i) {
-        double ey = ex * E1[i];
-        double r0[NR], ry[NR];
-#pragma unroll
-        for (int m = 0; m < NR; ++m) r0[m] = ry[m] = 0.0;
-        double jj = -hp;
-        for (int j = 0; j <= P; ++j) {
-            const double wgt = ey * E1[j];
-#pragma unroll
-            for (int m = 0; m < NR; ++m) {
-                const double wg = wgt * get(i, j, m);
-                r0[m] += wg;
-                ry[m] = fma(jj, wg, ry[m]);
-            }
-            ey *= e3y;
-            jj += 1.0;
-        }
-        const double ii = i - hp;
-#pragma unroll
-        for (int m = 0; m < NR; ++m) {
-            V[m] += r0[m];
-            X[m] = fma(ii, r0[m], X[m]);
-            Y[m] += ry[m];
-        }
-        ex *= e3x;
-    }
-}
-
 template <int NR, int MODE>
 __device__ inline void interp_adj_store(const InterpAdjArgs &a, int o, double px, double py,
                                         const double (&V)[NR], const double (&X)[NR],
                                         const double (&Y)[NR])
 {
-    const double k = -2.0 * a.c * a.scale;
-    const double ox = a.h * (0.5 * a.P) - a.w - px, oy = a.h * (0.5 * a.P) - a.w - py;
     double gx[NR], gy[NR];
-#pragma unroll
-    for (int m = 0; m < NR; ++m) {
-        gx[m] = k * (a.h * X[m] + ox * V[m]);
-        gy[m] = k * (a.h * Y[m] + oy * V[m]);
-    }
+    interp_grad<NR>(a.c, a.scale, a.h, a.P, a.w, px, py, V, X, Y, gx, gy);
     if (MODE == ADJ_TARGET) {
-        const double a1 = a.du[2 * o], a2 = a.du[2 * o + 1];
-        a.dpos[2 * o] = a1 * gx[0] + a2 * gx[1];
-        a.dpos[2 * o + 1] = a1 * gy[0] + a2 * gy[1];
+        target_grad(a.du[2 * o], a.du[2 * o + 1], gx[0], gx[NR > 1 ? 1 : 0], gy[0],
+                    gy[NR > 1 ? 1 : 0], a.dpos + 2 * o);
     } else if (MODE == ADJ_SOURCE_SLP) {
         const double f1 = a.f[2 * o], f2 = a.f[2 * o + 1];
         if (a.dpos) {
@@ -1265,6 +1294,16 @@ __global__ __launch_bounds__(256) void k_interp_adj_direct(InterpAdjArgs a)
     double V[NR], X[NR], Y[NR];
     interp_sums<NR, 0>(get, a.P, E1, bx, e3x, e3y, V, X, Y);
     interp_adj_store<NR, MODE>(a, o, px, py, V, X, Y);
+}
+
+// dL/dptar from the kept output gradients (k_interp_tile<PT, true>).
+__global__ void k_target_grad(const double *__restrict__ jac, const double *__restrict__ du,
+                              int64_t n, double *__restrict__ dptar)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double4 g = *(const double4 *)(jac + 4 * t);
+    target_grad(du[2 * t], du[2 * t + 1], g.x, g.y, g.z, g.w, dptar + 2 * t);
 }
 
 // ---- rocFFT plans ------------------------------------------------------------
@@ -1550,6 +1589,7 @@ struct KeptK {
     KeySort kss{};
     GridPass gp{};
     double *grids = nullptr;   // the two filtered grids
+    double *jac = nullptr;     // output gradients at the targets, or null
     bool match(int k, const KGeom &q, const double *ff, const double *nn) const
     {
         return valid && k == kind && q.psrc == g.psrc && q.ptar == g.psrc && q.ns == g.ns &&
@@ -1599,7 +1639,10 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     int2 *anchor;
     double *fgg, *q, *grids;
     double2 *spec;
+    double *jac = nullptr;   // kept output gradients (se2p_set_keep)
+    bool want_jac = false;
     auto layout = [&](Carve &cv) {
+        if (want_jac) jac = cv.take<double>(4 * (size_t)nt);
         char *p = cv.take<char>(keysort_bytes(ns, L.nbins));
         if (p) keysort_carve(p, ns, L.nbins, &kss);
         if (!same_points) {
@@ -1615,6 +1658,7 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     };
     // a call whose backward may follow keeps its state in a buffer of its own
     const bool keep = ctx->keep && !ctx->host_ptrs && same_points && nrhs == 1;
+    want_jac = keep && L.itile && L.P == 24;
     Carve sizing;
     layout(sizing);
     Carve cv;
@@ -1715,6 +1759,7 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         kk.kss = kss;
         kk.gp = gp;
         kk.grids = grids;
+        kk.jac = jac;
         kk.valid = true;
     }
 
@@ -1745,11 +1790,15 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         ia.scale = L.interp_scale;
         ia.u = u;
         ia.ustride = ustride;
-        if (L.P == 24)
-            hipLaunchKernelGGL(k_interp_tile<24>, dim3((unsigned)gp.max_interp_chunks, nrhs),
+        ia.jac = jac;
+        if (L.P == 24 && jac)
+            hipLaunchKernelGGL((k_interp_tile<24, true>), dim3((unsigned)gp.max_interp_chunks, nrhs),
+                               dim3(256), L.lds_itile, st, ia);
+        else if (L.P == 24)
+            hipLaunchKernelGGL((k_interp_tile<24, false>), dim3((unsigned)gp.max_interp_chunks, nrhs),
                                dim3(256), L.lds_itile, st, ia);
         else
-            hipLaunchKernelGGL(k_interp_tile<0>, dim3((unsigned)gp.max_interp_chunks, nrhs),
+            hipLaunchKernelGGL((k_interp_tile<0, false>), dim3((unsigned)gp.max_interp_chunks, nrhs),
                                dim3(256), L.lds_itile, st, ia);
     } else {
         for (int r = 0; r < nrhs; ++r) {
@@ -1916,7 +1965,14 @@ int kspace_adjoint(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, con
     ia.ch2 = L.ch2;
     ia.scale = L.interp_scale;
 
-    if (dptar && kept) {
+    if (dptar && kept && kept->jac) {
+        // the forward's output gradients at the targets, contracted with du
+        stage_begin(ctx, ST_INTERP);
+        hipLaunchKernelGGL(k_target_grad, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st,
+                           kept->jac, du, nt, dptar);
+        SE2P_HIP(hipGetLastError());
+        stage_end(ctx, ST_INTERP);
+    } else if (dptar && kept) {
         // the forward's filtered grids, interpolated with derivative weights
         stage_begin(ctx, ST_INTERP);
         InterpAdjArgs a = ia;
