@@ -888,6 +888,22 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs a)
 // nodes (i, j), W = ex E1[i] E1[j] e3x^i e3y^j the fast-Gaussian-gridding
 // weight (mex_stokes_slp_kspace.cpp:264-329).  get(i, j, m) returns grid m at
 // support node (i, j).
+// The NR values of a support node stored as NR consecutive doubles (LDS);
+// two grids are read as one 16-byte load.
+template <int NR>
+__device__ __forceinline__ void load_node(const double *p, double (&g)[NR])
+{
+    if constexpr (NR == 2) {
+        const double2 v = *(const double2 *)p;
+        g[0] = v.x;
+        g[1] = v.y;
+    } else {
+#pragma unroll
+        for (int m = 0; m < NR; ++m) g[m] = p[m];
+    }
+}
+
+// get(i, j, g) fills g[0..NR) with the grid values at support node (i, j).
 // DERIV false: V only (X, Y stay zero); V is computed the same way either way.
 template <int NR, int PT, class GET, bool DERIV = true>
 __device__ inline void interp_sums(const GET &get, int P, const double *E1, double bx, double e3x,
@@ -915,18 +931,19 @@ __device__ inline void interp_sums(const GET &get, int P, const double *E1, doub
             for (int m = 0; m < NR; ++m) ra[m] = rb[m] = ya[m] = yb[m] = 0.0;
 #pragma unroll
             for (int j = 0; j <= PT; j += 2) {
+                double g[NR];
+                get(i, j, g);
 #pragma unroll
                 for (int m = 0; m < NR; ++m) {
-                    const double g = get(i, j, m);
-                    ra[m] = fma(wy[j], g, ra[m]);
-                    if (DERIV) ya[m] = fma(wj[j], g, ya[m]);
+                    ra[m] = fma(wy[j], g[m], ra[m]);
+                    if (DERIV) ya[m] = fma(wj[j], g[m], ya[m]);
                 }
                 if (j + 1 <= PT) {
+                    get(i, j + 1, g);
 #pragma unroll
                     for (int m = 0; m < NR; ++m) {
-                        const double g = get(i, j + 1, m);
-                        rb[m] = fma(wy[j + 1], g, rb[m]);
-                        if (DERIV) yb[m] = fma(wj[j + 1], g, yb[m]);
+                        rb[m] = fma(wy[j + 1], g[m], rb[m]);
+                        if (DERIV) yb[m] = fma(wj[j + 1], g[m], yb[m]);
                     }
                 }
             }
@@ -961,9 +978,11 @@ __device__ inline void interp_sums(const GET &get, int P, const double *E1, doub
         double jj = -hp;
         for (int j = 0; j <= P; ++j) {
             const double wgt = ey * E1[j];
+            double gv[NR];
+            get(i, j, gv);
 #pragma unroll
             for (int m = 0; m < NR; ++m) {
-                const double wg = wgt * get(i, j, m);
+                const double wg = wgt * gv[m];
                 r0[m] += wg;
                 ry[m] = fma(jj, wg, ry[m]);
             }
@@ -1098,7 +1117,7 @@ __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
             const double2 *row = G + li * RP + lj;
             if constexpr (PT > 0) {
                 const double *base = (const double *)row;
-                auto get = [&](int i, int j, int m) { return base[(i * RP + j) * 2 + m]; };
+                auto get = [&](int i, int j, double (&g)[2]) { load_node<2>(base + (i * RP + j) * 2, g); };
                 double V[2], X[2], Y[2];
                 interp_sums<2, PT, decltype(get), DERIV>(get, PT, E1, bx, e3x, e3y, V, X, Y);
                 s1 = V[0];
@@ -1261,7 +1280,7 @@ __global__ __launch_bounds__(256) void k_interp_adj_tile(InterpAdjArgs a)
         const int li = ((mx % a.Mx) + a.Mx) % a.Mx - X0;
         const int lj = ((my % a.My) + a.My) % a.My - Y0;
         const double *base = G + (size_t)(li * RP + lj) * NR;
-        auto get = [&](int i, int j, int m) { return base[(i * RP + j) * NR + m]; };
+        auto get = [&](int i, int j, double (&g)[NR]) { load_node<NR>(base + (i * RP + j) * NR, g); };
         double V[NR], X[NR], Y[NR];
         interp_sums<NR, PT>(get, P, E1, bx, e3x, e3y, V, X, Y);
         interp_adj_store<NR, MODE>(a, o, px, py, V, X, Y);
@@ -1287,9 +1306,10 @@ __global__ __launch_bounds__(256) void k_interp_adj_direct(InterpAdjArgs a)
     fgg_factors(px, py, a.c, a.w, a.h, &bx, &e3x, &e3y);
     const int col0 = ((mx % a.Mx) + a.Mx) % a.Mx;
     const int row0 = ((my % a.My) + a.My) % a.My;
-    auto get = [&](int i, int j, int m) {
+    auto get = [&](int i, int j, double (&g)[NR]) {
         const int64_t idx = (int64_t)((col0 + i) % a.Mx) * a.My + (row0 + j) % a.My;
-        return a.grids[m * a.gstride + idx];
+#pragma unroll
+        for (int m = 0; m < NR; ++m) g[m] = a.grids[m * a.gstride + idx];
     };
     double V[NR], X[NR], Y[NR];
     interp_sums<NR, 0>(get, a.P, E1, bx, e3x, e3y, V, X, Y);
