@@ -400,6 +400,86 @@ def _autotune_variant(args, pe, np, torch, w, ps, f, nv, ctx, stream, fwd_ms):
                         "parameters from se2p_autotune"}
 
 
+def _at_scale_variant(args, pe, W, np, torch, dev):
+    """The at-scale workload: N = 4e6 uniform points in the unit box (targets
+    = sources, seed 5), tol 1e-12 parameters from the driver formulas; the
+    Stokeslet on a 2048^2 grid, the stresslet on the driver's 3060^2 grid
+    (2048^2 is below its 2 k_inf at this tolerance).  Forward step (real +
+    Fourier sums) and forward + adjoint step (both backward entries, every
+    gradient, forward state kept), each timed over a few steps after a
+    warm-up."""
+    from fasttools2d_amd import params as Pm
+
+    N = 4_000_000
+    w = W.uniform_box(N, seed=5)
+    du = torch.from_numpy(np.random.default_rng(78).standard_normal((N, 2))).to(dev)
+    ps = torch.from_numpy(np.ascontiguousarray(w.psrc)).to(dev)
+    f = torch.from_numpy(np.ascontiguousarray(w.f)).to(dev)
+    nv = torch.from_numpy(np.ascontiguousarray(w.n)).to(dev)
+    ur = torch.empty((N, 2), dtype=torch.float64, device=dev)
+    uk = torch.empty_like(ur)
+    out = {}
+    for kind, M in (("slp", 2048), ("dlp", None)):
+        p = Pm.select_params(kind, N, N, w.f, 1.0, 1.0, tol=1e-12, M=M)
+        ctx = pe.Context(0)
+        stream = torch.cuda.Stream(dev)
+
+        def fwd():
+            if kind == "slp":
+                pe.mex_stokes_slp_real(ps, ps, f, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, ctx=ctx, out=ur)
+                pe.mex_stokes_slp_kspace(ps, ps, p.xi, p.eta, f, p.Mx, p.My, 1.0, 1.0, p.w, p.P,
+                                         ctx=ctx, out=uk)
+            else:
+                pe.mex_stokes_dlp_real(ps, ps, f, nv, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, ctx=ctx,
+                                       out=ur)
+                pe.mex_stokes_dlp_kspace(ps, ps, p.xi, p.eta, f, nv, p.Mx, p.My, 1.0, 1.0, p.w, p.P,
+                                         ctx=ctx, out=uk)
+
+        def fwd_adj():
+            fwd()
+            if kind == "slp":
+                pe.mex_stokes_slp_real_backward(ps, ps, f, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, du,
+                                                ctx=ctx)
+                pe.mex_stokes_slp_kspace_backward(ps, ps, p.xi, p.eta, f, p.Mx, p.My, 1.0, 1.0,
+                                                  p.w, p.P, du, ctx=ctx)
+            else:
+                pe.mex_stokes_dlp_real_backward(ps, ps, f, nv, p.xi, p.nside_x, p.nside_y, 1.0, 1.0,
+                                                du, ctx=ctx)
+                pe.mex_stokes_dlp_kspace_backward(ps, ps, p.xi, p.eta, f, nv, p.Mx, p.My, 1.0, 1.0,
+                                                  p.w, p.P, du, ctx=ctx)
+
+        def timed(fn, reps=3):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+
+        with torch.cuda.stream(stream):
+            ctx.bind_torch_stream(stream)
+            ctx.set_async(True)
+            f_ms = timed(fwd)
+            ctx.set_keep(True)
+            fa_ms = timed(fwd_adj)
+            ctx.set_keep(False)
+            ctx.set_async(False)
+        torch.cuda.synchronize()
+        ctx.close()
+        print(f"# at scale {kind}: N=4e6 grid {p.Mx}^2 nside {p.nside_x}: forward {f_ms:.2f} ms, "
+              f"forward+adjoint {fa_ms:.2f} ms", file=sys.stderr, flush=True)
+        out[kind] = {"forward_ms": f_ms, "forward_evals_per_s": N / (f_ms * 1e-3),
+                     "forward_plus_adjoint_ms": fa_ms, "over_forward": fa_ms / f_ms,
+                     "params": {"Mx": p.Mx, "nside": p.nside_x, "xi": p.xi, "P": p.P,
+                                "tol": 1e-12}}
+    out["workload"] = ("N = 4e6 uniform points, unit box, targets = sources, tol 1e-12; "
+                       "Stokeslet grid 2048^2, stresslet grid 3060^2 (driver)")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,6 +491,8 @@ def main() -> int:
                     help="skip the plain-vs-batched right-hand-side comparison")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+adjoint variant")
+    ap.add_argument("--no-scale", action="store_true",
+                    help="skip the N = 4e6 forward / forward+adjoint variant")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (scales HBM counters)")
     args = ap.parse_args()
@@ -567,6 +649,8 @@ def main() -> int:
         variants["graph"] = _graph_variant(pe, torch, step, ctx, stream, N)
         variants["autotuned"] = _autotune_variant(args, pe, np, torch, w, ps, f, nv, ctx, stream,
                                                   ms_per_step)
+        if not args.no_scale:
+            variants["at_scale"] = _at_scale_variant(args, pe, W, np, torch, dev)
     adjoint = None
     if not args.no_adjoint:
         adjoint = _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, ms_per_step)
