@@ -1775,8 +1775,9 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         SE2P_HIP(hipGetLastError());
         return 0;
     }
-    const bool same_points = (g.ptar == g.psrc) && (nt == ns);
-    if (same_points) return real_adjoint_same(ctx, kind, g, f, n, du, dpsrc, dptar, df, dn);
+    if (g.ptar == g.psrc && nt == ns)   // targets are the sources
+        return real_adjoint_same(ctx, kind, g, f, n, du, dpsrc, dptar, df, dn);
+    // separate targets: a target pass and a source pass over their own cell lists
     if (kind == 0 && want_src && upload_table(ctx)) return SE2P_ERR_HIP;
     const int ncomp = kind == 0 ? 2 : 3;
     const int64_t max_t = (nt + 63) / 64 + ncell, max_s = (ns + 63) / 64 + ncell;
@@ -1791,12 +1792,10 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
     auto layout = [&](Carve &cv) {
         char *p = cv.take<char>(keysort_bytes(ns, nkeys));
         if (p) keysort_carve(p, ns, nkeys, &kss);
-        if (!same_points) {
-            p = cv.take<char>(keysort_bytes(nt, nkeys));
-            if (p) keysort_carve(p, nt, nkeys, &kst);
-        }
+        p = cv.take<char>(keysort_bytes(nt, nkeys));
+        if (p) keysort_carve(p, nt, nkeys, &kst);
         spos = cv.take<double2>(ns);
-        tpos = same_points ? spos : cv.take<double2>(nt);
+        tpos = cv.take<double2>(nt);
         q = cv.take<double>((size_t)ncomp * ns);
         dut = cv.take<double>(2 * (size_t)nt);
         coff_t = cv.take<int>(ncell + 1);
@@ -1804,7 +1803,7 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
         bmap_t = cv.take<int>(max_t);
         bmap_s = cv.take<int>(max_s);
         sbox = cv.take<double4>(nkeys);
-        tbox = same_points ? sbox : cv.take<double4>(nkeys);
+        tbox = cv.take<double4>(nkeys);
     };
     Carve sizing;
     layout(sizing);
@@ -1815,9 +1814,8 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
 
     stage_begin(ctx, ST_SORT);
     if (int rc = cell_sort(ctx, g.psrc, ns, g.Lx, g.Ly, g.nsx, g.nsy, kSubLog, &kss)) return rc;
-    if (!same_points)
-        if (int rc = cell_sort(ctx, g.ptar, nt, g.Lx, g.Ly, g.nsx, g.nsy, kSubLog, &kst)) return rc;
-    const KeySort &tks = same_points ? kss : kst;
+    if (int rc = cell_sort(ctx, g.ptar, nt, g.Lx, g.Ly, g.nsx, g.nsy, kSubLog, &kst)) return rc;
+    const KeySort &tks = kst;
     const int threads = 256;
     const unsigned sblocks = (unsigned)((ns + threads - 1) / threads);
     const unsigned tblocks = (unsigned)((nt + threads - 1) / threads);
@@ -1827,16 +1825,15 @@ int real_adjoint(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, co
     else
         hipLaunchKernelGGL(k_gather_sources<1>, dim3(sblocks), dim3(threads), 0, st, g.psrc, f, n,
                            kss.perm, (int)ns, 1, (int64_t)0, spos, q);
-    if (!same_points)
-        hipLaunchKernelGGL(k_gather_targets, dim3(tblocks), dim3(threads), 0, st, g.ptar, tks.perm,
-                           (int)nt, tpos);
+    hipLaunchKernelGGL(k_gather_targets, dim3(tblocks), dim3(threads), 0, st, g.ptar, tks.perm,
+                       (int)nt, tpos);
     hipLaunchKernelGGL(k_gather_pairs, dim3(tblocks), dim3(threads), 0, st, du, tks.perm, (int)nt, dut);
     const unsigned kblocks = (unsigned)((nkeys + 3) / 4);   // k_run_bbox: one wavefront per run
-    // run boxes of the uniform side: sources for the target pass, targets
-    // (the same array when targets are the sources) for the source pass
-    if (dptar || (want_src && same_points))
+    // run boxes of the uniform side: sources for the target pass, targets for
+    // the source pass
+    if (dptar)
         hipLaunchKernelGGL(k_run_bbox, dim3(kblocks), dim3(threads), 0, st, spos, kss.off, nkeys, sbox);
-    if (want_src && !same_points)
+    if (want_src)
         hipLaunchKernelGGL(k_run_bbox, dim3(kblocks), dim3(threads), 0, st, tpos, tks.off, nkeys, tbox);
     if (dptar) plan_chunks(st, nullptr, tks.off, sub, 64, ncell, coff_t, bmap_t);
     if (want_src) plan_chunks(st, nullptr, kss.off, sub, 64, ncell, coff_s, bmap_s);

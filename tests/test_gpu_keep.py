@@ -112,3 +112,44 @@ def test_kept_benchmark_adjoint_matches_oracle(gpu):
             print(f"  {k}: {e:.2e}", flush=True)
             assert e <= 1e-12, k
     ctx.close()
+
+
+def test_keep_mode_edge_cases(gpu):
+    """Calls that keep nothing or must not reuse: separate targets and a
+    replaced kept state (forward with f, forward with f2, backward with f)
+    give the same results with keeping on as off."""
+    pe = gpu
+    w = W.uniform_box(2000, seed=65, L=1.0, targets_equal_sources=False, Nt=700)
+    p = Pm.select_params("dlp", 2000, 700, w.f, 1.0, 1.0, tol=1e-10, M=64)
+    ps, pt, f, nv = _t(w.psrc), _t(w.ptar), _t(w.f), _t(w.n)
+    du_t = _t(np.random.default_rng(66).standard_normal((700, 2)))
+    plain = pe.Context(0)
+    keep = pe.Context(0)
+    keep.set_keep(True)
+    print("[keep edge: separate targets]", flush=True)
+    outs = []
+    for ctx in (plain, keep):
+        ur = pe.mex_stokes_dlp_real(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, ctx=ctx)
+        uk = pe.mex_stokes_dlp_kspace(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, 1.0, 1.0, p.w,
+                                      p.P, ctx=ctx)
+        gr = pe.mex_stokes_dlp_real_backward(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, 1.0,
+                                             1.0, du_t, ctx=ctx)
+        gk = pe.mex_stokes_dlp_kspace_backward(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, 1.0,
+                                               1.0, p.w, p.P, du_t, ctx=ctx)
+        outs.append([ur.cpu().numpy(), uk.cpu().numpy()] +
+                    [v.cpu().numpy() for g in (gr, gk) for v in g.values()])
+    assert all(np.array_equal(a, b) for a, b in zip(*outs))
+    print("[keep edge: replaced state]", flush=True)
+    q = Pm.select_params("dlp", 2000, 2000, w.f, 1.0, 1.0, tol=1e-10, M=64)
+    du = _t(np.random.default_rng(67).standard_normal((2000, 2)))
+    f2 = f * 0.5 + 0.25
+    _fwd(pe, "dlp_real", ps, f, nv, q, 1.0, 1.0, keep)
+    _fwd(pe, "dlp_kspace", ps, f, nv, q, 1.0, 1.0, keep)
+    _fwd(pe, "dlp_real", ps, f2, nv, q, 1.0, 1.0, keep)     # replaces the kept state
+    _fwd(pe, "dlp_kspace", ps, f2, nv, q, 1.0, 1.0, keep)
+    for kind in ("dlp_real", "dlp_kspace"):
+        a = _bwd(pe, kind, ps, f, nv, du, q, 1.0, 1.0, keep)
+        b = _bwd(pe, kind, ps, f, nv, du, q, 1.0, 1.0, plain)
+        assert all(np.array_equal(a[k], b[k]) for k in b), kind
+    keep.close()
+    plain.close()
