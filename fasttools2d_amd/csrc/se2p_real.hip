@@ -851,7 +851,8 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
 // so grad phi = sum_k w_k D_k with D_lk = d_l t_k: the 2 x 4 tensor D is
 // computed once per pair and contracted with the target side's weights
 // (lane du, uniform densities) and the source side's (uniform du, lane
-// densities); grad t is even in d, t is odd.  With t = alpha P3(d) + beta P1(d),
+// densities); grad t is even in d, t is odd.  The lane's factor of each
+// weight is constant over the pass, so it is applied once at the end.  With t = alpha P3(d) + beta P1(d),
 // alpha = -4 e2 (1 + x)/r^4, beta = 2 xi^2 e2, x = xi^2 r^2:
 //   D_lk = d_l [g1 P3_k + g2 P1_k] + alpha dP3_k/dd_l + beta dP1_k/dd_l,
 //   g1 = 8 e2 (x^2 + 2 x + 2)/r^6, g2 = -2 xi^2 beta,
@@ -872,7 +873,12 @@ template <int KIND>
 __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
 {
     constexpr int RN = SrcRec<KIND>::n;
-    constexpr int NV = KIND == 0 ? 6 : 7;   // target grad, source grad, density terms
+    // accumulators.  Stokeslet: target grad (2), source grad (2), density (2).
+    // Stresslet, with the lane's du and densities factored out of the sums
+    // (they are constant over the pass): T[2l + a] = sum_j (D q_j)-terms of
+    // target component l and lane du component a (4), U[2 + 3l + c] likewise
+    // for the source side and lane density c (6), then the density terms (3).
+    constexpr int NV = KIND == 0 ? 6 : 13;
     __shared__ double e1tab[KIND == 0 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
     __shared__ double etab[SE2P_EXP2_N];
     __shared__ double red[kWaves - 1][NV][64];
@@ -965,23 +971,24 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
                          D13 = t1 * h3;
             const double D20 = t2 * h0, D21 = t2 * h1 + b1, D22 = t2 * h2 + 2.0 * A12,
                          D23 = t2 * h3 + 3.0 * b3;
-            // target side: lane du, uniform densities
-            {
-                const double w0 = la1 * u.q0, w1 = la1 * u.q1 + la2 * u.q0,
-                             w2 = la1 * u.q2 + la2 * u.q1, w3 = la2 * u.q2;
-                acc[0] += D10 * w0 + D11 * w1 + D12 * w2 + D13 * w3;
-                acc[1] += D20 * w0 + D21 * w1 + D22 * w2 + D23 * w3;
-            }
-            // source side: uniform du, lane densities
-            {
-                const double w0 = u.b1 * lq0, w1 = u.b1 * lq1 + u.b2 * lq0,
-                             w2 = u.b1 * lq2 + u.b2 * lq1, w3 = u.b2 * lq2;
-                acc[2] += D10 * w0 + D11 * w1 + D12 * w2 + D13 * w3;
-                acc[3] += D20 * w0 + D21 * w1 + D22 * w2 + D23 * w3;
-            }
-            acc[4] -= u.b1 * tt0 + u.b2 * tt1;
-            acc[5] -= u.b1 * tt1 + u.b2 * tt2;
-            acc[6] -= u.b1 * tt2 + u.b2 * tt3;
+            // target side (uniform densities q): component l, lane du a:
+            //   sum_k w_k D_lk = a1 (D_l0 q0 + D_l1 q1 + D_l2 q2)
+            //                  + a2 (D_l1 q0 + D_l2 q1 + D_l3 q2)
+            acc[0] = fma(D12, u.q2, fma(D11, u.q1, fma(D10, u.q0, acc[0])));
+            acc[1] = fma(D13, u.q2, fma(D12, u.q1, fma(D11, u.q0, acc[1])));
+            acc[2] = fma(D22, u.q2, fma(D21, u.q1, fma(D20, u.q0, acc[2])));
+            acc[3] = fma(D23, u.q2, fma(D22, u.q1, fma(D21, u.q0, acc[3])));
+            // source side (uniform du b), lane density c: D_lc b1 + D_l,c+1 b2
+            acc[4] = fma(D11, u.b2, fma(D10, u.b1, acc[4]));
+            acc[5] = fma(D12, u.b2, fma(D11, u.b1, acc[5]));
+            acc[6] = fma(D13, u.b2, fma(D12, u.b1, acc[6]));
+            acc[7] = fma(D21, u.b2, fma(D20, u.b1, acc[7]));
+            acc[8] = fma(D22, u.b2, fma(D21, u.b1, acc[8]));
+            acc[9] = fma(D23, u.b2, fma(D22, u.b1, acc[9]));
+            // density gradient of the lane as source (tensor -t)
+            acc[10] = fma(-u.b2, tt1, fma(-u.b1, tt0, acc[10]));
+            acc[11] = fma(-u.b2, tt2, fma(-u.b1, tt1, acc[11]));
+            acc[12] = fma(-u.b2, tt3, fma(-u.b1, tt2, acc[12]));
         } else {
             double t1, t2, s1, s2;
             diff<0>(u.x, u.y, lx, ly, zx, zy, t1, t2);   // lane as target
@@ -1063,22 +1070,31 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
         for (int w = 0; w < kWaves - 1; ++w) acc[v] += red[w][v][lane];
     const int o = a.perm[l];
     const double s4 = 1.0 / (4 * SE2P_PI);
-    if (a.dptar) {
-        a.dptar[2 * o] = -acc[0] * s4;
-        a.dptar[2 * o + 1] = -acc[1] * s4;
-    }
-    if (a.dpsrc) {
-        a.dpsrc[2 * o] = acc[2] * s4;
-        a.dpsrc[2 * o + 1] = acc[3] * s4;
-    }
     if (KIND == 0) {
+        if (a.dptar) {
+            a.dptar[2 * o] = -acc[0] * s4;
+            a.dptar[2 * o + 1] = -acc[1] * s4;
+        }
+        if (a.dpsrc) {
+            a.dpsrc[2 * o] = acc[2] * s4;
+            a.dpsrc[2 * o + 1] = acc[3] * s4;
+        }
         if (a.df) {
             a.df[2 * o] = acc[4] * s4;
             a.df[2 * o + 1] = acc[5] * s4;
         }
     } else {
+        // the lane's du and densities multiplied back in
+        if (a.dptar) {
+            a.dptar[2 * o] = -(la1 * acc[0] + la2 * acc[1]) * s4;
+            a.dptar[2 * o + 1] = -(la1 * acc[2] + la2 * acc[3]) * s4;
+        }
+        if (a.dpsrc) {
+            a.dpsrc[2 * o] = (lq0 * acc[4] + lq1 * acc[5] + lq2 * acc[6]) * s4;
+            a.dpsrc[2 * o + 1] = (lq0 * acc[7] + lq1 * acc[8] + lq2 * acc[9]) * s4;
+        }
         // m = dL/d(S11, S12 + S21, S22), S = f (x) n
-        const double m1 = acc[4] * s4, m2 = acc[5] * s4, m3 = acc[6] * s4;
+        const double m1 = acc[10] * s4, m2 = acc[11] * s4, m3 = acc[12] * s4;
         const double f1 = a.f[2 * o], f2 = a.f[2 * o + 1];
         const double n1 = a.nv[2 * o], n2 = a.nv[2 * o + 1];
         if (a.df) {
