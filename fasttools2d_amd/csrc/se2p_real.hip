@@ -53,13 +53,11 @@ constexpr int kWaves = 4;
 __device__ __forceinline__ int nb_dx(int j) { return (j < 4 ? j : j + 1) / 3 - 1; }
 __device__ __forceinline__ int nb_dy(int j) { return (j < 4 ? j : j + 1) % 3 - 1; }
 
-// 1/x from the hardware estimate and two Newton steps (|x| normal, finite).
+// 1/x from the hardware estimate and one Newton step (|x| normal, finite).
 __device__ __forceinline__ double rcp_nr(double x)
 {
-    double y = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-x, y, 1.0);
+    const double y = __builtin_amdgcn_rcp(x);
+    const double e = fma(-x, y, 1.0);
     return fma(y, e, y);
 }
 
