@@ -166,9 +166,10 @@ __device__ __forceinline__ void slp_pair(double d1, double d2, double r2, double
         slp_add(d1, d2, hE, g, q[(2 * r) * qs], q[(2 * r + 1) * qs], acc[r][0], acc[r][1]);
 }
 
-// One right-hand side: each source is one record of SrcRec<KIND>::n doubles
-// (x, y and its densities: (f1, f2), or the five stresslet values and a pad),
-// fetched by one scalar load.
+// One right-hand side: each source is one record of SrcRec<KIND>::n doubles,
+// fetched by one scalar load: (x, y, f1, f2) for the Stokeslet; for the
+// stresslet (x, y, -4 s0, -4 s1, -4 s2, b c1, b c2, b s1) with
+// s = (S11, S12 + S21, S22), c1 = 3 s0 + s2, c2 = s0 + 3 s2, b = 2 xi^2.
 template <int KIND>
 struct SrcRec {
     static constexpr int n = KIND == 0 ? 4 : 8;
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
         // one source: its record (single density) or position + density array
         // a source's position and (one right-hand side) its record values
         struct Src {
-            double x, y, q[kDlpComp];
+            double x, y, q[kDlpComp + 1];
         };
         auto load = [&](int s) {
             Src o;
@@ -394,6 +395,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
                     o.q[2] = v1.x;
                     o.q[3] = v1.y;
                     o.q[4] = v1.z;
+                    o.q[5] = v1.w;
                 }
             } else {
                 const double2 sp = spos[s];
@@ -437,10 +439,14 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
                     slp_factors(r2, xi, e1tab, etab, hE, g);
                     slp_add(d1, d2, hE, g, q0[0], q0[1], acc[0][0], acc[0][1]);
                 } else {
-                    double g, pe;
-                    dlp_factors(r2, xi2, etab, g, pe);
-                    dlp_add(d1, d2, d11, d1 * d2, d22, g, pe, q0[0], q0[1], q0[2], q0[3], q0[4],
-                            acc[0][0], acc[0][1]);
+                    // record values carry -4 and 2 xi^2: u_i += G d_i (d.S'd) + e2 l'_i
+                    const double x = xi2 * r2;
+                    const double e2 = se2p_exp_neg(x, etab);
+                    const double ir2 = rcp_nr(r2);
+                    const double G = e2 * (1.0 + x) * (ir2 * ir2);
+                    const double gq = G * (d11 * q0[0] + (d1 * d2) * q0[1] + d22 * q0[2]);
+                    acc[0][0] += gq * d1 + e2 * (d1 * q0[3] + d2 * q0[5]);
+                    acc[0][1] += gq * d2 + e2 * (d1 * q0[5] + d2 * q0[4]);
                 }
             } else {
                 if (KIND == 0) slp_pair<RB>(d1, d2, r2, xi, e1tab, etab, qb + s, qs, acc);
@@ -907,7 +913,11 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
         ly = v0.y;
         lq0 = v0.z;
         lq1 = v0.w;
-        if (KIND == 1) lq2 = rp[4];
+        if (KIND == 1) {   // the record holds -4 S: exact scaling back
+            lq0 = -0.25 * v0.z;
+            lq1 = -0.25 * v0.w;
+            lq2 = -0.25 * rp[4];
+        }
         const double2 d = a.du[l];
         la1 = d.x;
         la2 = d.y;
@@ -1082,10 +1092,13 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
             a.df[2 * o + 1] = acc[5] * s4;
         }
     } else {
-        // the lane's du and densities multiplied back in
+        // the lane's du and densities multiplied back in; the target-side
+        // sums used the records' -4 S (exact scaling back)
         if (a.dptar) {
-            a.dptar[2 * o] = -(la1 * acc[0] + la2 * acc[1]) * s4;
-            a.dptar[2 * o + 1] = -(la1 * acc[2] + la2 * acc[3]) * s4;
+            const double t0 = -0.25 * acc[0], t1 = -0.25 * acc[1], t2 = -0.25 * acc[2],
+                         t3 = -0.25 * acc[3];
+            a.dptar[2 * o] = -(la1 * t0 + la2 * t1) * s4;
+            a.dptar[2 * o + 1] = -(la1 * t2 + la2 * t3) * s4;
         }
         if (a.dpsrc) {
             a.dpsrc[2 * o] = (lq0 * acc[4] + lq1 * acc[5] + lq2 * acc[6]) * s4;
@@ -1169,8 +1182,12 @@ __device__ __forceinline__ void gather_source(const BoxTable &boxes, const int *
         } else {
             const double n1 = B.nv[2 * o], n2 = B.nv[2 * o + 1];
             const double s0 = f1 * n1, s1 = f1 * n2 + f2 * n1, s2 = f2 * n2;
-            *(double4 *)rp = make_double4(x, y, s0, s1);
-            *(double4 *)(rp + 4) = make_double4(s2, 3.0 * s0 + s2, s0 + 3.0 * s2, 0.0);
+            // the pair kernel's constant factors folded in: -4 S (exactly
+            // -4 times, a power of two) and 2 xi^2 times (c1, c2, s1)
+            const double b = 2.0 * B.xi2;
+            *(double4 *)rp = make_double4(x, y, -4.0 * s0, -4.0 * s1);
+            *(double4 *)(rp + 4) =
+                make_double4(-4.0 * s2, b * (3.0 * s0 + s2), b * (s0 + 3.0 * s2), b * s1);
         }
         return;
     }
