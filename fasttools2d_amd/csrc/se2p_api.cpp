@@ -1214,6 +1214,8 @@ int se2p_autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const
     if (!ctx) return SE2P_ERR_ARG;
     begin_call(ctx);
     if (kind != 0 && kind != 1) return fail(ctx, SE2P_ERR_ARG, "kind must be 0 (SLP) or 1 (DLP)");
+    // the tuner writes no output array: ptar stands in for check_common's
+    // output pointer, so only the input pointers and counts are checked
     if (int rc = check_common(ctx, psrc, ns, ptar, nt, ptar)) return rc;
     if (ns > 0 && (!f || (kind == 1 && !n)))
         return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
