@@ -3,10 +3,11 @@ sources, one array), tol 1e-12 parameters from the driver formulas.  The
 Stokeslet runs on a 2048^2 grid (eta = 0.93); the stresslet on the driver's
 own 3060^2 grid (2048^2 is below its 2 k_inf at tol 1e-12 and would give
 eta > 1).  The GPU evaluates every target; 256 of them are compared with the
-oracle at 1e-12."""
+oracle at 1e-12.  The adjoint of both kernels is checked at the same size."""
 import numpy as np
 import pytest
 import torch
+from scipy.spatial import cKDTree
 
 from oracle import cref
 from fasttools2d_amd import params as Pm
@@ -59,54 +60,123 @@ def test_stresslet_4e6_points_driver_grid(gpu, big):
     assert er <= 1e-12 and ek <= 1e-12
 
 
-def test_adjoint_finite_differences_4e6_points(gpu, big):
-    """Loss 1/2 |u - u*|^2 on the full Stokeslet Ewald sum (real + Fourier) at
-    N = 4e6, with u* the sum at positions perturbed by N(0, (1e-4)^2) and
-    densities scaled by 1 + 1e-3 N(0,1).  The backward entries' position
-    gradient (dpsrc + dptar: targets are the sources) is checked against
-    central differences on 16 points (seed 55), step 1e-7, and the density
-    gradient by the exact linearity of the loss's first variation in f."""
-    w, _ = big
-    p = Pm.select_params("slp", N, N, w.f, 1.0, 1.0, tol=1e-12, M=2048)
-    rng = np.random.default_rng(55)
-
-    def forward(x, f):
-        xs, fs = _dev(x), _dev(f)
+def _full_sum(gpu, kind, p, x, f, nv):
+    """Real + Fourier Ewald sum at targets = sources x."""
+    xs, fs = _dev(x), _dev(f)
+    if kind == "slp":
         ur = gpu.mex_stokes_slp_real(xs, xs, fs, p.xi, p.nside_x, p.nside_y, 1.0, 1.0)
         uk = gpu.mex_stokes_slp_kspace(xs, xs, p.xi, p.eta, fs, p.Mx, p.My, 1.0, 1.0, p.w, p.P)
-        return (ur + uk).cpu().numpy()
+    else:
+        ns = _dev(nv)
+        ur = gpu.mex_stokes_dlp_real(xs, xs, fs, ns, p.xi, p.nside_x, p.nside_y, 1.0, 1.0)
+        uk = gpu.mex_stokes_dlp_kspace(xs, xs, p.xi, p.eta, fs, ns, p.Mx, p.My, 1.0, 1.0,
+                                       p.w, p.P)
+    return (ur + uk).cpu().numpy()
 
-    x0, f0 = w.psrc, w.f
+
+def _full_backward(gpu, kind, p, x, f, nv, du):
+    """Gradients of sum(du * full sum): positions (source + target sides, as
+    the targets are the sources), densities, normals (stresslet)."""
+    xs, fs, dd = _dev(x), _dev(f), _dev(du)
+    if kind == "slp":
+        gr = gpu.mex_stokes_slp_real_backward(xs, xs, fs, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, dd)
+        gk = gpu.mex_stokes_slp_kspace_backward(xs, xs, p.xi, p.eta, fs, p.Mx, p.My, 1.0, 1.0,
+                                                p.w, p.P, dd)
+    else:
+        ns = _dev(nv)
+        gr = gpu.mex_stokes_dlp_real_backward(xs, xs, fs, ns, p.xi, p.nside_x, p.nside_y,
+                                              1.0, 1.0, dd)
+        gk = gpu.mex_stokes_dlp_kspace_backward(xs, xs, p.xi, p.eta, fs, ns, p.Mx, p.My,
+                                                1.0, 1.0, p.w, p.P, dd)
+    g = {"pos": (gr["psrc"] + gr["ptar"] + gk["psrc"] + gk["ptar"]).cpu().numpy(),
+         "f": (gr["f"] + gk["f"]).cpu().numpy()}
+    if kind == "dlp":
+        g["n"] = (gr["n"] + gk["n"]).cpu().numpy()
+    return g
+
+
+@pytest.mark.parametrize("kind", ["slp", "dlp"])
+def test_adjoint_finite_differences_4e6_points(gpu, big, kind):
+    """Loss 1/2 |u - u*|^2 on the full Ewald sum (real + Fourier) at N = 4e6,
+    tol 1e-12 (Stokeslet on a 2048^2 grid, stresslet on the driver's 3060^2),
+    with u* the sum at positions perturbed by N(0, (1e-4)^2) and densities
+    scaled by 1 + 1e-3 N(0,1).  Checks of the backward entries' gradients:
+    - positions, on a subset of 1e4 points: central differences of the loss
+      along one random direction that moves all of them at once, against the
+      gradient's projection on it (steps 1e-7 and 2e-7 with Richardson
+      extrapolation, points 2e-4 from any other);
+    - positions, one point and coordinate at a time: central differences on
+      points whose moves cross no cell or grid-anchor boundary;
+    - densities (and normals for the stresslet): the loss's first variation is
+      exactly linear in them, d/de L(f + e v) = <du, u(v)>."""
+    w, _ = big
+    M = 2048 if kind == "slp" else None
+    p = Pm.select_params(kind, N, N, w.f, 1.0, 1.0, tol=1e-12, M=M)
+    rng = np.random.default_rng(55 if kind == "slp" else 56)
+    x0, f0, n0 = w.psrc, w.f, w.n
+
+    def total(x, f, nv):
+        return _full_sum(gpu, kind, p, x, f, nv)
+
     xstar = Pm.wrap(x0 + 1e-4 * rng.standard_normal(x0.shape), 1.0)
     fstar = f0 * (1 + 1e-3 * rng.standard_normal(f0.shape))
-    ustar = forward(xstar, fstar)
-    u0 = forward(x0, f0)
+    ustar = total(xstar, fstar, n0)
+    u0 = total(x0, f0, n0)
     du = u0 - ustar
-    xs, fs, dd = _dev(x0), _dev(f0), _dev(du)
-    gr = gpu.mex_stokes_slp_real_backward(xs, xs, fs, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, dd)
-    gk = gpu.mex_stokes_slp_kspace_backward(xs, xs, p.xi, p.eta, fs, p.Mx, p.My, 1.0, 1.0, p.w, p.P, dd)
-    gpos = (gr["psrc"] + gr["ptar"] + gk["psrc"] + gk["ptar"]).cpu().numpy()
-    gf = (gr["f"] + gk["f"]).cpu().numpy()
+    g = _full_backward(gpu, kind, p, x0, f0, n0, du)
+    gpos = g["pos"]
 
-    # density gradient: d/de L(f + e v) at e = 0 equals <du, u(v)> exactly
+    # densities and normals: exact linearity of the first variation
     v = rng.standard_normal(f0.shape)
-    uv = forward(x0, v)
-    lin = float(np.sum(du * uv))
-    assert abs(np.sum(gf * v) - lin) <= 1e-10 * np.sum(np.abs(gf * v)), (np.sum(gf * v), lin)
+    lin = float(np.sum(du * total(x0, v, n0)))
+    assert abs(np.sum(g["f"] * v) - lin) <= 1e-10 * np.sum(np.abs(g["f"] * v)), \
+        (np.sum(g["f"] * v), lin)
+    if kind == "dlp":
+        vn = rng.standard_normal(n0.shape)
+        lin = float(np.sum(du * total(x0, f0, vn)))
+        assert abs(np.sum(g["n"] * vn) - lin) <= 1e-10 * np.sum(np.abs(g["n"] * vn)), \
+            (np.sum(g["n"] * vn), lin)
+
+    def dloss(xp, xm, step):
+        """(L(xp) - L(xm)) / (2 step), as 1/2 sum (u+ - u-).(u+ + u- - 2 u*)."""
+        up, um = total(xp, f0, n0), total(xm, f0, n0)
+        return 0.5 * np.sum((up - um) * (up + um - 2 * ustar)) / (2 * step)
+
+    # Moved points stay inside the box and 2e-4 away from every other point:
+    # the closest pairs of 4e6 uniform points are about 1e-7 apart, where the
+    # kernels' 1/r and log r behaviour makes central differences with a 1e-7
+    # step inaccurate (third-order error (step/r)^2).
+    step = 1e-7
+    cand = rng.choice(np.flatnonzero(np.all(np.abs(x0) < 0.5 - 1e-4, axis=1)), 40_000,
+                      replace=False)
+    y = np.mod(x0 + 0.5, 1.0)
+    dist, _ = cKDTree(y, boxsize=1.0).query(y[cand], k=2)
+    inside = cand[dist[:, 1] > 2e-4]
+    sub = rng.choice(inside, 10_000, replace=False)
+    d = np.zeros_like(x0)
+    d[sub] = rng.standard_normal((sub.size, 2))
+    fd1 = dloss(x0 + step * d, x0 - step * d, step)
+    fd2 = dloss(x0 + 2 * step * d, x0 - 2 * step * d, 2 * step)
+    fd = (4 * fd1 - fd2) / 3           # Richardson: removes the step^2 term
+    proj = float(np.sum(gpos * d))
+    scale = float(np.sum(np.abs(gpos * d)))
+    err_dir = abs(fd - proj) / abs(proj)
+    print(f"[N=4e6 {kind} adjoint] 1e4-point direction: fd {fd:.12e} grad {proj:.12e} "
+          f"rel {err_dir:.2e} (step alone {abs(fd1 - proj) / abs(proj):.2e}; "
+          f"|proj|/sum|terms| {abs(proj) / scale:.2e})", flush=True)
+    assert err_dir < 1e-6, err_dir
 
     def stable(xp):
         cell = tuple(np.floor((xp + 0.5) * p.nside_x).astype(int))
         return cell, cref.find_closest_node(xp[0], xp[1], 1.0, 1.0, 1.0 / p.Mx, p.P)[:2]
 
-    step = 1e-7
+    want = 16 if kind == "slp" else 8
     checked = 0
     worst = 0.0
     floor = 1e-3 * float(np.sqrt(np.mean(gpos ** 2)))   # scale for near-zero components
-    for i in rng.choice(N, 64, replace=False):
-        if checked == 16:
+    for i in rng.choice(inside, 64, replace=False):
+        if checked == want:
             break
-        if np.any(np.abs(x0[i]) > 0.5 - 1e-4):
-            continue
         moves = [x0[i] + s * step * np.eye(2)[c] for c in range(2) for s in (1, -1)]
         if any(stable(m) != stable(x0[i]) for m in moves):
             continue
@@ -114,12 +184,10 @@ def test_adjoint_finite_differences_4e6_points(gpu, big):
             xp, xm = x0.copy(), x0.copy()
             xp[i, c] += step
             xm[i, c] -= step
-            up, um = forward(xp, f0), forward(xm, f0)
-            # L+ - L- = 1/2 sum (u+ - u-) . (u+ + u- - 2 u*)
-            fd = 0.5 * np.sum((up - um) * (up + um - 2 * ustar)) / (2 * step)
-            err = abs(fd - gpos[i, c]) / max(abs(gpos[i, c]), floor)
+            err = abs(dloss(xp, xm, step) - gpos[i, c]) / max(abs(gpos[i, c]), floor)
             worst = max(worst, err)
         checked += 1
-    print(f"[N=4e6 adjoint FD] {checked} points, worst relative error {worst:.2e}", flush=True)
-    assert checked >= 8
+    print(f"[N=4e6 {kind} adjoint] {checked} points one at a time, worst relative error "
+          f"{worst:.2e}", flush=True)
+    assert checked >= want // 2
     assert worst < 1e-4, worst
