@@ -188,7 +188,8 @@ def _multibox_variant(args, pe, W, np, torch, dev):
     N ~ log-uniform in [1e3, 2e5] uniform points each (1.41e6 in all; seed 4),
     per-box parameters from the driver formulas at tol 1e-10; one step is the
     stresslet real-space sums of every box (one launch) and their Fourier
-    sums (box by box on four streams).  Timed as eager async calls and as a
+    sums (packed spreading and interpolation launches; per-box sorts and
+    transforms on four streams).  Timed as eager async calls and as a
     replayed HIP graph of the same step."""
     boxes = W.multibox_config(seed=4, nbox=32)
     tb = []
@@ -214,25 +215,33 @@ def _multibox_variant(args, pe, W, np, torch, dev):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    with torch.cuda.stream(stream):
-        ctx.bind_torch_stream(stream)
-        ctx.set_async(True)
-        step()
-        step()
+    def eager_and_graph(fn):
+        fn()
+        fn()
         torch.cuda.synchronize()
-        eager_ms = timed(step)
+        eager = timed(fn)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=stream):
-            step()
+            fn()
         torch.cuda.synchronize()
         g.replay()
         torch.cuda.synchronize()
-        graph_ms = timed(g.replay)
+        return eager, timed(g.replay)
+
+    with torch.cuda.stream(stream):
+        ctx.bind_torch_stream(stream)
+        ctx.set_async(True)
+        eager_ms, graph_ms = eager_and_graph(step)
+        parts = {name: dict(zip(("eager_ms", "graph_ms"), eager_and_graph(fn))) for name, fn in (
+            ("real", lambda: pe.stokes_dlp_real_multibox(tb, ctx=ctx)),
+            ("fourier", lambda: pe.stokes_dlp_kspace_multibox(tb, ctx=ctx)))}
         ctx.set_async(False)
-    print(f"# multibox variant: eager {eager_ms:.3f} ms, graph {graph_ms:.3f} ms", file=sys.stderr,
-          flush=True)
+    print(f"# multibox variant: eager {eager_ms:.3f} ms, graph {graph_ms:.3f} ms (real "
+          f"{parts['real']['graph_ms']:.3f}, Fourier {parts['fourier']['graph_ms']:.3f} ms replayed)",
+          file=sys.stderr, flush=True)
     return {"eager": {"ms_per_step": eager_ms, "target_evals_per_s": ntot / (eager_ms * 1e-3)},
             "graph": {"ms_per_step": graph_ms, "target_evals_per_s": ntot / (graph_ms * 1e-3)},
+            "parts": parts,
             "workload": f"32 boxes, {ntot} points in all (targets = sources), stresslet real + Fourier sums, "
                         "per-box parameters at tol 1e-10"}
 
@@ -495,6 +504,8 @@ def main() -> int:
                     help="skip the N = 4e6 forward / forward+adjoint variant")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (scales HBM counters)")
+    ap.add_argument("--variant", choices=["batched", "multibox", "at_scale"], default=None,
+                    help="run only this variant and print its JSON line (for profiling)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -508,6 +519,15 @@ def main() -> int:
 
     from fasttools2d_amd import periodic_ewald as pe
     from fasttools2d_amd import workloads as W
+
+    if args.variant is not None:
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        run = {"batched": _batched_variant, "multibox": _multibox_variant,
+               "at_scale": _at_scale_variant}[args.variant]
+        print(json.dumps({"variant": args.variant, "result": run(args, pe, W, np, torch, dev)}),
+              flush=True)
+        return 0
 
     w, p = W.dlp_ellipses_config(seed=2, n_ell=200, n_nodes=500, tol=1e-10, M=512)
     N = len(w.psrc)

@@ -218,6 +218,22 @@ private:
     se2p_ctx *ctx = ctx_;   // for SE2P_HIP
 };
 
+int ensure_slots(se2p_ctx *ctx, int n)
+{
+    if (!ctx->fork_ev) SE2P_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    while ((int)ctx->slots.size() < n) {
+        se2p_ctx::Slot sl;
+        SE2P_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
+        SE2P_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+        const size_t fs_bytes = sizeof(int) * kFsMaxKeys;
+        SE2P_HIP(hipMalloc((void **)&sl.fs, fs_bytes));
+        SE2P_HIP(hipMemsetAsync(sl.fs, 0, fs_bytes, sl.st));
+        SE2P_HIP(hipStreamSynchronize(sl.st));
+        ctx->slots.push_back(sl);
+    }
+    return 0;
+}
+
 }  // namespace se2p
 
 using namespace se2p;
@@ -624,62 +640,6 @@ static int check_box(se2p_ctx *ctx, const se2p_box_desc &b, int i, bool dlp, boo
     return rc;
 }
 
-// The context's stream, workspace, sort state and plan set swapped for slot
-// k's while in scope (restored on every exit path).
-struct SlotScope {
-    se2p_ctx *ctx;
-    se2p_ctx::Slot &slot;
-    hipStream_t st;
-    char *ws;
-    size_t ws_bytes;
-    int *fs;
-    bool timing;
-    SlotScope(se2p_ctx *c, int k)
-        : ctx(c), slot(c->slots[(size_t)k]), st(c->stream), ws(c->ws), ws_bytes(c->ws_bytes),
-          fs(c->fs), timing(c->timing)
-    {
-        ctx->stream = slot.st;
-        ctx->ws = slot.ws;
-        ctx->ws_bytes = slot.ws_bytes;
-        ctx->fs = slot.fs;
-        ctx->timing = false;
-        ctx->plan_slot = k + 1;
-    }
-    ~SlotScope()
-    {
-        slot.ws = ctx->ws;   // ensure_workspace may have grown it
-        slot.ws_bytes = ctx->ws_bytes;
-        ctx->stream = st;
-        ctx->ws = ws;
-        ctx->ws_bytes = ws_bytes;
-        ctx->fs = fs;
-        ctx->timing = timing;
-        ctx->plan_slot = 0;
-    }
-};
-
-static int ensure_slots(se2p_ctx *ctx, int n)
-{
-    if (!ctx->fork_ev) SE2P_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
-    while ((int)ctx->slots.size() < n) {
-        se2p_ctx::Slot sl;
-        SE2P_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
-        SE2P_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-        const size_t fs_bytes = sizeof(int) * kFsMaxKeys;
-        SE2P_HIP(hipMalloc((void **)&sl.fs, fs_bytes));
-        SE2P_HIP(hipMemsetAsync(sl.fs, 0, fs_bytes, sl.st));
-        SE2P_HIP(hipStreamSynchronize(sl.st));
-        ctx->slots.push_back(sl);
-    }
-    return 0;
-}
-
-// Fourier-space sums of many boxes: boxes are independent, so they run on
-// kKspaceSlots streams side by side (largest first, each to the slot with the
-// least work so far), forked from and joined into the context's stream.
-// Every box is computed exactly as a single-box call computes it.
-constexpr int kKspaceSlots = 4;
-
 // Restores the context's keep flag: multi-box calls keep no forward state
 // (their boxes run on several streams).
 struct KeepOff {
@@ -692,38 +652,7 @@ struct KeepOff {
 static int kspace_multibox(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int nbox)
 {
     KeepOff no_keep(ctx);
-    auto run = [&](const se2p_box_desc &b) {
-        KGeom g{b.psrc, b.ptar, b.ns, b.nt, b.xi, b.eta, b.Lx, b.Ly, b.w, b.Mx, b.My, b.P};
-        return kspace_sum(ctx, kind, g, b.f, b.n, 1, 2 * b.ns, 2 * b.nt, b.u);
-    };
-    const int ns = std::min(kKspaceSlots, nbox);
-    if (ns <= 1) return run(boxes[0]);
-    if (int rc = ensure_slots(ctx, ns)) return rc;
-    std::vector<int> order((size_t)nbox);
-    std::vector<double> cost((size_t)nbox);
-    for (int i = 0; i < nbox; ++i) {
-        order[(size_t)i] = i;
-        cost[(size_t)i] = (double)boxes[i].ns + (double)boxes[i].nt +
-                          0.25 * (double)boxes[i].Mx * (double)boxes[i].My;
-    }
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return cost[(size_t)x] > cost[(size_t)y]; });
-    SE2P_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
-    for (int k = 0; k < ns; ++k) SE2P_HIP(hipStreamWaitEvent(ctx->slots[(size_t)k].st, ctx->fork_ev, 0));
-    std::vector<double> load((size_t)ns, 0.0);
-    int rc = 0;
-    for (int i : order) {
-        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[(size_t)k] += cost[(size_t)i];
-        SlotScope scope(ctx, k);
-        if ((rc = run(boxes[i])) != 0) break;
-    }
-    // join (also after an error, so that the context's stream stays ordered)
-    for (int k = 0; k < ns; ++k) {
-        SE2P_HIP(hipEventRecord(ctx->slots[(size_t)k].done, ctx->slots[(size_t)k].st));
-        SE2P_HIP(hipStreamWaitEvent(ctx->stream, ctx->slots[(size_t)k].done, 0));
-    }
-    return rc;
+    return kspace_sum_boxes(ctx, kind, boxes, nbox);
 }
 
 static int multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox, int kind, bool kspace)

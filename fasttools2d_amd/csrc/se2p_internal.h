@@ -96,6 +96,45 @@ int fail(se2p_ctx *ctx, int code, const char *fmt, ...);
 // Ensure the workspace holds at least `bytes`.
 int ensure_workspace(se2p_ctx *ctx, size_t bytes);
 
+// Stream slots: at least n of them, each with its stream, event and sort
+// state.  Multi-box Fourier-space sums use kKspaceSlots.
+int ensure_slots(se2p_ctx *ctx, int n);
+constexpr int kKspaceSlots = 4;
+
+// The context's stream, workspace, sort state and plan set swapped for slot
+// k's while in scope (restored on every exit path).
+struct SlotScope {
+    se2p_ctx *ctx;
+    se2p_ctx::Slot &slot;
+    hipStream_t st;
+    char *ws;
+    size_t ws_bytes;
+    int *fs;
+    bool timing;
+    SlotScope(se2p_ctx *c, int k)
+        : ctx(c), slot(c->slots[(size_t)k]), st(c->stream), ws(c->ws), ws_bytes(c->ws_bytes),
+          fs(c->fs), timing(c->timing)
+    {
+        ctx->stream = slot.st;
+        ctx->ws = slot.ws;
+        ctx->ws_bytes = slot.ws_bytes;
+        ctx->fs = slot.fs;
+        ctx->timing = false;
+        ctx->plan_slot = k + 1;
+    }
+    ~SlotScope()
+    {
+        slot.ws = ctx->ws;   // ensure_workspace may have grown it
+        slot.ws_bytes = ctx->ws_bytes;
+        ctx->stream = st;
+        ctx->ws = ws;
+        ctx->ws_bytes = ws_bytes;
+        ctx->fs = fs;
+        ctx->timing = timing;
+        ctx->plan_slot = 0;
+    }
+};
+
 // Timing helpers (no-ops unless ctx->timing).
 void stage_begin(se2p_ctx *ctx, Stage s);
 void stage_end(se2p_ctx *ctx, Stage s);
@@ -186,6 +225,8 @@ struct KGeom {
 };
 int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *n,
                int nrhs, int64_t dstride, int64_t ustride, double *u);
+// Fourier-space sums of many independent boxes (one right-hand side each).
+int kspace_sum_boxes(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int nbox);
 // Adjoint of kspace_sum (one right-hand side), outputs as for real_adjoint.
 int kspace_adjoint(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *n,
                    const double *du, double *dpsrc, double *dptar, double *df, double *dn);

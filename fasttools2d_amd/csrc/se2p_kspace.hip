@@ -189,6 +189,25 @@ struct SpreadArgs {
     int64_t max_chunks;                    // partial-buffer stride per group
 };
 
+// Launch arguments of up to kPack boxes, passed by value: box j owns the
+// workgroups [off[j], off[j+1]) of the launch.
+constexpr int kPack = 16;
+
+struct SpreadPack {
+    SpreadArgs a[kPack];
+    int off[kPack + 1];
+    int n;
+};
+static_assert(sizeof(SpreadPack) <= 4096, "kernel argument size");
+
+// The box owning workgroup b (off ascending, off[0] = 0); wave-uniform.
+__device__ __forceinline__ int pack_slot(const int *off, int n, int b)
+{
+    int j = 0;
+    while (j + 1 < n && b >= off[j + 1]) ++j;
+    return j;
+}
+
 // Anchor bins whose anchors a satisfy (t - a) mod M in [0, P] for some node t
 // of [t0, t0 + kTile), i.e. anchors in the cyclic range [t0 - P, t0 + kTile - 1]:
 // at most two runs of consecutive bins, (b0[0], n[0]) and (b0[1], n[1]), in
@@ -380,8 +399,10 @@ __host__ __device__ inline size_t strip_lds_doubles(int P, int NQ)
 // per axis and the test is one unsigned compare; the source loop has no
 // branch (a lane outside the support adds an exact zero), which lets the
 // LDS reads of the next sources run ahead.
+// One workgroup of the strip spreading: workgroup index b of the box's chunk
+// list, right-hand-side group grp.
 template <int NQ, bool WIDE>
-__global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
+__device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b, const int grp)
 {
     constexpr int kSB = strip_batch(NQ);
     extern __shared__ double lds[];
@@ -397,12 +418,10 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
     __shared__ int rstart[kMaxRanges], rpref[kMaxRanges + 1];
     __shared__ int nrange;
 
-    // blockIdx.y: group of right-hand sides (grids NQ*y .. NQ*y+NQ-1)
-    const int grp = blockIdx.y;
+    // grp: group of right-hand sides (grids NQ*grp .. NQ*grp+NQ-1)
     const double *qg = a.s.q + (int64_t)grp * NQ * a.s.ns;
     // block -> (tile, chunk of the tile's candidate list)
     const int ntiles = a.ntx * a.nty;
-    const int b = blockIdx.x;
     if (b >= a.chunk_off[ntiles]) return;   // uniform over the workgroup
     const int tile = a.block_tile[b];
     const int chunk = b - a.chunk_off[tile];
@@ -607,13 +626,26 @@ __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
     }
 }
 
+template <int NQ, bool WIDE>
+__global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
+{
+    spread_strip_wg<NQ, WIDE>(a, blockIdx.x, blockIdx.y);
+}
+
+// Many boxes in one launch: box j owns workgroups [off[j], off[j+1]) and each
+// runs exactly the workgroup a single-box launch would run.
+template <int NQ, bool WIDE>
+__global__ __launch_bounds__(256) void k_spread_strip_boxes(SpreadPack p)
+{
+    const int j = pack_slot(p.off, p.n, blockIdx.x);
+    spread_strip_wg<NQ, WIDE>(p.a[j], blockIdx.x - p.off[j], blockIdx.y);
+}
+
 // Tiles whose candidates were split over several workgroups: add the partial
 // tiles in chunk order (deterministic) and write the grid.
 template <int NQ>
-__global__ __launch_bounds__(64) void k_spread_combine(SpreadArgs a)
+__device__ __forceinline__ void spread_combine_wg(const SpreadArgs &a, const int tile, const int grp)
 {
-    const int tile = blockIdx.x;
-    const int grp = blockIdx.y;
     double *gg = a.grids + (int64_t)grp * NQ * a.gstride;
     const int b0 = a.chunk_off[tile], b1 = a.chunk_off[tile + 1];
     if (b1 - b0 <= 1) return;
@@ -641,6 +673,20 @@ __global__ __launch_bounds__(64) void k_spread_combine(SpreadArgs a)
 #pragma unroll
         for (int c = 0; c < NQ; ++c) gg[c * a.gstride + idx] = acc[k][c];
     }
+}
+
+template <int NQ>
+__global__ __launch_bounds__(64) void k_spread_combine(SpreadArgs a)
+{
+    spread_combine_wg<NQ>(a, blockIdx.x, blockIdx.y);
+}
+
+// Box j owns workgroups [off[j], off[j+1]), one per tile.
+template <int NQ>
+__global__ __launch_bounds__(64) void k_spread_combine_boxes(SpreadPack p)
+{
+    const int j = pack_slot(p.off, p.n, blockIdx.x);
+    spread_combine_wg<NQ>(p.a[j], blockIdx.x - p.off[j], blockIdx.y);
 }
 
 // Work split per tile (one wavefront per tile): the candidate runs of the
@@ -1053,7 +1099,7 @@ struct InterpTileArgs {
 // every target, jac[4 t] = (du1/dx, du2/dx, du1/dy, du2/dy) in caller order,
 // from which the backward call forms dL/dptar.
 template <int PT, bool DERIV>
-__global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
+__device__ __forceinline__ void interp_tile_wg(const InterpTileArgs &a, const int b, const int rhs)
 {
     extern __shared__ double lds[];
     const int P = PT > 0 ? PT : a.P;
@@ -1065,7 +1111,6 @@ __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
     double2 *G = (double2 *)lds;          // [R][RP]
     double *E1 = (double *)(G + R * RP);  // [P1]
     const int ntiles = a.ntx * a.nty;
-    const int b = blockIdx.x;
     if (b >= a.chunk_off[ntiles]) return;   // uniform over the workgroup
     const int tile = a.block_tile[b];
     const int chunk = b - a.chunk_off[tile];
@@ -1087,8 +1132,8 @@ __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
     const int v0 = chunk * 256, v1 = min(total, v0 + 256);
     fill_e1(E1, P, a.ch2);
     const int X0 = tx * kTile, Y0 = ty * kTile;
-    // blockIdx.y selects the right-hand side
-    for (int r = blockIdx.y; r < blockIdx.y + 1 && r < a.nr; ++r) {
+    // rhs selects the right-hand side
+    for (int r = rhs; r < rhs + 1 && r < a.nr; ++r) {
         const double *g1 = a.grids + (int64_t)(2 * r) * a.gstride;
         const double *g2 = g1 + a.gstride;
         __syncthreads();   // previous RHS done with G
@@ -1148,6 +1193,26 @@ __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
     }
 }
 
+template <int PT, bool DERIV>
+__global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
+{
+    interp_tile_wg<PT, DERIV>(a, blockIdx.x, blockIdx.y);
+}
+
+struct InterpPack {
+    InterpTileArgs a[kPack];
+    int off[kPack + 1];
+    int n;
+};
+static_assert(sizeof(InterpPack) <= 4096, "kernel argument size");
+
+// Many boxes in one launch, as k_spread_strip_boxes.
+template <int PT>
+__global__ __launch_bounds__(256) void k_interp_tile_boxes(InterpPack p)
+{
+    const int j = pack_slot(p.off, p.n, blockIdx.x);
+    interp_tile_wg<PT, false>(p.a[j], blockIdx.x - p.off[j], blockIdx.y);
+}
 
 // ---- adjoint interpolation -------------------------------------------------
 // For each point and each of NR grids g_m: V_m = sum W g_m and the sums
@@ -1628,40 +1693,32 @@ void free_kept_k(se2p_ctx *ctx)
 }
 
 // ---- forward -------------------------------------------------------------------
-int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *nv,
-               int nrhs, int64_t dstride, int64_t ustride, double *u)
-{
-    hipStream_t st = ctx->stream;
-    if (g.nt == 0 || nrhs == 0) return 0;
-    const int64_t ns = g.ns, nt = g.nt;
-    if (ns == 0) {
-        // no sources: every grid is zero, so is the result
-        for (int r = 0; r < nrhs; ++r) launch_zero(st, u + (int64_t)r * ustride, 2 * nt);
-        SE2P_HIP(hipGetLastError());
-        return 0;
-    }
-    const KLayout L = make_layout(g);
-    const int ncomp = kind == 0 ? 2 : 3;
-    // right-hand sides per spreading group (grids per group nq = ncomp*RB);
-    // the last group is padded with zero densities
-    const int RB = nrhs >= 2 ? 4 : 1;
-    const int ngrp = (nrhs + RB - 1) / RB;
-    const int nrp = ngrp * RB;   // padded right-hand sides
-    // targets are binned separately unless they are the source array itself
-    const bool same_points = (g.ptar == g.psrc) && (nt == ns);
-    GridPass gp;
-    plan_pass(L, ncomp * RB, ngrp, ns, nt, &gp);
-    const bool maps = gp.strip || L.itile;
+// One Fourier-space sum's plan and buffers: a single-box call, or one box of a
+// multi-box call.
+struct KRun {
+    int kind = 0, nrhs = 1, ncomp = 2, RB = 1, ngrp = 1, nrp = 1;
+    KGeom g{};
+    const double *f = nullptr, *nv = nullptr;
+    int64_t dstride = 0, ustride = 0;
+    double *u = nullptr;
+    bool same_points = false;   // targets are the source array: one set of bins
+    bool maps = false;          // tile work maps are used (strip spreading or tile interpolation)
+    bool gathered = false;      // the fused sort gathered the gridding data
+    bool want_jac = false;      // keep the output gradients for a backward call
+    KLayout L{};
+    GridPass gp{};
+    KeySort kss{}, kst{};
+    int2 *anchor = nullptr;
+    double *fgg = nullptr, *q = nullptr, *grids = nullptr, *jac = nullptr;
+    double2 *spec = nullptr;
+    SpreadArgs sa{};
     FftPlans *plans = nullptr;
-    if (int rc = get_plans(ctx, L.Mx, L.My, ncomp * nrp, 2 * nrp, &plans)) return rc;
 
-    KeySort kss, kst;
-    int2 *anchor;
-    double *fgg, *q, *grids;
-    double2 *spec;
-    double *jac = nullptr;   // kept output gradients (se2p_set_keep)
-    bool want_jac = false;
-    auto layout = [&](Carve &cv) {
+    const KeySort &tbins() const { return same_points ? kss : kst; }
+
+    void layout(Carve &cv)
+    {
+        const int64_t ns = g.ns, nt = g.nt;
         if (want_jac) jac = cv.take<double>(4 * (size_t)nt);
         char *p = cv.take<char>(keysort_bytes(ns, L.nbins));
         if (p) keysort_carve(p, ns, L.nbins, &kss);
@@ -1675,12 +1732,209 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         grids = cv.take<double>((size_t)ncomp * nrp * L.G);
         spec = cv.take<double2>((size_t)ncomp * nrp * L.GH);
         carve_pass(cv, L, &gp);
-    };
+    }
+};
+
+static void krun_init(KRun &r, int kind, const KGeom &g, const double *f, const double *nv,
+                      int nrhs, int64_t dstride, int64_t ustride, double *u)
+{
+    r.kind = kind;
+    r.g = g;
+    r.f = f;
+    r.nv = nv;
+    r.nrhs = nrhs;
+    r.dstride = dstride;
+    r.ustride = ustride;
+    r.u = u;
+    r.L = make_layout(g);
+    r.ncomp = kind == 0 ? 2 : 3;
+    // right-hand sides per spreading group (grids per group nq = ncomp*RB);
+    // the last group is padded with zero densities
+    r.RB = nrhs >= 2 ? 4 : 1;
+    r.ngrp = (nrhs + r.RB - 1) / r.RB;
+    r.nrp = r.ngrp * r.RB;
+    // targets are binned separately unless they are the source array itself
+    r.same_points = (g.ptar == g.psrc) && (g.nt == g.ns);
+    plan_pass(r.L, r.ncomp * r.RB, r.ngrp, g.ns, g.nt, &r.gp);
+    r.maps = r.gp.strip || r.L.itile;
+}
+
+// Bins of the sources (and of the targets unless they are the sources), the
+// sources' gridding data and the work maps.  The fused sort (se2p_fsort.h)
+// also gathers the sources' gridding data and, beside the sort that
+// completes the target bins, builds the tile work lists and chunk plans;
+// otherwise separate launches do.
+static int krun_sort(se2p_ctx *ctx, KRun &r)
+{
+    hipStream_t st = ctx->stream;
+    const KLayout &L = r.L;
+    const KGeom &g = r.g;
+    GridPass &gp = r.gp;
+    const int64_t ns = g.ns, nt = g.nt;
+    const KeySort &tbins = r.tbins();
+    SpreadArgs &sa = r.sa;
+    sa = spread_args(L, gp, r.kss.off, r.grids);
+    sa.s.anchor = r.anchor;
+    sa.s.fgg = r.fgg;
+    sa.s.q = r.q;
+    sa.s.ns = ns;
+    const TileWorkExtra tw{sa, tbins.off, gp.scnt, gp.icnt, gp.rng_start, gp.rng_len,
+                           r.maps ? L.ntiles : 0};
+    const FsPlanExtra pl{PlanArgs{gp.scnt, nullptr, 1, 1, L.ntiles, gp.soffc, gp.smap},
+                         PlanArgs{gp.icnt, nullptr, 1, 1, L.ntiles, gp.ioffc, gp.imap},
+                         r.maps ? 2 : 0};
+    const TileWorkExtra no_tw{sa, tbins.off, nullptr, nullptr, nullptr, nullptr, 0};
+    const FsPlanExtra no_pl{pl.p0, pl.p1, 0};
+    bool mapped = false;
+    r.gathered = false;
+    if (fsort_ok(ctx, ns, L.nbins)) {
+        const BinKeyFn kf = bin_key_fn(L, g, g.psrc);
+        int rc;
+        if (r.kind == 0)
+            rc = fsort_run(ctx, &r.kss, ns, L.nbins, kf,
+                           GridSrcGather<0>{g.psrc, r.f, r.nv, (int)ns, r.nrhs, r.nrp, r.dstride, g.Lx,
+                                            g.Ly, L.h, L.P, L.Mx, L.My, L.c, g.w, r.anchor, r.fgg, r.q},
+                           r.same_points ? tw : no_tw, r.same_points ? pl : no_pl);
+        else
+            rc = fsort_run(ctx, &r.kss, ns, L.nbins, kf,
+                           GridSrcGather<1>{g.psrc, r.f, r.nv, (int)ns, r.nrhs, r.nrp, r.dstride, g.Lx,
+                                            g.Ly, L.h, L.P, L.Mx, L.My, L.c, g.w, r.anchor, r.fgg, r.q},
+                           r.same_points ? tw : no_tw, r.same_points ? pl : no_pl);
+        if (rc) return rc;
+        r.gathered = true;
+        mapped = r.same_points;
+    } else if (int rc = bin_points(ctx, L, g, g.psrc, ns, &r.kss)) {
+        return rc;
+    }
+    if (!r.same_points) {
+        if (fsort_ok(ctx, nt, L.nbins)) {
+            if (int rc = fsort_run(ctx, &r.kst, nt, L.nbins, bin_key_fn(L, g, g.ptar), FsNoPayload{},
+                                   tw, pl))
+                return rc;
+            mapped = true;
+        } else if (int rc = bin_points(ctx, L, g, g.ptar, nt, &r.kst)) {
+            return rc;
+        }
+    }
+    if (r.maps && !mapped) tile_maps(st, L, gp, sa, tbins.off);
+    return 0;
+}
+
+// The sources' gridding data in bin order, unless the fused sort wrote it.
+static void krun_gather(se2p_ctx *ctx, const KRun &r)
+{
+    if (r.gathered) return;
+    const KLayout &L = r.L;
+    const KGeom &g = r.g;
+    const unsigned sblocks = (unsigned)((g.ns + 255) / 256);
+    if (r.kind == 0)
+        hipLaunchKernelGGL(k_gather_grid_src<0>, dim3(sblocks), dim3(256), 0, ctx->stream, g.psrc,
+                           r.f, r.nv, r.kss.perm, (int)g.ns, r.nrhs, r.nrp, r.dstride, g.Lx, g.Ly,
+                           L.h, L.P, L.Mx, L.My, L.c, g.w, r.anchor, r.fgg, r.q);
+    else
+        hipLaunchKernelGGL(k_gather_grid_src<1>, dim3(sblocks), dim3(256), 0, ctx->stream, g.psrc,
+                           r.f, r.nv, r.kss.perm, (int)g.ns, r.nrhs, r.nrp, r.dstride, g.Lx, g.Ly,
+                           L.h, L.P, L.Mx, L.My, L.c, g.w, r.anchor, r.fgg, r.q);
+}
+
+static InterpTileArgs interp_tile_args(const KRun &r)
+{
+    const KLayout &L = r.L;
+    const KeySort &tbins = r.tbins();
+    InterpTileArgs ia;
+    ia.ptar = r.g.ptar;
+    ia.perm = tbins.perm;
+    ia.boff = tbins.off;
+    ia.nbx = L.nbx;
+    ia.nby = L.nby;
+    ia.chunk_off = r.gp.ioffc;
+    ia.block_tile = r.gp.imap;
+    ia.ntx = L.ntx;
+    ia.nty = L.nty;
+    ia.grids = r.grids;
+    ia.gstride = L.G;
+    ia.Mx = L.Mx;
+    ia.My = L.My;
+    ia.P = L.P;
+    ia.nr = r.nrhs;
+    ia.Lx = r.g.Lx;
+    ia.Ly = r.g.Ly;
+    ia.h = L.h;
+    ia.c = L.c;
+    ia.w = r.g.w;
+    ia.ch2 = L.ch2;
+    ia.scale = L.interp_scale;
+    ia.u = r.u;
+    ia.ustride = r.ustride;
+    ia.jac = r.jac;
+    return ia;
+}
+
+// Interpolation of the filtered grids at the targets.
+static int krun_interp(se2p_ctx *ctx, const KRun &r)
+{
+    hipStream_t st = ctx->stream;
+    const KLayout &L = r.L;
+    const GridPass &gp = r.gp;
+    const int nrhs = r.nrhs;
+    if (L.itile) {
+        const InterpTileArgs ia = interp_tile_args(r);
+        if (L.P == 24 && r.jac)
+            hipLaunchKernelGGL((k_interp_tile<24, true>), dim3((unsigned)gp.max_interp_chunks, nrhs),
+                               dim3(256), L.lds_itile, st, ia);
+        else if (L.P == 24)
+            hipLaunchKernelGGL((k_interp_tile<24, false>), dim3((unsigned)gp.max_interp_chunks, nrhs),
+                               dim3(256), L.lds_itile, st, ia);
+        else
+            hipLaunchKernelGGL((k_interp_tile<0, false>), dim3((unsigned)gp.max_interp_chunks, nrhs),
+                               dim3(256), L.lds_itile, st, ia);
+    } else {
+        for (int rr = 0; rr < nrhs; ++rr) {
+            InterpArgs ia;
+            ia.ptar = r.g.ptar;
+            ia.nt = r.g.nt;
+            ia.grids = r.grids + (int64_t)(2 * rr) * L.G;
+            ia.gstride = L.G;
+            ia.Mx = L.Mx;
+            ia.My = L.My;
+            ia.P = L.P;
+            ia.nr = 1;
+            ia.Lx = r.g.Lx;
+            ia.Ly = r.g.Ly;
+            ia.h = L.h;
+            ia.c = L.c;
+            ia.w = r.g.w;
+            ia.ch2 = L.ch2;
+            ia.scale = L.interp_scale;
+            ia.u = r.u + (int64_t)rr * r.ustride;
+            ia.ustride = r.ustride;
+            hipLaunchKernelGGL(k_interp<1>, dim3((unsigned)((r.g.nt + 255) / 256)), dim3(256), 0, st, ia);
+        }
+    }
+    SE2P_HIP(hipGetLastError());
+    return 0;
+}
+
+int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *nv,
+               int nrhs, int64_t dstride, int64_t ustride, double *u)
+{
+    hipStream_t st = ctx->stream;
+    if (g.nt == 0 || nrhs == 0) return 0;
+    if (g.ns == 0) {
+        // no sources: every grid is zero, so is the result
+        for (int r = 0; r < nrhs; ++r) launch_zero(st, u + (int64_t)r * ustride, 2 * g.nt);
+        SE2P_HIP(hipGetLastError());
+        return 0;
+    }
+    KRun r;
+    krun_init(r, kind, g, f, nv, nrhs, dstride, ustride, u);
+    if (int rc = get_plans(ctx, r.L.Mx, r.L.My, r.ncomp * r.nrp, 2 * r.nrp, &r.plans)) return rc;
+
     // a call whose backward may follow keeps its state in a buffer of its own
-    const bool keep = ctx->keep && !ctx->host_ptrs && same_points && nrhs == 1;
-    want_jac = keep && L.itile && L.P == 24;
+    const bool keep = ctx->keep && !ctx->host_ptrs && r.same_points && nrhs == 1;
+    r.want_jac = keep && r.L.itile && r.L.P == 24;
     Carve sizing;
-    layout(sizing);
+    r.layout(sizing);
     Carve cv;
     if (keep) {
         if (!ctx->kept_k) ctx->kept_k = new KeptK;
@@ -1701,74 +1955,19 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         if (int rc = ensure_workspace(ctx, sizing.off)) return rc;
         cv.base = ctx->ws;
     }
-    layout(cv);
+    r.layout(cv);
 
-    // Bins of the sources (and of the targets unless they are the sources).
-    // The fused sort (se2p_fsort.h) also gathers the sources' gridding data
-    // and, beside the sort that completes the target bins, builds the tile
-    // work lists and chunk plans; otherwise separate launches do.
     stage_begin(ctx, ST_SORT);
-    const KeySort &tbins = same_points ? kss : kst;
-    SpreadArgs sa = spread_args(L, gp, kss.off, grids);
-    sa.s.anchor = anchor;
-    sa.s.fgg = fgg;
-    sa.s.q = q;
-    sa.s.ns = ns;
-    const TileWorkExtra tw{sa, tbins.off, gp.scnt, gp.icnt, gp.rng_start, gp.rng_len,
-                           maps ? L.ntiles : 0};
-    const FsPlanExtra pl{PlanArgs{gp.scnt, nullptr, 1, 1, L.ntiles, gp.soffc, gp.smap},
-                         PlanArgs{gp.icnt, nullptr, 1, 1, L.ntiles, gp.ioffc, gp.imap}, maps ? 2 : 0};
-    const TileWorkExtra no_tw{sa, tbins.off, nullptr, nullptr, nullptr, nullptr, 0};
-    const FsPlanExtra no_pl{pl.p0, pl.p1, 0};
-    bool gathered = false, mapped = false;
-    if (fsort_ok(ctx, ns, L.nbins)) {
-        const BinKeyFn kf = bin_key_fn(L, g, g.psrc);
-        int rc;
-        if (kind == 0)
-            rc = fsort_run(ctx, &kss, ns, L.nbins, kf,
-                           GridSrcGather<0>{g.psrc, f, nv, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly,
-                                            L.h, L.P, L.Mx, L.My, L.c, g.w, anchor, fgg, q},
-                           same_points ? tw : no_tw, same_points ? pl : no_pl);
-        else
-            rc = fsort_run(ctx, &kss, ns, L.nbins, kf,
-                           GridSrcGather<1>{g.psrc, f, nv, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly,
-                                            L.h, L.P, L.Mx, L.My, L.c, g.w, anchor, fgg, q},
-                           same_points ? tw : no_tw, same_points ? pl : no_pl);
-        if (rc) return rc;
-        gathered = true;
-        mapped = same_points;
-    } else if (int rc = bin_points(ctx, L, g, g.psrc, ns, &kss)) {
-        return rc;
-    }
-    if (!same_points) {
-        if (fsort_ok(ctx, nt, L.nbins)) {
-            if (int rc = fsort_run(ctx, &kst, nt, L.nbins, bin_key_fn(L, g, g.ptar), FsNoPayload{},
-                                   tw, pl))
-                return rc;
-            mapped = true;
-        } else if (int rc = bin_points(ctx, L, g, g.ptar, nt, &kst)) {
-            return rc;
-        }
-    }
-    if (maps && !mapped) tile_maps(st, L, gp, sa, tbins.off);
+    if (int rc = krun_sort(ctx, r)) return rc;
     stage_end(ctx, ST_SORT);
 
     stage_begin(ctx, ST_SPREAD);
-    const unsigned sblocks = (unsigned)((ns + 255) / 256);
-    if (gathered) {
-    } else if (kind == 0) {
-        hipLaunchKernelGGL(k_gather_grid_src<0>, dim3(sblocks), dim3(256), 0, st, g.psrc, f, nv,
-                           kss.perm, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly, L.h, L.P, L.Mx, L.My,
-                           L.c, g.w, anchor, fgg, q);
-    } else {
-        hipLaunchKernelGGL(k_gather_grid_src<1>, dim3(sblocks), dim3(256), 0, st, g.psrc, f, nv,
-                           kss.perm, (int)ns, nrhs, nrp, dstride, g.Lx, g.Ly, L.h, L.P, L.Mx, L.My,
-                           L.c, g.w, anchor, fgg, q);
-    }
-    if (int rc = spread(ctx, L, gp, sa)) return rc;
+    krun_gather(ctx, r);
+    if (int rc = spread(ctx, r.L, r.gp, r.sa)) return rc;
     stage_end(ctx, ST_SPREAD);
 
-    if (int rc = transform_filter(ctx, L, g, plans, kind == 0 ? FILTER_SLP : FILTER_DLP, nrp, grids, spec))
+    if (int rc = transform_filter(ctx, r.L, g, r.plans, kind == 0 ? FILTER_SLP : FILTER_DLP, r.nrp,
+                                  r.grids, r.spec))
         return rc;
     if (keep) {
         KeptK &kk = *ctx->kept_k;
@@ -1776,76 +1975,183 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         kk.g = g;
         kk.f = f;
         kk.nv = nv;
-        kk.kss = kss;
-        kk.gp = gp;
-        kk.grids = grids;
-        kk.jac = jac;
+        kk.kss = r.kss;
+        kk.gp = r.gp;
+        kk.grids = r.grids;
+        kk.jac = r.jac;
         kk.valid = true;
     }
 
     stage_begin(ctx, ST_INTERP);
-    if (L.itile) {
-        InterpTileArgs ia;
-        ia.ptar = g.ptar;
-        ia.perm = tbins.perm;
-        ia.boff = tbins.off;
-        ia.nbx = L.nbx;
-        ia.nby = L.nby;
-        ia.chunk_off = gp.ioffc;
-        ia.block_tile = gp.imap;
-        ia.ntx = L.ntx;
-        ia.nty = L.nty;
-        ia.grids = grids;
-        ia.gstride = L.G;
-        ia.Mx = L.Mx;
-        ia.My = L.My;
-        ia.P = L.P;
-        ia.nr = nrhs;
-        ia.Lx = g.Lx;
-        ia.Ly = g.Ly;
-        ia.h = L.h;
-        ia.c = L.c;
-        ia.w = g.w;
-        ia.ch2 = L.ch2;
-        ia.scale = L.interp_scale;
-        ia.u = u;
-        ia.ustride = ustride;
-        ia.jac = jac;
-        if (L.P == 24 && jac)
-            hipLaunchKernelGGL((k_interp_tile<24, true>), dim3((unsigned)gp.max_interp_chunks, nrhs),
-                               dim3(256), L.lds_itile, st, ia);
-        else if (L.P == 24)
-            hipLaunchKernelGGL((k_interp_tile<24, false>), dim3((unsigned)gp.max_interp_chunks, nrhs),
-                               dim3(256), L.lds_itile, st, ia);
-        else
-            hipLaunchKernelGGL((k_interp_tile<0, false>), dim3((unsigned)gp.max_interp_chunks, nrhs),
-                               dim3(256), L.lds_itile, st, ia);
-    } else {
-        for (int r = 0; r < nrhs; ++r) {
-            InterpArgs ia;
-            ia.ptar = g.ptar;
-            ia.nt = nt;
-            ia.grids = grids + (int64_t)(2 * r) * L.G;
-            ia.gstride = L.G;
-            ia.Mx = L.Mx;
-            ia.My = L.My;
-            ia.P = L.P;
-            ia.nr = 1;
-            ia.Lx = g.Lx;
-            ia.Ly = g.Ly;
-            ia.h = L.h;
-            ia.c = L.c;
-            ia.w = g.w;
-            ia.ch2 = L.ch2;
-            ia.scale = L.interp_scale;
-            ia.u = u + (int64_t)r * ustride;
-            ia.ustride = ustride;
-            hipLaunchKernelGGL(k_interp<1>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, ia);
-        }
-    }
-    SE2P_HIP(hipGetLastError());
+    if (int rc = krun_interp(ctx, r)) return rc;
     stage_end(ctx, ST_INTERP);
     return 0;
+}
+
+// ---- many boxes ------------------------------------------------------------------
+// Strip spreading of many boxes, kPack boxes per launch (each box's workgroups
+// run exactly as in its own launch, so the grids are bitwise those of
+// single-box calls); boxes the strip kernel does not take run one by one.
+static int spread_boxes(se2p_ctx *ctx, std::vector<KRun> &runs)
+{
+    hipStream_t st = ctx->stream;
+    std::vector<int> pk;
+    for (size_t k = 0; k < runs.size(); ++k) {
+        const KRun &r = runs[k];
+        const bool wide = r.L.Mx >= r.L.P + 16 && r.L.My >= r.L.P + 16;
+        if (r.gp.strip && wide && r.gp.ngrp == 1 && r.gp.nq == runs[0].gp.nq) pk.push_back((int)k);
+        else if (int rc = spread(ctx, r.L, r.gp, r.sa)) return rc;
+    }
+    for (size_t s0 = 0; s0 < pk.size(); s0 += kPack) {
+        const int n = (int)std::min<size_t>(kPack, pk.size() - s0);
+        SpreadPack sp{}, cp{};
+        size_t lds = 0;
+        for (int j = 0; j < n; ++j) {
+            const KRun &r = runs[(size_t)pk[s0 + (size_t)j]];
+            sp.a[j] = cp.a[j] = r.sa;
+            sp.off[j + 1] = sp.off[j] + (int)r.gp.max_spread_chunks;
+            cp.off[j + 1] = cp.off[j] + r.L.ntiles;
+            lds = std::max(lds, r.gp.lds_strip);
+        }
+        sp.n = cp.n = n;
+#define SE2P_SPREAD_BOXES(NQV)                                                                     \
+    hipLaunchKernelGGL((k_spread_strip_boxes<NQV, true>), dim3((unsigned)sp.off[n]), dim3(256),    \
+                       lds, st, sp);                                                               \
+    hipLaunchKernelGGL((k_spread_combine_boxes<NQV>), dim3((unsigned)cp.off[n]), dim3(64), 0, st, cp);
+        switch (runs[0].gp.nq) {
+        case 2: SE2P_SPREAD_BOXES(2); break;
+        case 3: SE2P_SPREAD_BOXES(3); break;
+        default: return fail(ctx, SE2P_ERR_ARG, "unsupported grid count %d", runs[0].gp.nq);
+        }
+#undef SE2P_SPREAD_BOXES
+        SE2P_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+// Tile interpolation of many boxes, kPack boxes per launch (bitwise as
+// single-box launches); the other boxes run one by one.
+static int interp_boxes(se2p_ctx *ctx, std::vector<KRun> &runs)
+{
+    hipStream_t st = ctx->stream;
+    std::vector<int> pk24, pk0;
+    for (size_t k = 0; k < runs.size(); ++k) {
+        const KRun &r = runs[k];
+        if (r.L.itile && r.nrhs == 1 && !r.jac) (r.L.P == 24 ? pk24 : pk0).push_back((int)k);
+        else if (int rc = krun_interp(ctx, r)) return rc;
+    }
+    for (int pt : {24, 0}) {
+        const std::vector<int> &pk = pt == 24 ? pk24 : pk0;
+        for (size_t s0 = 0; s0 < pk.size(); s0 += kPack) {
+            const int n = (int)std::min<size_t>(kPack, pk.size() - s0);
+            InterpPack ip{};
+            size_t lds = 0;
+            for (int j = 0; j < n; ++j) {
+                const KRun &r = runs[(size_t)pk[s0 + (size_t)j]];
+                ip.a[j] = interp_tile_args(r);
+                ip.off[j + 1] = ip.off[j] + (int)r.gp.max_interp_chunks;
+                lds = std::max(lds, r.L.lds_itile);
+            }
+            ip.n = n;
+            if (pt == 24)
+                hipLaunchKernelGGL((k_interp_tile_boxes<24>), dim3((unsigned)ip.off[n]), dim3(256), lds,
+                                   st, ip);
+            else
+                hipLaunchKernelGGL((k_interp_tile_boxes<0>), dim3((unsigned)ip.off[n]), dim3(256), lds,
+                                   st, ip);
+            SE2P_HIP(hipGetLastError());
+        }
+    }
+    return 0;
+}
+
+// Fourier-space sums of many independent boxes.  Every box gets a workspace
+// region of its own, so that all of them are resident at once:
+//   1. on the stream slots: each box's sorts, gathers and work maps;
+//   2. on the context stream: spreading, boxes packed into shared launches;
+//   3. on the slots: each box's transforms and filter;
+//   4. on the context stream: interpolation, packed as the spreading.
+// Each box is computed exactly as a single-box call computes it.
+int kspace_sum_boxes(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int nbox)
+{
+    std::vector<KRun> runs;
+    std::vector<double> cost;
+    for (int i = 0; i < nbox; ++i) {
+        const se2p_box_desc &b = boxes[i];
+        if (b.nt == 0) continue;
+        if (b.ns == 0) {
+            launch_zero(ctx->stream, b.u, 2 * b.nt);   // no sources: zero result
+            continue;
+        }
+        const KGeom g{b.psrc, b.ptar, b.ns, b.nt, b.xi, b.eta, b.Lx, b.Ly, b.w, b.Mx, b.My, b.P};
+        runs.emplace_back();
+        krun_init(runs.back(), kind, g, b.f, b.n, 1, 2 * b.ns, 2 * b.nt, b.u);
+        cost.push_back((double)b.ns + (double)b.nt + 0.25 * (double)b.Mx * (double)b.My);
+    }
+    SE2P_HIP(hipGetLastError());
+    const int nrun = (int)runs.size();
+    if (nrun == 0) return 0;
+    std::vector<size_t> base((size_t)nrun);
+    size_t total = 0;
+    for (int k = 0; k < nrun; ++k) {
+        Carve sizing;
+        runs[(size_t)k].layout(sizing);
+        base[(size_t)k] = total;
+        total += sizing.off;
+    }
+    if (int rc = ensure_workspace(ctx, total)) return rc;
+    for (int k = 0; k < nrun; ++k) {
+        Carve cv;
+        cv.base = ctx->ws + base[(size_t)k];
+        runs[(size_t)k].layout(cv);
+    }
+    // largest first, each to the slot with the least work so far
+    const int nsl = std::min(kKspaceSlots, nrun);
+    if (int rc = ensure_slots(ctx, nsl)) return rc;
+    std::vector<int> order((size_t)nrun), slot_of((size_t)nrun);
+    for (int k = 0; k < nrun; ++k) order[(size_t)k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return cost[(size_t)x] > cost[(size_t)y]; });
+    {
+        std::vector<double> load((size_t)nsl, 0.0);
+        for (int k : order) {
+            const int sl = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[(size_t)sl] += cost[(size_t)k];
+            slot_of[(size_t)k] = sl;
+        }
+    }
+    // per-box work on the slots, forked from and joined into the context stream
+    auto on_slots = [&](auto &&work) {
+        SE2P_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
+        for (int k = 0; k < nsl; ++k)
+            SE2P_HIP(hipStreamWaitEvent(ctx->slots[(size_t)k].st, ctx->fork_ev, 0));
+        int rc = 0;
+        for (int k : order) {
+            SlotScope scope(ctx, slot_of[(size_t)k]);
+            if ((rc = work(runs[(size_t)k])) != 0) break;
+        }
+        // join (also after an error, so that the context's stream stays ordered)
+        for (int k = 0; k < nsl; ++k) {
+            SE2P_HIP(hipEventRecord(ctx->slots[(size_t)k].done, ctx->slots[(size_t)k].st));
+            SE2P_HIP(hipStreamWaitEvent(ctx->stream, ctx->slots[(size_t)k].done, 0));
+        }
+        return rc;
+    };
+    int rc = on_slots([&](KRun &r) {
+        if (int e = get_plans(ctx, r.L.Mx, r.L.My, r.ncomp * r.nrp, 2 * r.nrp, &r.plans)) return e;
+        if (int e = krun_sort(ctx, r)) return e;
+        krun_gather(ctx, r);
+        SE2P_HIP(hipGetLastError());
+        return 0;
+    });
+    if (rc) return rc;
+    if ((rc = spread_boxes(ctx, runs)) != 0) return rc;
+    rc = on_slots([&](KRun &r) {
+        return transform_filter(ctx, r.L, r.g, r.plans, kind == 0 ? FILTER_SLP : FILTER_DLP, r.nrp,
+                                r.grids, r.spec);
+    });
+    if (rc) return rc;
+    return interp_boxes(ctx, runs);
 }
 
 // ---- adjoint -------------------------------------------------------------------
