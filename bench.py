@@ -361,14 +361,16 @@ def _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, fwd_
                         "their sorted state for the backward calls (se2p_set_keep)"}
 
 
-def _autotune_variant(args, pe, np, torch, w, ps, f, nv, ctx, stream, fwd_ms):
+def _autotune_variant(args, pe, np, torch, w, ps, f, nv, stream, fwd_ms):
     """The headline geometry with parameters chosen by se2p_autotune at the
-    same tolerance (1e-10): support P from the window rule, points per cell
-    by timing.  One step is the same two MEX-equivalent calls at the tuned
-    parameters; the output is checked against the FP64 direct Ewald sum on
+    same tolerance (1e-10): the drivers' support P = 24, points per cell and
+    spreading work split by timing, on a context of its own (the split is a
+    context setting).  One step is the same two MEX-equivalent calls at the
+    tuned parameters; the output is checked against the FP64 direct Ewald sum on
     the 256 truth targets (the real + Fourier total does not depend on the
     split, so the truth at the headline's xi serves any xi)."""
     N = ps.shape[0]
+    ctx = pe.Context(0)
     t0 = time.perf_counter()
     prm, info = pe.autotune("dlp", ps, ps, f, nv, w.Lx, w.Ly, tol=1e-10, ctx=ctx)
     tune_s = time.perf_counter() - t0
@@ -396,7 +398,8 @@ def _autotune_variant(args, pe, np, torch, w, ps, f, nv, ctx, stream, fwd_ms):
         ms = e0.elapsed_time(e1) / args.steps
         ctx.set_async(False)
     acc = _direct_ewald_error(np, ur.cpu().numpy(), uk.cpu().numpy())
-    print(f"# autotuned: P {prm.P}, nside {prm.nside_x}, grid {prm.Mx}^2: {ms:.3f} ms per step, "
+    print(f"# autotuned: P {prm.P}, nside {prm.nside_x}, grid {prm.Mx}^2, spread chunk "
+          f"{info['spread_chunk'] or 'default'}: {ms:.3f} ms per step, "
           f"error vs direct Ewald {acc['u'] if acc else float('nan'):.2e}", file=sys.stderr,
           flush=True)
     return {"ms_per_step": ms, "target_evals_per_s": N / (ms * 1e-3),
@@ -404,6 +407,7 @@ def _autotune_variant(args, pe, np, torch, w, ps, f, nv, ctx, stream, fwd_ms):
             "params": {"P": prm.P, "Nb": prm.Nb, "nside": [prm.nside_x, prm.nside_y],
                        "Mx": prm.Mx, "My": prm.My, "xi": prm.xi, "tol": 1e-10},
             "tuning_seconds": tune_s, "candidates": info["candidates"],
+            "spread_chunk": info["spread_chunk"], "spread_chunks": info["chunks"],
             "error_vs_direct_ewald": acc,
             "workload": "headline geometry (stresslet, 200 ellipses x 500 nodes), tol 1e-10, "
                         "parameters from se2p_autotune"}
@@ -667,7 +671,7 @@ def main() -> int:
         variants = _batched_variant(args, pe, W, np, torch, dev)
         variants["multibox"] = _multibox_variant(args, pe, W, np, torch, dev)
         variants["graph"] = _graph_variant(pe, torch, step, ctx, stream, N)
-        variants["autotuned"] = _autotune_variant(args, pe, np, torch, w, ps, f, nv, ctx, stream,
+        variants["autotuned"] = _autotune_variant(args, pe, np, torch, w, ps, f, nv, stream,
                                                   ms_per_step)
         if not args.no_scale:
             variants["at_scale"] = _at_scale_variant(args, pe, W, np, torch, dev)

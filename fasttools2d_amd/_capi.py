@@ -41,6 +41,7 @@ SIGNATURES = {
     "se2p_set_host_pointers": (_I, [_P, _I]),
     "se2p_set_sort_mode": (_I, [_P, _I]),
     "se2p_set_keep": (_I, [_P, _I]),
+    "se2p_set_spread_chunk": (_I, [_P, _I]),
     "se2p_set_timing": (_I, [_P, _I]),
     "se2p_stage_times": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
     "se2p_slp_real": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _I, _I, _D, _D, _P]),
@@ -95,7 +96,9 @@ class Se2pTuneInfo(ctypes.Structure):
     """se2p_tune_info of include/se2p.h."""
     _fields_ = [("ncand", ctypes.c_int), ("best", ctypes.c_int), ("cached", ctypes.c_int),
                 ("P", ctypes.c_int), ("Nb", ctypes.c_double * 16), ("nside_x", ctypes.c_int * 16),
-                ("Mx", ctypes.c_int * 16), ("ms", ctypes.c_double * 16)]
+                ("Mx", ctypes.c_int * 16), ("ms", ctypes.c_double * 16),
+                ("spread_chunk", ctypes.c_int), ("chunk", ctypes.c_int * 3),
+                ("chunk_ms", ctypes.c_double * 3)]
 
 
 SIGNATURES.update({

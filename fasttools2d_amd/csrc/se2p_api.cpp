@@ -380,6 +380,21 @@ int se2p_set_keep(se2p_ctx *ctx, int enable)
     return SE2P_OK;
 }
 
+int se2p_set_spread_chunk(se2p_ctx *ctx, int chunk)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    if (chunk != 0 && (chunk < SE2P_SPREAD_CHUNK_MIN || chunk > SE2P_SPREAD_CHUNK_MAX || chunk % 16 != 0))
+        return fail(ctx, SE2P_ERR_ARG, "spread chunk %d: 0, or a multiple of 16 in [%d, %d]", chunk,
+                    SE2P_SPREAD_CHUNK_MIN, SE2P_SPREAD_CHUNK_MAX);
+    if (chunk != ctx->spread_chunk && ctx->device != SE2P_HOST_ONLY) {
+        // kept Fourier-space state carries the old work split
+        if (ctx->stream) hipStreamSynchronize(ctx->stream);
+        free_kept_k(ctx);
+    }
+    ctx->spread_chunk = chunk;
+    return SE2P_OK;
+}
+
 int se2p_set_timing(se2p_ctx *ctx, int enable)
 {
     if (!ctx) return SE2P_ERR_ARG;
@@ -974,7 +989,11 @@ struct TuneKey {
     }
 };
 std::mutex g_tune_mu;
-std::map<TuneKey, double> g_tune_nb;   // key -> chosen points per cell
+struct TuneChoice {
+    double Nb;   // points per cell
+    int chunk;   // spreading chunk
+};
+std::map<TuneKey, TuneChoice> g_tune_nb;   // key -> choice
 
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
@@ -1040,8 +1059,12 @@ static int autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, con
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = g_tune_nb.find(key);
         if (it != g_tune_nb.end()) {
-            params_for(it->second, prm);
-            if (info) info->cached = 1;
+            params_for(it->second.Nb, prm);
+            if (int rc = se2p_set_spread_chunk(ctx, it->second.chunk)) return rc;
+            if (info) {
+                info->cached = 1;
+                info->spread_chunk = it->second.chunk;
+            }
             return io.done(SE2P_OK);
         }
     }
@@ -1067,10 +1090,10 @@ static int autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, con
     EventPair ev;
     SE2P_HIP(hipEventCreate(&ev.a));
     SE2P_HIP(hipEventCreate(&ev.b));
-    int best = 0;
-    double best_ms = std::numeric_limits<double>::infinity();
-    for (size_t i = 0; i < cand.size(); ++i) {
-        const se2p_params &c = cand[i];
+    // real + Fourier step of parameter set c: one warm-up step (it sizes the
+    // workspace and makes the FFT plans), then the best of kBlocks blocks of
+    // kReps back-to-back steps
+    auto time_step = [&](const se2p_params &c, double *t) -> int {
         const RealGeom rg{b.src, b.tar, ns, nt, c.xi, Lx, Ly, c.nside_x, c.nside_y};
         const KGeom kg{b.src, b.tar, ns, nt, c.xi, c.eta, Lx, Ly, c.w, c.Mx, c.My, c.P};
         if (int rc = check_kspace_params(ctx, kg)) return rc;
@@ -1078,10 +1101,9 @@ static int autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, con
             if (int rc = real_sum(ctx, kind, rg, f, n, 1, 2 * ns, 2 * nt, b.ur, nullptr)) return rc;
             return kspace_sum(ctx, kind, kg, f, n, 1, 2 * ns, 2 * nt, b.uk);
         };
-        if (int rc = step()) return rc;   // sizes the workspace, makes the FFT plans
+        if (int rc = step()) return rc;
         SE2P_HIP(hipStreamSynchronize(st));
-        // the best of kBlocks blocks of kReps back-to-back steps
-        double t = std::numeric_limits<double>::infinity();
+        *t = std::numeric_limits<double>::infinity();
         for (int blk = 0; blk < kBlocks; ++blk) {
             SE2P_HIP(hipEventRecord(ev.a, st));
             for (int r = 0; r < kReps; ++r)
@@ -1090,8 +1112,18 @@ static int autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, con
             SE2P_HIP(hipEventSynchronize(ev.b));
             float ms = 0.0f;
             SE2P_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
-            t = std::min(t, (double)ms / kReps);
+            *t = std::min(*t, (double)ms / kReps);
         }
+        return 0;
+    };
+    // 1. the cell size, at the default spreading chunk
+    if (int rc = se2p_set_spread_chunk(ctx, 0)) return rc;
+    int best = 0;
+    double best_ms = std::numeric_limits<double>::infinity();
+    for (size_t i = 0; i < cand.size(); ++i) {
+        const se2p_params &c = cand[i];
+        double t;
+        if (int rc = time_step(c, &t)) return rc;
         if (info) {
             info->Nb[i] = c.Nb;
             info->nside_x[i] = c.nside_x;
@@ -1103,14 +1135,36 @@ static int autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, con
             best = (int)i;
         }
     }
+    // 2. the spreading work split for that cell size (the default was timed above)
+    // (0: the library's default, 256 candidates for one density)
+    const int chunks[SE2P_TUNE_CHUNKS] = {0, 128, 512};
+    int best_chunk = chunks[0];
+    double chunk_best = best_ms;
+    for (int k = 0; k < SE2P_TUNE_CHUNKS; ++k) {
+        double t = best_ms;
+        if (k > 0) {
+            if (int rc = se2p_set_spread_chunk(ctx, chunks[k])) return rc;
+            if (int rc = time_step(cand[(size_t)best], &t)) return rc;
+        }
+        if (info) {
+            info->chunk[k] = chunks[k];
+            info->chunk_ms[k] = t;
+        }
+        if (t < chunk_best) {
+            chunk_best = t;
+            best_chunk = chunks[k];
+        }
+    }
+    if (int rc = se2p_set_spread_chunk(ctx, best_chunk)) return rc;
     SE2P_HIP(hipGetLastError());
     *prm = cand[(size_t)best];
     if (info) {
         info->ncand = (int)cand.size();
         info->best = best;
+        info->spread_chunk = best_chunk;
     }
     std::lock_guard<std::mutex> lock(g_tune_mu);
-    g_tune_nb[key] = prm->Nb;
+    g_tune_nb[key] = TuneChoice{prm->Nb, best_chunk};
     return io.done(SE2P_OK);
 }
 

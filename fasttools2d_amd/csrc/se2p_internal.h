@@ -67,6 +67,7 @@ struct se2p_ctx {
     size_t io_bytes = 0;
     int *fs = nullptr;        // fused-sort state (se2p_fsort.h), zero between sorts
     int sort_mode = 0;        // SE2P_SORT_* bits
+    int spread_chunk = 0;     // candidate sources per spreading workgroup (0: the default)
     // Stream slots for independent work run side by side (multi-box Fourier
     // sums): each has its own stream, workspace, sort state and FFT plans
     // (plan_slot in the plan key).  Slot work forks from and joins into

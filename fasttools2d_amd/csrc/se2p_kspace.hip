@@ -1500,7 +1500,8 @@ struct GridPass {
     double *partial;
 };
 
-static void plan_pass(const KLayout &L, int nq, int ngrp, int64_t nsp, int64_t nip, GridPass *gp)
+static void plan_pass(const KLayout &L, int nq, int ngrp, int64_t nsp, int64_t nip, int chunk,
+                      GridPass *gp)
 {
     const int P = L.P;
     gp->nq = nq;
@@ -1509,7 +1510,8 @@ static void plan_pass(const KLayout &L, int nq, int ngrp, int64_t nsp, int64_t n
     gp->strip = (L.Mx > P) && (L.My > P + kPad) && gp->lds_strip <= kLdsMax;
     gp->lds_gen = sizeof(double) * ((size_t)2 * kBatch * (P + 1) + (size_t)kBatch * nq + (P + 1)) +
                   sizeof(int) * 2 * kBatch;
-    gp->chunk = 4 * strip_batch(nq) * 4;   // 4 wavefronts x 4 batches
+    // candidates per workgroup: the context's setting, else 4 wavefronts x 4 batches
+    gp->chunk = chunk > 0 ? chunk : 4 * strip_batch(nq) * 4;
     const int64_t tps_x = std::min<int64_t>(L.ntx, (kTile + P + kBin - 1) / kTile + 1);
     const int64_t tps_y = std::min<int64_t>(L.nty, (kTile + P + kBin - 1) / kTile + 1);
     gp->max_spread_chunks = L.ntiles + (tps_x * tps_y * nsp + gp->chunk - 1) / gp->chunk;
@@ -1736,7 +1738,7 @@ struct KRun {
 };
 
 static void krun_init(KRun &r, int kind, const KGeom &g, const double *f, const double *nv,
-                      int nrhs, int64_t dstride, int64_t ustride, double *u)
+                      int nrhs, int64_t dstride, int64_t ustride, double *u, int chunk)
 {
     r.kind = kind;
     r.g = g;
@@ -1755,7 +1757,7 @@ static void krun_init(KRun &r, int kind, const KGeom &g, const double *f, const 
     r.nrp = r.ngrp * r.RB;
     // targets are binned separately unless they are the source array itself
     r.same_points = (g.ptar == g.psrc) && (g.nt == g.ns);
-    plan_pass(r.L, r.ncomp * r.RB, r.ngrp, g.ns, g.nt, &r.gp);
+    plan_pass(r.L, r.ncomp * r.RB, r.ngrp, g.ns, g.nt, chunk, &r.gp);
     r.maps = r.gp.strip || r.L.itile;
 }
 
@@ -1927,7 +1929,7 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         return 0;
     }
     KRun r;
-    krun_init(r, kind, g, f, nv, nrhs, dstride, ustride, u);
+    krun_init(r, kind, g, f, nv, nrhs, dstride, ustride, u, ctx->spread_chunk);
     if (int rc = get_plans(ctx, r.L.Mx, r.L.My, r.ncomp * r.nrp, 2 * r.nrp, &r.plans)) return rc;
 
     // a call whose backward may follow keeps its state in a buffer of its own
@@ -2085,7 +2087,7 @@ int kspace_sum_boxes(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int nb
         }
         const KGeom g{b.psrc, b.ptar, b.ns, b.nt, b.xi, b.eta, b.Lx, b.Ly, b.w, b.Mx, b.My, b.P};
         runs.emplace_back();
-        krun_init(runs.back(), kind, g, b.f, b.n, 1, 2 * b.ns, 2 * b.nt, b.u);
+        krun_init(runs.back(), kind, g, b.f, b.n, 1, 2 * b.ns, 2 * b.nt, b.u, ctx->spread_chunk);
         cost.push_back((double)b.ns + (double)b.nt + 0.25 * (double)b.Mx * (double)b.My);
     }
     SE2P_HIP(hipGetLastError());
@@ -2199,8 +2201,8 @@ int kspace_adjoint(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, con
     const KeptK *kept = same_points && ctx->kept_k && ctx->kept_k->match(kind, g, f, nv)
                             ? ctx->kept_k : nullptr;
     GridPass gt, gs;   // target pass, source pass
-    plan_pass(L, ncomp, 1, ns, nt, &gt);
-    plan_pass(L, 2, 1, nt, ns, &gs);
+    plan_pass(L, ncomp, 1, ns, nt, ctx->spread_chunk, &gt);
+    plan_pass(L, 2, 1, nt, ns, ctx->spread_chunk, &gs);
     // the source pass spreads over the same bins with the same chunking as
     // the forward: its work plans are the forward's
     const bool share_maps = kept && gs.strip == kept->gp.strip && gs.chunk == kept->gp.chunk &&

@@ -93,6 +93,12 @@ class Context:
         forward's arrays must stay unchanged until the backward call."""
         self._check(self._lib.se2p_set_keep(self.handle, int(bool(flag))))
 
+    def set_spread_chunk(self, chunk: int) -> None:
+        """Candidate sources per spreading workgroup (se2p_set_spread_chunk):
+        0 for the default, else a multiple of 16 in [64, 2048].  Results for
+        different values differ by rounding."""
+        self._check(self._lib.se2p_set_spread_chunk(self.handle, int(chunk)))
+
     def set_sort_mode(self, mode: int) -> None:
         """Sorting of cell lists and bins: 0 (default) uses the fused sort
         where it fits; SORT_RADIX / SORT_BOX_KERNEL force the separate
@@ -524,7 +530,9 @@ def autotune(kind, psrc, ptar, f, n, Lx, Ly, *, tol, P=0, ctx=None):
     nc = info.ncand
     out = {"P": info.P, "cached": bool(info.cached), "best": info.best,
            "candidates": [{"Nb": info.Nb[i], "nside_x": info.nside_x[i], "Mx": info.Mx[i],
-                           "ms": info.ms[i]} for i in range(nc)]}
+                           "ms": info.ms[i]} for i in range(nc)],
+           "spread_chunk": info.spread_chunk,
+           "chunks": [{"chunk": info.chunk[k], "ms": info.chunk_ms[k]} for k in range(3)] if nc else []}
     return prm, out
 
 

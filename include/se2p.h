@@ -98,6 +98,17 @@ int se2p_set_sort_mode(se2p_ctx *ctx, int mode);
  * nothing.  Disabling frees the state.  Default 0.  No reference
  * counterpart. */
 int se2p_set_keep(se2p_ctx *ctx, int enable);
+/* Spreading work split: candidate sources per spreading workgroup (a tile's
+ * candidate list is cut into chunks of this many, whose partial tiles are
+ * added in chunk order).  0 selects the default (256 for one density, 128
+ * for batched densities); otherwise a multiple of 16 in
+ * [SE2P_SPREAD_CHUNK_MIN, SE2P_SPREAD_CHUNK_MAX].  The grids' summation order
+ * follows the split, so results for different values differ by rounding;
+ * for one value they are bitwise reproducible.  A change drops kept
+ * Fourier-space state.  se2p_autotune sets it.  No reference counterpart. */
+#define SE2P_SPREAD_CHUNK_MIN 64
+#define SE2P_SPREAD_CHUNK_MAX 2048
+int se2p_set_spread_chunk(se2p_ctx *ctx, int chunk);
 /* Per-stage device timing with HIP events on the context stream.  When
  * enabled, se2p_stage_times fills up to n floats (milliseconds) for the
  * most recent call: sort, real, spread, fft_fwd, filter, fft_inv, interp
@@ -244,6 +255,7 @@ int se2p_dlp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *
 
 /* ---- parameter autotuning (no reference counterpart) --------------------- */
 #define SE2P_TUNE_MAX 16
+#define SE2P_TUNE_CHUNKS 3
 typedef struct se2p_tune_info {
     int ncand;                 /* candidates timed (0 on a cache hit) */
     int best;                  /* index of the chosen candidate */
@@ -253,6 +265,9 @@ typedef struct se2p_tune_info {
     int nside_x[SE2P_TUNE_MAX];
     int Mx[SE2P_TUNE_MAX];
     double ms[SE2P_TUNE_MAX];  /* measured real + Fourier step of each candidate, ms */
+    int spread_chunk;          /* chosen spreading work split (0: the default) */
+    int chunk[SE2P_TUNE_CHUNKS];        /* spreading splits timed at the chosen Nb */
+    double chunk_ms[SE2P_TUNE_CHUNKS];  /* their real + Fourier step, ms */
 } se2p_tune_info;
 
 /* Chooses the drivers' free parameter for one geometry at prm->tol
@@ -264,12 +279,15 @@ typedef struct se2p_tune_info {
  * (Ns+Nt)/4), each as the best of three blocks of three back-to-back
  * steps after a warm-up step, and the fastest is kept.  Every candidate meets the drivers'
  * truncation estimates at tol; candidates whose grid exceeds 4096^2 points
- * are skipped.  P is prm->P, or the drivers' 24 when 0.  prm->M must be 0
+ * are skipped.  Then, at the chosen Nb, the spreading work split
+ * (se2p_set_spread_chunk) is timed at the default, 128 and 512, and the
+ * fastest is set on the context.  P is prm->P, or the drivers' 24 when 0.  prm->M must be 0
  * (the grid follows from the cell size).  On return *prm holds the chosen
  * candidate's parameters as se2p_select_params fills them for its Nb and P:
  * passing *prm on to se2p_slp_ewald / se2p_dlp_ewald reproduces them.
  * Choices are cached per process by (device, kind, Ns, Nt,
- * targets-are-sources, Lx, Ly, tol, P requested); a cache hit times nothing.
+ * targets-are-sources, Lx, Ly, tol, P requested); a cache hit times nothing
+ * and sets the cached spreading split on the context.
  * Points are the caller's (unwrapped) points, as for the driver entries.
  * Synchronous.  info may be NULL. */
 int se2p_autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const double *ptar,

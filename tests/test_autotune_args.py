@@ -32,3 +32,18 @@ def test_autotune_argument_checks():
         assert b"host-only" in lib.se2p_last_error(h)
     finally:
         lib.se2p_destroy(h)
+
+
+def test_spread_chunk_argument_checks():
+    lib = _capi.load()
+    h = ctypes.c_void_p()
+    assert lib.se2p_create(ctypes.byref(h), _capi.SE2P_HOST_ONLY) == 0
+    try:
+        for ok in (0, 64, 128, 256, 512, 2048):
+            assert lib.se2p_set_spread_chunk(h, ok) == 0, ok
+        for bad in (-16, 16, 48, 100, 2064, 4096):
+            assert lib.se2p_set_spread_chunk(h, bad) == _capi.SE2P_ERR_ARG, bad
+            assert b"spread chunk" in lib.se2p_last_error(h)
+        assert lib.se2p_set_spread_chunk(None, 0) == _capi.SE2P_ERR_ARG
+    finally:
+        lib.se2p_destroy(h)

@@ -38,23 +38,31 @@ def test_autotune_stresslet_benchmark_meets_tol(gpu):
     w, _ = W.dlp_ellipses_config()
     ps, f, n = _t(w.psrc), _t(w.f), _t(w.n)
     print("[autotune dlp ellipses N=1e5 tol 1e-10]", flush=True)
-    prm, info = pe.autotune("dlp", ps, ps, f, n, 1.0, 1.0, tol=1e-10)
+    ctx = pe.Context(0)
+    prm, info = pe.autotune("dlp", ps, ps, f, n, 1.0, 1.0, tol=1e-10, ctx=ctx)
     assert info["P"] == 24 and prm.P == 24 and not info["cached"]
+    # the spreading split: the default (0), 128 and 512 timed at the chosen Nb
+    cms = [c["ms"] for c in info["chunks"]]
+    assert [c["chunk"] for c in info["chunks"]] == [0, 128, 512]
+    assert info["spread_chunk"] == info["chunks"][int(np.argmin(cms))]["chunk"]
+    print(f"  spread chunks {info['chunks']}", flush=True)
     ms = [c["ms"] for c in info["candidates"]]
     assert len(ms) >= 4 and info["best"] == int(np.argmin(ms))
     assert prm.nside_x == info["candidates"][info["best"]]["nside_x"]
     print(f"  candidates {info['candidates']}", flush=True)
     pt = _t(w.ptar[np.array(d["targets"])])
-    ur = pe.mex_stokes_dlp_real(ps, pt, f, n, prm.xi, prm.nside_x, prm.nside_y, 1.0, 1.0)
+    ur = pe.mex_stokes_dlp_real(ps, pt, f, n, prm.xi, prm.nside_x, prm.nside_y, 1.0, 1.0, ctx=ctx)
     uk = pe.mex_stokes_dlp_kspace(ps, pt, prm.xi, prm.eta, f, n, prm.Mx, prm.My, 1.0, 1.0,
-                                  prm.w, prm.P)
+                                  prm.w, prm.P, ctx=ctx)
     truth = np.array(d["u_real"]) + np.array(d["u_kspace"])
     err = relerr((ur + uk).cpu().numpy(), truth)
     print(f"  tuned Nb {prm.Nb:.1f} nside {prm.nside_x} M {prm.Mx}: error vs direct Ewald "
           f"{err:.3e}", flush=True)
     assert err <= 1e-10
-    prm2, info2 = pe.autotune("dlp", ps, ps, f, n, 1.0, 1.0, tol=1e-10)
+    ctx2 = pe.Context(0)
+    prm2, info2 = pe.autotune("dlp", ps, ps, f, n, 1.0, 1.0, tol=1e-10, ctx=ctx2)
     assert info2["cached"] and info2["candidates"] == []
+    assert info2["spread_chunk"] == info["spread_chunk"]
     assert (prm2.Nb, prm2.xi, prm2.Mx, prm2.nside_x) == (prm.Nb, prm.xi, prm.Mx, prm.nside_x)
 
 
@@ -66,7 +74,8 @@ def test_autotune_separate_targets_driver_parity(gpu):
     xt = rng.uniform(-0.5, 1.5, (1000, 2)) * [Lx, Ly]
     f = rng.standard_normal((3000, 2))
     print("[autotune slp 3000 sources / 1000 targets, 1x2 box]", flush=True)
-    prm, info = pe.autotune("slp", _t(xs), _t(xt), _t(f), None, Lx, Ly, tol=1e-10)
+    prm, info = pe.autotune("slp", _t(xs), _t(xt), _t(f), None, Lx, Ly, tol=1e-10,
+                            ctx=pe.Context(0))
     assert prm.My == 2 * prm.Mx and prm.nside_y == 2 * prm.nside_x
     u1, u2, ur, uk, xi, prm2 = pe.StokesSLP_ewald_2p(xs[:, 0], xs[:, 1], xt[:, 0], xt[:, 1],
                                                      f[:, 0], f[:, 1], Lx, Ly, P=prm.P,
@@ -83,9 +92,10 @@ def test_autotune_slp_matches_tighter_tolerance(gpu):
     x = w.psrc_unwrapped
     print("[autotune slp N=2e4 L=1.3 tol 1e-8]", flush=True)
     xt = _t(x)
-    prm, info = pe.autotune("slp", xt, xt, _t(w.f), None, 1.3, 1.3, tol=1e-8)
+    ctx = pe.Context(0)
+    prm, info = pe.autotune("slp", xt, xt, _t(w.f), None, 1.3, 1.3, tol=1e-8, ctx=ctx)
     assert info["P"] == 24
-    kw = dict(ctx=None, return_params=True)
+    kw = dict(ctx=ctx, return_params=True)
     a = pe.StokesSLP_ewald_2p(x[:, 0], x[:, 1], x[:, 0], x[:, 1], w.f[:, 0], w.f[:, 1], 1.3, 1.3,
                               P=prm.P, Nb=prm.Nb, tol=1e-8, **kw)
     b = pe.StokesSLP_ewald_2p(x[:, 0], x[:, 1], x[:, 0], x[:, 1], w.f[:, 0], w.f[:, 1], 1.3, 1.3,
@@ -95,3 +105,43 @@ def test_autotune_slp_matches_tighter_tolerance(gpu):
     err = relerr(ua, ub)
     print(f"  tuned Nb {prm.Nb:.1f} M {prm.Mx}: vs tol 1e-13 run {err:.3e}", flush=True)
     assert err <= 1e-8
+
+
+def test_spread_chunk_settings_match_oracle(gpu):
+    """The spreading work split changes only the summation order: each
+    setting is within 1e-12 of the oracle, bitwise repeatable, and a change
+    drops kept state (the backward call then rebuilds with the new split)."""
+    from oracle import cref
+    from fasttools2d_amd import params as Pm
+
+    pe = gpu
+    w = W.uniform_box(3000, seed=44)
+    p = Pm.select_params("dlp", 3000, 3000, w.f, 1.0, 1.0, tol=1e-10, M=96)
+    ps, f, n = _t(w.psrc), _t(w.f), _t(w.n)
+    sub = np.arange(0, 3000, 10)
+    ref = cref.dlp_kspace(w.psrc, w.ptar[sub], p.xi, p.eta, w.f, w.n, p.Mx, p.My, 1.0, 1.0, p.w, p.P)
+    ctx = pe.Context(0)
+    outs = {}
+    for chunk in (0, 64, 128, 512, 2048, 128):
+        ctx.set_spread_chunk(chunk)
+        u = pe.mex_stokes_dlp_kspace(ps, ps, p.xi, p.eta, f, n, p.Mx, p.My, 1.0, 1.0, p.w, p.P,
+                                     ctx=ctx).cpu().numpy()
+        err = relerr(u[sub], ref)
+        print(f"[spread chunk {chunk}] vs oracle {err:.2e}", flush=True)
+        assert err <= 1e-12
+        if chunk in outs:
+            assert np.array_equal(outs[chunk], u)
+        outs[chunk] = u
+    # kept state made with one split, backward after a change: equal to a
+    # backward that rebuilds with the new split
+    du = _t(np.random.default_rng(45).standard_normal((3000, 2)))
+    args = (ps, ps, p.xi, p.eta, f, n, p.Mx, p.My, 1.0, 1.0, p.w, p.P)
+    ctx.set_keep(True)
+    ctx.set_spread_chunk(0)
+    pe.mex_stokes_dlp_kspace(*args, ctx=ctx)
+    ctx.set_spread_chunk(512)
+    g_kept = pe.mex_stokes_dlp_kspace_backward(*args, du, ctx=ctx)
+    ctx.set_keep(False)
+    g_new = pe.mex_stokes_dlp_kspace_backward(*args, du, ctx=ctx)
+    for k in g_new:
+        assert torch.equal(g_kept[k], g_new[k]), k
