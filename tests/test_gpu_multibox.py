@@ -116,3 +116,39 @@ def test_multibox_many_boxes_equals_single(gpu):
             one = _single(gpu, kind, b).cpu().numpy()
             assert np.array_equal(outs[i], one), (kind, i, relerr(outs[i], one))
         _say(f"[multibox 32] {kind} identical to single-box calls")
+
+
+def test_multibox_packs_with_mixed_grids(gpu):
+    """20 boxes, more than one packed launch (16 boxes each): grids from 32
+    to 320 points a side, so that some boxes take the packed strip spreading
+    and others the single-box kernels (grids narrower than P + 16), square
+    and rectangular, targets = sources and separate targets.  Every box
+    bitwise equal to its single-box call, Fourier space of both kernels."""
+    rng = np.random.default_rng(46)
+    boxes = []
+    for i in range(20):
+        Lx = float(rng.choice([0.5, 1.0, 2.0]))
+        Ly = Lx if i % 3 else 2 * Lx
+        N = int(rng.integers(50, 2000))
+        x = np.stack([Pm.wrap(Lx * rng.random(N), Lx), Pm.wrap(Ly * rng.random(N), Ly)], 1)
+        t = x if i % 2 else np.stack([Pm.wrap(Lx * rng.random(N // 2 + 1), Lx),
+                                      Pm.wrap(Ly * rng.random(N // 2 + 1), Ly)], 1)
+        f = rng.standard_normal((N, 2))
+        nv = rng.standard_normal((N, 2))
+        nv /= np.linalg.norm(nv, axis=1, keepdims=True)
+        M = int(rng.choice([20, 32, 48, 64, 96, 160]))
+        p = Pm.select_params("dlp", N, len(t), f, Lx, Ly, tol=1e-10, M=M)
+        if p.eta >= 1:   # grid below the driver's: take the driver's
+            p = Pm.select_params("dlp", N, len(t), f, Lx, Ly, tol=1e-10)
+        boxes.append({"psrc": x, "ptar": t, "f": f, "n": nv, "Lx": Lx, "Ly": Ly, "xi": p.xi,
+                      "nside_x": p.nside_x, "nside_y": p.nside_y, "eta": p.eta, "w": p.w,
+                      "Mx": p.Mx, "My": p.My, "P": p.P})
+    narrow = sum(min(b["Mx"], b["My"]) < b["P"] + 16 for b in boxes)
+    _say(f"[multibox packs] 20 boxes, {narrow} with grids narrower than P + 16")
+    assert 0 < narrow < 20
+    for kind in ("slp_kspace", "dlp_kspace"):
+        outs = _multi(gpu, kind, boxes)
+        for i, b in enumerate(boxes):
+            one = _single(gpu, kind, b).cpu().numpy()
+            assert np.array_equal(outs[i], one), (kind, i, relerr(outs[i], one))
+        _say(f"[multibox packs] {kind} identical to single-box calls")
