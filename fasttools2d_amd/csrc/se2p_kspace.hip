@@ -370,6 +370,15 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Lane l's value of v as a wave-uniform value (l uniform).
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Strip variant (grids with Mx > P and My > P + kPad): a lane owns a column
 // strip of 4 nodes (X, Ys..Ys+3) of the 16x16 tile, so the x factor and the
 // window test are shared by 4 nodes, and the y factors are zero-padded by
@@ -545,7 +554,47 @@ __device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b
         }
         wave_lds_sync();
         const int nb = min(kSB, c1 - base);
-        if (WIDE) {
+        if constexpr (WIDE && NQ <= 3) {
+            // the batch's densities and tile offsets are the same for every
+            // lane: lane s holds source s's, and v_readlane hands them to the
+            // source loop as wave-uniform values (an LDS read of them would
+            // return the same 8 bytes to all 64 lanes)
+            const int lx = lane & 15, ly4 = 4 * (lane >> 4);
+            const unsigned Pu = (unsigned)a.P, Pj = (unsigned)(a.P + kPad);
+            double qv[NQ];
+            int dxv = 0, dyv = 0;
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) qv[c] = 0.0;
+            if (lane < kSB) {
+#pragma unroll
+                for (int c = 0; c < NQ; ++c) qv[c] = Q[lane * NQ + c];
+                dxv = AXY[2 * lane];
+                dyv = AXY[2 * lane + 1];
+            }
+            // constant trip count and a uniform guard: a loop holding
+            // v_readlane (convergent) is unrolled only when fully unrollable
+#pragma unroll
+            for (int s = 0; s < kSB; ++s) {
+                if (s < nb) {
+                    const unsigned iu = (unsigned)(lx + __builtin_amdgcn_readlane(dxv, s));
+                    const unsigned ju =
+                        (unsigned)(ly4 + __builtin_amdgcn_readlane(dyv, s) + kPad);
+                    const bool hit = iu <= Pu && ju <= Pj;
+                    const double wx = WX[s * P1 + min(iu, Pu)];
+                    const double wxe = hit ? wx : 0.0;
+                    const double *wy = WY + s * PW + min(ju, Pj);   // zero-padded row
+                    double q[NQ];
+#pragma unroll
+                    for (int c = 0; c < NQ; ++c) q[c] = readlane_f64(qv[c], s);
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        const double wgt = wxe * wy[kk];
+#pragma unroll
+                        for (int c = 0; c < NQ; ++c) acc[kk][c] += wgt * q[c];
+                    }
+                }
+            }
+        } else if (WIDE) {
             const int lx = lane & 15, ly4 = 4 * (lane >> 4);
             const unsigned Pu = (unsigned)a.P, Pj = (unsigned)(a.P + kPad);
 #pragma unroll 4
