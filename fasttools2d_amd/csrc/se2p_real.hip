@@ -439,14 +439,16 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
                     slp_factors(r2, xi, e1tab, etab, hE, g);
                     slp_add(d1, d2, hE, g, q0[0], q0[1], acc[0][0], acc[0][1]);
                 } else {
-                    // record values carry -4 and 2 xi^2: u_i += G d_i (d.S'd) + e2 l'_i
+                    // record values carry -4 and 2 xi^2:
+                    //   u_i += e2 (H d_i + l'_i),  H = (1 + x)/r^4 (d.S'd),
+                    // with (1 + x)/r^4 = (1/r^2)(1/r^2 + xi^2)
                     const double x = xi2 * r2;
                     const double e2 = se2p_exp_neg(x, etab);
                     const double ir2 = rcp_nr(r2);
-                    const double G = e2 * (1.0 + x) * (ir2 * ir2);
-                    const double gq = G * (d11 * q0[0] + (d1 * d2) * q0[1] + d22 * q0[2]);
-                    acc[0][0] += gq * d1 + e2 * (d1 * q0[3] + d2 * q0[5]);
-                    acc[0][1] += gq * d2 + e2 * (d1 * q0[5] + d2 * q0[4]);
+                    const double H = (ir2 * (ir2 + xi2)) *
+                                     (d11 * q0[0] + (d1 * d2) * q0[1] + d22 * q0[2]);
+                    acc[0][0] = fma(e2, fma(H, d1, d1 * q0[3] + d2 * q0[5]), acc[0][0]);
+                    acc[0][1] = fma(e2, fma(H, d2, d1 * q0[5] + d2 * q0[4]), acc[0][1]);
                 }
             } else {
                 if (KIND == 0) slp_pair<RB>(d1, d2, r2, xi, e1tab, etab, qb + s, qs, acc);
