@@ -139,7 +139,7 @@ struct RealArgs {
 __device__ __forceinline__ void slp_factors(double r2, double xi, const double *e1tab,
                                             const double *etab, double &hE, double &g)
 {
-    const double x = r2 * xi * xi;
+    const double x = r2 * (xi * xi);   // xi^2 is loop-invariant: one multiply per pair
     const double e = se2p_exp_neg(x, etab);
     hE = 0.5 * se2p_e1(x, e, e1tab);
     g = e * rcp_nr(r2);
@@ -150,8 +150,8 @@ __device__ __forceinline__ void slp_add(double d1, double d2, double hE, double 
                                         double f2, double &a1, double &a2)
 {
     const double c = d1 * f2 - d2 * f1;
-    a1 += hE * f1 + (g * d2) * c;
-    a2 += hE * f2 - (g * d1) * c;
+    a1 = fma(hE, f1, fma(g * d2, c, a1));
+    a2 = fma(hE, f2, fma(-(g * d1), c, a2));
 }
 
 template <int RB>
@@ -875,10 +875,19 @@ struct RealAdjSameArgs {
     double *dpsrc, *dptar, *df, *dn;   // caller order; may be null
 };
 
-template <int KIND>
+// TG: targets per wavefront.  TG = 64: the wavefronts of a workgroup share
+// its 64 targets and split every source run four ways (partial sums added
+// through LDS).  TG = 16: each wavefront owns 16 of the targets and its four
+// 16-lane rows take four consecutive sources per step (lane = target + 16 x
+// row); a wavefront's reach is then a quarter of the chunk, so fewer of its
+// steps carry no pair in range.  The rows' sums are added across lanes in a
+// fixed order.  The Stokeslet uses TG = 16 (5% faster at N = 4e6), the
+// stresslet TG = 64 (TG = 16 measured no faster).
+template <int KIND, int TG>
 __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
 {
     constexpr int RN = SrcRec<KIND>::n;
+    static_assert(TG == 64 || TG == 16, "targets per wavefront");
     // accumulators.  Stokeslet: target grad (2), source grad (2), density (2).
     // Stresslet, with the lane's du and densities factored out of the sums
     // (they are constant over the pass): T[2l + a] = sum_j (D q_j)-terms of
@@ -887,7 +896,7 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
     constexpr int NV = KIND == 0 ? 6 : 13;
     __shared__ double e1tab[KIND == 0 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
     __shared__ double etab[SE2P_EXP2_N];
-    __shared__ double red[kWaves - 1][NV][64];
+    __shared__ double red[TG == 64 ? kWaves - 1 : 1][NV][64];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (KIND == 0)
@@ -904,7 +913,9 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
     const int sub = 1 << (2 * kSubLog);
     const int l0 = a.off[cell * sub] + 64 * (b - a.chunk_off[cell]);
     const int l1 = a.off[(cell + 1) * sub];
-    const int l = l0 + lane;
+    if (TG == 16 && l0 + TG * wave >= l1) return;   // no target in this wavefront
+    const int l = TG == 64 ? l0 + lane : l0 + TG * wave + (lane % TG);
+    const int row = lane / TG;   // TG = 16: source phase of the lane
     const bool active = l < l1;
     // lane point: position, densities, du
     double lx, ly, lq0 = 0.0, lq1 = 0.0, lq2 = 0.0, la1 = 0.0, la2 = 0.0;
@@ -1042,7 +1053,23 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
             u = un;
         }
     };
-    run(a.off[cell * sub] + wave, a.off[(cell + 1) * sub], 0.0, 0.0, true);
+    // TG = 16: points ub + row, ub = u0, u0 + 4, ... < u1 (lanes past u1 idle;
+    // their loads are clamped to u1 - 1)
+    auto run_rows = [&](int u0, int u1, double zx, double zy, bool own) {
+        if (u0 >= u1) return;
+        int ub = u0;
+        U cur = load(min(ub + row, u1 - 1));
+        for (;;) {
+            const int un = ub + 4;
+            const U nxt = load(min(un + row, u1 - 1));
+            if (ub + row < u1) pair(cur, zx, zy, own);
+            if (un >= u1) break;
+            cur = nxt;
+            ub = un;
+        }
+    };
+    if (TG == 64) run(a.off[cell * sub] + wave, a.off[(cell + 1) * sub], 0.0, 0.0, true);
+    else run_rows(a.off[cell * sub], a.off[(cell + 1) * sub], 0.0, 0.0, true);
     double bx0, by0, bx1, by1;
     wave_bbox(active, lx, ly, bx0, by0, bx1, by1);
     const double lim = cut2 * (1.0 + 1e-9);
@@ -1065,19 +1092,30 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
             const int rb = ra + __builtin_ctz(~(bits >> ra));
             bits &= ~((1u << rb) - 1u);
             const int u0 = a.off[ub * sub + ra], u1 = a.off[ub * sub + rb];
-            run(u0 + ((wave - cnt) & (kWaves - 1)), u1, zx, zy, false);
+            if (TG == 64) run(u0 + ((wave - cnt) & (kWaves - 1)), u1, zx, zy, false);
+            else run_rows(u0, u1, zx, zy, false);
             cnt += u1 - u0;
         }
     }
-    if (wave > 0)
+    if (TG == 64) {
+        if (wave > 0)
 #pragma unroll
-        for (int v = 0; v < NV; ++v) red[wave - 1][v][lane] = acc[v];
-    __syncthreads();
-    if (wave != 0 || !active) return;
+            for (int v = 0; v < NV; ++v) red[wave - 1][v][lane] = acc[v];
+        __syncthreads();
+        if (wave != 0 || !active) return;
 #pragma unroll
-    for (int v = 0; v < NV; ++v)
+        for (int v = 0; v < NV; ++v)
 #pragma unroll
-        for (int w = 0; w < kWaves - 1; ++w) acc[v] += red[w][v][lane];
+            for (int w = 0; w < kWaves - 1; ++w) acc[v] += red[w][v][lane];
+    } else {
+        // rows 0..3: (row 0 + row 2) + (row 1 + row 3), in lanes 0..15
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            acc[v] += __shfl_down(acc[v], 32);
+            acc[v] += __shfl_down(acc[v], 16);
+        }
+        if (row != 0 || !active) return;
+    }
     const int o = a.perm[l];
     const double s4 = 1.0 / (4 * SE2P_PI);
     if (KIND == 0) {
@@ -1783,8 +1821,8 @@ static int real_adjoint_same(se2p_ctx *ctx, int kind, const RealGeom &g, const d
     a.df = df;
     a.dn = dn;
     stage_begin(ctx, ST_REAL);
-    if (kind == 0) hipLaunchKernelGGL(k_real_adj_same<0>, dim3(p.grid8), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(k_real_adj_same<1>, dim3(p.grid8), dim3(256), 0, st, a);
+    if (kind == 0) hipLaunchKernelGGL((k_real_adj_same<0, 16>), dim3(p.grid8), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_real_adj_same<1, 64>), dim3(p.grid8), dim3(256), 0, st, a);
     stage_end(ctx, ST_REAL);
     SE2P_HIP(hipGetLastError());
     return 0;
