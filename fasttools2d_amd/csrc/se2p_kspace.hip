@@ -1026,6 +1026,10 @@ __device__ inline void interp_sums(const GET &get, int P, const double *E1, doub
             for (int m = 0; m < NR; ++m) ra[m] = rb[m] = ya[m] = yb[m] = 0.0;
 #pragma unroll
             for (int j = 0; j <= PT; j += 2) {
+                // without the derivative sums: half a row of node loads in
+                // flight at a time (120 VGPRs, four waves per SIMD, instead
+                // of 146 and three)
+                if (!DERIV && j == 2 * ((PT + 2) / 4)) __builtin_amdgcn_sched_barrier(0);
                 double g[NR];
                 get(i, j, g);
 #pragma unroll
