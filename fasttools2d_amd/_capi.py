@@ -29,6 +29,7 @@ SE2P_ERR_PARAM = 6
 SE2P_ERR_NOCONV = 7
 SE2P_ERR_HIP = 10
 SE2P_ERR_FFT = 11
+SE2P_ERR_FROZEN = 12
 
 # every symbol include/se2p.h declares, with (restype, argtypes)
 SIGNATURES = {
@@ -43,6 +44,7 @@ SIGNATURES = {
     "se2p_set_keep": (_I, [_P, _I]),
     "se2p_set_spread_chunk": (_I, [_P, _I]),
     "se2p_set_timing": (_I, [_P, _I]),
+    "se2p_set_frozen": (_I, [_P, _I]),
     "se2p_stage_times": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
     "se2p_slp_real": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _I, _I, _D, _D, _P]),
     "se2p_slp_kspace": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _I, _I, _D, _D, _D, _I, _P]),

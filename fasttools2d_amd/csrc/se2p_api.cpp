@@ -28,9 +28,19 @@ int fail(se2p_ctx *ctx, int code, const char *fmt, ...)
     return code;
 }
 
+int refuse_if_frozen(se2p_ctx *ctx, const char *what, size_t bytes)
+{
+    if (!ctx->frozen) return 0;
+    return fail(ctx, SE2P_ERR_FROZEN,
+                "context is frozen (se2p_set_frozen): %s would allocate or free device memory "
+                "(%zu bytes); make the same call once before freezing so that it is sized",
+                what, bytes);
+}
+
 int ensure_workspace(se2p_ctx *ctx, size_t bytes)
 {
     if (bytes <= ctx->ws_bytes) return 0;
+    if (int rc = refuse_if_frozen(ctx, "growing the workspace", bytes)) return rc;
     if (ctx->ws) {
         SE2P_HIP(hipStreamSynchronize(ctx->stream));
         SE2P_HIP(hipFree(ctx->ws));
@@ -107,7 +117,11 @@ static int check_common(se2p_ctx *ctx, const double *psrc, int64_t ns, const dou
 static int check_real(se2p_ctx *ctx, double xi, int nsx, int nsy, double Lx, double Ly)
 {
     if (nsx < 1 || nsy < 1) return fail(ctx, SE2P_ERR_NSIDE, "nside_x and nside_y must be >= 1");
-    if ((long long)nsx * nsy > (1LL << 28)) return fail(ctx, SE2P_ERR_NSIDE, "too many cells");
+    // cell-list keys are (cell, 4x4 sub-cell) pairs in 32-bit ints: every
+    // entry (forward, batched, backward, pair count) needs nsx*nsy*16 to fit
+    if ((long long)nsx * nsy * 16 > 0x7fffffffLL)
+        return fail(ctx, SE2P_ERR_NSIDE,
+                    "too many cells: nside_x * nside_y * 16 must fit in a 32-bit int");
     if (!(xi > 0) || !(Lx > 0) || !(Ly > 0) || !std::isfinite(xi) || !std::isfinite(Lx) ||
         !std::isfinite(Ly))
         return fail(ctx, SE2P_ERR_PARAM, "xi, Lx and Ly must be positive and finite");
@@ -136,10 +150,18 @@ class HostIo {
 public:
     explicit HostIo(se2p_ctx *ctx) : ctx_(ctx), on_(ctx->host_ptrs) {}
 
-    // declare an array (input or output) and its size in bytes
+    // declare an array (input or output) and its size in bytes; an array
+    // declared twice (ptar == psrc) is staged once, at the larger size
     void want(const void *p, size_t bytes)
     {
-        if (!on_ || !p || bytes == 0 || find(p)) return;
+        if (!on_ || !p || bytes == 0) return;
+        if (Item *it = find(p)) {
+            if (bytes > it->bytes) {
+                need_ += ((bytes + 255) & ~size_t(255)) - ((it->bytes + 255) & ~size_t(255));
+                it->bytes = bytes;
+            }
+            return;
+        }
         items_.push_back({p, nullptr, bytes, false});
         need_ += (bytes + 255) & ~size_t(255);
     }
@@ -148,6 +170,8 @@ public:
     {
         if (!on_) return 0;
         if (need_ > ctx_->io_bytes) {
+            if (int rc = refuse_if_frozen(ctx_, "growing the host-pointer staging area", need_))
+                return rc;
             if (ctx_->io) {
                 SE2P_HIP(hipStreamSynchronize(ctx_->stream));
                 SE2P_HIP(hipFree(ctx_->io));
@@ -220,6 +244,8 @@ private:
 
 int ensure_slots(se2p_ctx *ctx, int n)
 {
+    if ((int)ctx->slots.size() < n)
+        if (int rc = refuse_if_frozen(ctx, "creating stream slots", sizeof(int) * kFsMaxKeys)) return rc;
     if (!ctx->fork_ev) SE2P_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     while ((int)ctx->slots.size() < n) {
         se2p_ctx::Slot sl;
@@ -311,6 +337,7 @@ int se2p_destroy(se2p_ctx *ctx)
         if (sl.st) hipStreamDestroy(sl.st);
     }
     if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
+    if (ctx->switch_ev) hipEventDestroy(ctx->switch_ev);
     for (auto &stage : ctx->ev)
         for (auto &visit : stage)
             for (hipEvent_t &e : visit)
@@ -330,16 +357,32 @@ int se2p_set_stream(se2p_ctx *ctx, void *s)
 {
     if (!ctx) return SE2P_ERR_ARG;
     if (ctx->device == SE2P_HOST_ONLY) return fail(ctx, SE2P_ERR_HIP, "host-only context");
-    if (ctx->own_stream && ctx->stream) hipStreamSynchronize(ctx->stream);
-    if (s) {
-        if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
-        ctx->stream = (hipStream_t)s;
-        ctx->own_stream = false;
-    } else if (!ctx->own_stream) {
-        hipSetDevice(ctx->device);
-        if (hipStreamCreate(&ctx->stream) != hipSuccess) return SE2P_ERR_HIP;
-        ctx->own_stream = true;
+    if (s && (hipStream_t)s == ctx->stream) return SE2P_OK;
+    if (!s && ctx->own_stream) return SE2P_OK;
+    SE2P_HIP(hipSetDevice(ctx->device));
+    // The context's workspace, sort state and kept buffers are shared by all
+    // its calls: work enqueued on the new stream must not overlap what is
+    // still pending on the old one, so the new stream waits for an event
+    // recorded on the old one (no host synchronisation).  While either stream
+    // is being captured into a graph no such link is made (it would join the
+    // capture or wait on an event from outside it): the capturing caller
+    // orders the streams.
+    const hipStream_t old = ctx->stream;
+    hipStream_t next = (hipStream_t)s;
+    if (!s) SE2P_HIP(hipStreamCreate(&next));
+    hipStreamCaptureStatus cs_old = hipStreamCaptureStatusNone, cs_new = hipStreamCaptureStatusNone;
+    if (old) SE2P_HIP(hipStreamIsCapturing(old, &cs_old));
+    SE2P_HIP(hipStreamIsCapturing(next, &cs_new));
+    if (old && cs_old == hipStreamCaptureStatusNone && cs_new == hipStreamCaptureStatusNone) {
+        if (!ctx->switch_ev)
+            SE2P_HIP(hipEventCreateWithFlags(&ctx->switch_ev, hipEventDisableTiming));
+        SE2P_HIP(hipEventRecord(ctx->switch_ev, old));
+        SE2P_HIP(hipStreamWaitEvent(next, ctx->switch_ev, 0));
     }
+    // a stream the context created is released once its pending work is done
+    if (ctx->own_stream && old) SE2P_HIP(hipStreamDestroy(old));
+    ctx->stream = next;
+    ctx->own_stream = !s;
     return SE2P_OK;
 }
 
@@ -371,6 +414,8 @@ int se2p_set_host_pointers(se2p_ctx *ctx, int enable)
 int se2p_set_keep(se2p_ctx *ctx, int enable)
 {
     if (!ctx) return SE2P_ERR_ARG;
+    if (enable == 0 && (ctx->kept_real || ctx->kept_k))
+        if (int rc = refuse_if_frozen(ctx, "freeing the kept forward state", 0)) return rc;
     ctx->keep = enable != 0;
     if (!ctx->keep && ctx->device != SE2P_HOST_ONLY) {
         if (ctx->stream) hipStreamSynchronize(ctx->stream);
@@ -386,12 +431,22 @@ int se2p_set_spread_chunk(se2p_ctx *ctx, int chunk)
     if (chunk != 0 && (chunk < SE2P_SPREAD_CHUNK_MIN || chunk > SE2P_SPREAD_CHUNK_MAX || chunk % 16 != 0))
         return fail(ctx, SE2P_ERR_ARG, "spread chunk %d: 0, or a multiple of 16 in [%d, %d]", chunk,
                     SE2P_SPREAD_CHUNK_MIN, SE2P_SPREAD_CHUNK_MAX);
+    if (chunk != ctx->spread_chunk && ctx->kept_k)
+        if (int rc = refuse_if_frozen(ctx, "changing the split (it frees the kept state)", 0))
+            return rc;
     if (chunk != ctx->spread_chunk && ctx->device != SE2P_HOST_ONLY) {
         // kept Fourier-space state carries the old work split
         if (ctx->stream) hipStreamSynchronize(ctx->stream);
         free_kept_k(ctx);
     }
     ctx->spread_chunk = chunk;
+    return SE2P_OK;
+}
+
+int se2p_set_frozen(se2p_ctx *ctx, int enable)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    ctx->frozen = enable != 0;
     return SE2P_OK;
 }
 
@@ -880,6 +935,7 @@ int se2p_select_params(int kind, int64_t ns, int64_t nt, double Q, double Lx, do
 static int ensure_driver_buffer(se2p_ctx *ctx, size_t bytes)
 {
     if (bytes <= ctx->drv_bytes) return 0;
+    if (int rc = refuse_if_frozen(ctx, "growing the driver buffers", bytes)) return rc;
     if (ctx->drv) {
         SE2P_HIP(hipStreamSynchronize(ctx->stream));
         SE2P_HIP(hipFree(ctx->drv));

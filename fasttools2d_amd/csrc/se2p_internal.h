@@ -81,10 +81,14 @@ struct se2p_ctx {
     };
     std::vector<Slot> slots;
     hipEvent_t fork_ev = nullptr;
+    hipEvent_t switch_ev = nullptr;   // orders a newly bound stream after the previous one
     int plan_slot = 0;        // 0: the context's own plans
     bool keep = false;        // forward calls keep their sorted state (se2p_set_keep)
     se2p::KeptReal *kept_real = nullptr;
     se2p::KeptK *kept_k = nullptr;
+    // se2p_set_frozen: calls that would allocate or free device memory fail
+    // with SE2P_ERR_FROZEN instead (captured graphs keep their addresses)
+    bool frozen = false;
 };
 
 namespace se2p {
@@ -93,6 +97,8 @@ namespace se2p {
 constexpr int kFsMaxKeys = 14336;
 
 int fail(se2p_ctx *ctx, int code, const char *fmt, ...);
+// 0, or SE2P_ERR_FROZEN (with a message naming `what`) on a frozen context.
+int refuse_if_frozen(se2p_ctx *ctx, const char *what, size_t bytes);
 
 // Ensure the workspace holds at least `bytes`.
 int ensure_workspace(se2p_ctx *ctx, size_t bytes);

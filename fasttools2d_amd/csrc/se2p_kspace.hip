@@ -1453,6 +1453,7 @@ static int get_plans(se2p_ctx *ctx, int Mx, int My, int nfwd, int ninv, FftPlans
         *out = &it->second;
         return 0;
     }
+    if (int rc = refuse_if_frozen(ctx, "creating rocFFT plans for a new grid", 0)) return rc;
     FftPlans p;
     size_t lengths[2] = {(size_t)My, (size_t)Mx};   // fastest dimension first
     rocfft_status s;
@@ -1996,6 +1997,8 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
         KeptK &kk = *ctx->kept_k;
         kk.valid = false;
         if (sizing.off > kk.bytes) {
+            if (int rc = refuse_if_frozen(ctx, "growing the kept Fourier-space state", sizing.off))
+                return rc;
             if (kk.buf) {
                 SE2P_HIP(hipStreamSynchronize(st));
                 SE2P_HIP(hipFree(kk.buf));

@@ -1612,6 +1612,8 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
         KeptReal &kr = *ctx->kept_real;
         kr.valid = false;
         if (sizing.off > kr.bytes) {
+            if (int rc = refuse_if_frozen(ctx, "growing the kept real-space state", sizing.off))
+                return rc;
             if (kr.buf) {
                 SE2P_HIP(hipStreamSynchronize(st));
                 SE2P_HIP(hipFree(kr.buf));
@@ -1734,8 +1736,7 @@ int real_sum_multibox(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int n
     return real_sum_boxes(ctx, kind, rb.data(), nbox, 1, 0, 0, nullptr);
 }
 
-int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
-             int nrhs, int64_t dstride, int64_t ustride, double *u, int64_t *pair_counts)
+static RealBox one_box(const RealGeom &g, const double *f, const double *n, double *u)
 {
     RealBox b{};
     b.psrc = g.psrc;
@@ -1750,6 +1751,13 @@ int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const 
     b.Lx = g.Lx;
     b.Ly = g.Ly;
     b.xi = g.xi;
+    return b;
+}
+
+int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
+             int nrhs, int64_t dstride, int64_t ustride, double *u, int64_t *pair_counts)
+{
+    RealBox b = one_box(g, f, n, u);
     return real_sum_boxes(ctx, kind, &b, 1, nrhs, dstride, ustride, pair_counts);
 }
 

@@ -75,6 +75,12 @@ class Context:
     def set_async(self, flag: bool) -> None:
         self._check(self._lib.se2p_set_async(self.handle, 1 if flag else 0))
 
+    def set_frozen(self, flag: bool) -> None:
+        """Refuse (SE2P_ERR_FROZEN) every call that would allocate or free
+        device memory, e.g. while HIP graphs captured from this context
+        exist (se2p_set_frozen)."""
+        self._check(self._lib.se2p_set_frozen(self.handle, 1 if flag else 0))
+
     def set_timing(self, flag: bool) -> None:
         self._check(self._lib.se2p_set_timing(self.handle, 1 if flag else 0))
 
@@ -144,6 +150,31 @@ def _as_points(a, name: str, device) -> torch.Tensor:
     return a
 
 
+def _output(out, nt: int, device) -> torch.Tensor:
+    """The result tensor: a new (Nt, 2) float64 device tensor, or the caller's
+    ``out``, which the library overwrites through its data pointer and which
+    must therefore be exactly that: contiguous float64 (Nt, 2) on the device
+    of the inputs."""
+    if out is None:
+        return torch.empty((nt, 2), dtype=torch.float64, device=device)
+    if not isinstance(out, torch.Tensor):
+        raise ValueError("out must be a torch tensor.")
+    if out.dtype != torch.float64 or tuple(out.shape) != (nt, 2):
+        raise ValueError(f"out must be a float64 tensor of shape ({nt}, 2).")
+    if out.device != torch.device(device):
+        raise ValueError(f"out must be on {device}.")
+    if not out.is_contiguous():
+        raise ValueError("out must be contiguous.")
+    return out
+
+
+def _same_size(ps, a, name: str) -> None:
+    """The reference's size check of a per-source array (any array type:
+    checked before anything is moved to the device)."""
+    if a is not None and np.shape(a)[:1] != np.shape(ps)[:1]:
+        raise ValueError(f"psrc and {name} must be the same size.")
+
+
 def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr() if t.numel() > 0 else 0)
 
@@ -169,8 +200,8 @@ def mex_stokes_slp_real(psrc, ptar, f, xi, nside_x, nside_y, Lx, Ly, *, ctx=None
     ff = _as_points(f, "f", dev)
     if ff.shape[0] != ps.shape[0]:
         raise ValueError("psrc and f must be the same size.")
+    u = _output(out, pt.shape[0], dev)
     ctx = _prepare(ctx, dev)
-    u = out if out is not None else torch.empty((pt.shape[0], 2), dtype=torch.float64, device=dev)
     ctx._check(ctx._lib.se2p_slp_real(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0],
                                       _ptr(ff), float(xi), int(nside_x), int(nside_y),
                                       float(Lx), float(Ly), _ptr(u)))
@@ -183,8 +214,8 @@ def mex_stokes_slp_kspace(psrc, ptar, xi, eta, f, Mx, My, Lx, Ly, w, P, *, ctx=N
     ff = _as_points(f, "f", dev)
     if ff.shape[0] != ps.shape[0]:
         raise ValueError("psrc and f must be the same size.")
+    u = _output(out, pt.shape[0], dev)
     ctx = _prepare(ctx, dev)
-    u = out if out is not None else torch.empty((pt.shape[0], 2), dtype=torch.float64, device=dev)
     ctx._check(ctx._lib.se2p_slp_kspace(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0],
                                         float(xi), float(eta), _ptr(ff), int(Mx), int(My),
                                         float(Lx), float(Ly), float(w), int(P), _ptr(u)))
@@ -199,8 +230,8 @@ def mex_stokes_dlp_real(psrc, ptar, f, n, xi, nside_x, nside_y, Lx, Ly, *, ctx=N
         raise ValueError("psrc and f must be the same size.")
     if nn.shape[0] != ps.shape[0]:
         raise ValueError("psrc and n must be the same size.")
+    u = _output(out, pt.shape[0], dev)
     ctx = _prepare(ctx, dev)
-    u = out if out is not None else torch.empty((pt.shape[0], 2), dtype=torch.float64, device=dev)
     ctx._check(ctx._lib.se2p_dlp_real(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0],
                                       _ptr(ff), _ptr(nn), float(xi), int(nside_x), int(nside_y),
                                       float(Lx), float(Ly), _ptr(u)))
@@ -216,8 +247,8 @@ def mex_stokes_dlp_kspace(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w, P, *, ct
         raise ValueError("psrc and f must be the same size.")
     if nn.shape[0] != ps.shape[0]:
         raise ValueError("psrc and n must be the same size.")
+    u = _output(out, pt.shape[0], dev)
     ctx = _prepare(ctx, dev)
-    u = out if out is not None else torch.empty((pt.shape[0], 2), dtype=torch.float64, device=dev)
     ctx._check(ctx._lib.se2p_dlp_kspace(ctx.handle, _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0],
                                         float(xi), float(eta), _ptr(ff), _ptr(nn), int(Mx), int(My),
                                         float(Lx), float(Ly), float(w), int(P), _ptr(u)))
@@ -472,6 +503,9 @@ def mex_stokes_dlp_kspace_backward(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w,
 
 # -- the reference drivers as single calls --------------------------------------
 def _driver(kind, psrc, ptar, f, n, Lx, Ly, P, Nb, tol, M, ctx, same):
+    _same_size(psrc, f, "f")
+    if kind == 1:
+        _same_size(psrc, n, "n")
     dev = _device_of(psrc, ptar, f, n)
     ps = _as_points(psrc, "psrc", dev)
     pt = ps if same else _as_points(ptar, "ptar", dev)
@@ -514,6 +548,11 @@ def autotune(kind, psrc, ptar, f, n, Lx, Ly, *, tol, P=0, ctx=None):
     them) and a dict of the candidates timed."""
     k = {"slp": 0, "dlp": 1}[kind]
     same = ptar is psrc
+    _same_size(psrc, f, "f")
+    if k == 1:
+        if n is None:
+            raise ValueError("the stresslet needs normals n.")
+        _same_size(psrc, n, "n")
     dev = _device_of(psrc, ptar, f, n) if n is not None else _device_of(psrc, ptar, f)
     ps = _as_points(psrc, "psrc", dev)
     pt = ps if same else _as_points(ptar, "ptar", dev)

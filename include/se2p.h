@@ -51,6 +51,7 @@ typedef struct se2p_ctx se2p_ctx;
 #define SE2P_ERR_NOCONV 7  /* parameter Newton iteration did not converge */
 #define SE2P_ERR_HIP 10    /* HIP runtime error */
 #define SE2P_ERR_FFT 11    /* rocFFT error */
+#define SE2P_ERR_FROZEN 12 /* the call would allocate or free on a frozen context */
 
 /* ---- context ----------------------------------------------------------- */
 /* device >= 0: a HIP device.  device == SE2P_HOST_ONLY: a context that only
@@ -61,7 +62,10 @@ int se2p_create(se2p_ctx **ctx, int device);
 int se2p_destroy(se2p_ctx *ctx);
 const char *se2p_last_error(const se2p_ctx *ctx);
 /* Use a caller-provided hipStream_t (passed as void*); NULL restores the
- * context's own stream. */
+ * context's own stream.  Work enqueued afterwards waits (by an event, without
+ * blocking the host) for the work already enqueued on the previous stream,
+ * since both use the context's workspace; while either stream is being
+ * captured into a graph the caller orders them instead. */
 int se2p_set_stream(se2p_ctx *ctx, void *hip_stream);
 void *se2p_get_stream(se2p_ctx *ctx);
 int se2p_set_async(se2p_ctx *ctx, int async_flag);
@@ -109,6 +113,21 @@ int se2p_set_keep(se2p_ctx *ctx, int enable);
 #define SE2P_SPREAD_CHUNK_MIN 64
 #define SE2P_SPREAD_CHUNK_MAX 2048
 int se2p_set_spread_chunk(se2p_ctx *ctx, int chunk);
+/* Graph capture.  In async mode a call that finds its workspace, rocFFT
+ * plans and kept state already sized neither allocates nor synchronises, so
+ * it can be captured in a HIP graph.  A captured graph holds the device
+ * addresses of the context's workspace, stream-slot workspaces, kept state
+ * and rocFFT work buffers: it is valid only while those stay where they
+ * are.  A later call that needs a larger workspace, a new grid size (new
+ * plans), se2p_set_keep(ctx, 0), a change of se2p_set_spread_chunk and
+ * se2p_destroy all move or free them.  With enable != 0 the context is
+ * frozen: every call or setting that would allocate or free device memory
+ * fails with SE2P_ERR_FROZEN instead (the outputs of such a call are
+ * undefined; nothing is freed).  Freeze after
+ * the warm-up calls that size the context and while graphs captured from it
+ * exist; se2p_destroy is not guarded.  Default 0.  No reference
+ * counterpart. */
+int se2p_set_frozen(se2p_ctx *ctx, int enable);
 /* Per-stage device timing with HIP events on the context stream.  When
  * enabled, se2p_stage_times fills up to n floats (milliseconds) for the
  * most recent call: sort, real, spread, fft_fwd, filter, fft_inv, interp

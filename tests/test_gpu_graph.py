@@ -98,3 +98,83 @@ def test_multibox_graph_replays_match_eager(gpu):
         assert all(torch.equal(u, r) for u, r in zip(eager, ref)), rep
     print(f"[multibox graph] {len(boxes)} boxes, 4 replays bitwise equal", flush=True)
     ctx.set_async(False)
+
+
+def test_frozen_context_refuses_reallocation(gpu):
+    """se2p_set_frozen: after the warm-up, the same calls still run, while a
+    call that would grow the kept state, plan a new grid or free the kept
+    state fails with SE2P_ERR_FROZEN."""
+    from fasttools2d_amd import _capi
+
+    pe = gpu
+    dev = torch.device("cuda", 0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    w = W.uniform_box(2000, seed=9)
+    p = Pm.select_params("dlp", 2000, 2000, w.f, 1.0, 1.0, tol=1e-10, M=64)
+    ps, f, nv = t(w.psrc), t(w.f), t(w.n)
+    big = W.uniform_box(20000, seed=10)
+    pb = Pm.select_params("dlp", 20000, 20000, big.f, 1.0, 1.0, tol=1e-10, M=128)
+    bs, bf, bn = t(big.psrc), t(big.f), t(big.n)
+    ctx = pe.Context(0)
+    ctx.set_keep(True)
+    ur = pe.mex_stokes_dlp_real(ps, ps, f, nv, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, ctx=ctx).clone()
+    uk = pe.mex_stokes_dlp_kspace(ps, ps, p.xi, p.eta, f, nv, p.Mx, p.My, 1.0, 1.0, p.w, p.P,
+                                  ctx=ctx).clone()
+    ctx.set_frozen(True)
+    again = pe.mex_stokes_dlp_real(ps, ps, f, nv, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, ctx=ctx)
+    assert torch.equal(again, ur)
+    with pytest.raises(_capi.Se2pError) as e:
+        pe.mex_stokes_dlp_real(bs, bs, bf, bn, pb.xi, pb.nside_x, pb.nside_y, 1.0, 1.0, ctx=ctx)
+    assert e.value.code == _capi.SE2P_ERR_FROZEN
+    with pytest.raises(_capi.Se2pError) as e:
+        pe.mex_stokes_dlp_kspace(ps, ps, p.xi, p.eta, f, nv, 2 * p.Mx, 2 * p.My, 1.0, 1.0,
+                                 p.w / 2, p.P, ctx=ctx)
+    assert e.value.code == _capi.SE2P_ERR_FROZEN
+    with pytest.raises(_capi.Se2pError) as e:
+        ctx.set_keep(False)
+    assert e.value.code == _capi.SE2P_ERR_FROZEN
+    # the sized calls still run and agree with the warm-up
+    again = pe.mex_stokes_dlp_kspace(ps, ps, p.xi, p.eta, f, nv, p.Mx, p.My, 1.0, 1.0, p.w, p.P,
+                                     ctx=ctx)
+    assert torch.equal(again, uk)
+    ctx.set_frozen(False)
+    ctx.set_keep(False)
+    out = pe.mex_stokes_dlp_real(bs, bs, bf, bn, pb.xi, pb.nside_x, pb.nside_y, 1.0, 1.0, ctx=ctx)
+    assert torch.isfinite(out).all()
+
+
+def test_stream_switch_orders_calls(gpu):
+    """Calls on one context bound to two streams in turn, async: the call on
+    the second stream waits for the first (both use the context's workspace),
+    so both results equal synchronous calls."""
+    pe = gpu
+    dev = torch.device("cuda", 0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    wa = W.uniform_box(60000, seed=11)
+    pa = Pm.select_params("dlp", 60000, 60000, wa.f, 1.0, 1.0, tol=1e-10, M=256)
+    wb = W.uniform_box(30000, seed=12)
+    pb = Pm.select_params("dlp", 30000, 30000, wb.f, 1.0, 1.0, tol=1e-10, M=128)
+    A = [t(a) for a in (wa.psrc, wa.f, wa.n)]
+    B = [t(a) for a in (wb.psrc, wb.f, wb.n)]
+    ref = pe.Context(0)
+    ra = pe.mex_stokes_dlp_real(A[0], A[0], A[1], A[2], pa.xi, pa.nside_x, pa.nside_y, 1.0, 1.0, ctx=ref)
+    rb = pe.mex_stokes_dlp_real(B[0], B[0], B[1], B[2], pb.xi, pb.nside_x, pb.nside_y, 1.0, 1.0, ctx=ref)
+    torch.cuda.synchronize()
+    ctx = pe.Context(0)
+    s1, s2 = torch.cuda.Stream(dev), torch.cuda.Stream(dev)
+    ua = torch.empty_like(ra)
+    ub = torch.empty_like(rb)
+    for rep in range(3):
+        ua.zero_()
+        ub.zero_()
+        torch.cuda.synchronize()
+        ctx.set_async(True)
+        with torch.cuda.stream(s1):
+            pe.mex_stokes_dlp_real(A[0], A[0], A[1], A[2], pa.xi, pa.nside_x, pa.nside_y, 1.0,
+                                   1.0, ctx=ctx, out=ua)
+        with torch.cuda.stream(s2):
+            pe.mex_stokes_dlp_real(B[0], B[0], B[1], B[2], pb.xi, pb.nside_x, pb.nside_y, 1.0,
+                                   1.0, ctx=ctx, out=ub)
+        torch.cuda.synchronize()
+        ctx.set_async(False)
+        assert torch.equal(ua, ra) and torch.equal(ub, rb), rep

@@ -83,3 +83,37 @@ def test_host_pointer_calls_match_device_calls(gpu):
         assert np.all(np.isfinite(u_host)) and np.abs(u_host).max() > 0
     finally:
         lib.se2p_destroy(h)
+
+
+def test_host_pointer_same_array_with_more_targets_than_sources(gpu):
+    """psrc and ptar the same host array with Nt > Ns (the first Ns points are
+    the sources, all Nt points the targets): the array is staged once, at the
+    larger of the two sizes, and the results match device-pointer calls."""
+    pe = gpu
+    w = W.uniform_box(1200, seed=17)
+    ns, nt = 500, 1200
+    pts = np.ascontiguousarray(w.psrc)
+    f, nv = np.ascontiguousarray(w.f[:ns]), np.ascontiguousarray(w.n[:ns])
+    p = Pm.select_params("dlp", ns, nt, f, 1.0, 1.0, tol=1e-10, M=64)
+    dev = torch.device("cuda", 0)
+    t = lambda a: torch.from_numpy(a).to(dev)
+    dp = t(pts)
+    ref_r = pe.mex_stokes_dlp_real(dp[:ns], dp, t(f), t(nv), p.xi, p.nside_x, p.nside_y, 1.0, 1.0).cpu().numpy()
+    ref_k = pe.mex_stokes_dlp_kspace(dp[:ns], dp, p.xi, p.eta, t(f), t(nv), p.Mx, p.My, 1.0, 1.0,
+                                     p.w, p.P).cpu().numpy()
+    torch.cuda.synchronize()
+    lib = _capi.load()
+    h = ctypes.c_void_p()
+    assert lib.se2p_create(ctypes.byref(h), 0) == 0
+    try:
+        assert lib.se2p_set_host_pointers(h, 1) == 0
+        u = np.zeros((nt, 2))
+        assert lib.se2p_dlp_real(h, _h(pts), ns, _h(pts), nt, _h(f), _h(nv), p.xi, p.nside_x,
+                                 p.nside_y, 1.0, 1.0, _h(u)) == 0, lib.se2p_last_error(h)
+        assert np.array_equal(u, ref_r)
+        u = np.zeros((nt, 2))
+        assert lib.se2p_dlp_kspace(h, _h(pts), ns, _h(pts), nt, p.xi, p.eta, _h(f), _h(nv), p.Mx,
+                                   p.My, 1.0, 1.0, p.w, p.P, _h(u)) == 0, lib.se2p_last_error(h)
+        assert np.array_equal(u, ref_k)
+    finally:
+        lib.se2p_destroy(h)

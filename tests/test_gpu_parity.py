@@ -171,3 +171,25 @@ def test_device_sort_is_stable_counting_order(gpu):
         assert np.array_equal(perm.cpu().numpy(), np.argsort(keys, kind="stable"))
         ref_off = np.concatenate([[0], np.cumsum(np.bincount(keys, minlength=nkeys))])
         assert np.array_equal(off.cpu().numpy(), ref_off)
+
+
+@pytest.mark.parametrize("kind", ["slp", "dlp"])
+def test_truncation_rule_cases(gpu, kind):
+    """The hand-built truncation cases of the oracle pins (own-cell pairs
+    beyond the cutoff, neighbour pairs at 0.99, 1.01 and exactly 1 cutoff,
+    image revisits at nside 1 and 2, coincident pairs) through the HIP path,
+    against the closed-form pair sums over the hand-listed pairs."""
+    from tests.test_oracle_pins import (TRUNCATION_CASES, truncation_case_arrays,
+                                        truncation_expected)
+    pe = gpu
+    for c in TRUNCATION_CASES:
+        ps, pt, f, nv = truncation_case_arrays(c)
+        L, ns, xi = c["L"], c["nside"], c["xi"]
+        _say(f"[truncation {kind}] {c['name']}")
+        if kind == "slp":
+            u = pe.mex_stokes_slp_real(_dev(ps), _dev(pt), _dev(f), xi, ns, ns, L, L).cpu().numpy()
+        else:
+            u = pe.mex_stokes_dlp_real(_dev(ps), _dev(pt), _dev(f), _dev(nv), xi, ns, ns, L,
+                                       L).cpu().numpy()
+        ref = truncation_expected(kind, c, ps, pt, f, nv)
+        assert np.abs(u - ref).max() <= 1e-13 * np.abs(ref).max(), (c["name"], u, ref)

@@ -15,7 +15,15 @@ Each check is chosen so that a plausible mistake fails it:
   wavenumber convention, scaling, Re/Im);
 * the stresslet identity of PeriodicEwald/tests/stresslet_identity_test.m;
 * the invariants of PeriodicEwald/tests/consistency_test.m (independence of
-  the cell count Nb, periodic replication of the box).
+  the cell count Nb, periodic replication of the box);
+* the real-space truncation rule itself, on tiny hand-built cases whose
+  included (source, image) pairs are listed by hand from
+  mex_stokes_slp_real.cpp:150-151,233-264 and mex_stokes_dlp_real.cpp:113-123,
+  143-167: own-cell pairs beyond the cutoff (included), neighbour pairs just
+  inside, just outside and exactly at it (strict <), the image revisits at
+  nside = 1 and 2, and the coincident pair (Stokeslet self term; skipped by
+  the stresslet).  The expected sums add the closed-form pair terms of
+  stokes_{slp,dlp}_real_ds.m over exactly those pairs.
 """
 import mpmath
 import numpy as np
@@ -256,3 +264,105 @@ def test_oracle_against_direct_ewald_truth_at_benchmark_size():
     assert err_r + err_k < 1e-10
     assert err_k < 1e-13
     assert 1e-13 < err_r   # the cutoff truncation is visible, not hidden
+
+
+# ---- the truncation rule on hand-built cases --------------------------------
+# Each case: square box side L, nside x nside cells, splitting xi, sources
+# with densities and unit normals, targets, and per target the pairs the MEX
+# rule includes, listed by hand: (source index, image offset in units of L),
+# or ("self", source index) for a coincident pair.  The rule: every pair of
+# the target's own cell, no cutoff (mex_stokes_slp_real.cpp:150-151); the
+# eight neighbour cells in the reference's order with periodic offset
+# L*(q - wrap(q))/nside, only if r^2 < L^2/nside^2 (strict, :233-264;
+# mex_stokes_dlp_real.cpp:143-167); r^2 == 0 gives the Stokeslet self term and
+# is skipped by the stresslet (mex_stokes_dlp_real.cpp:118).
+def _unit(a):
+    return [np.cos(a), np.sin(a)]
+
+
+TRUNCATION_CASES = [
+    dict(name="own cell pair at 1.23 rc is included (nside 3)", L=1.0, nside=3, xi=3.0,
+         src=[(-0.19, -0.19)], tar=[(-0.48, -0.48)],
+         pairs=[[(0, (0, 0))]]),
+    dict(name="neighbour pairs at 0.99 rc (in), 1.01 rc (out) and exactly rc (out)",
+         L=3.0, nside=3, xi=1.0,   # cells of side 1: cut2 = 3*3/3/3 = 1 exactly
+         src=[(-0.26, 0.2), (-0.24, 0.2), (-0.25, 0.2)], tar=[(-1.25, 0.2)],
+         pairs=[[(0, (0, 0))]]),
+    dict(name="nside 1: own pair plus the two images within L; coincident point",
+         L=1.0, nside=1, xi=1.5,
+         # dyadic coordinates: every image difference is exact in either
+         # kernel's arithmetic, so the coincident target's four axis images
+         # sit at r^2 == L^2 exactly and are excluded
+         src=[(0.25, 0.125)], tar=[(-0.1, -0.05), (0.25, 0.125)],
+         pairs=[[(0, (0, 0)), (0, (-1, 0)), (0, (0, -1))], [("self", 0)]]),
+    dict(name="nside 2: a neighbour cell visited twice, at two offsets", L=1.0, nside=2, xi=4.0,
+         src=[(0.45, -0.2), (0.45, 0.45)], tar=[(-0.45, -0.2)],
+         pairs=[[(0, (-1, 0)), (1, (-1, -1))]]),
+    dict(name="coincident pair and an own-cell pair (nside 3)", L=1.0, nside=3, xi=3.0,
+         src=[(0.05, 0.05), (0.1, -0.1)], tar=[(0.05, 0.05)],
+         pairs=[[("self", 0), (1, (0, 0))]]),
+]
+
+
+def truncation_case_arrays(c):
+    """Arrays of a truncation case: psrc, ptar, f, n (seeded densities)."""
+    rng = np.random.default_rng(len(c["src"]) * 7 + c["nside"])
+    ps = np.array(c["src"], dtype=np.float64)
+    pt = np.array(c["tar"], dtype=np.float64)
+    f = rng.standard_normal(ps.shape)
+    nv = np.array([_unit(a) for a in rng.uniform(0, 2 * np.pi, len(ps))])
+    return ps, pt, f, nv
+
+
+def truncation_expected(kind, c, ps, pt, f, nv):
+    """Sum of isolated closed-form pair terms over the listed pairs: the
+    direct sums of stokes_{slp,dlp}_real_ds.m on one source and one target
+    in a box so large that no other image contributes."""
+    big = 1e4
+    out = np.zeros((len(pt), 2))
+    for t, plist in enumerate(c["pairs"]):
+        for item in plist:
+            if item[0] == "self":
+                s, z = item[1], (0, 0)
+            else:
+                s, z = item
+            xs = ps[s:s + 1] + np.array(z, dtype=np.float64) * c["L"]
+            if kind == "slp":
+                out[t] += ds.slp_real_ds(xs, pt[t:t + 1], f[s:s + 1], big, big, c["xi"])[0]
+            else:
+                out[t] += ds.dlp_real_ds(xs, pt[t:t + 1], f[s:s + 1], nv[s:s + 1], big, big, c["xi"])[0]
+    return out
+
+
+def test_truncation_cases_are_as_described():
+    """The hand-listed pairs rest on the cell of each point and the pair
+    distances; check those facts directly."""
+    def cell(x, y, L, ns):
+        return (min(max(int(np.floor(ns * (x / L + 0.5))), 0), ns - 1),
+                min(max(int(np.floor(ns * (y / L + 0.5))), 0), ns - 1))
+    c0, c1, c2, c3, _ = TRUNCATION_CASES
+    assert cell(*c0["src"][0], 1.0, 3) == cell(*c0["tar"][0], 1.0, 3)
+    r = np.hypot(*np.subtract(c0["src"][0], c0["tar"][0]))
+    assert 1.2 < r * 3 < 1.4
+    t = c1["tar"][0]
+    assert cell(*t, 3.0, 3) == (0, 1)
+    d2 = [np.subtract(s, t)[0] ** 2 for s in c1["src"]]
+    assert all(cell(*s, 3.0, 3) == (1, 1) for s in c1["src"])
+    assert d2[0] < 1.0 < d2[1] and d2[2] == 1.0
+    assert cell(*c3["tar"][0], 1.0, 2) == (0, 0)
+    assert cell(*c3["src"][0], 1.0, 2) == (1, 0) and cell(*c3["src"][1], 1.0, 2) == (1, 1)
+
+
+@pytest.mark.parametrize("kind", ["slp", "dlp"])
+@pytest.mark.parametrize("ci", range(len(TRUNCATION_CASES)))
+def test_oracle_truncation_rule(kind, ci):
+    c = TRUNCATION_CASES[ci]
+    ps, pt, f, nv = truncation_case_arrays(c)
+    L, ns, xi = c["L"], c["nside"], c["xi"]
+    if kind == "slp":
+        u = cref.slp_real(ps, pt, f, xi, ns, ns, L, L)
+    else:
+        u = cref.dlp_real(ps, pt, f, nv, xi, ns, ns, L, L)
+    ref = truncation_expected(kind, c, ps, pt, f, nv)
+    assert np.abs(ref).max() > 1e-3, c["name"]
+    assert np.abs(u - ref).max() <= 1e-13 * np.abs(ref).max(), (c["name"], u, ref)

@@ -254,3 +254,70 @@ def test_struct_layouts_match_header(tmp_path):
     want = [ctypes.sizeof(_capi.Se2pParams), ctypes.sizeof(_capi.Se2pBoxDesc),
             _capi.Se2pBoxDesc.Lx.offset, _capi.Se2pBoxDesc.Mx.offset, _capi.Se2pBoxDesc.u.offset]
     assert got == want, (got, want)
+
+
+def test_cell_count_overflow_rejected_by_every_real_entry(hctx):
+    """nside_x * nside_y * 16 (cell and 4x4 sub-cell keys) must fit in a
+    32-bit int; every real-space entry refuses more cells before any device
+    work, the backward entries included."""
+    lib, h = hctx
+    x = _fake(1)
+    L = 1.0
+    for ns in (16384, 11586):
+        assert lib.se2p_slp_real(h, x, 4, x, 4, x, 10.0, ns, ns, L, L, x) == _capi.SE2P_ERR_NSIDE
+        assert lib.se2p_dlp_real(h, x, 4, x, 4, x, x, 10.0, ns, ns, L, L, x) == _capi.SE2P_ERR_NSIDE
+        assert lib.se2p_slp_real_backward(h, x, 4, x, 3, x, 10.0, ns, ns, L, L, x, x, x, x) == \
+            _capi.SE2P_ERR_NSIDE
+        assert lib.se2p_dlp_real_backward(h, x, 4, x, 3, x, x, 10.0, ns, ns, L, L, x, x, x, x, x) == \
+            _capi.SE2P_ERR_NSIDE
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        assert lib.se2p_count_pairs(h, x, 4, x, 4, ns, ns, L, L, ctypes.byref(a), ctypes.byref(b)) == \
+            _capi.SE2P_ERR_NSIDE
+        assert b"32-bit" in lib.se2p_last_error(h)
+    # the largest square cell grid that fits reaches the device step
+    assert lib.se2p_slp_real_backward(h, x, 4, x, 3, x, 10.0, 11585, 11585, L, L, x, x, x, x) == \
+        _capi.SE2P_ERR_HIP
+
+
+def test_frozen_flag_setter(hctx):
+    lib, h = hctx
+    assert lib.se2p_set_frozen(h, 1) == 0 and lib.se2p_set_frozen(h, 0) == 0
+    assert lib.se2p_set_frozen(None, 1) == _capi.SE2P_ERR_ARG
+
+
+def test_python_out_argument_checked():
+    """out= is written through its data pointer by the library, so anything but
+    a contiguous float64 (Nt, 2) tensor on the inputs' device is refused."""
+    import torch
+
+    from fasttools2d_amd import periodic_ewald as pe
+
+    dev = torch.device("cpu")
+    good = torch.empty((5, 2), dtype=torch.float64)
+    assert pe._output(good, 5, dev) is good
+    assert pe._output(None, 5, dev).shape == (5, 2)
+    bad = [torch.empty((5, 2), dtype=torch.float32), torch.empty((4, 2), dtype=torch.float64),
+           torch.empty((2, 5), dtype=torch.float64).t(), torch.empty((10,), dtype=torch.float64),
+           np.zeros((5, 2))]
+    for b in bad:
+        with pytest.raises(ValueError):
+            pe._output(b, 5, dev)
+    with pytest.raises(ValueError, match="out must be on"):
+        pe._output(good, 5, torch.device("meta"))
+
+
+def test_python_driver_and_autotune_check_density_sizes():
+    """The drivers and the autotuner refuse densities or normals whose length
+    differs from the sources' before anything reaches the device."""
+    from fasttools2d_amd import periodic_ewald as pe
+
+    x = np.zeros(10)
+    with pytest.raises(ValueError, match="psrc and f must be the same size"):
+        pe.StokesSLP_ewald_2p(x, x, x, x, np.zeros(9), np.zeros(9), 1.0, 1.0)
+    with pytest.raises(ValueError, match="psrc and n must be the same size"):
+        pe.StokesDLP_ewald_2p(x, x, x, x, np.zeros(11), np.zeros(11), x, x, 1.0, 1.0)
+    ps = np.zeros((10, 2))
+    with pytest.raises(ValueError, match="psrc and f must be the same size"):
+        pe.autotune("slp", ps, ps, np.zeros((8, 2)), None, 1.0, 1.0, tol=1e-8)
+    with pytest.raises(ValueError, match="psrc and n must be the same size"):
+        pe.autotune("dlp", ps, ps, np.zeros((10, 2)), np.zeros((3, 2)), 1.0, 1.0, tol=1e-8)
