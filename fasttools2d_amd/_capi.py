@@ -49,6 +49,7 @@ SIGNATURES = {
     "se2p_slp_real": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _I, _I, _D, _D, _P]),
     "se2p_slp_kspace": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _I, _I, _D, _D, _D, _I, _P]),
     "se2p_dlp_real": (_I, [_P, _P, _I64, _P, _I64, _P, _P, _D, _I, _I, _D, _D, _P]),
+    "se2p_dlp_real_noalloc": (_I, [_P, _P, _I64, _P, _I64, _P, _P, _D, _I, _I, _D, _D, _P]),
     "se2p_dlp_kspace": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _P, _I, _I, _D, _D, _D, _I, _P]),
     "se2p_slp_real_batched": (_I, [_P, _P, _I64, _P, _I64, _P, _I, _I64, _D, _I, _I, _D, _D,
                                    _P, _I64]),
@@ -76,6 +77,7 @@ SIGNATURES = {
 }
 
 SE2P_HOST_ONLY = -1
+SE2P_NOALLOC_SCRATCH_BYTES = 32 * 1024 * 1024
 
 class Se2pParams(ctypes.Structure):
     """se2p_params of include/se2p.h."""

@@ -302,6 +302,10 @@ int se2p_create(se2p_ctx **out, int device)
     for (auto &stage : ctx->ev)
         for (auto &visit : stage)
             for (hipEvent_t &e : visit) hipEventCreate(&e);
+    // this device's address of the no-allocation entry's fixed scratch; on
+    // failure that entry reports it and every other entry works as before
+    if (noalloc_scratch_init(ctx) != 0) ctx->scratch = nullptr;
+    ctx->err.clear();
     *out = ctx;
     return SE2P_OK;
 }
@@ -514,6 +518,36 @@ int se2p_dlp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *p
     RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
     if (int rc = real_sum(ctx, 1, g, f, n, 1, 2 * ns, 2 * nt, u, nullptr)) return rc;
     return io.done(finish(ctx));
+}
+
+int se2p_dlp_real_noalloc(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                          int64_t nt, const double *f, const double *n, double xi, int nside_x,
+                          int nside_y, double Lx, double Ly, double *u)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
+    if (ns > 0 && (!f || !n)) return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
+    if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
+    if (ctx->host_ptrs)
+        return fail(ctx, SE2P_ERR_ARG,
+                    "se2p_dlp_real_noalloc takes device pointers only (host-pointer mode "
+                    "stages arrays in allocated memory)");
+    RealGeom g{psrc, ptar, ns, nt, xi, Lx, Ly, nside_x, nside_y};
+    size_t need = 0;
+    if (int rc = real_scratch_bytes(ctx, 1, g, &need)) return rc;
+    if (need > (size_t)SE2P_NOALLOC_SCRATCH_BYTES)
+        return fail(ctx, SE2P_ERR_ARG,
+                    "se2p_dlp_real_noalloc: Ns = %lld, Nt = %lld on %d x %d cells need %zu bytes "
+                    "of scratch; the fixed scratch holds %zu (use se2p_dlp_real)",
+                    (long long)ns, (long long)nt, nside_x, nside_y, need,
+                    (size_t)SE2P_NOALLOC_SCRATCH_BYTES);
+    if (int rc = set_device(ctx)) return rc;
+    if (!ctx->scratch)
+        return fail(ctx, SE2P_ERR_HIP, "no fixed scratch address on device %d", ctx->device);
+    if (int rc = real_sum_fixed(ctx, 1, g, f, n, u, ctx->scratch, SE2P_NOALLOC_SCRATCH_BYTES))
+        return rc;
+    return finish(ctx);
 }
 
 int se2p_slp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,

@@ -86,6 +86,9 @@ struct se2p_ctx {
     bool keep = false;        // forward calls keep their sorted state (se2p_set_keep)
     se2p::KeptReal *kept_real = nullptr;
     se2p::KeptK *kept_k = nullptr;
+    // This device's address of the fixed scratch of se2p_dlp_real_noalloc
+    // (looked up once per device at se2p_create; null if that failed).
+    char *scratch = nullptr;
     // se2p_set_frozen: calls that would allocate or free device memory fail
     // with SE2P_ERR_FROZEN instead (captured graphs keep their addresses)
     bool frozen = false;
@@ -214,9 +217,20 @@ int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const 
 // Real-space sum over several independent boxes (host records of the boxes;
 // se2p_real.hip fills in their global bases).
 struct RealBox;
+// fixed != null: the sorted state is carved from fixed[0..fixed_bytes) and
+// nothing is allocated (SE2P_ERR_ARG if it does not fit).
 int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, int64_t dstride,
-                   int64_t ustride, int64_t *pair_counts);
+                   int64_t ustride, int64_t *pair_counts, char *fixed = nullptr,
+                   size_t fixed_bytes = 0);
 int real_sum_multibox(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int nbox);
+// The no-allocation form of real_sum (one density): the bytes of sorted state
+// a call needs, and the call itself with that state in caller-given scratch.
+int real_scratch_bytes(se2p_ctx *ctx, int kind, const RealGeom &g, size_t *bytes);
+int real_sum_fixed(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
+                   double *u, char *scratch_base, size_t scratch_bytes);
+// Looks up this device's address of the fixed scratch into ctx->scratch
+// (se2p_real.hip, where the array lives); called by se2p_create.
+int noalloc_scratch_init(se2p_ctx *ctx);
 // Adjoint of real_sum (one right-hand side): with L = sum_t du_t . u_t, the
 // gradients dL/dpsrc, dL/dptar, dL/df and (stresslet) dL/dn; null outputs are
 // skipped.

@@ -45,6 +45,40 @@ namespace se2p {
 
 static __constant__ double c_e1_table[SE2P_E1_NPIECE * SE2P_E1_ROW];
 
+// ---- fixed scratch of the no-allocation entry point (se2p_dlp_real_noalloc)
+// The sorted state of that entry point (cell lists, source records, chunk
+// plans, run boxes) is carved from this array instead of the context's
+// workspace.  Each GPU holds its own copy from module load until the process
+// ends.  On the host the name is only a stand-in for the symbol, not a device
+// address: noalloc_scratch_init looks up each device's address once, with
+// that device current, and host code uses only that address afterwards.
+static __device__ __align__(256) char scratch[SE2P_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kScratchDevices = 64;
+
+int noalloc_scratch_init(se2p_ctx *ctx)
+{
+    static std::mutex mu;
+    static char *addr[kScratchDevices] = {};
+    std::lock_guard<std::mutex> lock(mu);
+    const int d = ctx->device;
+    if (d < 0 || d >= kScratchDevices)
+        return fail(ctx, SE2P_ERR_HIP, "no fixed scratch for device %d (devices 0..%d)", d,
+                    kScratchDevices - 1);
+    if (!addr[d]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();   // leave no sticky error for later calls
+            return fail(ctx, SE2P_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                        hipGetErrorString(e));
+        }
+        addr[d] = (char *)p;
+    }
+    ctx->scratch = addr[d];
+    return 0;
+}
+
 constexpr int kSubLog = 2;   // 4x4 sub-cells per cell for target ordering
 constexpr int kWaves = 4;
 
@@ -1593,7 +1627,7 @@ void free_kept_real(se2p_ctx *ctx)
 // kernel.  boxes[] are host records; their bases and derived parameters are
 // filled in here.
 int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, int64_t dstride,
-                   int64_t ustride, int64_t *pair_counts)
+                   int64_t ustride, int64_t *pair_counts, char *fixed, size_t fixed_bytes)
 {
     hipStream_t st = ctx->stream;
     RealPlan p;
@@ -1603,11 +1637,18 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
     if (kind == 0 && upload_table(ctx)) return SE2P_ERR_HIP;
     // a call whose backward may follow keeps its plan in a buffer of its own
     const bool keep = ctx->keep && !ctx->host_ptrs && nbox == 1 && nrhs == 1 && !counting &&
-                      p.same_points;
+                      p.same_points && !fixed;
     Carve sizing;
     real_layout(sizing, p);
     Carve cv;
-    if (keep) {
+    if (fixed) {
+        // caller-provided scratch: nothing is allocated
+        if (sizing.off > fixed_bytes)
+            return fail(ctx, SE2P_ERR_ARG,
+                        "the sorted state needs %zu bytes of scratch; the fixed scratch holds %zu",
+                        sizing.off, fixed_bytes);
+        cv.base = fixed;
+    } else if (keep) {
         if (!ctx->kept_real) ctx->kept_real = new KeptReal;
         KeptReal &kr = *ctx->kept_real;
         kr.valid = false;
@@ -1759,6 +1800,25 @@ int real_sum(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const 
 {
     RealBox b = one_box(g, f, n, u);
     return real_sum_boxes(ctx, kind, &b, 1, nrhs, dstride, ustride, pair_counts);
+}
+
+int real_scratch_bytes(se2p_ctx *ctx, int kind, const RealGeom &g, size_t *bytes)
+{
+    RealBox b = one_box(g, nullptr, nullptr, nullptr);
+    RealPlan p;
+    if (int rc = real_plan(ctx, kind, &b, 1, 1, 2 * g.ns, &p)) return rc;
+    Carve sizing;
+    real_layout(sizing, p);
+    *bytes = g.nt > 0 ? sizing.off : 0;
+    return 0;
+}
+
+int real_sum_fixed(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, const double *n,
+                   double *u, char *scratch_base, size_t scratch_bytes)
+{
+    RealBox b = one_box(g, f, n, u);
+    return real_sum_boxes(ctx, kind, &b, 1, 1, 2 * g.ns, 2 * g.nt, nullptr, scratch_base,
+                          scratch_bytes);
 }
 
 // Adjoint when the targets are the sources: the forward's plan (kept by the

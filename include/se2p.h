@@ -152,6 +152,33 @@ int se2p_dlp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double 
                     int64_t nt, double xi, double eta, const double *f, const double *n,
                     int Mx, int My, double Lx, double Ly, double w, int P, double *u);
 
+/* ---- the stresslet real-space sum without allocation ----------------------- */
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * during a call: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Same arguments, kernels and (bitwise) results as
+ * se2p_dlp_real, but the call's sorted state (cell lists, source records,
+ * chunk plans, run boxes) lives in a fixed device array of
+ * SE2P_NOALLOC_SCRATCH_BYTES that the library holds on each GPU from its
+ * first se2p_create on that GPU until the process ends.  The call makes no
+ * hipMalloc, hipFree or other allocating call; the caller passes inputs and
+ * the output as device pointers.
+ *  - Limit: about 100 bytes per point (targets that are the sources; more
+ *    with separate targets) plus 36 bytes per (cell, 4x4 sub-cell) key, so
+ *    about 3.3e5 points on 22 x 22 cells.  Larger problems return
+ *    SE2P_ERR_ARG with the bytes needed in se2p_last_error; use
+ *    se2p_dlp_real for them.
+ *  - Host-pointer mode (se2p_set_host_pointers) is rejected with
+ *    SE2P_ERR_ARG: it stages arrays in allocated memory.
+ *  - Every call on one GPU, from any context, uses that GPU's one scratch
+ *    array, which keeps the last call's contents: calls that overlap in time
+ *    on one GPU (two threads, or two streams not ordered by the caller) race
+ *    and give wrong results.  Calls ordered on one stream are safe.
+ *  - Nothing is kept for a backward call (se2p_set_keep does not apply). */
+#define SE2P_NOALLOC_SCRATCH_BYTES (32u * 1024u * 1024u)
+int se2p_dlp_real_noalloc(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
+                          int64_t nt, const double *f, const double *n, double xi, int nside_x,
+                          int nside_y, double Lx, double Ly, double *u);
+
 /* ---- batched right-hand sides -------------------------------------------- */
 /* nrhs density vectors on one geometry (same sources, targets and, for the
  * DLP, normals).  Density r is at f + r*f_stride and result r at

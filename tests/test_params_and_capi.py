@@ -270,6 +270,8 @@ def test_cell_count_overflow_rejected_by_every_real_entry(hctx):
             _capi.SE2P_ERR_NSIDE
         assert lib.se2p_dlp_real_backward(h, x, 4, x, 3, x, x, 10.0, ns, ns, L, L, x, x, x, x, x) == \
             _capi.SE2P_ERR_NSIDE
+        assert lib.se2p_dlp_real_noalloc(h, x, 4, x, 4, x, x, 10.0, ns, ns, L, L, x) == \
+            _capi.SE2P_ERR_NSIDE
         a, b = ctypes.c_int64(), ctypes.c_int64()
         assert lib.se2p_count_pairs(h, x, 4, x, 4, ns, ns, L, L, ctypes.byref(a), ctypes.byref(b)) == \
             _capi.SE2P_ERR_NSIDE
@@ -277,6 +279,27 @@ def test_cell_count_overflow_rejected_by_every_real_entry(hctx):
     # the largest square cell grid that fits reaches the device step
     assert lib.se2p_slp_real_backward(h, x, 4, x, 3, x, 10.0, 11585, 11585, L, L, x, x, x, x) == \
         _capi.SE2P_ERR_HIP
+
+
+def test_noalloc_entry_validation(hctx):
+    """se2p_dlp_real_noalloc: the fixed scratch's size limit and the refusal of
+    host-pointer mode are checked before any device work."""
+    lib, h = hctx
+    x = _fake(1)
+    assert lib.se2p_dlp_real_noalloc(h, x, 400000, x, 400000, x, x, 100.0, 22, 22, 1.0, 1.0, x) == \
+        _capi.SE2P_ERR_ARG
+    msg = lib.se2p_last_error(h).decode()
+    need = int(re.search(r"need (\d+) bytes", msg).group(1))
+    assert need > _capi.SE2P_NOALLOC_SCRATCH_BYTES and "fixed scratch holds 33554432" in msg
+    # the benchmark's size fits (and then needs a device)
+    assert lib.se2p_dlp_real_noalloc(h, x, 100000, x, 100000, x, x, 100.0, 22, 22, 1.0, 1.0, x) == \
+        _capi.SE2P_ERR_HIP
+    assert lib.se2p_dlp_real_noalloc(h, x, 0, x, 0, None, None, 100.0, 22, 22, 1.0, 1.0, x) == \
+        _capi.SE2P_ERR_HIP
+    assert lib.se2p_set_host_pointers(h, 1) == 0
+    assert lib.se2p_dlp_real_noalloc(h, x, 10, x, 10, x, x, 10.0, 1, 1, 1.0, 1.0, x) == \
+        _capi.SE2P_ERR_ARG
+    assert b"device pointers only" in lib.se2p_last_error(h)
 
 
 def test_frozen_flag_setter(hctx):
