@@ -415,16 +415,23 @@ def _autotune_variant(args, pe, np, torch, w, ps, f, nv, stream, fwd_ms):
 
 def _at_scale_variant(args, pe, W, np, torch, dev):
     """The at-scale workload: N = 4e6 uniform points in the unit box (targets
-    = sources, seed 5), tol 1e-12 parameters from the driver formulas; the
-    Stokeslet on a 2048^2 grid, the stresslet on the driver's 3060^2 grid
-    (2048^2 is below its 2 k_inf at this tolerance).  Forward step (real +
-    Fourier sums) and forward + adjoint step (both backward entries, every
-    gradient, forward state kept), each timed over a few steps after a
-    warm-up."""
+    = sources, seed 5), tol 1e-12 on a 2048^2 grid for both kernels: the
+    grid is below the drivers' 2 k_inf at this tolerance, so xi and the cells
+    come from the fixed-grid mode (the k-space estimate inverted at
+    k_inf = 1024, the real-space estimate for the cells).  Forward step (real
+    + Fourier sums) and forward + adjoint step (both backward entries, every
+    gradient), with the forward state kept for the backward calls and
+    rebuilt by them, each timed over a few steps after a warm-up.  The error
+    of the forward total on 256 targets is measured against a tighter run
+    (tests/golden/truth_uniform_N4e6.json: the oracle at tol 1e-14, 4096^2
+    grid, the drivers' 121 x 121 cells)."""
     from fasttools2d_amd import params as Pm
 
     N = 4_000_000
     w = W.uniform_box(N, seed=5)
+    truth = json.load(open(os.path.join(ROOT, "tests", "golden", "truth_uniform_N4e6.json")))
+    sub = np.array(truth["targets"])
+    assert np.array_equal(sub, np.random.default_rng(5).choice(N, 256, replace=False))
     du = torch.from_numpy(np.random.default_rng(78).standard_normal((N, 2))).to(dev)
     ps = torch.from_numpy(np.ascontiguousarray(w.psrc)).to(dev)
     f = torch.from_numpy(np.ascontiguousarray(w.f)).to(dev)
@@ -432,8 +439,8 @@ def _at_scale_variant(args, pe, W, np, torch, dev):
     ur = torch.empty((N, 2), dtype=torch.float64, device=dev)
     uk = torch.empty_like(ur)
     out = {}
-    for kind, M in (("slp", 2048), ("dlp", None)):
-        p = Pm.select_params(kind, N, N, w.f, 1.0, 1.0, tol=1e-12, M=M)
+    for kind in ("slp", "dlp"):
+        p = Pm.select_params_grid(kind, N, N, w.f, 1.0, 1.0, M=2048, tol=1e-12)
         ctx = pe.Context(0)
         stream = torch.cuda.Stream(dev)
 
@@ -476,20 +483,33 @@ def _at_scale_variant(args, pe, W, np, torch, dev):
             ctx.bind_torch_stream(stream)
             ctx.set_async(True)
             f_ms = timed(fwd)
+            fr_ms = timed(fwd_adj)      # the backward calls rebuild the forward state
             ctx.set_keep(True)
-            fa_ms = timed(fwd_adj)
+            fa_ms = timed(fwd_adj)      # the backward calls reuse the kept state
             ctx.set_keep(False)
             ctx.set_async(False)
         torch.cuda.synchronize()
+        fwd()
+        u = (ur + uk).cpu().numpy()[sub]
+        torch.cuda.synchronize()
         ctx.close()
-        print(f"# at scale {kind}: N=4e6 grid {p.Mx}^2 nside {p.nside_x}: forward {f_ms:.2f} ms, "
-              f"forward+adjoint {fa_ms:.2f} ms", file=sys.stderr, flush=True)
+        ut = np.array(truth[kind]["u"])
+        err = float(np.abs(u - ut).max() / np.abs(ut).max())
+        print(f"# at scale {kind}: N=4e6 grid {p.Mx}^2 nside {p.nside_x} xi {p.xi:.2f}: forward "
+              f"{f_ms:.2f} ms, forward+adjoint {fa_ms:.2f} ms kept / {fr_ms:.2f} ms rebuilt; "
+              f"error vs tighter run {err:.2e}", file=sys.stderr, flush=True)
         out[kind] = {"forward_ms": f_ms, "forward_evals_per_s": N / (f_ms * 1e-3),
                      "forward_plus_adjoint_ms": fa_ms, "over_forward": fa_ms / f_ms,
-                     "params": {"Mx": p.Mx, "nside": p.nside_x, "xi": p.xi, "P": p.P,
-                                "tol": 1e-12}}
-    out["workload"] = ("N = 4e6 uniform points, unit box, targets = sources, tol 1e-12; "
-                       "Stokeslet grid 2048^2, stresslet grid 3060^2 (driver)")
+                     "forward_plus_adjoint_rebuilt_ms": fr_ms,
+                     "over_forward_rebuilt": fr_ms / f_ms,
+                     "error_vs_tighter_run": err,
+                     "params": {"Mx": p.Mx, "nside": p.nside_x, "xi": p.xi, "eta": p.eta,
+                                "P": p.P, "tol": 1e-12, "mode": "fixed grid"}}
+    out["workload"] = ("N = 4e6 uniform points, unit box, targets = sources, tol 1e-12, "
+                       "grid 2048^2 for both kernels (fixed-grid parameters)")
+    out["error_reference"] = ("forward total (real + Fourier) on 256 targets against the oracle "
+                              "at tol 1e-14 on a 4096^2 grid with the drivers' 121 x 121 cells "
+                              "(tests/golden/truth_uniform_N4e6.json)")
     return out
 
 

@@ -85,7 +85,7 @@ class Se2pParams(ctypes.Structure):
                 ("M", ctypes.c_int), ("nside_x", ctypes.c_int), ("nside_y", ctypes.c_int),
                 ("Mx", ctypes.c_int), ("My", ctypes.c_int), ("kinf", ctypes.c_int),
                 ("converged", ctypes.c_int), ("rc", ctypes.c_double), ("xi", ctypes.c_double),
-                ("eta", ctypes.c_double), ("w", ctypes.c_double)]
+                ("eta", ctypes.c_double), ("w", ctypes.c_double), ("fixed_grid", ctypes.c_int)]
 
 
 SIGNATURES.update({

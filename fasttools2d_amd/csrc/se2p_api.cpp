@@ -962,8 +962,8 @@ int se2p_select_params(int kind, int64_t ns, int64_t nt, double Q, double Lx, do
         !(Q >= 1) || !std::isfinite(Q))
         return SE2P_ERR_ARG;
     if (p->P < 0 || (p->P % 2) != 0) return SE2P_ERR_P;
-    select_params(kind, ns, nt, Q, Lx, Ly, p);
-    return SE2P_OK;
+    if (p->fixed_grid && (p->M < 2 || p->M % 2 != 0)) return SE2P_ERR_GRID;
+    return select_params(kind, ns, nt, Q, Lx, Ly, p);
 }
 
 static int ensure_driver_buffer(se2p_ctx *ctx, size_t bytes)
@@ -1014,6 +1014,8 @@ static int check_driver(se2p_ctx *ctx, int64_t ns, int64_t nt, double Lx, double
     if (!(Lx > 0) || !(Ly > 0) || !std::isfinite(Lx) || !std::isfinite(Ly))
         return fail(ctx, SE2P_ERR_PARAM, "Lx and Ly must be positive and finite");
     if (prm->P < 0 || (prm->P % 2) != 0) return fail(ctx, SE2P_ERR_P, "P must be even");
+    if (prm->fixed_grid && (prm->M < 2 || prm->M % 2 != 0))
+        return fail(ctx, SE2P_ERR_GRID, "fixed-grid mode needs an even grid M >= 2 (got %d)", prm->M);
     return 0;
 }
 
@@ -1052,7 +1054,9 @@ static int ewald_full(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, c
     double hs[3];
     SE2P_HIP(hipMemcpyAsync(hs, b.sums, sizeof(hs), hipMemcpyDeviceToHost, st));
     SE2P_HIP(hipStreamSynchronize(st));
-    select_params(kind, ns, nt, hs[0] + 1.0, Lx, Ly, prm);
+    if (select_params(kind, ns, nt, hs[0] + 1.0, Lx, Ly, prm) != 0)
+        return fail(ctx, SE2P_ERR_PARAM, "no parameters meet tol %g on the fixed %d grid", prm->tol,
+                    prm->M);
 
     launch_wrap(st, psrc, ns, Lx, Ly, b.src);
     if (!same) launch_wrap(st, ptar, nt, Lx, Ly, b.tar);

@@ -55,9 +55,69 @@ static bool newton(F f, DF df, double x0, double tol, double *x_out)
     return ok;
 }
 
+// Fixed-grid mode (p->fixed_grid, p->M > 0): the drivers' k-space estimate
+// inverted for xi at kinf = M/2, the real-space estimate inverted for the
+// cutoff at that xi, cells at least that wide.  Closed forms except the
+// stresslet's real-space estimate, x e^{-x^2} s = tol with x = xi rc, solved
+// by Newton on x^2 - ln x = ln(s / tol) on its decreasing branch.
+static int select_params_grid(int kind, int64_t ns, int64_t nt, double Q, double Lx, double Ly,
+                              se2p_params *p)
+{
+    const int P = p->P > 0 ? p->P : 24;
+    const double tol = p->tol > 0 ? p->tol : 1e-16;
+    const int Mb = p->M;
+    long long A, B;
+    mat_rat(Lx / Ly, &A, &B);
+    const double kinf = Mb / 2;
+    const double k = 2 * SE2P_PI * kinf / Lx;
+    double xi, rc_min;
+    bool ok = true;
+    if (kind == 0) {
+        const double c = 4 * Q * SE2P_PI * Lx / (Lx * Lx * Lx * Lx * Lx * Lx);
+        xi = k / (2 * std::sqrt(std::log(std::sqrt(c / k) / tol)));
+        rc_min = std::sqrt(std::log(std::sqrt(Q * SE2P_PI / (4 * Lx * Ly * xi)) / tol)) / xi;
+    } else {
+        const double c = 8 * SE2P_PI * Q * Lx / (Lx * Lx * Lx) * (Lx * Lx * Lx);
+        xi = k / (2 * std::sqrt(std::log(std::sqrt(k * c) / tol)));
+        const double b = std::log(std::sqrt(2 * SE2P_PI * Q / (Lx * Ly)) / tol);
+        double x = std::sqrt(std::max(b, 0.5)) + 1.0;
+        ok = false;
+        for (int it = 0; it < 100; ++it) {
+            const double g = x * x - std::log(x) - b;
+            const double xn = x - g / (2 * x - 1 / x);
+            if (std::fabs(xn - x) <= 1e-15 * x) {
+                x = xn;
+                ok = true;
+                break;
+            }
+            x = std::max(xn, 0.75);
+        }
+        rc_min = x / xi;
+    }
+    if (!(xi > 0) || !std::isfinite(xi) || !(rc_min > 0) || !std::isfinite(rc_min))
+        return SE2P_ERR_PARAM;   // tol too loose for this grid (estimate below tol everywhere)
+    const long long a = std::max(1LL, (long long)std::floor(Lx / (A * rc_min) * (1 + 1e-12)));
+    const double m = 0.95 * std::sqrt(SE2P_PI * P);
+    p->P = P;
+    p->tol = tol;
+    p->nside_x = (int)(a * A);
+    p->nside_y = (int)(a * B);
+    p->Nb = (double)(ns + nt) / ((double)p->nside_x * p->nside_y);
+    p->rc = Lx / p->nside_x;
+    p->xi = xi;
+    p->kinf = (int)kinf;
+    p->Mx = (int)(A * Mb);
+    p->My = (int)(B * Mb);
+    p->w = P * Lx / p->Mx / 2;
+    p->eta = std::pow(2 * xi * p->w / m, 2);
+    p->converged = ok ? 1 : 0;
+    return 0;
+}
+
 int select_params(int kind, int64_t ns, int64_t nt, double Q, double Lx, double Ly,
                   se2p_params *p)
 {
+    if (p->fixed_grid) return select_params_grid(kind, ns, nt, Q, Lx, Ly, p);
     const int P = p->P > 0 ? p->P : 24;
     const double npts = (double)(ns + nt);
     double Nb = p->Nb;

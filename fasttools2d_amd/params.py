@@ -68,8 +68,31 @@ def _newton(fun, dfun, x0, tol, what):
 
 
 # ---- Stokeslet (SLP) estimates, StokesSLP_ewald_2p.m:147-203 ------------------
+def slp_real_estimate(Q, Lx, Ly, xi, rc):
+    """Real-space truncation estimate of StokesSLP_ewald_2p.m:147-176."""
+    return math.sqrt(Q * math.pi / (4 * Lx * Ly * xi)) * math.exp(-xi * xi * rc * rc)
+
+
+def slp_kspace_estimate(Q, Lx, Ly, xi, k):
+    """k-space truncation estimate of StokesSLP_ewald_2p.m:178-203 at wavenumber k."""
+    c = 4 * Q * math.pi * max(Lx, Ly) / (Lx ** 3 * Ly ** 3)
+    return math.sqrt(c / k) * math.exp(-k * k / (4 * xi * xi))
+
+
+def dlp_real_estimate(Q, Lx, Ly, xi, rc):
+    """Real-space truncation estimate of StokesDLP_ewald_2p.m:171-198."""
+    return math.exp(-xi * xi * rc * rc) * xi * rc * math.sqrt(2 * math.pi * Q / (Lx * Ly))
+
+
+def dlp_kspace_estimate(Q, Lx, Ly, xi, k):
+    """k-space truncation estimate of StokesDLP_ewald_2p.m:200-225, with MATLAB's
+    precedence in sqrt(k*8*pi*Q*max(Lx,Ly)/Lx^3*Ly^3)."""
+    c = 8 * math.pi * Q * max(Lx, Ly) / Lx ** 3 * Ly ** 3
+    return math.sqrt(k * c) * math.exp(-k * k / (4 * xi * xi))
+
+
 def slp_find_xi(Q, Lx, Ly, rc, tol):
-    f = lambda x: math.sqrt(Q * math.pi / (4 * Lx * Ly * x)) * math.exp(-x * x * rc * rc) - tol
+    f = lambda x: slp_real_estimate(Q, Lx, Ly, x, rc) - tol
     fp = lambda x: (-math.sqrt(Q * math.pi / (4 * Lx * Ly)) * math.exp(-x * x * rc * rc)
                     * (0.5 * x ** -1.5 + 2 * rc * rc * x * math.sqrt(1 / x)))
     return _newton(f, fp, 4 / rc, tol, "xi")
@@ -77,7 +100,7 @@ def slp_find_xi(Q, Lx, Ly, rc, tol):
 
 def slp_find_kinfb(Q, Lx, Ly, xi, tol):
     c = 4 * Q * math.pi * max(Lx, Ly) / (Lx ** 3 * Ly ** 3)
-    f = lambda k: math.sqrt(c / k) * math.exp(-k * k / (4 * xi * xi)) - tol
+    f = lambda k: slp_kspace_estimate(Q, Lx, Ly, xi, k) - tol
     fp = lambda k: (-math.sqrt(c) * math.exp(-k * k / (4 * xi * xi))
                     * (0.5 * k ** -1.5 + math.sqrt(1 / k) * 2 * k / (4 * xi * xi)))
     k = _newton(f, fp, matlab_round(5 * xi), tol, "k")
@@ -87,7 +110,7 @@ def slp_find_kinfb(Q, Lx, Ly, xi, tol):
 # ---- stresslet (DLP) estimates, StokesDLP_ewald_2p.m:171-225 ------------------
 def dlp_find_xi(Q, Lx, Ly, rc, tol):
     s = math.sqrt(2 * math.pi * Q / (Lx * Ly))
-    f = lambda a: math.exp(-a * a * rc * rc) * a * rc * s - tol
+    f = lambda a: dlp_real_estimate(Q, Lx, Ly, a, rc) - tol
     fp = lambda a: (1 - 2 * a * a * rc * rc) * math.exp(-a * a * rc * rc) * s * rc
     return _newton(f, fp, 1 / rc, tol, "xi")
 
@@ -95,7 +118,7 @@ def dlp_find_xi(Q, Lx, Ly, rc, tol):
 def dlp_find_kinfb(Q, Lx, Ly, xi, tol):
     # MATLAB: sqrt(k*8*pi*Q*max(Lx,Ly)/Lx^3*Ly^3) -- division binds to Lx^3 only
     c = 8 * math.pi * Q * max(Lx, Ly) / Lx ** 3 * Ly ** 3
-    f = lambda k: math.sqrt(k * c) * math.exp(-k * k / (4 * xi * xi)) - tol
+    f = lambda k: dlp_kspace_estimate(Q, Lx, Ly, xi, k) - tol
     fp = lambda k: (math.sqrt(c) * math.exp(-k * k / (4 * xi * xi))
                     * (0.5 * k ** -0.5 - 2 * k ** 1.5 / (4 * xi * xi)))
     k = _newton(f, fp, matlab_round(5 * xi), tol, "k")
@@ -159,5 +182,85 @@ def select_params(kind: str, nsrc: int, ntar: int, f, Lx: float, Ly: float, *, P
     Mx = A * Mb
     w = P * Lx / Mx / 2
     eta = (2 * xi * w / m) ** 2
+    return EwaldParams(P=P, Nb=Nb, tol=tol, nside_x=nside_x, nside_y=nside_y, rc=rc, xi=xi,
+                       kinf=kinf, Mx=Mx, My=My, w=w, eta=eta, A=A, B=B)
+
+
+# ---- fixed-grid mode -----------------------------------------------------------
+# The drivers pick the cell size first (Nb points per cell), then xi from the
+# real-space estimate and the grid from the k-space estimate.  With the grid
+# given instead, the same two estimates are inverted the other way round: xi
+# so that the k-space estimate at the grid's k_inf = M/2 equals tol, then the
+# smallest cutoff rc whose real-space estimate at that xi is tol, and the
+# largest cell count whose cells are at least rc wide.
+def slp_xi_from_kinf(Q, Lx, kinf, tol):
+    """Inverse of the Stokeslet k-space estimate (StokesSLP_ewald_2p.m:178-203,
+    with Lx passed for both sides as the driver does) at k = 2 pi kinf / Lx."""
+    k = 2 * math.pi * kinf / Lx
+    c = 4 * Q * math.pi * max(Lx, Lx) / (Lx ** 3 * Lx ** 3)
+    return k / (2 * math.sqrt(math.log(math.sqrt(c / k) / tol)))
+
+
+def dlp_xi_from_kinf(Q, Lx, kinf, tol):
+    """Inverse of the stresslet k-space estimate (StokesDLP_ewald_2p.m:200-225,
+    the driver's Lx-twice and precedence forms) at k = 2 pi kinf / Lx."""
+    k = 2 * math.pi * kinf / Lx
+    c = 8 * math.pi * Q * max(Lx, Lx) / Lx ** 3 * Lx ** 3
+    return k / (2 * math.sqrt(math.log(math.sqrt(k * c) / tol)))
+
+
+def slp_rc_from_xi(Q, Lx, Ly, xi, tol):
+    """Inverse of the Stokeslet real-space estimate (StokesSLP_ewald_2p.m:147-176)."""
+    return math.sqrt(math.log(math.sqrt(Q * math.pi / (4 * Lx * Ly * xi)) / tol)) / xi
+
+
+def dlp_rc_from_xi(Q, Lx, Ly, xi, tol):
+    """Inverse of the stresslet real-space estimate (StokesDLP_ewald_2p.m:171-198):
+    x e^{-x^2} s = tol with x = xi rc, s = sqrt(2 pi Q / (Lx Ly)), on the
+    decreasing branch x > 1/sqrt(2) (Newton on x^2 - ln x = ln(s / tol))."""
+    b = math.log(math.sqrt(2 * math.pi * Q / (Lx * Ly)) / tol)
+    x = math.sqrt(max(b, 0.5)) + 1.0
+    for _ in range(100):
+        g = x * x - math.log(x) - b
+        x_new = x - g / (2 * x - 1 / x)
+        if abs(x_new - x) <= 1e-15 * x:
+            x = x_new
+            break
+        x = max(x_new, 0.75)
+    return x / xi
+
+
+def select_params_grid(kind: str, nsrc: int, ntar: int, f, Lx: float, Ly: float, *, M: int,
+                       P: int = 24, tol: float = 1e-16):
+    """Parameters for a given grid (fixed-grid mode; no reference counterpart,
+    built from the drivers' own estimates).  ``M`` is the base grid (Mx = A*M,
+    My = B*M for the box aspect A:B, as in the drivers); xi comes from the
+    k-space estimate at k_inf = M/2, the cutoff from the real-space estimate
+    at that xi, and nside_x = a*A, nside_y = a*B with the largest a whose cells
+    are at least the cutoff wide.  ``Nb`` reports the resulting points per
+    cell."""
+    M = int(M)
+    if M < 2 or M % 2:
+        raise ValueError("M must be even and >= 2")
+    A, B = matlab_rat(Lx / Ly)
+    f = np.asarray(f, dtype=np.float64)
+    Q = float(np.sum(f * f)) + 1.0
+    kinf = M // 2
+    if kind == "slp":
+        xi = slp_xi_from_kinf(Q, Lx, kinf, tol)
+        rc_min = slp_rc_from_xi(Q, Lx, Ly, xi, tol)
+    elif kind == "dlp":
+        xi = dlp_xi_from_kinf(Q, Lx, kinf, tol)
+        rc_min = dlp_rc_from_xi(Q, Lx, Ly, xi, tol)
+    else:
+        raise ValueError("kind must be 'slp' or 'dlp'")
+    a = max(1, int(math.floor(Lx / (A * rc_min) * (1 + 1e-12))))
+    nside_x, nside_y = a * A, a * B
+    rc = Lx / nside_x
+    m = 0.95 * math.sqrt(math.pi * P)
+    Mx, My = A * M, B * M
+    w = P * Lx / Mx / 2
+    eta = (2 * xi * w / m) ** 2
+    Nb = (nsrc + ntar) / (nside_x * nside_y)
     return EwaldParams(P=P, Nb=Nb, tol=tol, nside_x=nside_x, nside_y=nside_y, rc=rc, xi=xi,
                        kinf=kinf, Mx=Mx, My=My, w=w, eta=eta, A=A, B=B)

@@ -502,7 +502,7 @@ def mex_stokes_dlp_kspace_backward(psrc, ptar, xi, eta, f, n, Mx, My, Lx, Ly, w,
 
 
 # -- the reference drivers as single calls --------------------------------------
-def _driver(kind, psrc, ptar, f, n, Lx, Ly, P, Nb, tol, M, ctx, same):
+def _driver(kind, psrc, ptar, f, n, Lx, Ly, P, Nb, tol, M, ctx, same, fixed_grid=False):
     _same_size(psrc, f, "f")
     if kind == 1:
         _same_size(psrc, n, "n")
@@ -517,6 +517,7 @@ def _driver(kind, psrc, ptar, f, n, Lx, Ly, P, Nb, tol, M, ctx, same):
     prm.Nb = float(Nb) if Nb is not None else 0.0
     prm.tol = float(tol)
     prm.M = int(M) if M is not None else 0
+    prm.fixed_grid = 1 if fixed_grid else 0
     nt = pt.shape[0]
     u = torch.empty((nt, 2), dtype=torch.float64, device=dev)
     ur = torch.empty_like(u)
@@ -580,29 +581,32 @@ def _cols(*arrays):
 
 
 def StokesSLP_ewald_2p(xsrc, ysrc, xtar, ytar, f1, f2, Lx, Ly, *, P=24, Nb=None, tol=1e-16,
-                       M=None, ctx=None, return_params=False):
+                       M=None, ctx=None, return_params=False, fixed_grid=False):
     """[u1, u2, ur, uk, xi] of PeriodicEwald/src/matlab/StokesSLP_ewald_2p.m:1-138.
 
     Inputs are 1-D arrays (MATLAB column vectors); ur and uk are (Nt, 2)
     device tensors, u1/u2 device tensors.  ``M`` overrides the grid the
-    driver would choose; ``return_params`` appends the se2p_params used."""
+    driver would choose; with ``fixed_grid`` the grid M is fixed and xi and
+    the cells follow from it at ``tol`` (se2p_params.fixed_grid; no reference
+    counterpart).  ``return_params`` appends the se2p_params used."""
     same = (xsrc is xtar) and (ysrc is ytar)
     ps = _cols(xsrc, ysrc)
     pt = ps if same else _cols(xtar, ytar)
-    u, ur, uk, prm = _driver(0, ps, pt, _cols(f1, f2), None, Lx, Ly, P, Nb, tol, M, ctx, same)
+    u, ur, uk, prm = _driver(0, ps, pt, _cols(f1, f2), None, Lx, Ly, P, Nb, tol, M, ctx, same,
+                             fixed_grid)
     out = (u[:, 0], u[:, 1], ur, uk, prm.xi)
     return out + (prm,) if return_params else out
 
 
 def StokesDLP_ewald_2p(xsrc, ysrc, xtar, ytar, n1, n2, f1, f2, Lx, Ly, *, P=24, Nb=None, tol=1e-16,
-                       M=None, ctx=None, return_params=False):
+                       M=None, ctx=None, return_params=False, fixed_grid=False):
     """[u1, u2, ur, uk, xi] of PeriodicEwald/src/matlab/StokesDLP_ewald_2p.m:1-163
     (uk includes the k = 0 mode; u = -(ur + uk)).  Normals are used as given
-    (no renormalisation)."""
+    (no renormalisation).  ``M`` and ``fixed_grid`` as for StokesSLP_ewald_2p."""
     same = (xsrc is xtar) and (ysrc is ytar)
     ps = _cols(xsrc, ysrc)
     pt = ps if same else _cols(xtar, ytar)
     u, ur, uk, prm = _driver(1, ps, pt, _cols(f1, f2), _cols(n1, n2), Lx, Ly, P, Nb, tol, M, ctx,
-                             same)
+                             same, fixed_grid)
     out = (u[:, 0], u[:, 1], ur, uk, prm.xi)
     return out + (prm,) if return_params else out

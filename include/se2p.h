@@ -266,7 +266,16 @@ int se2p_dlp_kspace_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, cons
 /* Parameters of the spectral Ewald split.  Inputs left at 0 take the
  * reference driver's defaults: P = 24, Nb = min(P*round(log2(Ns+Nt)), (Ns+Nt)/4),
  * tol = 1e-16, grid M = min(2*kinf, 10000) (Mx = A*M, My = B*M with
- * [A,B] = rat(Lx/Ly)).  The other fields are outputs. */
+ * [A,B] = rat(Lx/Ly)).  The other fields are outputs.
+ *
+ * fixed_grid != 0 (input; no reference counterpart) with M > 0 given: the
+ * grid is fixed and the drivers' estimates are inverted the other way round:
+ * xi so that the k-space estimate at kinf = M/2 equals tol
+ * (StokesSLP_ewald_2p.m:178-203, StokesDLP_ewald_2p.m:200-225), then the
+ * smallest cutoff whose real-space estimate at that xi is tol (:147-176,
+ * :171-198), then the largest nside_x = a*A, nside_y = a*B whose cells are at
+ * least that cutoff wide.  Nb is ignored and returned as the resulting points
+ * per cell; rc is the cell side. */
 typedef struct se2p_params {
     int P;
     double Nb;
@@ -275,6 +284,7 @@ typedef struct se2p_params {
     int nside_x, nside_y, Mx, My, kinf;
     int converged;   /* 0 if a Newton solve hit its iteration cap (the reference warns) */
     double rc, xi, eta, w;
+    int fixed_grid;  /* input: 1 selects the fixed-grid mode above (needs M > 0) */
 } se2p_params;
 
 /* Parameter selection of StokesSLP_ewald_2p.m:27-104 (kind 0) or

@@ -344,3 +344,57 @@ def test_python_driver_and_autotune_check_density_sizes():
         pe.autotune("slp", ps, ps, np.zeros((8, 2)), None, 1.0, 1.0, tol=1e-8)
     with pytest.raises(ValueError, match="psrc and n must be the same size"):
         pe.autotune("dlp", ps, ps, np.zeros((10, 2)), np.zeros((3, 2)), 1.0, 1.0, tol=1e-8)
+
+
+@pytest.mark.parametrize("kind", ["slp", "dlp"])
+def test_fixed_grid_mode_inverts_the_driver_estimates(kind):
+    """Fixed-grid parameters (grid given, xi and cells derived): the drivers'
+    own Newton solves, run forward on the result, give back the grid (k_inf =
+    M/2) and a cutoff no wider than the cells; one more cell per side would
+    be narrower than the cutoff the estimate needs.  The C selection agrees
+    with the Python restatement."""
+    lib = _capi.load()
+    find_xi = Pm.slp_find_xi if kind == "slp" else Pm.dlp_find_xi
+    find_k = Pm.slp_find_kinfb if kind == "slp" else Pm.dlp_find_kinfb
+    est_r = Pm.slp_real_estimate if kind == "slp" else Pm.dlp_real_estimate
+    est_k = Pm.slp_kspace_estimate if kind == "slp" else Pm.dlp_kspace_estimate
+    for (N, Lx, Ly, tol, M, seed) in ((4000, 1.0, 1.0, 1e-12, 128, 1), (20000, 2.0, 1.0, 1e-10, 256, 2),
+                                      (1000, 1.0, 1.0, 1e-8, 64, 3)):
+        rng = np.random.default_rng(seed)
+        f = rng.standard_normal((N, 2))
+        Q = float(np.sum(f * f)) + 1
+        p = Pm.select_params_grid(kind, N, N, f, Lx, Ly, M=M, tol=tol)
+        A, B = Pm.matlab_rat(Lx / Ly)
+        assert (p.Mx, p.My) == (A * M, B * M)
+        # the k-space estimate (Lx passed twice, as the drivers do) is tol at
+        # k_inf = M/2, and the drivers' Newton solve lands on that grid (it
+        # stops anywhere the estimate is within [0, 2 tol], so one below)
+        k = 2 * np.pi * (M // 2) / Lx
+        assert abs(est_k(Q, Lx, Lx, p.xi, k) / tol - 1) < 1e-9
+        assert find_k(Q, Lx, Lx, p.xi, tol) in (M // 2 - 1, M // 2)
+        # the real-space estimate is at most tol at the cell side, and one more
+        # cell per side would make the cells narrower than the cutoff it needs
+        assert est_r(Q, Lx, Ly, p.xi, p.rc) <= tol * (1 + 1e-9)
+        a = p.nside_x // A
+        assert est_r(Q, Lx, Ly, p.xi, Lx / ((a + 1) * A)) > tol
+        assert find_xi(Q, Lx, Ly, p.rc, tol) <= p.xi * (1 + 1e-9)
+        prm = _capi.Se2pParams()
+        prm.tol, prm.M, prm.fixed_grid = tol, M, 1
+        assert lib.se2p_select_params(0 if kind == "slp" else 1, N, N, Q, Lx, Ly, ctypes.byref(prm)) == 0
+        assert (prm.nside_x, prm.nside_y, prm.Mx, prm.My, prm.kinf) == \
+            (p.nside_x, p.nside_y, p.Mx, p.My, p.kinf)
+        assert abs(prm.xi - p.xi) <= 1e-12 * p.xi and abs(prm.eta - p.eta) <= 1e-12 * p.eta
+        assert prm.converged == 1
+    prm = _capi.Se2pParams()
+    prm.tol, prm.M, prm.fixed_grid = 1e-10, 63, 1
+    assert lib.se2p_select_params(0, 100, 100, 200.0, 1.0, 1.0, ctypes.byref(prm)) == _capi.SE2P_ERR_GRID
+
+
+def test_fixed_grid_config5_values():
+    """N = 4e6 uniform points, tol 1e-12, grid 2048^2: the grid is below the
+    drivers' 2 k_inf (2440 and 3060), so xi drops and the cells grow."""
+    f = np.random.default_rng(5).standard_normal((4_000_000, 2))
+    s = Pm.select_params_grid("slp", 4_000_000, 4_000_000, f, 1.0, 1.0, M=2048, tol=1e-12)
+    d = Pm.select_params_grid("dlp", 4_000_000, 4_000_000, f, 1.0, 1.0, M=2048, tol=1e-12)
+    assert (s.nside_x, s.Mx) == (99, 2048) and abs(s.xi - 564.65) < 0.01 and s.eta < 1
+    assert (d.nside_x, d.Mx) == (80, 2048) and abs(d.xi - 498.92) < 0.01 and d.eta < 1
