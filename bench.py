@@ -183,6 +183,106 @@ def _batched_variant(args, pe, W, np, torch, dev, nrhs=64):
     return res
 
 
+def _timed_ms(torch, stream, fn, reps=3):
+    """Average milliseconds of fn() on `stream` (HIP events), after one warm-up."""
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _batched_ellipses_variant(args, pe, W, np, torch, dev, nrhs=64):
+    """Per-RHS throughput of 64 densities on the benchmark's stresslet
+    geometry (200 ellipses x 500 nodes, N = 1e5, grid 512^2, tol 1e-10), where
+    a plain call is compute-bound rather than launch-bound: 64 plain calls of
+    the two stresslet entry points against one call of each batched entry
+    point.  Densities: N(0,1) times the trapezoid weights."""
+    w, p = W.dlp_ellipses_config(seed=2, n_ell=200, n_nodes=500, tol=1e-10, M=512)
+    N = len(w.psrc)
+    wt = np.linalg.norm(w.f, axis=1, keepdims=True)
+    rng = np.random.default_rng(101)
+    F = torch.from_numpy(rng.standard_normal((nrhs, N, 2)) * wt[None]).to(dev)
+    ps = torch.from_numpy(np.ascontiguousarray(w.psrc)).to(dev)
+    nv = torch.from_numpy(np.ascontiguousarray(w.n)).to(dev)
+    ctx = pe.Context(0)
+    stream = torch.cuda.Stream(dev)
+    ur = torch.empty((N, 2), dtype=torch.float64, device=dev)
+    uk = torch.empty_like(ur)
+
+    def plain():
+        for r in range(nrhs):
+            pe.mex_stokes_dlp_real(ps, ps, F[r], nv, p.xi, p.nside_x, p.nside_y, 1.0, 1.0,
+                                   ctx=ctx, out=ur)
+            pe.mex_stokes_dlp_kspace(ps, ps, p.xi, p.eta, F[r], nv, p.Mx, p.My, 1.0, 1.0, p.w,
+                                     p.P, ctx=ctx, out=uk)
+
+    def batched():
+        pe.stokes_dlp_real_batched(ps, ps, F, nv, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, ctx=ctx)
+        pe.stokes_dlp_kspace_batched(ps, ps, p.xi, p.eta, F, nv, p.Mx, p.My, 1.0, 1.0, p.w, p.P,
+                                     ctx=ctx)
+
+    res = {}
+    with torch.cuda.stream(stream):
+        ctx.bind_torch_stream(stream)
+        ctx.set_async(True)
+        for name, fn in (("plain", plain), ("batched", batched)):
+            ms = _timed_ms(torch, stream, fn)
+            res[name] = {"ms_per_call_set": ms, "target_evals_per_s": N * nrhs / (ms * 1e-3)}
+        ctx.set_async(False)
+    torch.cuda.synchronize()
+    ctx.close()
+    res["batched_over_plain"] = res["batched"]["target_evals_per_s"] / res["plain"]["target_evals_per_s"]
+    res["workload"] = (f"stresslet, 200 ellipses x 500 nodes (N=1e5), grid 512^2, tol 1e-10, "
+                       f"{nrhs} densities (real + Fourier sums)")
+    print(f"# batched on the ellipses: plain {res['plain']['ms_per_call_set']:.2f} ms, batched "
+          f"{res['batched']['ms_per_call_set']:.2f} ms ({res['batched_over_plain']:.2f}x per density)",
+          file=sys.stderr, flush=True)
+    return res
+
+
+def _fixed_grid_variant(args, pe, np, torch, w, ps, f, nv, stream, fwd_ms):
+    """The benchmark workload on its stated 512^2 grid at tol 1e-10 with the
+    fixed-grid parameters: xi from the drivers' k-space estimate at
+    k_inf = 256 and the cells from their real-space estimate (the headline
+    uses the drivers' cell size with the grid raised to 512^2, which their
+    own rule would size at 368^2).  Step time and error against the FP64
+    direct Ewald truth on 256 targets (the total is independent of xi)."""
+    from fasttools2d_amd import params as Pm
+
+    N = len(w.psrc)
+    p = Pm.select_params_grid("dlp", N, N, w.f, 1.0, 1.0, M=512, tol=1e-10)
+    ctx = pe.Context(0)
+    ur = torch.empty((N, 2), dtype=torch.float64, device=ps.device)
+    uk = torch.empty_like(ur)
+
+    def step():
+        pe.mex_stokes_dlp_real(ps, ps, f, nv, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, ctx=ctx, out=ur)
+        pe.mex_stokes_dlp_kspace(ps, ps, p.xi, p.eta, f, nv, p.Mx, p.My, 1.0, 1.0, p.w, p.P,
+                                 ctx=ctx, out=uk)
+
+    with torch.cuda.stream(stream):
+        ctx.bind_torch_stream(stream)
+        ctx.set_async(True)
+        ms = _timed_ms(torch, stream, step, reps=10)
+        ctx.set_async(False)
+    torch.cuda.synchronize()
+    err = _direct_ewald_error(np, ur.cpu().numpy(), uk.cpu().numpy())
+    ctx.close()
+    print(f"# fixed-grid parameters on 512^2: nside {p.nside_x} xi {p.xi:.2f}: {ms:.3f} ms "
+          f"({fwd_ms / ms:.2f}x the headline step), error vs direct Ewald "
+          f"{err['u'] if err else float('nan'):.2e}", file=sys.stderr, flush=True)
+    return {"ms_per_step": ms, "target_evals_per_s": N / (ms * 1e-3),
+            "over_headline": fwd_ms / ms,
+            "error_vs_direct_ewald": err["u"] if err else None,
+            "params": {"nside": p.nside_x, "xi": p.xi, "eta": p.eta, "Mx": p.Mx, "tol": 1e-10,
+                       "mode": "fixed grid"}}
+
+
 def _multibox_variant(args, pe, W, np, torch, dev):
     """The many-box workload: 32 independent periodic boxes, L ~ U[0.5, 4],
     N ~ log-uniform in [1e3, 2e5] uniform points each (1.41e6 in all; seed 4),
@@ -528,7 +628,8 @@ def main() -> int:
                     help="skip the N = 4e6 forward / forward+adjoint variant")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (scales HBM counters)")
-    ap.add_argument("--variant", choices=["batched", "multibox", "at_scale"], default=None,
+    ap.add_argument("--variant", choices=["batched", "batched_ellipses", "multibox", "at_scale"],
+                    default=None,
                     help="run only this variant and print its JSON line (for profiling)")
     args = ap.parse_args()
 
@@ -547,8 +648,8 @@ def main() -> int:
     if args.variant is not None:
         dev = torch.device("cuda", 0)
         torch.cuda.set_device(dev)
-        run = {"batched": _batched_variant, "multibox": _multibox_variant,
-               "at_scale": _at_scale_variant}[args.variant]
+        run = {"batched": _batched_variant, "batched_ellipses": _batched_ellipses_variant,
+               "multibox": _multibox_variant, "at_scale": _at_scale_variant}[args.variant]
         print(json.dumps({"variant": args.variant, "result": run(args, pe, W, np, torch, dev)}),
               flush=True)
         return 0
@@ -643,8 +744,15 @@ def main() -> int:
     cpu = None
     parity = None
     if not args.no_cpu:
+        # threads spread over the cores, bound before the oracle's OpenMP
+        # runtime starts; the oracle build for the host's instruction set
+        # (all builds give bitwise the same results)
+        os.environ.setdefault("OMP_PROC_BIND", "spread")
+        os.environ.setdefault("OMP_PLACES", "cores")
         from oracle import cref
 
+        isa = cref.host_isa()
+        cref.use_isa(isa)
         sub = np.random.default_rng(55).choice(N, 256, replace=False)
         c = (cref.dlp_real(w.psrc, w.ptar[sub], w.f, w.n, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly) +
              cref.dlp_kspace(w.psrc, w.ptar[sub], p.xi, p.eta, w.f, w.n, p.Mx, p.My, w.Lx, w.Ly,
@@ -680,6 +788,9 @@ def main() -> int:
         cpu = {"value": N / med, "unit": "target evals/s", "cores": threads, "kind": "port",
                "sample": f"full workload (N={N} targets, real + Fourier sums), median of "
                          f"{args.cpu_runs} runs of each sum after 1 warm-up, {threads} OpenMP threads",
+               "build": f"gcc -O3 -march={isa} -fopenmp -ffp-contract=off",
+               "binding": {"OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+                           "OMP_PLACES": os.environ.get("OMP_PLACES")},
                "seconds_per_step": med,
                "stage_seconds": {"real": real_s, "fourier": k_s},
                "one_thread": {"value": N / (real1 + k1), "seconds_per_step": real1 + k1,
@@ -689,6 +800,9 @@ def main() -> int:
     variants = None
     if not args.no_variants:
         variants = _batched_variant(args, pe, W, np, torch, dev)
+        variants["batched_ellipses"] = _batched_ellipses_variant(args, pe, W, np, torch, dev)
+        variants["fixed_grid"] = _fixed_grid_variant(args, pe, np, torch, w, ps, f, nv, stream,
+                                                     ms_per_step)
         variants["multibox"] = _multibox_variant(args, pe, W, np, torch, dev)
         variants["graph"] = _graph_variant(pe, torch, step, ctx, stream, N)
         variants["autotuned"] = _autotune_variant(args, pe, np, torch, w, ps, f, nv, stream,

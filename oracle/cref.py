@@ -13,8 +13,43 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libse2p_oracle.so")
+# builds of the same source by instruction set (oracle/Makefile); the tests use
+# the portable x86-64-v2 build, the CPU-baseline leg the best the host runs
+_BUILDS = {"x86-64-v2": "libse2p_oracle.so", "x86-64-v3": "libse2p_oracle_v3.so",
+           "x86-64-v4": "libse2p_oracle_v4.so"}
+_LIB_PATH = os.path.join(_HERE, "_build", _BUILDS["x86-64-v2"])
 _lib = None
+
+
+def host_isa() -> str:
+    """The highest x86-64 level of this host's CPU (from /proc/cpuinfo):
+    x86-64-v4 (AVX-512 F/BW/CD/DQ/VL), x86-64-v3 (AVX2, FMA, BMI1/2, MOVBE,
+    F16C) or x86-64-v2."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            flags = set()
+            for line in fh:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+    except OSError:
+        return "x86-64-v2"
+    v3 = {"avx2", "fma", "bmi1", "bmi2", "movbe", "f16c", "abm"}
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+    if v3 <= flags and v4 <= flags:
+        return "x86-64-v4"
+    if v3 <= flags:
+        return "x86-64-v3"
+    return "x86-64-v2"
+
+
+def use_isa(isa: str) -> None:
+    """Load the build for instruction set `isa` (before the first call)."""
+    global _LIB_PATH
+    path = os.path.join(_HERE, "_build", _BUILDS[isa])
+    if _lib is not None and path != _LIB_PATH:
+        raise RuntimeError("the oracle is already loaded; choose the build before the first call")
+    _LIB_PATH = path
 
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -33,6 +68,8 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
+        if not os.path.exists(_LIB_PATH):
+            raise ImportError(f"{_LIB_PATH} is missing: build it with `make -C {_HERE}`")
         L = ctypes.CDLL(_LIB_PATH)
         L.or_e1.restype = _D
         L.or_e1.argtypes = [_D]
