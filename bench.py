@@ -283,28 +283,20 @@ def _fixed_grid_variant(args, pe, np, torch, w, ps, f, nv, stream, fwd_ms):
                        "mode": "fixed grid"}}
 
 
-def _multibox_variant(args, pe, W, np, torch, dev):
+def _multibox_variant(args, pe, W, np, torch, dev, tune=False):
     """The many-box workload: 32 independent periodic boxes, L ~ U[0.5, 4],
     N ~ log-uniform in [1e3, 2e5] uniform points each (1.41e6 in all; seed 4),
-    per-box parameters from the driver formulas at tol 1e-10; one step is the
-    stresslet real-space sums of every box (one launch) and their Fourier
-    sums (packed spreading and interpolation launches; per-box sorts and
-    transforms on four streams).  Timed as eager async calls and as a
-    replayed HIP graph of the same step."""
-    boxes = W.multibox_config(seed=4, nbox=32)
-    tb = []
-    for b in boxes:
-        d = dict(b)
-        for k in ("psrc", "f", "n"):
-            d[k] = torch.from_numpy(np.ascontiguousarray(b[k])).to(dev)
-        d["ptar"] = d["psrc"]
-        tb.append(d)
-    ntot = sum(len(b["psrc"]) for b in boxes)
+    N(0,1) forces, per-box parameters from the driver formulas at tol 1e-10.
+    One step is the real-space sums of every box (one launch set) and their
+    Fourier sums (packed spreading and interpolation launches; per-box sorts
+    and transforms on four streams).  Timed as eager async calls and as a
+    replayed HIP graph of the same step; the Stokeslet (the workload's
+    forces) first, then the stresslet on the same boxes with its own driver
+    parameters.  With ``tune``, also the same step at the parameters and
+    spreading split from se2p_autotune_multibox (its tuning takes minutes on
+    a fresh process: every candidate grid size builds rocFFT plans)."""
     ctx = pe.Context(0)
     stream = torch.cuda.Stream(dev)
-
-    def step():
-        return (pe.stokes_dlp_real_multibox(tb, ctx=ctx) + pe.stokes_dlp_kspace_multibox(tb, ctx=ctx))
 
     def timed(fn, reps=5):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -328,22 +320,65 @@ def _multibox_variant(args, pe, W, np, torch, dev):
         torch.cuda.synchronize()
         return eager, timed(g.replay)
 
-    with torch.cuda.stream(stream):
-        ctx.bind_torch_stream(stream)
-        ctx.set_async(True)
-        eager_ms, graph_ms = eager_and_graph(step)
-        parts = {name: dict(zip(("eager_ms", "graph_ms"), eager_and_graph(fn))) for name, fn in (
-            ("real", lambda: pe.stokes_dlp_real_multibox(tb, ctx=ctx)),
-            ("fourier", lambda: pe.stokes_dlp_kspace_multibox(tb, ctx=ctx)))}
-        ctx.set_async(False)
-    print(f"# multibox variant: eager {eager_ms:.3f} ms, graph {graph_ms:.3f} ms (real "
-          f"{parts['real']['graph_ms']:.3f}, Fourier {parts['fourier']['graph_ms']:.3f} ms replayed)",
-          file=sys.stderr, flush=True)
-    return {"eager": {"ms_per_step": eager_ms, "target_evals_per_s": ntot / (eager_ms * 1e-3)},
+    out = {}
+    for kernel in ("slp", "dlp"):
+        boxes = W.multibox_config(seed=4, nbox=32, kind=kernel)
+        tb = []
+        for b in boxes:
+            d = dict(b)
+            for k in ("psrc", "f", "n"):
+                d[k] = torch.from_numpy(np.ascontiguousarray(b[k])).to(dev)
+            d["ptar"] = d["psrc"]
+            if kernel == "slp":
+                d.pop("n")
+            tb.append(d)
+        ntot = sum(len(b["psrc"]) for b in boxes)
+        real = pe.stokes_slp_real_multibox if kernel == "slp" else pe.stokes_dlp_real_multibox
+        fourier = pe.stokes_slp_kspace_multibox if kernel == "slp" else pe.stokes_dlp_kspace_multibox
+
+        def step():
+            return real(tb, ctx=ctx) + fourier(tb, ctx=ctx)
+
+        with torch.cuda.stream(stream):
+            ctx.bind_torch_stream(stream)
+            ctx.set_async(True)
+            eager_ms, graph_ms = eager_and_graph(step)
+            parts = {name: dict(zip(("eager_ms", "graph_ms"), eager_and_graph(fn))) for name, fn in (
+                ("real", lambda: real(tb, ctx=ctx)), ("fourier", lambda: fourier(tb, ctx=ctx)))}
+            ctx.set_async(False)
+        name = "Stokeslet" if kernel == "slp" else "stresslet"
+        autotuned = None
+        if tune:
+            # the same step at parameters from se2p_autotune_multibox (own
+            # context: the spreading split is a context setting)
+            tctx = pe.Context(0)
+            tuned = [dict(d) for d in tb]
+            t0 = time.perf_counter()
+            tinfo = pe.autotune_multibox(kernel, tuned, tol=1e-10, ctx=tctx)
+            tune_s = time.perf_counter() - t0
+            with torch.cuda.stream(stream):
+                tctx.bind_torch_stream(stream)
+                tctx.set_async(True)
+                t_eager, t_graph = eager_and_graph(lambda: real(tuned, ctx=tctx) + fourier(tuned, ctx=tctx))
+                tctx.set_async(False)
+            print(f"# multibox variant {name} autotuned ({tune_s:.1f} s): eager {t_eager:.3f} ms, "
+                  f"graph {t_graph:.3f} ms", file=sys.stderr, flush=True)
+            autotuned = {"eager_ms": t_eager, "graph_ms": t_graph,
+                         "target_evals_per_s": ntot / (min(t_eager, t_graph) * 1e-3),
+                         "tuning_seconds": tune_s, "tuner": tinfo}
+        print(f"# multibox variant {name}: eager {eager_ms:.3f} ms, graph {graph_ms:.3f} ms (real "
+              f"{parts['real']['graph_ms']:.3f}, Fourier {parts['fourier']['graph_ms']:.3f} ms replayed)",
+              file=sys.stderr, flush=True)
+        out[kernel] = {
+            "eager": {"ms_per_step": eager_ms, "target_evals_per_s": ntot / (eager_ms * 1e-3)},
             "graph": {"ms_per_step": graph_ms, "target_evals_per_s": ntot / (graph_ms * 1e-3)},
             "parts": parts,
-            "workload": f"32 boxes, {ntot} points in all (targets = sources), stresslet real + Fourier sums, "
-                        "per-box parameters at tol 1e-10"}
+            "autotuned": autotuned,
+            "grids": sorted({int(b["Mx"]) for b in boxes}),
+            "workload": f"32 boxes, {ntot} points in all (targets = sources), {name} real + Fourier "
+                        "sums, per-box parameters from the {kernel} driver formulas at tol 1e-10"
+                        .replace("{kernel}", kernel)}
+    return out
 
 
 def _graph_variant(pe, torch, step, ctx, stream, N, reps=10):
@@ -463,8 +498,9 @@ def _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, fwd_
 
 def _autotune_variant(args, pe, np, torch, w, ps, f, nv, stream, fwd_ms):
     """The headline geometry with parameters chosen by se2p_autotune at the
-    same tolerance (1e-10): the drivers' support P = 24, points per cell and
-    spreading work split by timing, on a context of its own (the split is a
+    same tolerance (1e-10): points per cell, spreading work split and
+    support/grid (P 16, 20 or the drivers' 24, on oversampled grids, checked
+    against P = 24) by timing, on a context of its own (the split is a
     context setting).  One step is the same two MEX-equivalent calls at the
     tuned parameters; the output is checked against the FP64 direct Ewald sum on
     the 256 truth targets (the real + Fourier total does not depend on the
@@ -508,6 +544,7 @@ def _autotune_variant(args, pe, np, torch, w, ps, f, nv, stream, fwd_ms):
                        "Mx": prm.Mx, "My": prm.My, "xi": prm.xi, "tol": 1e-10},
             "tuning_seconds": tune_s, "candidates": info["candidates"],
             "spread_chunk": info["spread_chunk"], "spread_chunks": info["chunks"],
+            "supports": info["supports"],
             "error_vs_direct_ewald": acc,
             "workload": "headline geometry (stresslet, 200 ellipses x 500 nodes), tol 1e-10, "
                         "parameters from se2p_autotune"}
@@ -649,7 +686,8 @@ def main() -> int:
         dev = torch.device("cuda", 0)
         torch.cuda.set_device(dev)
         run = {"batched": _batched_variant, "batched_ellipses": _batched_ellipses_variant,
-               "multibox": _multibox_variant, "at_scale": _at_scale_variant}[args.variant]
+               "multibox": lambda *a: _multibox_variant(*a, tune=True),
+               "at_scale": _at_scale_variant}[args.variant]
         print(json.dumps({"variant": args.variant, "result": run(args, pe, W, np, torch, dev)}),
               flush=True)
         return 0

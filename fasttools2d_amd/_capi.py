@@ -102,7 +102,16 @@ class Se2pTuneInfo(ctypes.Structure):
                 ("P", ctypes.c_int), ("Nb", ctypes.c_double * 16), ("nside_x", ctypes.c_int * 16),
                 ("Mx", ctypes.c_int * 16), ("ms", ctypes.c_double * 16),
                 ("spread_chunk", ctypes.c_int), ("chunk", ctypes.c_int * 3),
-                ("chunk_ms", ctypes.c_double * 3)]
+                ("chunk_ms", ctypes.c_double * 3), ("npc", ctypes.c_int),
+                ("pc_P", ctypes.c_int * 6), ("pc_Mx", ctypes.c_int * 6),
+                ("pc_ms", ctypes.c_double * 6), ("pc_dev", ctypes.c_double * 6)]
+
+
+class Se2pTuneBoxesInfo(ctypes.Structure):
+    """se2p_tune_boxes_info of include/se2p.h."""
+    _fields_ = [("drivers_ms", ctypes.c_double), ("tuned_ms", ctypes.c_double),
+                ("kept_drivers", ctypes.c_int), ("spread_chunk", ctypes.c_int),
+                ("chunk", ctypes.c_int * 3), ("chunk_ms", ctypes.c_double * 3)]
 
 
 SIGNATURES.update({
@@ -126,6 +135,10 @@ SIGNATURES.update({
     name: (_I, [_P, ctypes.POINTER(Se2pBoxDesc), _I])
     for name in ("se2p_slp_real_multibox", "se2p_slp_kspace_multibox",
                  "se2p_dlp_real_multibox", "se2p_dlp_kspace_multibox")
+})
+SIGNATURES.update({
+    "se2p_autotune_multibox": (_I, [_P, _I, ctypes.POINTER(Se2pBoxDesc), _I, _D,
+                                    ctypes.POINTER(Se2pTuneBoxesInfo)]),
 })
 
 STAGES = ("sort", "real", "spread", "fft_fwd", "filter", "fft_inv", "interp")

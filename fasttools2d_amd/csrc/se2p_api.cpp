@@ -1086,6 +1086,7 @@ std::mutex g_tune_mu;
 struct TuneChoice {
     double Nb;   // points per cell
     int chunk;   // spreading chunk
+    int P, M;    // support and grid override (M = 0: the drivers' grid)
 };
 std::map<TuneKey, TuneChoice> g_tune_nb;   // key -> choice
 
@@ -1105,21 +1106,219 @@ struct TimingOff {
     explicit TimingOff(se2p_ctx *c) : ctx(c), was(c->timing) { c->timing = false; }
     ~TimingOff() { ctx->timing = was; }
 };
+
+// One geometry to tune: wrapped device points, densities, Q = sum f^2 + 1,
+// and output scratch (2 x nt doubles each).
+struct TuneGeom {
+    int kind;
+    const double *src, *tar;
+    int64_t ns, nt;
+    const double *f, *n;
+    double Lx, Ly, Q, tol;
+    double *ur, *uk;
+};
+
+enum TuneStage : unsigned { TUNE_NB = 1, TUNE_CHUNK = 2, TUNE_SUPPORT = 4 };
+constexpr int kTuneReps = 3, kTuneBlocks = 3;
+constexpr double kTuneMaxGrid = 4096.0 * 4096.0;
 }  // namespace
+
+// Driver parameters of geometry tg for points per cell Nb, support P and grid
+// override M (0: the drivers' grid).
+static void tune_params(const TuneGeom &tg, double Nb, int P, int M, se2p_params *c)
+{
+    *c = se2p_params{};
+    c->P = P;
+    c->Nb = Nb;
+    c->tol = tg.tol;
+    c->M = M;
+    select_params(tg.kind, tg.ns, tg.nt, tg.Q, tg.Lx, tg.Ly, c);
+    c->M = M;
+}
+
+// Best of kTuneBlocks blocks of kTuneReps back-to-back real + Fourier steps
+// at parameters c, after one warm-up step (it sizes the workspace and makes
+// the FFT plans); HIP events on the context stream.
+static int tune_time(se2p_ctx *ctx, const TuneGeom &tg, const se2p_params &c, EventPair &ev, double *t)
+{
+    hipStream_t st = ctx->stream;
+    const RealGeom rg{tg.src, tg.tar, tg.ns, tg.nt, c.xi, tg.Lx, tg.Ly, c.nside_x, c.nside_y};
+    const KGeom kg{tg.src, tg.tar, tg.ns, tg.nt, c.xi, c.eta, tg.Lx, tg.Ly, c.w, c.Mx, c.My, c.P};
+    if (int rc = check_kspace_params(ctx, kg)) return rc;
+    auto step = [&]() -> int {
+        if (int rc = real_sum(ctx, tg.kind, rg, tg.f, tg.n, 1, 2 * tg.ns, 2 * tg.nt, tg.ur, nullptr)) return rc;
+        return kspace_sum(ctx, tg.kind, kg, tg.f, tg.n, 1, 2 * tg.ns, 2 * tg.nt, tg.uk);
+    };
+    if (int rc = step()) return rc;
+    SE2P_HIP(hipStreamSynchronize(st));
+    *t = std::numeric_limits<double>::infinity();
+    for (int blk = 0; blk < kTuneBlocks; ++blk) {
+        SE2P_HIP(hipEventRecord(ev.a, st));
+        for (int r = 0; r < kTuneReps; ++r)
+            if (int rc = step()) return rc;
+        SE2P_HIP(hipEventRecord(ev.b, st));
+        SE2P_HIP(hipEventSynchronize(ev.b));
+        float ms = 0.0f;
+        SE2P_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+        *t = std::min(*t, (double)ms / kTuneReps);
+    }
+    return 0;
+}
+
+// The tuning stages of one geometry (see se2p_autotune).  Pin: the caller's
+// support (0: the drivers' 24, and stage 3 may change it).  Sets the chosen
+// spreading chunk on the context when stage 2 runs.
+static int tune_geometry(se2p_ctx *ctx, const TuneGeom &tg, int Pin, unsigned stages,
+                         se2p_params *out, int *chunk_out, se2p_tune_info *info)
+{
+    const int P0 = Pin > 0 ? Pin : 24;   // the drivers' default support
+    if (info) info->P = P0;
+    TimingOff quiet(ctx);
+    KeepOff no_keep(ctx);
+    EventPair ev;
+    SE2P_HIP(hipEventCreate(&ev.a));
+    SE2P_HIP(hipEventCreate(&ev.b));
+
+    // 1. the cell size: points per cell around the drivers' default, one
+    //    candidate per cell grid
+    const double npts = (double)(tg.ns + tg.nt);
+    const double Nb0 = std::min(P0 * std::round(std::log2(npts)), npts / 4);
+    std::vector<se2p_params> cand;
+    if (stages & TUNE_NB) {
+        for (int k = -6; k <= 2 && (int)cand.size() < SE2P_TUNE_MAX; ++k) {
+            se2p_params c;
+            tune_params(tg, Nb0 * std::pow(2.0, 0.5 * k), P0, 0, &c);
+            if ((double)c.Mx * (double)c.My > kTuneMaxGrid) continue;
+            bool dup = false;
+            for (const se2p_params &o : cand) dup = dup || (o.nside_x == c.nside_x && o.nside_y == c.nside_y);
+            if (!dup) cand.push_back(c);
+        }
+        if (cand.empty()) return fail(ctx, SE2P_ERR_PARAM, "no candidate grid within 4096^2 points");
+    } else {
+        se2p_params c;
+        tune_params(tg, 0.0, P0, 0, &c);   // the drivers' default cells
+        cand.push_back(c);
+    }
+    if (stages & TUNE_CHUNK)
+        if (int rc = se2p_set_spread_chunk(ctx, 0)) return rc;
+    int best = 0;
+    double best_ms = std::numeric_limits<double>::infinity();
+    for (size_t i = 0; i < cand.size(); ++i) {
+        const se2p_params &c = cand[i];
+        double t;
+        if (int rc = tune_time(ctx, tg, c, ev, &t)) return rc;
+        if (info && i < SE2P_TUNE_MAX) {
+            info->Nb[i] = c.Nb;
+            info->nside_x[i] = c.nside_x;
+            info->Mx[i] = c.Mx;
+            info->ms[i] = t;
+        }
+        if (t < best_ms) {
+            best_ms = t;
+            best = (int)i;
+        }
+    }
+    if (info) {
+        info->ncand = (int)cand.size();
+        info->best = best;
+    }
+    // 2. the spreading work split for that cell size (0: the library's
+    //    default, 256 candidates for one density; timed above)
+    int best_chunk = 0;
+    if (stages & TUNE_CHUNK) {
+        const int chunks[SE2P_TUNE_CHUNKS] = {0, 128, 512};
+        double chunk_best = best_ms;
+        for (int k = 0; k < SE2P_TUNE_CHUNKS; ++k) {
+            double t = best_ms;
+            if (k > 0) {
+                if (int rc = se2p_set_spread_chunk(ctx, chunks[k])) return rc;
+                if (int rc = tune_time(ctx, tg, cand[(size_t)best], ev, &t)) return rc;
+            }
+            if (info) {
+                info->chunk[k] = chunks[k];
+                info->chunk_ms[k] = t;
+            }
+            if (t < chunk_best) {
+                chunk_best = t;
+                best_chunk = chunks[k];
+            }
+        }
+        if (int rc = se2p_set_spread_chunk(ctx, best_chunk)) return rc;
+        best_ms = chunk_best;
+        if (info) info->spread_chunk = best_chunk;
+    }
+    se2p_params chosen = cand[(size_t)best];
+    // 3. a narrower support on an oversampled grid: supports 16 and 20 on
+    //    grids 1, 1.2 and 1.4 times the chosen one.  A candidate faster than
+    //    the best so far is kept only if its Fourier-space sum differs from
+    //    the chosen P = 24 one by at most tol/4 of the largest |ur + uk|: the
+    //    Fourier-space sum converges as the support and the grid grow, so
+    //    the difference bounds what the candidate adds to the error.
+    if ((stages & TUNE_SUPPORT) && Pin == 0) {
+        hipStream_t st = ctx->stream;
+        const size_t nv = 2 * (size_t)tg.nt;
+        std::vector<double> ur_h(nv), uk_ref(nv), uk_c(nv);
+        {
+            double t;
+            if (int rc = tune_time(ctx, tg, chosen, ev, &t)) return rc;
+        }
+        SE2P_HIP(hipMemcpyAsync(ur_h.data(), tg.ur, nv * sizeof(double), hipMemcpyDeviceToHost, st));
+        SE2P_HIP(hipMemcpyAsync(uk_ref.data(), tg.uk, nv * sizeof(double), hipMemcpyDeviceToHost, st));
+        SE2P_HIP(hipStreamSynchronize(st));
+        double scale = 0.0;
+        for (size_t i = 0; i < nv; ++i) scale = std::max(scale, std::fabs(ur_h[i] + uk_ref[i]));
+        const int Mb0 = std::min(2 * chosen.kinf, 10000);   // the drivers' grid per unit of A
+        const int supports[2] = {16, 20};
+        const double over[3] = {1.0, 1.2, 1.4};
+        int np = 0;
+        for (int ps : supports)
+            for (double s : over) {
+                const int M = 2 * (int)std::ceil(s * Mb0 / 2.0);
+                se2p_params c;
+                tune_params(tg, chosen.Nb, ps, M, &c);
+                if ((double)c.Mx * (double)c.My > kTuneMaxGrid || c.Mx < ps / 2 + 2 || c.My < ps / 2 + 2)
+                    continue;
+                double t;
+                if (int rc = tune_time(ctx, tg, c, ev, &t)) return rc;
+                double dev = -1.0;
+                if (t < best_ms) {
+                    SE2P_HIP(hipMemcpyAsync(uk_c.data(), tg.uk, nv * sizeof(double), hipMemcpyDeviceToHost, st));
+                    SE2P_HIP(hipStreamSynchronize(st));
+                    double dmax = 0.0;
+                    for (size_t i = 0; i < nv; ++i) dmax = std::max(dmax, std::fabs(uk_c[i] - uk_ref[i]));
+                    dev = scale > 0 ? dmax / scale : dmax;
+                    if (dev <= 0.25 * tg.tol) {
+                        chosen = c;
+                        best_ms = t;
+                    }
+                }
+                if (info && np < SE2P_TUNE_PCAND) {
+                    info->pc_P[np] = c.P;
+                    info->pc_Mx[np] = c.Mx;
+                    info->pc_ms[np] = t;
+                    info->pc_dev[np] = dev;
+                }
+                ++np;
+            }
+        if (info) info->npc = std::min(np, SE2P_TUNE_PCAND);
+    }
+    SE2P_HIP(hipGetLastError());
+    if (info) info->P = chosen.P;
+    *out = chosen;
+    if (chunk_out) *chunk_out = best_chunk;
+    return 0;
+}
 
 static int autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const double *ptar,
                     int64_t nt, const double *f, const double *n, double Lx, double Ly,
                     se2p_params *prm, se2p_tune_info *info)
 {
-    constexpr int kReps = 3, kBlocks = 3;
-    constexpr double kMaxGrid = 4096.0 * 4096.0;
     if (int rc = check_driver(ctx, ns, nt, Lx, Ly, prm)) return rc;
     if (!(prm->tol > 0) || !(prm->tol < 1))
         return fail(ctx, SE2P_ERR_PARAM, "tol must be in (0, 1)");
     if (prm->M != 0) return fail(ctx, SE2P_ERR_ARG, "M must be 0: the grid follows from the tuned cell size");
+    if (prm->fixed_grid) return fail(ctx, SE2P_ERR_ARG, "the fixed-grid mode is not tuned");
     if (info) std::memset(info, 0, sizeof(*info));
-    const int P = prm->P > 0 ? prm->P : 24;   // the drivers' default support
-    if (info) info->P = P;
     if (int rc = set_device(ctx)) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
@@ -1139,127 +1338,167 @@ static int autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, con
     double hs[3];
     SE2P_HIP(hipMemcpyAsync(hs, b.sums, sizeof(hs), hipMemcpyDeviceToHost, st));
     SE2P_HIP(hipStreamSynchronize(st));
-    const double Q = hs[0] + 1.0;
-    const double tol = prm->tol;
-    auto params_for = [&](double Nb, se2p_params *c) {
-        *c = se2p_params{};
-        c->P = P;
-        c->Nb = Nb;
-        c->tol = tol;
-        select_params(kind, ns, nt, Q, Lx, Ly, c);
-    };
-    const TuneKey key{ctx->device, kind, prm->P, ns, nt, same ? 1 : 0, Lx, Ly, tol};
+    const TuneGeom tg{kind, b.src, b.tar, ns, nt, f, n, Lx, Ly, hs[0] + 1.0, prm->tol, b.ur, b.uk};
+    const TuneKey key{ctx->device, kind, prm->P, ns, nt, same ? 1 : 0, Lx, Ly, prm->tol};
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = g_tune_nb.find(key);
         if (it != g_tune_nb.end()) {
-            params_for(it->second.Nb, prm);
+            tune_params(tg, it->second.Nb, it->second.P, it->second.M, prm);
             if (int rc = se2p_set_spread_chunk(ctx, it->second.chunk)) return rc;
             if (info) {
                 info->cached = 1;
+                info->P = prm->P;
                 info->spread_chunk = it->second.chunk;
             }
             return io.done(SE2P_OK);
         }
     }
-
-    // candidates around the drivers' default points per cell, one per cell grid
-    const double npts = (double)(ns + nt);
-    const double Nb0 = std::min(P * std::round(std::log2(npts)), npts / 4);
-    std::vector<se2p_params> cand;
-    for (int k = -6; k <= 2 && (int)cand.size() < SE2P_TUNE_MAX; ++k) {
-        se2p_params c;
-        params_for(Nb0 * std::pow(2.0, 0.5 * k), &c);
-        if ((double)c.Mx * (double)c.My > kMaxGrid) continue;
-        bool dup = false;
-        for (const se2p_params &o : cand) dup = dup || (o.nside_x == c.nside_x && o.nside_y == c.nside_y);
-        if (!dup) cand.push_back(c);
-    }
-    if (cand.empty()) return fail(ctx, SE2P_ERR_PARAM, "no candidate grid within 4096^2 points");
-
     launch_wrap(st, psrc, ns, Lx, Ly, b.src);
     if (!same) launch_wrap(st, ptar, nt, Lx, Ly, b.tar);
-    TimingOff quiet(ctx);
-    KeepOff no_keep(ctx);
+    int chunk = 0;
+    if (int rc = tune_geometry(ctx, tg, prm->P, TUNE_NB | TUNE_CHUNK | TUNE_SUPPORT, prm, &chunk, info))
+        return rc;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune_nb[key] = TuneChoice{prm->Nb, chunk, prm->P, prm->M};
+    return io.done(SE2P_OK);
+}
+
+// Multi-box tuning: each box's cell size (stage 1 of the single-box tuner at
+// its drivers' P = 24), written into its descriptor, then the spreading work
+// split of the packed Fourier-space call over all boxes.
+static int autotune_boxes(se2p_ctx *ctx, int kind, se2p_box_desc *boxes, int nbox, double tol,
+                          se2p_tune_boxes_info *info)
+{
+    if (!(tol > 0) || !(tol < 1)) return fail(ctx, SE2P_ERR_PARAM, "tol must be in (0, 1)");
+    if (nbox < 0 || (nbox > 0 && !boxes)) return fail(ctx, SE2P_ERR_ARG, "bad box list");
+    if (ctx->host_ptrs) return fail(ctx, SE2P_ERR_ARG, "multi-box tuning needs device arrays");
+    if (info) std::memset(info, 0, sizeof(*info));
+    for (int i = 0; i < nbox; ++i) {
+        const se2p_box_desc &d = boxes[i];
+        if (int rc = check_common(ctx, d.psrc, d.ns, d.ptar, d.nt, d.u)) {
+            ctx->err = "box " + std::to_string(i) + ": " + ctx->err;
+            return rc;
+        }
+        if (d.ns > 0 && (!d.f || (kind == 1 && !d.n)))
+            return fail(ctx, SE2P_ERR_ARG, "box %d: null density pointer", i);
+        if (d.ns + d.nt < 2) return fail(ctx, SE2P_ERR_ARG, "box %d: need at least two points in total", i);
+        if (!(d.Lx > 0) || !(d.Ly > 0) || !std::isfinite(d.Lx) || !std::isfinite(d.Ly))
+            return fail(ctx, SE2P_ERR_PARAM, "box %d: Lx and Ly must be positive and finite", i);
+    }
+    if (int rc = set_device(ctx)) return rc;
+    hipStream_t st = ctx->stream;
     EventPair ev;
     SE2P_HIP(hipEventCreate(&ev.a));
     SE2P_HIP(hipEventCreate(&ev.b));
-    // real + Fourier step of parameter set c: one warm-up step (it sizes the
-    // workspace and makes the FFT plans), then the best of kBlocks blocks of
-    // kReps back-to-back steps
-    auto time_step = [&](const se2p_params &c, double *t) -> int {
-        const RealGeom rg{b.src, b.tar, ns, nt, c.xi, Lx, Ly, c.nside_x, c.nside_y};
-        const KGeom kg{b.src, b.tar, ns, nt, c.xi, c.eta, Lx, Ly, c.w, c.Mx, c.My, c.P};
-        if (int rc = check_kspace_params(ctx, kg)) return rc;
-        auto step = [&]() -> int {
-            if (int rc = real_sum(ctx, kind, rg, f, n, 1, 2 * ns, 2 * nt, b.ur, nullptr)) return rc;
-            return kspace_sum(ctx, kind, kg, f, n, 1, 2 * ns, 2 * nt, b.uk);
-        };
+    auto step = [&]() -> int {
+        if (int rc = real_sum_multibox(ctx, kind, boxes, nbox)) return rc;
+        return kspace_multibox(ctx, kind, boxes, nbox);
+    };
+    auto time_boxes = [&](double *t) -> int {
         if (int rc = step()) return rc;
         SE2P_HIP(hipStreamSynchronize(st));
         *t = std::numeric_limits<double>::infinity();
-        for (int blk = 0; blk < kBlocks; ++blk) {
+        for (int blk = 0; blk < kTuneBlocks; ++blk) {
             SE2P_HIP(hipEventRecord(ev.a, st));
-            for (int r = 0; r < kReps; ++r)
+            for (int r = 0; r < kTuneReps; ++r)
                 if (int rc = step()) return rc;
             SE2P_HIP(hipEventRecord(ev.b, st));
             SE2P_HIP(hipEventSynchronize(ev.b));
             float ms = 0.0f;
             SE2P_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
-            *t = std::min(*t, (double)ms / kReps);
+            *t = std::min(*t, (double)ms / kTuneReps);
         }
         return 0;
     };
-    // 1. the cell size, at the default spreading chunk
+    TimingOff quiet(ctx);
     if (int rc = se2p_set_spread_chunk(ctx, 0)) return rc;
-    int best = 0;
-    double best_ms = std::numeric_limits<double>::infinity();
-    for (size_t i = 0; i < cand.size(); ++i) {
-        const se2p_params &c = cand[i];
-        double t;
-        if (int rc = time_step(c, &t)) return rc;
-        if (info) {
-            info->Nb[i] = c.Nb;
-            info->nside_x[i] = c.nside_x;
-            info->Mx[i] = c.Mx;
-            info->ms[i] = t;
+    // the drivers' parameters first, for the record
+    std::vector<se2p_params> drv((size_t)nbox), tuned((size_t)nbox);
+    std::vector<double> Q((size_t)nbox, 1.0);
+    for (int i = 0; i < nbox; ++i) {
+        se2p_box_desc &d = boxes[i];
+        const bool same = d.psrc == d.ptar && d.ns == d.nt;
+        DrvBufs b;
+        if (int rc = driver_buffers(ctx, d.ns, d.nt, same, &b)) return rc;
+        launch_driver_sums(st, d.f, kind == 1 ? d.n : nullptr, d.psrc, d.ns, b.partial, b.sums);
+        double hs[3];
+        SE2P_HIP(hipMemcpyAsync(hs, b.sums, sizeof(hs), hipMemcpyDeviceToHost, st));
+        SE2P_HIP(hipStreamSynchronize(st));
+        Q[(size_t)i] = hs[0] + 1.0;
+        const TuneGeom tg{kind, d.psrc, d.ptar, d.ns, d.nt, d.f, d.n, d.Lx, d.Ly, Q[(size_t)i], tol, b.ur, b.uk};
+        tune_params(tg, 0.0, 24, 0, &drv[(size_t)i]);
+    }
+    auto apply = [&](const std::vector<se2p_params> &ps) {
+        for (int i = 0; i < nbox; ++i) {
+            const se2p_params &c = ps[(size_t)i];
+            se2p_box_desc &d = boxes[i];
+            d.xi = c.xi;
+            d.nside_x = c.nside_x;
+            d.nside_y = c.nside_y;
+            d.eta = c.eta;
+            d.w = c.w;
+            d.Mx = c.Mx;
+            d.My = c.My;
+            d.P = c.P;
         }
-        if (t < best_ms) {
-            best_ms = t;
-            best = (int)i;
+    };
+    apply(drv);
+    for (int i = 0; i < nbox; ++i)
+        for (bool ks : {false, true})
+            if (int rc = check_box(ctx, boxes[i], i, kind == 1, ks)) return rc;
+    double t_drv = 0.0;
+    if (int rc = time_boxes(&t_drv)) return rc;
+    // each box's cell size, timed on its own
+    for (int i = 0; i < nbox; ++i) {
+        se2p_box_desc &d = boxes[i];
+        tuned[(size_t)i] = drv[(size_t)i];
+        if (d.ns == 0 || d.nt == 0) continue;
+        const bool same = d.psrc == d.ptar && d.ns == d.nt;
+        DrvBufs b;
+        if (int rc = driver_buffers(ctx, d.ns, d.nt, same, &b)) return rc;
+        const TuneGeom tg{kind, d.psrc, d.ptar, d.ns, d.nt, d.f, d.n, d.Lx, d.Ly, Q[(size_t)i], tol, b.ur, b.uk};
+        if (int rc = tune_geometry(ctx, tg, 24, TUNE_NB, &tuned[(size_t)i], nullptr, nullptr)) {
+            ctx->err = "box " + std::to_string(i) + ": " + ctx->err;
+            return rc;
         }
     }
-    // 2. the spreading work split for that cell size (the default was timed above)
-    // (0: the library's default, 256 candidates for one density)
+    apply(tuned);
+    for (int i = 0; i < nbox; ++i)
+        for (bool ks : {false, true})
+            if (int rc = check_box(ctx, boxes[i], i, kind == 1, ks)) return rc;
+    // the spreading work split of the packed call
     const int chunks[SE2P_TUNE_CHUNKS] = {0, 128, 512};
-    int best_chunk = chunks[0];
-    double chunk_best = best_ms;
+    int best_chunk = 0;
+    double best = std::numeric_limits<double>::infinity();
     for (int k = 0; k < SE2P_TUNE_CHUNKS; ++k) {
-        double t = best_ms;
-        if (k > 0) {
-            if (int rc = se2p_set_spread_chunk(ctx, chunks[k])) return rc;
-            if (int rc = time_step(cand[(size_t)best], &t)) return rc;
-        }
+        if (int rc = se2p_set_spread_chunk(ctx, chunks[k])) return rc;
+        double t;
+        if (int rc = time_boxes(&t)) return rc;
         if (info) {
             info->chunk[k] = chunks[k];
             info->chunk_ms[k] = t;
         }
-        if (t < chunk_best) {
-            chunk_best = t;
+        if (t < best) {
+            best = t;
             best_chunk = chunks[k];
         }
     }
-    if (int rc = se2p_set_spread_chunk(ctx, best_chunk)) return rc;
-    SE2P_HIP(hipGetLastError());
-    *prm = cand[(size_t)best];
-    if (info) {
-        info->ncand = (int)cand.size();
-        info->best = best;
-        info->spread_chunk = best_chunk;
+    // keep the drivers' cells where the tuned ones are not faster as a whole
+    if (!(best < t_drv)) {
+        apply(drv);
+        best_chunk = 0;
+        best = t_drv;
     }
-    std::lock_guard<std::mutex> lock(g_tune_mu);
-    g_tune_nb[key] = TuneChoice{prm->Nb, best_chunk};
-    return io.done(SE2P_OK);
+    if (int rc = se2p_set_spread_chunk(ctx, best_chunk)) return rc;
+    if (info) {
+        info->drivers_ms = t_drv;
+        info->tuned_ms = best;
+        info->spread_chunk = best_chunk;
+        info->kept_drivers = best == t_drv ? 1 : 0;
+    }
+    SE2P_HIP(hipGetLastError());
+    return finish(ctx);
 }
 
 int se2p_slp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
@@ -1297,6 +1536,15 @@ int se2p_autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const
     if (ns > 0 && (!f || (kind == 1 && !n)))
         return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
     return autotune(ctx, kind, psrc, ns, ptar, nt, f, n, Lx, Ly, prm, info);
+}
+
+int se2p_autotune_multibox(se2p_ctx *ctx, int kind, se2p_box_desc *boxes, int nbox, double tol,
+                           se2p_tune_boxes_info *info)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    begin_call(ctx);
+    if (kind != 0 && kind != 1) return fail(ctx, SE2P_ERR_ARG, "kind must be 0 (SLP) or 1 (DLP)");
+    return autotune_boxes(ctx, kind, boxes, nbox, tol, info);
 }
 
 int se2p_count_pairs(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,

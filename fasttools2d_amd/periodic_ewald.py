@@ -543,10 +543,12 @@ def autotune(kind, psrc, ptar, f, n, Lx, Ly, *, tol, P=0, ctx=None):
 
     ``kind`` is 'slp' or 'dlp'; points are the caller's (N, 2) arrays
     (unwrapped, as for the drivers); ``ptar is psrc`` means targets are the
-    sources.  P = 0 takes the drivers' support, 24.  Returns
-    ``(params, info)``: the se2p_params to pass on (``Nb`` and ``P`` set, so
-    ``StokesSLP_ewald_2p(..., P=params.P, Nb=params.Nb, tol=tol)`` reproduces
-    them) and a dict of the candidates timed."""
+    sources.  P = 0 starts from the drivers' support, 24, and lets the
+    tuner try narrower supports on oversampled grids.  Returns
+    ``(params, info)``: the se2p_params to pass on (``Nb``, ``P`` and ``M``
+    set, so ``StokesSLP_ewald_2p(..., P=params.P, Nb=params.Nb,
+    M=params.M or None, tol=tol)`` reproduces them) and a dict of the
+    candidates timed."""
     k = {"slp": 0, "dlp": 1}[kind]
     same = ptar is psrc
     _same_size(psrc, f, "f")
@@ -572,8 +574,46 @@ def autotune(kind, psrc, ptar, f, n, Lx, Ly, *, tol, P=0, ctx=None):
            "candidates": [{"Nb": info.Nb[i], "nside_x": info.nside_x[i], "Mx": info.Mx[i],
                            "ms": info.ms[i]} for i in range(nc)],
            "spread_chunk": info.spread_chunk,
-           "chunks": [{"chunk": info.chunk[k], "ms": info.chunk_ms[k]} for k in range(3)] if nc else []}
+           "chunks": [{"chunk": info.chunk[k], "ms": info.chunk_ms[k]} for k in range(3)] if nc else [],
+           "supports": [{"P": info.pc_P[i], "Mx": info.pc_Mx[i], "ms": info.pc_ms[i],
+                         "dev": info.pc_dev[i]} for i in range(info.npc)]}
     return prm, out
+
+
+def autotune_multibox(kind, boxes, *, tol, ctx=None):
+    """Tunes a multi-box call (se2p_autotune_multibox): each box's cells by
+    timing, then the spreading split of the packed call.  ``boxes`` are dicts
+    as for the multi-box entries (device (N, 2) tensors "psrc", "ptar", "f",
+    "n" for the stresslet, "Lx", "Ly"); their parameter keys (xi, nside_x,
+    nside_y, eta, w, Mx, My, P) are overwritten with the tuned values.
+    Returns a dict of the timings."""
+    k = {"slp": 0, "dlp": 1}[kind]
+    if not boxes:
+        raise ValueError("no boxes")
+    dev = boxes[0]["psrc"].device
+    ctx = _prepare(ctx, dev)
+    arr = (_capi.Se2pBoxDesc * len(boxes))()
+    keep = []
+    for i, b in enumerate(boxes):
+        ps = _as_points(b["psrc"], "psrc", dev)
+        pt = ps if b["ptar"] is b["psrc"] else _as_points(b["ptar"], "ptar", dev)
+        ff = _as_points(b["f"], "f", dev)
+        nn = _as_points(b["n"], "n", dev) if k == 1 else None
+        u = torch.empty((pt.shape[0], 2), dtype=torch.float64, device=dev)
+        keep += [ps, pt, ff, nn, u]
+        d = arr[i]
+        d.psrc, d.ns, d.ptar, d.nt = _ptr(ps), ps.shape[0], _ptr(pt), pt.shape[0]
+        d.f, d.n = _ptr(ff), (_ptr(nn) if nn is not None else None)
+        d.Lx, d.Ly, d.u = float(b["Lx"]), float(b["Ly"]), _ptr(u)
+    info = _capi.Se2pTuneBoxesInfo()
+    ctx._check(ctx._lib.se2p_autotune_multibox(ctx.handle, k, arr, len(boxes), float(tol),
+                                               ctypes.byref(info)))
+    for b, d in zip(boxes, arr):
+        b.update(xi=d.xi, nside_x=d.nside_x, nside_y=d.nside_y, eta=d.eta, w=d.w, Mx=d.Mx,
+                 My=d.My, P=d.P)
+    return {"drivers_ms": info.drivers_ms, "tuned_ms": info.tuned_ms,
+            "kept_drivers": bool(info.kept_drivers), "spread_chunk": info.spread_chunk,
+            "chunks": [{"chunk": info.chunk[i], "ms": info.chunk_ms[i]} for i in range(3)]}
 
 
 def _cols(*arrays):

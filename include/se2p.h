@@ -210,9 +210,10 @@ int se2p_dlp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const
  * and Ewald parameters (for instance many periodic cells of a simulation).
  * Result b equals the single-box entry point's result on box b (the same
  * kernels; to rounding).  All pointers are device arrays.  The real-space
- * sums of all boxes run as one sort and one pair-kernel launch; the
- * Fourier-space sums run box after box on the context stream.  The reference
- * has no multi-box form; its callers loop over boxes. */
+ * sums of all boxes run as one sort and one pair-kernel launch; in Fourier
+ * space every box's sorts and transforms run on four stream slots and the
+ * spreading and interpolation of up to 16 boxes share a launch.  The
+ * reference has no multi-box form; its callers loop over boxes. */
 typedef struct se2p_box_desc {
     const double *psrc; int64_t ns;   /* Ns x 2 sources */
     const double *ptar; int64_t nt;   /* Nt x 2 targets */
@@ -312,6 +313,7 @@ int se2p_dlp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *
 /* ---- parameter autotuning (no reference counterpart) --------------------- */
 #define SE2P_TUNE_MAX 16
 #define SE2P_TUNE_CHUNKS 3
+#define SE2P_TUNE_PCAND 6
 typedef struct se2p_tune_info {
     int ncand;                 /* candidates timed (0 on a cache hit) */
     int best;                  /* index of the chosen candidate */
@@ -324,31 +326,68 @@ typedef struct se2p_tune_info {
     int spread_chunk;          /* chosen spreading work split (0: the default) */
     int chunk[SE2P_TUNE_CHUNKS];        /* spreading splits timed at the chosen Nb */
     double chunk_ms[SE2P_TUNE_CHUNKS];  /* their real + Fourier step, ms */
+    int npc;                            /* support/grid candidates timed (stage 3) */
+    int pc_P[SE2P_TUNE_PCAND];          /* their support P */
+    int pc_Mx[SE2P_TUNE_PCAND];         /* their grid */
+    double pc_ms[SE2P_TUNE_PCAND];      /* their real + Fourier step, ms */
+    double pc_dev[SE2P_TUNE_PCAND];     /* max |uk - uk(P = 24)| / max |ur + uk(P = 24)|;
+                                           -1: not checked (not faster) */
 } se2p_tune_info;
 
-/* Chooses the drivers' free parameter for one geometry at prm->tol
- * (required, 0 < tol < 1): the points per cell Nb, which sets the cell grid
- * (nside), the splitting xi and the Fourier grid through the driver formulas
- * (StokesSLP_ewald_2p.m:33-104, StokesDLP_ewald_2p.m:33-125).  The real- plus
- * Fourier-space sums are timed on this context for Nb = Nb0 * 2^(k/2),
- * k = -6..2, around the drivers' default Nb0 = min(P*round(log2(Ns+Nt)),
- * (Ns+Nt)/4), each as the best of three blocks of three back-to-back
- * steps after a warm-up step, and the fastest is kept.  Every candidate meets the drivers'
- * truncation estimates at tol; candidates whose grid exceeds 4096^2 points
- * are skipped.  Then, at the chosen Nb, the spreading work split
- * (se2p_set_spread_chunk) is timed at the default, 128 and 512, and the
- * fastest is set on the context.  P is prm->P, or the drivers' 24 when 0.  prm->M must be 0
- * (the grid follows from the cell size).  On return *prm holds the chosen
- * candidate's parameters as se2p_select_params fills them for its Nb and P:
- * passing *prm on to se2p_slp_ewald / se2p_dlp_ewald reproduces them.
- * Choices are cached per process by (device, kind, Ns, Nt,
- * targets-are-sources, Lx, Ly, tol, P requested); a cache hit times nothing
- * and sets the cached spreading split on the context.
- * Points are the caller's (unwrapped) points, as for the driver entries.
- * Synchronous.  info may be NULL. */
+/* Chooses the drivers' free parameters for one geometry at prm->tol
+ * (required, 0 < tol < 1) by timing the real- plus Fourier-space sums on
+ * this context (each candidate: a warm-up step, then the best of three
+ * blocks of three back-to-back steps).  Three stages:
+ *  1. the points per cell Nb, which sets the cell grid (nside), the
+ *     splitting xi and the Fourier grid through the driver formulas
+ *     (StokesSLP_ewald_2p.m:33-104, StokesDLP_ewald_2p.m:33-125):
+ *     Nb = Nb0 * 2^(k/2), k = -6..2, around the drivers' default
+ *     Nb0 = min(P*round(log2(Ns+Nt)), (Ns+Nt)/4), one candidate per cell
+ *     grid, grids above 4096^2 points skipped.  Every candidate meets the
+ *     drivers' truncation estimates at tol.
+ *  2. at that Nb, the spreading work split (se2p_set_spread_chunk): the
+ *     default, 128 and 512; the fastest is set on the context.
+ *  3. only when prm->P is 0: supports P = 16 and 20 on grids 1, 1.2 and 1.4
+ *     times the drivers' grid (the grid override M).  A candidate faster
+ *     than the best so far is taken only if its Fourier-space sum differs
+ *     from the P = 24 one by at most tol/4 of max |ur + uk| (P = 24): the
+ *     Fourier-space sum converges as the support and the grid grow, so that
+ *     difference bounds the error the candidate adds.
+ * P is prm->P, or the drivers' 24 when 0.  prm->M must be 0 and
+ * prm->fixed_grid 0.  On return *prm holds the chosen parameters as
+ * se2p_select_params fills them for its Nb, P and M: passing *prm on to
+ * se2p_slp_ewald / se2p_dlp_ewald reproduces them.  Choices are cached per
+ * process by (device, kind, Ns, Nt, targets-are-sources, Lx, Ly, tol, P
+ * requested); a cache hit times nothing and sets the cached spreading split
+ * on the context.  Points are the caller's (unwrapped) points, as for the
+ * driver entries.  Synchronous.  info may be NULL. */
 int se2p_autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const double *ptar,
                   int64_t nt, const double *f, const double *n, double Lx, double Ly,
                   se2p_params *prm, se2p_tune_info *info);
+
+/* Multi-box tuning (no reference counterpart).  Writes each box's Ewald
+ * parameters (xi, nside_x/_y, eta, w, Mx, My, P) into its descriptor: the
+ * drivers' parameters at tol with P = 24 (se2p_select_params, Nb default)
+ * timed as one multi-box step (real- plus Fourier-space multi-box calls),
+ * then each box's points per cell tuned on its own as stage 1 of
+ * se2p_autotune, and the spreading work split (default, 128, 512) of the
+ * multi-box step at those parameters.  The tuned set is kept if its best
+ * step is faster than the drivers' set, otherwise the drivers' parameters
+ * and the default split are.  The split is set on the context.  Every box
+ * needs at least two points; points are wrapped, as for the multi-box
+ * entries; u is overwritten.  Device arrays only.  Synchronous; info may be
+ * NULL. */
+typedef struct se2p_tune_boxes_info {
+    double drivers_ms;                  /* multi-box step at the drivers' parameters */
+    double tuned_ms;                    /* best step at the parameters written */
+    int kept_drivers;                   /* 1: the drivers' parameters were kept */
+    int spread_chunk;                   /* the split set on the context */
+    int chunk[SE2P_TUNE_CHUNKS];        /* splits timed at the tuned cells */
+    double chunk_ms[SE2P_TUNE_CHUNKS];  /* their multi-box step, ms */
+} se2p_tune_boxes_info;
+
+int se2p_autotune_multibox(se2p_ctx *ctx, int kind, se2p_box_desc *boxes, int nbox, double tol,
+                           se2p_tune_boxes_info *info);
 
 /* ---- pair statistics (no reference counterpart) ------------------------- */
 /* Number of source-target pairs the real-space sum evaluates (pairs_in: same

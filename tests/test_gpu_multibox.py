@@ -107,10 +107,13 @@ def test_multibox_equals_single_and_oracle(gpu, kind):
             assert np.all(outs[i] == 0)
 
 
-def test_multibox_many_boxes_equals_single(gpu):
-    boxes = W.multibox_config(seed=4, nbox=32)
-    _say(f"[multibox 32] N total={sum(len(b['psrc']) for b in boxes)}")
-    for kind in ("slp_real", "slp_kspace"):
+@pytest.mark.parametrize("kernel", ["slp", "dlp"])
+def test_multibox_many_boxes_equals_single(gpu, kernel):
+    """The 32-box workload (L ~ U[0.5, 4], N log-uniform in [1e3, 2e5]), each
+    kernel with its own driver parameters per box."""
+    boxes = W.multibox_config(seed=4, nbox=32, kind=kernel)
+    _say(f"[multibox 32 {kernel}] N total={sum(len(b['psrc']) for b in boxes)}")
+    for kind in (kernel + "_real", kernel + "_kspace"):
         outs = _multi(gpu, kind, boxes)
         for i, b in enumerate(boxes):
             one = _single(gpu, kind, b).cpu().numpy()
@@ -152,3 +155,40 @@ def test_multibox_packs_with_mixed_grids(gpu):
             one = _single(gpu, kind, b).cpu().numpy()
             assert np.array_equal(outs[i], one), (kind, i, relerr(outs[i], one))
         _say(f"[multibox packs] {kind} identical to single-box calls")
+
+
+def test_multibox_autotune(gpu):
+    """se2p_autotune_multibox on 8 boxes of the many-box workload (Stokeslet,
+    tol 1e-10): the parameters it writes give a total (real + Fourier) within
+    5e-10 of the drivers' parameters' total on every box (the Ewald total
+    does not depend on the split; each meets tol), and the smallest box's
+    real- and Fourier-space sums at the tuned parameters match the oracle at
+    1e-12."""
+    pe = gpu
+    boxes = W.multibox_config(seed=4, nbox=8)
+    _say(f"[multibox autotune] N total={sum(len(b['psrc']) for b in boxes)}")
+    dev_boxes = []
+    for b in boxes:
+        d = {k: b[k] for k in ("Lx", "Ly")}
+        d["psrc"] = d["ptar"] = _dev(b["psrc"])
+        d["f"] = _dev(b["f"])
+        dev_boxes.append(d)
+    ctx = pe.Context(0)
+    info = pe.autotune_multibox("slp", dev_boxes, tol=1e-10, ctx=ctx)
+    _say(f"  drivers {info['drivers_ms']:.3f} ms, tuned {info['tuned_ms']:.3f} ms, split "
+         f"{info['spread_chunk']}, kept drivers {info['kept_drivers']}")
+    assert info["spread_chunk"] in (0, 128, 512)
+    assert info["tuned_ms"] <= info["drivers_ms"]
+    tuned = []
+    for b, d in zip(boxes, dev_boxes):
+        t = dict(b)
+        t.update({k: d[k] for k in ("xi", "nside_x", "nside_y", "eta", "w", "Mx", "My", "P")})
+        tuned.append(t)
+    ur_t, uk_t = _multi(pe, "slp_real", tuned), _multi(pe, "slp_kspace", tuned)
+    ur_d, uk_d = _multi(pe, "slp_real", boxes), _multi(pe, "slp_kspace", boxes)
+    for i in range(len(boxes)):
+        e = relerr(ur_t[i] + uk_t[i], ur_d[i] + uk_d[i])
+        assert e <= 5e-10, (i, e, tuned[i]["nside_x"], boxes[i]["nside_x"])
+    i = int(np.argmin([len(b["psrc"]) for b in boxes]))
+    assert relerr(ur_t[i], _oracle("slp_real", tuned[i])) <= 1e-12
+    assert relerr(uk_t[i], _oracle("slp_kspace", tuned[i])) <= 1e-12
