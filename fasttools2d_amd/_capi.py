@@ -41,6 +41,7 @@ SIGNATURES = {
     "se2p_set_async": (_I, [_P, _I]),
     "se2p_set_host_pointers": (_I, [_P, _I]),
     "se2p_set_sort_mode": (_I, [_P, _I]),
+    "se2p_set_adjoint_passes": (_I, [_P, _I]),
     "se2p_set_keep": (_I, [_P, _I]),
     "se2p_set_spread_chunk": (_I, [_P, _I]),
     "se2p_set_timing": (_I, [_P, _I]),

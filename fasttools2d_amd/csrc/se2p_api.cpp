@@ -401,6 +401,14 @@ int se2p_set_sort_mode(se2p_ctx *ctx, int mode)
     return SE2P_OK;
 }
 
+int se2p_set_adjoint_passes(se2p_ctx *ctx, int passes)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    if (passes != 1 && passes != 2) return fail(ctx, SE2P_ERR_ARG, "passes must be 1 or 2 (got %d)", passes);
+    ctx->split_adjoint = passes == 2;
+    return SE2P_OK;
+}
+
 int se2p_set_async(se2p_ctx *ctx, int a)
 {
     if (!ctx) return SE2P_ERR_ARG;

@@ -84,6 +84,7 @@ struct se2p_ctx {
     hipEvent_t switch_ev = nullptr;   // orders a newly bound stream after the previous one
     int plan_slot = 0;        // 0: the context's own plans
     bool keep = false;        // forward calls keep their sorted state (se2p_set_keep)
+    bool split_adjoint = false;   // se2p_set_adjoint_passes(ctx, 2)
     se2p::KeptReal *kept_real = nullptr;
     se2p::KeptK *kept_k = nullptr;
     // This device's address of the fixed scratch of se2p_dlp_real_noalloc

@@ -920,6 +920,76 @@ __global__ void k_filter_dlp_adj(FilterArgs a)
     a.spec[2 * a.sstride + p] = make_double2(-o3i, o3r);
 }
 
+// Forward and transposed filters in one pass over the spectra of one
+// spreading of 2 + 2 (Stokeslet) or 3 + 2 (stresslet) grids at the same
+// points: the densities' spectra [0, ncomp) filtered as k_filter_slp /
+// k_filter_dlp into [0, 2), the spread du's spectra [ncomp, ncomp + 2) by the
+// transposed filter (k_filter_slp again, the Stokeslet filter being its own
+// transpose; k_filter_dlp_adj for the stresslet) into [2, 2 + ncomp).  Each
+// thread reads its element of every input before it writes any output.
+template <int KIND>
+__global__ void k_filter_both(FilterArgs a)
+{
+    const int Mh = a.My / 2 + 1;
+    const int64_t n = (int64_t)a.Mx * Mh;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double2 *S = a.spec + p;
+    const int64_t ss = a.sstride;
+    if constexpr (KIND == 0) {
+        const int j = (int)(p / Mh), k = (int)(p % Mh);
+        const double k1 = wavenumber(j, a.Mx, a.Lx);
+        const double k2 = 2.0 * SE2P_PI / a.Ly * k;
+        const double K2 = k1 * k1 + k2 * k2;
+        double e = 0.0, a11 = 0.0, a22 = 0.0, a12 = 0.0;
+        if (p != 0) {
+            e = (1.0 / (K2 * K2) + 0.25 / (K2 * a.xi * a.xi)) *
+                exp(-0.25 * (1 - a.eta) / (a.xi * a.xi) * K2) * a.scale;
+            const bool nyx = (2 * j == a.Mx), nyy = (2 * k == a.My);
+            a11 = k2 * k2;
+            a22 = k1 * k1;
+            a12 = (nyx != nyy) ? 0.0 : k1 * k2;
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const double2 h1 = S[(2 * r) * ss], h2 = S[(2 * r + 1) * ss];
+            S[(2 * r) * ss] = make_double2((a11 * h1.x - a12 * h2.x) * e, (a11 * h1.y - a12 * h2.y) * e);
+            S[(2 * r + 1) * ss] = make_double2((a22 * h2.x - a12 * h1.x) * e, (a22 * h2.y - a12 * h1.y) * e);
+        }
+    } else {
+        const DlpCoef c = dlp_coef(a, (int)(p / Mh), (int)(p % Mh));
+        const double2 A = S[0], B = S[ss], C = S[2 * ss], Z1 = S[3 * ss], Z2 = S[4 * ss];
+        const double o1r = c.et * (c.b111 * A.x + c.b112 * B.x + c.b122 * C.x);
+        const double o1i = c.et * (c.b111 * A.y + c.b112 * B.y + c.b122 * C.y);
+        const double o2r = c.et * (c.b112 * A.x + c.b122 * B.x + c.b222 * C.x);
+        const double o2i = c.et * (c.b112 * A.y + c.b122 * B.y + c.b222 * C.y);
+        const double p1r = c.et * (c.b111 * Z1.x + c.b112 * Z2.x);
+        const double p1i = c.et * (c.b111 * Z1.y + c.b112 * Z2.y);
+        const double p2r = c.et * (c.b112 * Z1.x + c.b122 * Z2.x);
+        const double p2i = c.et * (c.b112 * Z1.y + c.b122 * Z2.y);
+        const double p3r = c.et * (c.b122 * Z1.x + c.b222 * Z2.x);
+        const double p3i = c.et * (c.b122 * Z1.y + c.b222 * Z2.y);
+        S[0] = make_double2(o1i, -o1r);
+        S[ss] = make_double2(o2i, -o2r);
+        S[2 * ss] = make_double2(-p1i, p1r);
+        S[3 * ss] = make_double2(-p2i, p2r);
+        S[4 * ss] = make_double2(-p3i, p3r);
+    }
+}
+
+// du of the points in bin order as two density columns of the gridding
+// data: out[i] = du[2 perm[i]], out[n + i] = du[2 perm[i] + 1].
+__global__ void k_gather_du_cols(const double *__restrict__ du, const int *__restrict__ perm, int n,
+                                 double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = perm[i];
+    const double2 v = *(const double2 *)(du + 2 * (int64_t)o);
+    out[i] = v.x;
+    out[n + i] = v.y;
+}
+
 // ---- interpolation ---------------------------------------------------------
 struct InterpArgs {
     const double *ptar;
@@ -1658,6 +1728,8 @@ static int spread(se2p_ctx *ctx, const KLayout &L, const GridPass &gp, const Spr
     switch (gp.nq) {
     case 2: SE2P_SPREAD(2); break;
     case 3: SE2P_SPREAD(3); break;
+    case 4: SE2P_SPREAD(4); break;
+    case 5: SE2P_SPREAD(5); break;
     case 8: SE2P_SPREAD(8); break;
     case 12: SE2P_SPREAD(12); break;
     default: return fail(ctx, SE2P_ERR_ARG, "unsupported grid count %d", gp.nq);
@@ -1667,7 +1739,7 @@ static int spread(se2p_ctx *ctx, const KLayout &L, const GridPass &gp, const Spr
     return 0;
 }
 
-enum FilterKind { FILTER_SLP, FILTER_DLP, FILTER_DLP_ADJ };
+enum FilterKind { FILTER_SLP, FILTER_DLP, FILTER_DLP_ADJ, FILTER_BOTH_SLP, FILTER_BOTH_DLP };
 
 // Forward transforms, filter, inverse transforms; grids in, filtered real
 // grids out (spec is scratch).
@@ -1700,7 +1772,9 @@ static int transform_filter(se2p_ctx *ctx, const KLayout &L, const KGeom &g, Fft
     const unsigned fblocks = (unsigned)((L.GH + 255) / 256);
     if (fk == FILTER_SLP) hipLaunchKernelGGL(k_filter_slp, dim3(fblocks), dim3(256), 0, st, fa);
     else if (fk == FILTER_DLP) hipLaunchKernelGGL(k_filter_dlp, dim3(fblocks), dim3(256), 0, st, fa);
-    else hipLaunchKernelGGL(k_filter_dlp_adj, dim3(fblocks), dim3(256), 0, st, fa);
+    else if (fk == FILTER_DLP_ADJ) hipLaunchKernelGGL(k_filter_dlp_adj, dim3(fblocks), dim3(256), 0, st, fa);
+    else if (fk == FILTER_BOTH_SLP) hipLaunchKernelGGL(k_filter_both<0>, dim3(fblocks), dim3(256), 0, st, fa);
+    else hipLaunchKernelGGL(k_filter_both<1>, dim3(fblocks), dim3(256), 0, st, fa);
     SE2P_HIP(hipGetLastError());
     stage_end(ctx, ST_FILTER);
 
@@ -2234,6 +2308,98 @@ static void launch_interp_adj(hipStream_t st, const KLayout &L, const GridPass &
                            dim3(256), 0, st, ia);
 }
 
+// Targets are the sources, both sides wanted and no kept grids: the two
+// passes share their points, so they run as one.  The densities' ncomp
+// gridding columns and du's two are spread together (one sort and gather, one
+// spreading of ncomp + 2 grids), one batched transform pair carries both
+// filters (k_filter_both), and the interpolation forms the target part from
+// grids 0, 1 and the source part from grids 2.. .
+static int kspace_adjoint_same(se2p_ctx *ctx, int kind, const KGeom &g, const double *f,
+                               const double *nv, const double *du, double *dpsrc, double *dptar,
+                               double *df, double *dn)
+{
+    hipStream_t st = ctx->stream;
+    const int64_t n = g.ns;
+    const int ncomp = kind == 0 ? 2 : 3, nq = ncomp + 2;
+    KRun r;
+    krun_init(r, kind, g, f, nv, 1, 2 * n, 2 * n, nullptr, ctx->spread_chunk);
+    plan_pass(r.L, nq, 1, n, n, ctx->spread_chunk, &r.gp);
+    // tile interpolation (and its work maps) where the staged grids fit
+    const bool tile = interp_adj_lds(r.L.P, 2) <= kLdsMax;
+    r.maps = r.gp.strip || tile;
+    if (int rc = get_plans(ctx, r.L.Mx, r.L.My, nq, nq, &r.plans)) return rc;
+    // the single-density run's buffers, with nq columns of gridding data,
+    // grids and spectra
+    r.ncomp = nq;
+    Carve sizing;
+    r.layout(sizing);
+    if (int rc = ensure_workspace(ctx, sizing.off)) return rc;
+    Carve cv;
+    cv.base = ctx->ws;
+    r.layout(cv);
+    r.ncomp = ncomp;
+
+    stage_begin(ctx, ST_SORT);
+    if (int rc = krun_sort(ctx, r)) return rc;
+    stage_end(ctx, ST_SORT);
+
+    stage_begin(ctx, ST_SPREAD);
+    krun_gather(ctx, r);
+    hipLaunchKernelGGL(k_gather_du_cols, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, du,
+                       r.kss.perm, (int)n, r.q + (int64_t)ncomp * n);
+    if (int rc = spread(ctx, r.L, r.gp, r.sa)) return rc;
+    stage_end(ctx, ST_SPREAD);
+
+    if (int rc = transform_filter(ctx, r.L, g, r.plans, kind == 0 ? FILTER_BOTH_SLP : FILTER_BOTH_DLP,
+                                  1, r.grids, r.spec))
+        return rc;
+
+    stage_begin(ctx, ST_INTERP);
+    const KLayout &L = r.L;
+    InterpAdjArgs a{};
+    a.pts = g.psrc;
+    a.perm = r.kss.perm;
+    a.boff = r.kss.off;
+    a.nbx = L.nbx;
+    a.nby = L.nby;
+    a.chunk_off = r.gp.ioffc;
+    a.block_tile = r.gp.imap;
+    a.ntx = L.ntx;
+    a.nty = L.nty;
+    a.n = n;
+    a.grids = r.grids;
+    a.gstride = L.G;
+    a.Mx = L.Mx;
+    a.My = L.My;
+    a.P = L.P;
+    a.Lx = g.Lx;
+    a.Ly = g.Ly;
+    a.h = L.h;
+    a.c = L.c;
+    a.w = g.w;
+    a.ch2 = L.ch2;
+    a.scale = L.interp_scale;
+    a.du = du;
+    a.f = f;
+    a.nv = nv;
+    // target part from grids 0, 1, source part from grids 2.. (one launch of
+    // all ncomp + 2 grids holds too many registers per thread)
+    InterpAdjArgs at = a, as = a;
+    at.dpos = dptar;
+    as.grids = r.grids + 2 * L.G;
+    as.dpos = dpsrc;
+    as.df = df;
+    as.dn = dn;
+    launch_interp_adj<2, ADJ_TARGET>(st, L, r.gp, tile, at);
+    if (kind == 0)
+        launch_interp_adj<2, ADJ_SOURCE_SLP>(st, L, r.gp, tile, as);
+    else
+        launch_interp_adj<3, ADJ_SOURCE_DLP>(st, L, r.gp, tile && interp_adj_lds(L.P, 3) <= kLdsMax, as);
+    SE2P_HIP(hipGetLastError());
+    stage_end(ctx, ST_INTERP);
+    return 0;
+}
+
 int kspace_adjoint(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const double *nv,
                    const double *du, double *dpsrc, double *dptar, double *df, double *dn)
 {
@@ -2256,6 +2422,8 @@ int kspace_adjoint(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, con
     // the filtered grids the target pass interpolates
     const KeptK *kept = same_points && ctx->kept_k && ctx->kept_k->match(kind, g, f, nv)
                             ? ctx->kept_k : nullptr;
+    if (same_points && !kept && dptar && want_src && !ctx->split_adjoint)
+        return kspace_adjoint_same(ctx, kind, g, f, nv, du, dpsrc, dptar, df, dn);
     GridPass gt, gs;   // target pass, source pass
     plan_pass(L, ncomp, 1, ns, nt, ctx->spread_chunk, &gt);
     plan_pass(L, 2, 1, nt, ns, ctx->spread_chunk, &gs);

@@ -111,6 +111,12 @@ class Context:
         kernels.  Every mode gives bitwise identical results."""
         self._check(self._lib.se2p_set_sort_mode(self.handle, int(mode)))
 
+    def set_adjoint_passes(self, passes: int) -> None:
+        """Fourier-space backward calls with targets = sources and both
+        sides wanted: 1 (default) runs both passes as one, 2 one after the
+        other (se2p_set_adjoint_passes).  Results agree to rounding."""
+        self._check(self._lib.se2p_set_adjoint_passes(self.handle, int(passes)))
+
     def stage_times(self) -> dict:
         buf = (ctypes.c_float * len(_capi.STAGES))()
         self._check(self._lib.se2p_stage_times(self.handle, buf, len(_capi.STAGES)))

@@ -87,6 +87,13 @@ int se2p_set_host_pointers(se2p_ctx *ctx, int enable);
 #define SE2P_SORT_RADIX 1
 #define SE2P_SORT_BOX_KERNEL 2
 int se2p_set_sort_mode(se2p_ctx *ctx, int mode);
+/* Passes of the Fourier-space backward call when the targets are the sources
+ * and both sides' gradients are wanted without kept grids: 1 (the default)
+ * runs both passes as one (the densities and du spread together, one
+ * transform pair with both filters, one interpolation); 2 runs the target
+ * pass and the source pass one after the other.  The results agree to
+ * rounding.  For validation and measurement.  No reference counterpart. */
+int se2p_set_adjoint_passes(se2p_ctx *ctx, int passes);
 /* Forward state kept for the backward call.  With enable != 0, a
  * single-density forward call whose targets are its sources (ptar == psrc,
  * Nt == Ns; se2p_slp_real, se2p_dlp_real, se2p_slp_kspace, se2p_dlp_kspace)

@@ -198,3 +198,54 @@ def test_adjoint_ellipses_full_size(gpu):
         if kind == "dlp_real":
             tot = g["psrc"].sum(axis=0) + g["ptar"].sum(axis=0)
             assert np.abs(tot).max() <= 1e-12 * np.abs(g["psrc"]).sum(), tot
+
+
+def test_adjoint_one_pass_equals_two_passes(gpu):
+    """Targets = sources with both sides wanted: the Fourier-space backward
+    runs both passes as one (densities and du spread together, one transform
+    pair with both filters, one interpolation).  Against the two-pass form
+    (se2p_set_adjoint_passes(ctx, 2)): within 1e-13 on every output, and both
+    within 1e-12 of the oracle; grids of 48 (strip spreading, tile
+    interpolation) and 14 and 24 (support wrapping onto itself, generic
+    spreading); a partial output set gives the full call's values bitwise."""
+    pe = gpu
+    rng = np.random.default_rng(57)
+    P, L = 24, 1.0
+    m = 0.95 * np.sqrt(np.pi * P)
+    one, two = pe.Context(0), pe.Context(0)
+    two.set_adjoint_passes(2)
+    for N, MM in ((800, 48), (40, 14), (150, 24)):
+        x = Pm.wrap(rng.random((N, 2)), L)
+        f = rng.standard_normal((N, 2))
+        nv = rng.standard_normal((N, 2))
+        nv /= np.linalg.norm(nv, axis=1, keepdims=True)
+        du = rng.standard_normal((N, 2))
+        w = P * L / MM / 2
+        xi = np.sqrt(0.5) * m / (2 * w)
+        ps, ff, nn, dd = _dev(x), _dev(f), _dev(nv), _dev(du)
+        for kind in ("slp_kspace", "dlp_kspace"):
+            _say(f"[one pass] {kind} N={N} M={MM}")
+
+            def run(ctx, wrt=("psrc", "ptar", "f", "n")):
+                if kind == "slp_kspace":
+                    g = pe.mex_stokes_slp_kspace_backward(ps, ps, xi, 0.5, ff, MM, MM, L, L, w, P, dd,
+                                                          wrt=tuple(k for k in wrt if k != "n"),
+                                                          ctx=ctx)
+                else:
+                    g = pe.mex_stokes_dlp_kspace_backward(ps, ps, xi, 0.5, ff, nn, MM, MM, L, L, w, P,
+                                                          dd, wrt=wrt, ctx=ctx)
+                return _np(g)
+
+            g1, g2 = run(one), run(two)
+            if kind == "slp_kspace":
+                c = cref.slp_kspace_adj(x, x, xi, 0.5, f, MM, MM, L, L, w, P, du)
+            else:
+                c = cref.dlp_kspace_adj(x, x, xi, 0.5, f, nv, MM, MM, L, L, w, P, du)
+            names = {"dpsrc": "psrc", "dptar": "ptar", "df": "f", "dn": "n"}
+            for k, v in c.items():
+                k = names[k]
+                e12, e1 = relerr(g1[k], g2[k]), relerr(g1[k], v)
+                assert e12 <= 1e-13 and e1 <= TOL and relerr(g2[k], v) <= TOL, (kind, N, MM, k, e12, e1)
+            part = run(one, wrt=("ptar", "f"))
+            for k in part:
+                assert np.array_equal(part[k], g1[k]), (kind, k)

@@ -1,9 +1,11 @@
 """Kept forward state (se2p_set_keep): backward calls that reuse the forward
 call's sorted state give bitwise the results of backward calls that rebuild
-it, for all four sums; a backward call whose arguments differ from the kept
-forward call's (another density array) does not reuse it; forward results
-are unchanged by keeping; the stresslet benchmark (N = 1e5, grid 512^2)
-with kept state matches the oracle adjoints on a target subset."""
+it in two passes (se2p_set_adjoint_passes(ctx, 2)), for all four sums, and
+agree to 1e-13 with the default one-pass rebuild; a backward call whose
+arguments differ from the kept forward call's (another density array) does
+not reuse it; forward results are unchanged by keeping; the stresslet
+benchmark (N = 1e5, grid 512^2) with kept state matches the oracle adjoints
+on a target subset."""
 import numpy as np
 import pytest
 import torch
@@ -61,8 +63,11 @@ def test_kept_backward_is_bitwise_the_rebuilt_one(gpu, kind):
     du = _t(np.random.default_rng(62).standard_normal((3000, 2)))
     print(f"[keep {kind}]", flush=True)
     plain = pe.Context(0)
+    plain.set_adjoint_passes(2)
     keep = pe.Context(0)
+    keep.set_adjoint_passes(2)
     keep.set_keep(True)
+    one = pe.Context(0)
     u0 = _fwd(pe, kind, ps, f, nv, p, 1.0, 1.0, plain).cpu().numpy()
     g0 = _bwd(pe, kind, ps, f, nv, du, p, 1.0, 1.0, plain)
     u1 = _fwd(pe, kind, ps, f, nv, p, 1.0, 1.0, keep).cpu().numpy()
@@ -70,6 +75,11 @@ def test_kept_backward_is_bitwise_the_rebuilt_one(gpu, kind):
     assert np.array_equal(u0, u1)
     for name in g0:
         assert np.array_equal(g0[name], g1[name]), name
+    # the default rebuild (one pass in Fourier space) agrees to rounding
+    g3 = _bwd(pe, kind, ps, f, nv, du, p, 1.0, 1.0, one)
+    for name in g0:
+        assert relerr(g3[name], g0[name]) <= 1e-13, name
+    one.close()
     # a second backward with another du reuses the same kept state
     du2 = _t(np.random.default_rng(63).standard_normal((3000, 2)))
     assert all(np.array_equal(a, b) for a, b in
