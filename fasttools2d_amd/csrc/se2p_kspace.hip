@@ -410,9 +410,20 @@ __host__ __device__ inline size_t strip_lds_doubles(int P, int NQ)
 // LDS reads of the next sources run ahead.
 // One workgroup of the strip spreading: workgroup index b of the box's chunk
 // list, right-hand-side group grp.
-template <int NQ, bool WIDE>
+typedef double se2p_v4d __attribute__((ext_vector_type(4)));
+
+// MF (WIDE only): the source loop on the matrix cores.  For a group of four
+// sources g = 0..3 of the batch the tile update is the product
+//   H[y][x] += sum_g wy_g(y) (q_g wx_g(x)),   y, x = 0..15 (tile offsets),
+// one v_mfma_f64_16x16x4f64 per grid: lane l supplies A[y = l % 16][g = l / 16]
+// (a y weight) and B[g = l / 16][x = l % 16] (a density times an x weight),
+// and receives H[y = l / 16 + 4 r][x = l % 16] in accumulator r.  A weight
+// outside the source's support, and every weight of a source past the
+// batch's end, is zero.
+template <int NQ, bool WIDE, bool MF>
 __device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b, const int grp)
 {
+    static_assert(!MF || WIDE, "the matrix-core loop needs one support interval per axis");
     constexpr int kSB = strip_batch(NQ);
     extern __shared__ double lds[];
     const int P1 = a.P + 1, PW = P1 + 2 * kPad;
@@ -469,6 +480,9 @@ __device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b
     for (int k = 0; k < 4; ++k)
 #pragma unroll
         for (int c = 0; c < NQ; ++c) acc[k][c] = 0.0;
+    se2p_v4d cm[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) cm[c] = se2p_v4d{0.0, 0.0, 0.0, 0.0};
 
     // Wavefront w takes batches w, w+4, ... of the chunk.  Each wave fills and
     // reads only its own batch area, so the waves run independently, with
@@ -554,7 +568,25 @@ __device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b
         }
         wave_lds_sync();
         const int nb = min(kSB, c1 - base);
-        if constexpr (WIDE && NQ <= 3) {
+        if constexpr (MF) {
+            const int t = lane & 15, g = lane >> 4;
+            for (int s0 = 0; s0 < nb; s0 += 4) {   // uniform trip count: every lane runs the MFMAs
+                const int sg = s0 + g;
+                const bool valid = sg < nb;
+                const int sc = valid ? sg : 0;   // an LDS index inside the batch area
+                const int2 dd = *(const int2 *)(AXY + 2 * sc);
+                const int iu = t + dd.x, ju = t + dd.y;
+                const bool hx = valid && (unsigned)iu <= (unsigned)a.P;
+                const bool hy = (unsigned)ju <= (unsigned)a.P;
+                const double wx = WX[sc * P1 + (hx ? iu : 0)];
+                const double wy = WY[sc * PW + kPad + (hy ? ju : 0)];
+                const double av = hy ? wy : 0.0;
+                const double bx = hx ? wx : 0.0;
+#pragma unroll
+                for (int c = 0; c < NQ; ++c)
+                    cm[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Q[sc * NQ + c] * bx, cm[c], 0, 0, 0);
+            }
+        } else if constexpr (WIDE && NQ <= 3) {
             // the batch's densities and tile offsets are the same for every
             // lane: lane s holds source s's, and v_readlane hands them to the
             // source loop as wave-uniform values (an LDS read of them would
@@ -633,6 +665,13 @@ __device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b
         }
         wave_lds_sync();   // reads of this batch done before the next fill
     }
+    if constexpr (MF) {
+        // accumulator r of lane l holds node (x, y) = (l % 16, l / 16 + 4 r)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) acc[k][c] = cm[c][k];
+    }
     __syncthreads();   // every wave is done with its batch area (reused below)
     // fixed-order reduction of the four wavefronts' tiles, one wave at a time
     double *red = lds;   // [64][4*NQ]; the batch areas are dead now
@@ -654,7 +693,26 @@ __device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b
     }
     if (wave != 0) return;
     double *gg = a.grids + (int64_t)grp * NQ * a.gstride;
-    if (nchunk == 1) {
+    if (MF && nchunk == 1) {
+        if (X < a.Mx) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int Y = Y0 + (lane >> 4) + 4 * k;
+                if (Y >= a.My) break;
+                const int64_t idx = (int64_t)X * a.My + Y;
+#pragma unroll
+                for (int c = 0; c < NQ; ++c) gg[c * a.gstride + idx] = acc[k][c];
+            }
+        }
+    } else if (MF) {
+        // partial tile in the layout k_spread_combine reads (node (x, y) at
+        // lane x + 16 (y / 4), slot y % 4)
+        double *pt = a.partial + ((int64_t)grp * a.max_chunks + b) * 256 * NQ;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) pt[(c * 4 + (lane >> 4)) * 64 + (lane & 15) + 16 * k] = acc[k][c];
+    } else if (nchunk == 1) {
         if (X < a.Mx) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -675,10 +733,13 @@ __device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b
     }
 }
 
+// Strip spreading on grids with Mx, My >= P + 16 uses the matrix cores.
+constexpr bool kSpreadMfma = true;
+
 template <int NQ, bool WIDE>
 __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
 {
-    spread_strip_wg<NQ, WIDE>(a, blockIdx.x, blockIdx.y);
+    spread_strip_wg<NQ, WIDE, WIDE && kSpreadMfma>(a, blockIdx.x, blockIdx.y);
 }
 
 // Many boxes in one launch: box j owns workgroups [off[j], off[j+1]) and each
@@ -687,7 +748,7 @@ template <int NQ, bool WIDE>
 __global__ __launch_bounds__(256) void k_spread_strip_boxes(SpreadPack p)
 {
     const int j = pack_slot(p.off, p.n, blockIdx.x);
-    spread_strip_wg<NQ, WIDE>(p.a[j], blockIdx.x - p.off[j], blockIdx.y);
+    spread_strip_wg<NQ, WIDE, WIDE && kSpreadMfma>(p.a[j], blockIdx.x - p.off[j], blockIdx.y);
 }
 
 // Tiles whose candidates were split over several workgroups: add the partial
