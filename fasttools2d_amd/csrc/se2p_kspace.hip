@@ -1896,6 +1896,7 @@ struct KRun {
     bool maps = false;          // tile work maps are used (strip spreading or tile interpolation)
     bool gathered = false;      // the fused sort gathered the gridding data
     bool want_jac = false;      // keep the output gradients for a backward call
+    bool shared_sort = false;   // bins and gridding data come from a sort of many boxes
     KLayout L{};
     GridPass gp{};
     KeySort kss{}, kst{};
@@ -1911,15 +1912,17 @@ struct KRun {
     {
         const int64_t ns = g.ns, nt = g.nt;
         if (want_jac) jac = cv.take<double>(4 * (size_t)nt);
-        char *p = cv.take<char>(keysort_bytes(ns, L.nbins));
-        if (p) keysort_carve(p, ns, L.nbins, &kss);
-        if (!same_points) {
-            p = cv.take<char>(keysort_bytes(nt, L.nbins));
-            if (p) keysort_carve(p, nt, L.nbins, &kst);
+        if (!shared_sort) {
+            char *p = cv.take<char>(keysort_bytes(ns, L.nbins));
+            if (p) keysort_carve(p, ns, L.nbins, &kss);
+            if (!same_points) {
+                p = cv.take<char>(keysort_bytes(nt, L.nbins));
+                if (p) keysort_carve(p, nt, L.nbins, &kst);
+            }
+            anchor = cv.take<int2>(ns);
+            fgg = cv.take<double>(3 * (size_t)ns);
+            q = cv.take<double>((size_t)ncomp * nrp * ns);
         }
-        anchor = cv.take<int2>(ns);
-        fgg = cv.take<double>(3 * (size_t)ns);
-        q = cv.take<double>((size_t)ncomp * nrp * ns);
         grids = cv.take<double>((size_t)ncomp * nrp * L.G);
         spec = cv.take<double2>((size_t)ncomp * nrp * L.GH);
         carve_pass(cv, L, &gp);
@@ -2181,6 +2184,220 @@ int kspace_sum(se2p_ctx *ctx, int kind, const KGeom &g, const double *f, const d
     return 0;
 }
 
+// ---- many boxes: one bin sort ----------------------------------------------------
+// When every box's targets are its sources, the bins of all boxes come from
+// one stable sort of global keys bin_base[box] + bin, so each box's points
+// keep the order, and its gridding data the values, of a single-box call.
+// The gridding data are gathered into arrays over all points (sorted
+// positions, component stride n); a box's bin offsets are its slice of the
+// global offsets (global positions), and its permutation entries are
+// rewritten to box-local point indices.
+struct MbBox {
+    const double *p, *f, *nv;
+    double Lx, Ly, h, c, w;
+    int P, Mx, My, nby;
+    int s_base, bin_base;
+};
+struct MbBoxBatch {
+    MbBox b[kPack];
+    int first, count;
+};
+static_assert(sizeof(MbBoxBatch) <= 4096, "kernel argument size");
+
+__global__ void k_write_mb(MbBoxBatch bb, MbBox *__restrict__ tab, int *__restrict__ s_off, int nbox,
+                           int n)
+{
+    const int i = threadIdx.x;
+    if (i < bb.count) {
+        tab[bb.first + i] = bb.b[i];
+        s_off[bb.first + i] = bb.b[i].s_base;
+    }
+    if (i == 0 && bb.first + bb.count == nbox) s_off[nbox] = n;
+}
+
+// Box of global point g: the largest b with s_off[b] <= g (every box has
+// points, so the offsets ascend strictly).
+__device__ __forceinline__ int mb_box_of(const int *s_off, int nbox, int g)
+{
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_off[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct MbKeyFn {
+    const MbBox *tab;
+    const int *s_off;
+    int nbox;
+    __device__ int operator()(int g) const
+    {
+        const int b = mb_box_of(s_off, nbox, g);
+        const MbBox &B = tab[b];
+        const BinKeyFn kf{B.p, B.Lx, B.Ly, B.h, B.P, B.Mx, B.My, B.nby};
+        return B.bin_base + kf(g - s_off[b]);
+    }
+};
+
+__global__ void k_mb_keys(MbKeyFn kf, int n, int *__restrict__ keys)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) keys[g] = kf(g);
+}
+
+// Gridding data at sorted position i (as a single-box gather) and the
+// permutation entry made box-local.
+template <int KIND>
+__global__ void k_mb_gather(const MbBox *__restrict__ tab, const int *__restrict__ s_off, int nbox,
+                            int n, int *__restrict__ perm, int2 *__restrict__ anchor,
+                            double *__restrict__ fgg, double *__restrict__ q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = perm[i];
+    const int b = mb_box_of(s_off, nbox, g);
+    const MbBox &B = tab[b];
+    const int o = g - s_off[b];
+    const int64_t nsb = s_off[b + 1] - s_off[b];
+    gather_grid_source<KIND>(B.p, B.f, B.nv, o, i, n, 1, 1, 2 * nsb, B.Lx, B.Ly, B.h, B.P, B.Mx,
+                             B.My, B.c, B.w, anchor, fgg, q);
+    perm[i] = o;
+}
+
+// Tile work lists of up to kPack boxes (one tile per workgroup) and their
+// chunk plans (one plan per workgroup).
+struct TileWorkPack {
+    SpreadArgs a[kPack];
+    int *scnt[kPack], *icnt[kPack], *rs[kPack], *rl[kPack];
+    int off[kPack + 1];
+    int n;
+};
+static_assert(sizeof(TileWorkPack) <= 4096, "kernel argument size");
+
+__global__ __launch_bounds__(64) void k_tile_work_boxes(TileWorkPack p)
+{
+    const int j = pack_slot(p.off, p.n, blockIdx.x);
+    tile_work(p.a[j], p.a[j].boff, blockIdx.x - p.off[j], threadIdx.x, p.scnt[j], p.icnt[j],
+              p.rs[j], p.rl[j]);
+}
+
+struct PlanPack {
+    PlanArgs p[2 * kPack];
+    int n;
+};
+static_assert(sizeof(PlanPack) <= 4096, "kernel argument size");
+
+__global__ __launch_bounds__(256) void k_plan_boxes(PlanPack pk)
+{
+    __shared__ int wsum[4];
+    if ((int)blockIdx.x < pk.n) plan_body<4>(pk.p[blockIdx.x], wsum);
+}
+
+// The one-sort path's buffers over all points.
+struct MbSort {
+    int n = 0, nkeys = 0;
+    MbBox *tab = nullptr;
+    int *s_off = nullptr;
+    KeySort ks{};
+    int2 *anchor = nullptr;
+    double *fgg = nullptr, *q = nullptr;
+    void layout(Carve &cv, int nbox, int ncomp)
+    {
+        tab = cv.take<MbBox>((size_t)nbox);
+        s_off = cv.take<int>((size_t)nbox + 1);
+        char *p = cv.take<char>(keysort_bytes(n, nkeys));
+        if (p) keysort_carve(p, n, nkeys, &ks);
+        anchor = cv.take<int2>((size_t)n);
+        fgg = cv.take<double>(3 * (size_t)n);
+        q = cv.take<double>((size_t)ncomp * n);
+    }
+};
+
+// Sorts and gathers the points of every run and builds their work maps; the
+// runs then spread and interpolate as after krun_sort.
+static int mb_sort(se2p_ctx *ctx, int kind, std::vector<KRun> &runs, MbSort &m)
+{
+    hipStream_t st = ctx->stream;
+    const int nrun = (int)runs.size();
+    for (int b0 = 0; b0 < nrun; b0 += kPack) {
+        MbBoxBatch bb{};
+        bb.first = b0;
+        bb.count = std::min(kPack, nrun - b0);
+        int s_base = 0, bin_base = 0;
+        for (int k = 0; k < b0; ++k) {
+            s_base += (int)runs[(size_t)k].g.ns;
+            bin_base += runs[(size_t)k].L.nbins;
+        }
+        for (int i = 0; i < bb.count; ++i) {
+            const KRun &r = runs[(size_t)(b0 + i)];
+            bb.b[i] = MbBox{r.g.psrc, r.f, r.nv, r.g.Lx, r.g.Ly, r.L.h, r.L.c, r.g.w, r.L.P, r.L.Mx,
+                            r.L.My, r.L.nby, s_base, bin_base};
+            s_base += (int)r.g.ns;
+            bin_base += r.L.nbins;
+        }
+        hipLaunchKernelGGL(k_write_mb, dim3(1), dim3(kPack), 0, st, bb, m.tab, m.s_off, nrun, m.n);
+    }
+    const MbKeyFn kf{m.tab, m.s_off, nrun};
+    if (fsort_ok(ctx, m.n, m.nkeys)) {
+        if (int rc = fsort_run(ctx, &m.ks, m.n, m.nkeys, kf, FsNoPayload{}, FsNoExtra{}, FsNoExtra{}))
+            return rc;
+    } else {
+        hipLaunchKernelGGL(k_mb_keys, dim3((unsigned)((m.n + 255) / 256)), dim3(256), 0, st, kf, m.n,
+                           m.ks.key);
+        if (int rc = keysort_run(ctx, &m.ks, m.n, m.nkeys)) return rc;
+    }
+    const unsigned blocks = (unsigned)((m.n + 255) / 256);
+    if (kind == 0)
+        hipLaunchKernelGGL(k_mb_gather<0>, dim3(blocks), dim3(256), 0, st, m.tab, m.s_off, nrun, m.n,
+                           m.ks.perm, m.anchor, m.fgg, m.q);
+    else
+        hipLaunchKernelGGL(k_mb_gather<1>, dim3(blocks), dim3(256), 0, st, m.tab, m.s_off, nrun, m.n,
+                           m.ks.perm, m.anchor, m.fgg, m.q);
+    // each run's views: global positions throughout, bins sliced per box
+    std::vector<int> mapped;
+    int bin_base = 0;
+    for (int k = 0; k < nrun; ++k) {
+        KRun &r = runs[(size_t)k];
+        r.kss.perm = m.ks.perm;
+        r.kss.off = m.ks.off + bin_base;
+        bin_base += r.L.nbins;
+        r.anchor = m.anchor;
+        r.fgg = m.fgg;
+        r.q = m.q;
+        r.gathered = true;
+        r.sa = spread_args(r.L, r.gp, r.kss.off, r.grids);
+        r.sa.s.anchor = m.anchor;
+        r.sa.s.fgg = m.fgg;
+        r.sa.s.q = m.q;
+        r.sa.s.ns = m.n;
+        if (r.maps) mapped.push_back(k);
+    }
+    for (size_t s0 = 0; s0 < mapped.size(); s0 += kPack) {
+        const int cnt = (int)std::min<size_t>(kPack, mapped.size() - s0);
+        TileWorkPack tp{};
+        PlanPack pp{};
+        for (int j = 0; j < cnt; ++j) {
+            KRun &r = runs[(size_t)mapped[s0 + (size_t)j]];
+            tp.a[j] = r.sa;
+            tp.scnt[j] = r.gp.scnt;
+            tp.icnt[j] = r.gp.icnt;
+            tp.rs[j] = r.gp.rng_start;
+            tp.rl[j] = r.gp.rng_len;
+            tp.off[j + 1] = tp.off[j] + r.L.ntiles;
+            pp.p[2 * j] = PlanArgs{r.gp.scnt, nullptr, 1, 1, r.L.ntiles, r.gp.soffc, r.gp.smap};
+            pp.p[2 * j + 1] = PlanArgs{r.gp.icnt, nullptr, 1, 1, r.L.ntiles, r.gp.ioffc, r.gp.imap};
+        }
+        tp.n = cnt;
+        pp.n = 2 * cnt;
+        hipLaunchKernelGGL(k_tile_work_boxes, dim3((unsigned)tp.off[cnt]), dim3(64), 0, st, tp);
+        hipLaunchKernelGGL(k_plan_boxes, dim3((unsigned)pp.n), dim3(256), 0, st, pp);
+    }
+    SE2P_HIP(hipGetLastError());
+    return 0;
+}
+
 // ---- many boxes ------------------------------------------------------------------
 // Strip spreading of many boxes, kPack boxes per launch (each box's workgroups
 // run exactly as in its own launch, so the grids are bitwise those of
@@ -2260,7 +2477,10 @@ static int interp_boxes(se2p_ctx *ctx, std::vector<KRun> &runs)
 
 // Fourier-space sums of many independent boxes.  Every box gets a workspace
 // region of its own, so that all of them are resident at once:
-//   1. on the stream slots: each box's sorts, gathers and work maps;
+//   1. when every box's targets are its sources, one sort, gather and work
+//      map sequence over all boxes on the context stream (mb_sort);
+//      otherwise, on the stream slots, each box's sorts, gathers and work
+//      maps;
 //   2. on the context stream: spreading, boxes packed into shared launches;
 //   3. on the slots: each box's transforms and filter;
 //   4. on the context stream: interpolation, packed as the spreading.
@@ -2284,6 +2504,21 @@ int kspace_sum_boxes(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int nb
     SE2P_HIP(hipGetLastError());
     const int nrun = (int)runs.size();
     if (nrun == 0) return 0;
+    // one sort over all boxes when every box's targets are its sources
+    MbSort ms;
+    bool shared = nrun > 1;
+    {
+        int64_t n = 0, nk = 0;
+        for (const KRun &r : runs) {
+            shared = shared && r.same_points && r.nrhs == 1;
+            n += r.g.ns;
+            nk += r.L.nbins;
+        }
+        shared = shared && n <= 0x7fffffffLL && nk < 0x7fffffffLL;
+        ms.n = (int)n;
+        ms.nkeys = (int)nk;
+    }
+    for (KRun &r : runs) r.shared_sort = shared;
     std::vector<size_t> base((size_t)nrun);
     size_t total = 0;
     for (int k = 0; k < nrun; ++k) {
@@ -2292,11 +2527,23 @@ int kspace_sum_boxes(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int nb
         base[(size_t)k] = total;
         total += sizing.off;
     }
+    size_t mbase = total;
+    if (shared) {
+        Carve sizing;
+        ms.layout(sizing, nrun, runs[0].ncomp);
+        total += sizing.off;
+    }
     if (int rc = ensure_workspace(ctx, total)) return rc;
     for (int k = 0; k < nrun; ++k) {
         Carve cv;
         cv.base = ctx->ws + base[(size_t)k];
         runs[(size_t)k].layout(cv);
+    }
+    if (shared) {
+        Carve cv;
+        cv.base = ctx->ws + mbase;
+        ms.layout(cv, nrun, runs[0].ncomp);
+        if (int rc = mb_sort(ctx, kind, runs, ms)) return rc;
     }
     // largest first, each to the slot with the least work so far
     const int nsl = std::min(kKspaceSlots, nrun);
@@ -2332,6 +2579,7 @@ int kspace_sum_boxes(se2p_ctx *ctx, int kind, const se2p_box_desc *boxes, int nb
     };
     int rc = on_slots([&](KRun &r) {
         if (int e = get_plans(ctx, r.L.Mx, r.L.My, r.ncomp * r.nrp, 2 * r.nrp, &r.plans)) return e;
+        if (shared) return 0;   // sorted, gathered and mapped above
         if (int e = krun_sort(ctx, r)) return e;
         krun_gather(ctx, r);
         SE2P_HIP(hipGetLastError());

@@ -121,12 +121,15 @@ def test_multibox_many_boxes_equals_single(gpu, kernel):
         _say(f"[multibox 32] {kind} identical to single-box calls")
 
 
-def test_multibox_packs_with_mixed_grids(gpu):
+@pytest.mark.parametrize("targets", ["mixed", "sources"])
+def test_multibox_packs_with_mixed_grids(gpu, targets):
     """20 boxes, more than one packed launch (16 boxes each): grids from 32
     to 320 points a side, so that some boxes take the packed strip spreading
     and others the single-box kernels (grids narrower than P + 16), square
-    and rectangular, targets = sources and separate targets.  Every box
-    bitwise equal to its single-box call, Fourier space of both kernels."""
+    and rectangular; targets = sources in every other box ("mixed": each box
+    sorted on its own) or in every box ("sources": one sort over all boxes).
+    Every box bitwise equal to its single-box call, Fourier space of both
+    kernels."""
     rng = np.random.default_rng(46)
     boxes = []
     for i in range(20):
@@ -134,8 +137,8 @@ def test_multibox_packs_with_mixed_grids(gpu):
         Ly = Lx if i % 3 else 2 * Lx
         N = int(rng.integers(50, 2000))
         x = np.stack([Pm.wrap(Lx * rng.random(N), Lx), Pm.wrap(Ly * rng.random(N), Ly)], 1)
-        t = x if i % 2 else np.stack([Pm.wrap(Lx * rng.random(N // 2 + 1), Lx),
-                                      Pm.wrap(Ly * rng.random(N // 2 + 1), Ly)], 1)
+        t = x if (i % 2 or targets == "sources") else np.stack(
+            [Pm.wrap(Lx * rng.random(N // 2 + 1), Lx), Pm.wrap(Ly * rng.random(N // 2 + 1), Ly)], 1)
         f = rng.standard_normal((N, 2))
         nv = rng.standard_normal((N, 2))
         nv /= np.linalg.norm(nv, axis=1, keepdims=True)
@@ -147,7 +150,7 @@ def test_multibox_packs_with_mixed_grids(gpu):
                       "nside_x": p.nside_x, "nside_y": p.nside_y, "eta": p.eta, "w": p.w,
                       "Mx": p.Mx, "My": p.My, "P": p.P})
     narrow = sum(min(b["Mx"], b["My"]) < b["P"] + 16 for b in boxes)
-    _say(f"[multibox packs] 20 boxes, {narrow} with grids narrower than P + 16")
+    _say(f"[multibox packs {targets}] 20 boxes, {narrow} with grids narrower than P + 16")
     assert 0 < narrow < 20
     for kind in ("slp_kspace", "dlp_kspace"):
         outs = _multi(gpu, kind, boxes)
