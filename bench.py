@@ -354,6 +354,14 @@ def _multibox_variant(args, pe, W, np, torch, dev, tune=False):
             tctx = pe.Context(0)
             tuned = [dict(d) for d in tb]
             t0 = time.perf_counter()
+            # each box's cell size first (se2p_autotune at P = 24, cached per
+            # process, which the multi-box tuner then reuses), one line per box
+            for i, d in enumerate(tuned):
+                prm, _ = pe.autotune(kernel, d["psrc"], d["psrc"], d["f"], d.get("n"), d["Lx"],
+                                     d["Ly"], tol=1e-10, P=24, ctx=tctx)
+                print(f"# multibox {name} tuning box {i + 1}/{len(tuned)}: N {d['psrc'].shape[0]}, "
+                      f"Nb {prm.Nb:.0f}, grid {prm.Mx}, {time.perf_counter() - t0:.1f} s",
+                      file=sys.stderr, flush=True)
             tinfo = pe.autotune_multibox(kernel, tuned, tol=1e-10, ctx=tctx)
             tune_s = time.perf_counter() - t0
             with torch.cuda.stream(stream):
@@ -668,6 +676,8 @@ def main() -> int:
     ap.add_argument("--variant", choices=["batched", "batched_ellipses", "multibox", "at_scale"],
                     default=None,
                     help="run only this variant and print its JSON line (for profiling)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="--variant multibox: skip the multi-box tuner (it takes minutes)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -686,7 +696,7 @@ def main() -> int:
         dev = torch.device("cuda", 0)
         torch.cuda.set_device(dev)
         run = {"batched": _batched_variant, "batched_ellipses": _batched_ellipses_variant,
-               "multibox": lambda *a: _multibox_variant(*a, tune=True),
+               "multibox": lambda *a: _multibox_variant(*a, tune=not args.no_tune),
                "at_scale": _at_scale_variant}[args.variant]
         print(json.dumps({"variant": args.variant, "result": run(args, pe, W, np, torch, dev)}),
               flush=True)

@@ -112,7 +112,8 @@ class Se2pTuneBoxesInfo(ctypes.Structure):
     """se2p_tune_boxes_info of include/se2p.h."""
     _fields_ = [("drivers_ms", ctypes.c_double), ("tuned_ms", ctypes.c_double),
                 ("kept_drivers", ctypes.c_int), ("spread_chunk", ctypes.c_int),
-                ("chunk", ctypes.c_int * 3), ("chunk_ms", ctypes.c_double * 3)]
+                ("chunk", ctypes.c_int * 3), ("chunk_ms", ctypes.c_double * 3),
+                ("smooth_ms", ctypes.c_double), ("smooth_grids", ctypes.c_int)]
 
 
 SIGNATURES.update({

@@ -1372,6 +1372,17 @@ static int autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, con
     return io.done(SE2P_OK);
 }
 
+// Smallest even length >= M whose prime factors are 2, 3, 5 and 7.
+static int smooth_even_at_least(int M)
+{
+    for (int m = M + (M & 1);; m += 2) {
+        int r = m;
+        for (int f : {2, 3, 5, 7})
+            while (r % f == 0) r /= f;
+        if (r == 1) return m;
+    }
+}
+
 // Multi-box tuning: each box's cell size (stage 1 of the single-box tuner at
 // its drivers' P = 24), written into its descriptor, then the spreading work
 // split of the packed Fourier-space call over all boxes.
@@ -1466,10 +1477,22 @@ static int autotune_boxes(se2p_ctx *ctx, int kind, se2p_box_desc *boxes, int nbo
         DrvBufs b;
         if (int rc = driver_buffers(ctx, d.ns, d.nt, same, &b)) return rc;
         const TuneGeom tg{kind, d.psrc, d.ptar, d.ns, d.nt, d.f, d.n, d.Lx, d.Ly, Q[(size_t)i], tol, b.ur, b.uk};
-        if (int rc = tune_geometry(ctx, tg, 24, TUNE_NB, &tuned[(size_t)i], nullptr, nullptr)) {
-            ctx->err = "box " + std::to_string(i) + ": " + ctx->err;
-            return rc;
+        // a cell size se2p_autotune chose for this geometry (P = 24) in this process
+        bool cached = false;
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mu);
+            const TuneKey key{ctx->device, kind, 24, d.ns, d.nt, same ? 1 : 0, d.Lx, d.Ly, tol};
+            auto it = g_tune_nb.find(key);
+            if (it != g_tune_nb.end()) {
+                tune_params(tg, it->second.Nb, 24, 0, &tuned[(size_t)i]);
+                cached = true;
+            }
         }
+        if (!cached)
+            if (int rc = tune_geometry(ctx, tg, 24, TUNE_NB, &tuned[(size_t)i], nullptr, nullptr)) {
+                ctx->err = "box " + std::to_string(i) + ": " + ctx->err;
+                return rc;
+            }
     }
     apply(tuned);
     for (int i = 0; i < nbox; ++i)
@@ -1493,17 +1516,50 @@ static int autotune_boxes(se2p_ctx *ctx, int kind, se2p_box_desc *boxes, int nbo
         }
     }
     // keep the drivers' cells where the tuned ones are not faster as a whole
-    if (!(best < t_drv)) {
-        apply(drv);
+    std::vector<se2p_params> cur = tuned;
+    const bool use_drv = !(best < t_drv);
+    if (use_drv) {
+        cur = drv;
         best_chunk = 0;
         best = t_drv;
     }
+    apply(cur);
     if (int rc = se2p_set_spread_chunk(ctx, best_chunk)) return rc;
+    // grids rounded up to lengths 2^a 3^b 5^c 7^d (rocFFT transforms them
+    // without Bluestein stages): the same xi, cells and support on a finer
+    // grid, so the Fourier-space truncation error can only fall
+    std::vector<se2p_params> smooth = cur;
+    int nround = 0;
+    for (int i = 0; i < nbox; ++i) {
+        const se2p_params &c = cur[(size_t)i];
+        const int Mb = c.M > 0 ? c.M : std::min(2 * c.kinf, 10000);
+        const int Ms = smooth_even_at_least(Mb);
+        if (Ms == Mb) continue;
+        const se2p_box_desc &d = boxes[i];
+        const TuneGeom tg{kind, d.psrc, d.ptar, d.ns, d.nt, d.f, d.n, d.Lx, d.Ly, Q[(size_t)i], tol,
+                          nullptr, nullptr};
+        tune_params(tg, c.Nb, c.P, Ms, &smooth[(size_t)i]);
+        if ((double)smooth[(size_t)i].Mx * smooth[(size_t)i].My > kTuneMaxGrid) smooth[(size_t)i] = c;
+        else ++nround;
+    }
+    double t_smooth = -1.0;
+    bool use_smooth = false;
+    if (nround > 0) {
+        apply(smooth);
+        for (int i = 0; i < nbox; ++i)
+            if (int rc = check_box(ctx, boxes[i], i, kind == 1, true)) return rc;
+        if (int rc = time_boxes(&t_smooth)) return rc;
+        use_smooth = t_smooth < best;
+        if (use_smooth) best = t_smooth;
+        else apply(cur);
+    }
     if (info) {
         info->drivers_ms = t_drv;
         info->tuned_ms = best;
         info->spread_chunk = best_chunk;
-        info->kept_drivers = best == t_drv ? 1 : 0;
+        info->kept_drivers = use_drv && !use_smooth ? 1 : 0;
+        info->smooth_grids = use_smooth ? nround : 0;
+        info->smooth_ms = t_smooth;
     }
     SE2P_HIP(hipGetLastError());
     return finish(ctx);

@@ -619,7 +619,8 @@ def autotune_multibox(kind, boxes, *, tol, ctx=None):
                  My=d.My, P=d.P)
     return {"drivers_ms": info.drivers_ms, "tuned_ms": info.tuned_ms,
             "kept_drivers": bool(info.kept_drivers), "spread_chunk": info.spread_chunk,
-            "chunks": [{"chunk": info.chunk[i], "ms": info.chunk_ms[i]} for i in range(3)]}
+            "chunks": [{"chunk": info.chunk[i], "ms": info.chunk_ms[i]} for i in range(3)],
+            "smooth_ms": info.smooth_ms, "smooth_grids": info.smooth_grids}
 
 
 def _cols(*arrays):

@@ -377,10 +377,14 @@ int se2p_autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, const
  * drivers' parameters at tol with P = 24 (se2p_select_params, Nb default)
  * timed as one multi-box step (real- plus Fourier-space multi-box calls),
  * then each box's points per cell tuned on its own as stage 1 of
- * se2p_autotune, and the spreading work split (default, 128, 512) of the
+ * se2p_autotune (or taken from se2p_autotune's per-process cache when that
+ * geometry was tuned there with P = 24 at the same tol), and the spreading work split (default, 128, 512) of the
  * multi-box step at those parameters.  The tuned set is kept if its best
  * step is faster than the drivers' set, otherwise the drivers' parameters
- * and the default split are.  The split is set on the context.  Every box
+ * and the default split are.  Last, every grid is rounded up to the next
+ * even length 2^a 3^b 5^c 7^d (the grid override M; xi, the cells and P
+ * unchanged, so the Fourier-space truncation error can only fall), kept if
+ * the step is faster.  The split is set on the context.  Every box
  * needs at least two points; points are wrapped, as for the multi-box
  * entries; u is overwritten.  Device arrays only.  Synchronous; info may be
  * NULL. */
@@ -391,6 +395,9 @@ typedef struct se2p_tune_boxes_info {
     int spread_chunk;                   /* the split set on the context */
     int chunk[SE2P_TUNE_CHUNKS];        /* splits timed at the tuned cells */
     double chunk_ms[SE2P_TUNE_CHUNKS];  /* their multi-box step, ms */
+    double smooth_ms;                   /* step with the grids rounded up to 2^a 3^b 5^c 7^d
+                                           (-1: every grid already was) */
+    int smooth_grids;                   /* grids rounded up in the parameters written (0: none) */
 } se2p_tune_boxes_info;
 
 int se2p_autotune_multibox(se2p_ctx *ctx, int kind, se2p_box_desc *boxes, int nbox, double tol,

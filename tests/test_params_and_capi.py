@@ -250,14 +250,14 @@ def test_struct_layouts_match_header(tmp_path):
         'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(se2p_params), sizeof(se2p_box_desc),'
         ' offsetof(se2p_box_desc, Lx), offsetof(se2p_box_desc, Mx), offsetof(se2p_box_desc, u),'
         ' sizeof(se2p_tune_info), offsetof(se2p_tune_info, pc_dev), sizeof(se2p_tune_boxes_info),'
-        ' offsetof(se2p_tune_boxes_info, chunk_ms));return 0;}\n')
+        ' offsetof(se2p_tune_boxes_info, smooth_grids));return 0;}\n')
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(_capi.Se2pParams), ctypes.sizeof(_capi.Se2pBoxDesc),
             _capi.Se2pBoxDesc.Lx.offset, _capi.Se2pBoxDesc.Mx.offset, _capi.Se2pBoxDesc.u.offset,
             ctypes.sizeof(_capi.Se2pTuneInfo), _capi.Se2pTuneInfo.pc_dev.offset,
-            ctypes.sizeof(_capi.Se2pTuneBoxesInfo), _capi.Se2pTuneBoxesInfo.chunk_ms.offset]
+            ctypes.sizeof(_capi.Se2pTuneBoxesInfo), _capi.Se2pTuneBoxesInfo.smooth_grids.offset]
     assert got == want, (got, want)
 
 
