@@ -52,10 +52,12 @@ def test_graph_replays_match_eager(gpu):
     ctx.set_async(False)
 
 
-def test_multibox_graph_replays_match_eager(gpu):
+@pytest.mark.parametrize("targets", ["mixed", "sources"])
+def test_multibox_graph_replays_match_eager(gpu, targets):
     """Many boxes (real-space sums in one launch, Fourier-space sums on
-    several streams forked from and joined into the captured stream): a
-    captured sequence replays bitwise equal to eager calls."""
+    several streams forked from and joined into the captured stream; with
+    targets = sources in every box, one Fourier-space sort over all boxes):
+    a captured sequence replays bitwise equal to eager calls."""
     pe = gpu
     dev = torch.device("cuda", 0)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
@@ -65,7 +67,8 @@ def test_multibox_graph_replays_match_eager(gpu):
         w = W.uniform_box(n, seed=40 + i, L=L)
         p = Pm.select_params("dlp", n, n, w.f, L, L, tol=1e-10)
         ps = t(w.psrc)
-        boxes.append(dict(psrc=ps, ptar=ps if i % 2 == 0 else t(w.ptar), f=t(w.f), n=t(w.n),
+        same = i % 2 == 0 or targets == "sources"
+        boxes.append(dict(psrc=ps, ptar=ps if same else t(w.ptar), f=t(w.f), n=t(w.n),
                           Lx=L, Ly=L, xi=p.xi, nside_x=p.nside_x, nside_y=p.nside_y, eta=p.eta,
                           w=p.w, Mx=p.Mx, My=p.My, P=p.P))
     ctx = pe.Context(0)
@@ -96,7 +99,7 @@ def test_multibox_graph_replays_match_eager(gpu):
             eager = step()   # eager work between replays
         torch.cuda.synchronize()
         assert all(torch.equal(u, r) for u, r in zip(eager, ref)), rep
-    print(f"[multibox graph] {len(boxes)} boxes, 4 replays bitwise equal", flush=True)
+    print(f"[multibox graph {targets}] {len(boxes)} boxes, 4 replays bitwise equal", flush=True)
     ctx.set_async(False)
 
 
