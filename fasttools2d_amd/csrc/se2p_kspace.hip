@@ -389,9 +389,11 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
 constexpr int kPad = 3;
 
 // sources per wavefront batch.  The four batch areas set the workgroup's LDS
-// (about 30 KB at P = 24), which keeps five workgroups (five waves per SIMD)
-// resident per CU; more grids per pass leave room for fewer sources.
-__host__ __device__ constexpr int strip_batch(int NQ) { return NQ <= 4 ? 16 : 8; }
+// (about 30 KB at P = 24 with three grids, 36 KB with twelve).  With the tile
+// update on the matrix cores, batches of 16 also for five and more grids
+// (fewer fills and chunks) took the twelve-grid spreading of 64 densities on
+// a 1e5-point, 512^2 workload from 4.70 to 3.15 ms.
+__host__ __device__ constexpr int strip_batch(int) { return 16; }
 
 __host__ __device__ inline size_t strip_lds_doubles(int P, int NQ)
 {
