@@ -348,6 +348,9 @@ __global__ __launch_bounds__(256) void k_run_bbox(const double2 *__restrict__ po
 }
 
 // ---- the cell-list kernel ------------------------------------------------
+// Right-hand sides per pass of the several-density pair kernel (8 or 4).
+constexpr int kBatchRB = 8;
+
 template <int KIND, int RB, bool REC>
 __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restrict__ spos,
                                               const double *__restrict__ qsrc,
@@ -390,7 +393,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
         tx = ty = __builtin_nan("");   // inactive lanes compute garbage, never stored
     }
     const int ncomp = KIND == 0 ? 2 : kDlpComp;
-    const int64_t qs = a.ns;
+    const int64_t qrow = (int64_t)ncomp * a.nrhs;   // several densities: one source's values
     const int lcell = cell - cbase;
     const int cx = lcell % nsx, cy = lcell / nsx;
 
@@ -409,7 +412,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
         double acc[RB][2];
 #pragma unroll
         for (int r = 0; r < RB; ++r) acc[r][0] = acc[r][1] = 0.0;
-        const double *qb = qsrc + (int64_t)r0 * ncomp * qs;
+        const double *qb = qsrc + (int64_t)r0 * ncomp;   // + s * qrow: source s's group
         // one source: its record (single density) or position + density array
         // a source's position and (one right-hand side) its record values
         struct Src {
@@ -460,8 +463,8 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
                     } else {
 #pragma unroll
                         for (int r = 0; r < RB; ++r) {
-                            acc[r][0] += self * qb[(2 * r) * qs + s];
-                            acc[r][1] += self * qb[(2 * r + 1) * qs + s];
+                            acc[r][0] += self * qb[s * qrow + 2 * r];
+                            acc[r][1] += self * qb[s * qrow + 2 * r + 1];
                         }
                     }
                 }
@@ -485,8 +488,8 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
                     acc[0][1] = fma(e2, fma(H, d2, d1 * q0[5] + d2 * q0[4]), acc[0][1]);
                 }
             } else {
-                if (KIND == 0) slp_pair<RB>(d1, d2, r2, xi, e1tab, etab, qb + s, qs, acc);
-                else dlp_pair<RB>(d1, d2, d11, d22, r2, xi2, etab, qb + s, qs, acc);
+                if (KIND == 0) slp_pair<RB>(d1, d2, r2, xi, e1tab, etab, qb + s * qrow, 1, acc);
+                else dlp_pair<RB>(d1, d2, d11, d22, r2, xi2, etab, qb + s * qrow, 1, acc);
             }
         };
 
@@ -1236,13 +1239,12 @@ __global__ void k_box_cell_keys(const BoxTable boxes, const int *__restrict__ pt
     key[i] = box_cell_key(boxes, pt_box, which, i);
 }
 
-// sources: position and per-RHS densities of input point g at sorted slot i.
-// With one right-hand side (rec != null) the sources are written as records
-// (SrcRec) instead of the [nrhs][ncomp][n] density array.
+// sources: position of input point g at sorted slot i and, with one
+// right-hand side (rec != null), its record (SrcRec).  Several right-hand
+// sides: the densities are written by k_gather_densities.
 template <int KIND>
 __device__ __forceinline__ void gather_source(const BoxTable &boxes, const int *pt_box, int g, int i,
-                                              int n, int nrhs, int64_t dstride, double2 *pos,
-                                              double *q, double *rec)
+                                              double2 *pos, double *rec)
 {
     const RealBox &B = boxes.get(box_of(pt_box, g));
     const int o = g - B.s_base;
@@ -1265,39 +1267,50 @@ __device__ __forceinline__ void gather_source(const BoxTable &boxes, const int *
         }
         return;
     }
+}
+
+// Several right-hand sides: the densities of sorted slot i and right-hand
+// side r, source-major, q[(i * nrhs + r) * ncomp + c], so that a source's
+// values for a group of right-hand sides are contiguous.  One thread per
+// (i, r), consecutive threads writing consecutive rows.
+template <int KIND>
+__global__ void k_gather_densities(const BoxTable boxes, const int *__restrict__ pt_box,
+                                   const int *__restrict__ perm, int n, int nrhs, int64_t dstride,
+                                   double *__restrict__ q)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * nrhs) return;
+    const int i = (int)(t / nrhs), r = (int)(t % nrhs);
+    const int g = perm[i];
+    const RealBox &B = boxes.get(box_of(pt_box, g));
+    const int o = g - B.s_base;
+    const double *fr = B.f + (int64_t)r * dstride;
+    const double f1 = fr[2 * o], f2 = fr[2 * o + 1];
     if (KIND == 0) {
-        for (int r = 0; r < nrhs; ++r) {
-            const double *fr = B.f + (int64_t)r * dstride;
-            q[(int64_t)(2 * r) * n + i] = fr[2 * o];
-            q[(int64_t)(2 * r + 1) * n + i] = fr[2 * o + 1];
-        }
+        q[2 * t] = f1;
+        q[2 * t + 1] = f2;
     } else {
         const double n1 = B.nv[2 * o], n2 = B.nv[2 * o + 1];
-        for (int r = 0; r < nrhs; ++r) {
-            const double *fr = B.f + (int64_t)r * dstride;
-            const double f1 = fr[2 * o], f2 = fr[2 * o + 1];
-            // S = f (x) n (mex_stokes_dlp_real.cpp:86-89); S12 + S21 merged,
-            // then the two combinations the pair kernel uses
-            const double s0 = f1 * n1, s1 = f1 * n2 + f2 * n1, s2 = f2 * n2;
-            double *qr = q + (int64_t)(kDlpComp * r) * n + i;
-            qr[0] = s0;
-            qr[(int64_t)n] = s1;
-            qr[2 * (int64_t)n] = s2;
-            qr[3 * (int64_t)n] = 3.0 * s0 + s2;
-            qr[4 * (int64_t)n] = s0 + 3.0 * s2;
-        }
+        // S = f (x) n (mex_stokes_dlp_real.cpp:86-89); S12 + S21 merged,
+        // then the two combinations the pair kernel uses
+        const double s0 = f1 * n1, s1 = f1 * n2 + f2 * n1, s2 = f2 * n2;
+        double *qr = q + kDlpComp * t;
+        qr[0] = s0;
+        qr[1] = s1;
+        qr[2] = s2;
+        qr[3] = 3.0 * s0 + s2;
+        qr[4] = s0 + 3.0 * s2;
     }
 }
 
 template <int KIND>
 __global__ void k_box_gather_sources(const BoxTable boxes,
                                      const int *__restrict__ pt_box, const int *__restrict__ perm,
-                                     int n, int nrhs, int64_t dstride, double2 *__restrict__ pos,
-                                     double *__restrict__ q, double *__restrict__ rec)
+                                     int n, double2 *__restrict__ pos, double *__restrict__ rec)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    gather_source<KIND>(boxes, pt_box, perm[i], i, n, nrhs, dstride, pos, q, rec);
+    gather_source<KIND>(boxes, pt_box, perm[i], i, pos, rec);
 }
 
 __device__ __forceinline__ void gather_target(const BoxTable &boxes, const int *pt_box, int g, int i,
@@ -1332,14 +1345,12 @@ struct SrcGather {
     static constexpr bool kBox = true;
     BoxTable boxes;
     const int *pt_box;
-    int n, nrhs;
-    int64_t dstride;
     double2 *pos;
-    double *q, *rec;
+    double *rec;
     double4 *box;
     __device__ double2 operator()(int i, int g, int) const
     {
-        gather_source<KIND>(boxes, pt_box, g, i, n, nrhs, dstride, pos, q, rec);
+        gather_source<KIND>(boxes, pt_box, g, i, pos, rec);
         const RealBox &B = boxes.get(box_of(pt_box, g));
         const int o = g - B.s_base;
         return make_double2(B.psrc[2 * o], B.psrc[2 * o + 1]);
@@ -1546,12 +1557,10 @@ static int real_prepare(se2p_ctx *ctx, const RealBox *boxes, RealPlan &p)
         int rc;
         if (kind == 0)
             rc = fsort_run(ctx, &p.kss, ns, nkeys, kf,
-                           SrcGather<0>{btab, p.s_box, (int)ns, nrhs, dstride, p.spos, p.q, p.rec,
-                                        p.sbox}, x0, FsNoExtra{});
+                           SrcGather<0>{btab, p.s_box, p.spos, p.rec, p.sbox}, x0, FsNoExtra{});
         else
             rc = fsort_run(ctx, &p.kss, ns, nkeys, kf,
-                           SrcGather<1>{btab, p.s_box, (int)ns, nrhs, dstride, p.spos, p.q, p.rec,
-                                        p.sbox}, x0, FsNoExtra{});
+                           SrcGather<1>{btab, p.s_box, p.spos, p.rec, p.sbox}, x0, FsNoExtra{});
         if (rc) return rc;
         planned = p.same_points;
         boxed = !box_kernel;
@@ -1564,10 +1573,10 @@ static int real_prepare(se2p_ctx *ctx, const RealBox *boxes, RealPlan &p)
             const unsigned blocks = (unsigned)((ns + threads - 1) / threads);
             if (kind == 0)
                 hipLaunchKernelGGL(k_box_gather_sources<0>, dim3(blocks), dim3(threads), 0, st, btab,
-                                   p.s_box, p.kss.perm, (int)ns, nrhs, dstride, p.spos, p.q, p.rec);
+                                   p.s_box, p.kss.perm, (int)ns, p.spos, p.rec);
             else
                 hipLaunchKernelGGL(k_box_gather_sources<1>, dim3(blocks), dim3(threads), 0, st, btab,
-                                   p.s_box, p.kss.perm, (int)ns, nrhs, dstride, p.spos, p.q, p.rec);
+                                   p.s_box, p.kss.perm, (int)ns, p.spos, p.rec);
         }
     }
     if (!p.same_points) {
@@ -1584,6 +1593,15 @@ static int real_prepare(se2p_ctx *ctx, const RealBox *boxes, RealPlan &p)
             hipLaunchKernelGGL(k_box_gather_targets, dim3((unsigned)((nt + threads - 1) / threads)),
                                dim3(threads), 0, st, btab, p.t_box, tks.perm, (int)nt, p.tpos);
         }
+    }
+    if (p.q && ns > 0) {   // several right-hand sides
+        const unsigned blocks = (unsigned)(((int64_t)ns * nrhs + threads - 1) / threads);
+        if (kind == 0)
+            hipLaunchKernelGGL(k_gather_densities<0>, dim3(blocks), dim3(threads), 0, st, btab, p.s_box,
+                               p.kss.perm, (int)ns, nrhs, dstride, p.q);
+        else
+            hipLaunchKernelGGL(k_gather_densities<1>, dim3(blocks), dim3(threads), 0, st, btab, p.s_box,
+                               p.kss.perm, (int)ns, nrhs, dstride, p.q);
     }
     if (!planned) plan_chunks(st, nullptr, tks.off, sub, 64, (int)ncell, p.coff, p.bmap);   // chunks of 64 targets
     if (!boxed)
@@ -1719,13 +1737,21 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
         SE2P_HIP(hipMemsetAsync(p.cnt, 0, 2 * sizeof(unsigned long long), st));
         hipLaunchKernelGGL(k_real_count, dim3((unsigned)p.max_chunks), dim3(64), 0, st, a);
     } else {
-        // groups of four right-hand sides share every pair evaluation; the
-        // remainder goes one at a time
-        const int n4 = nrhs / 4 * 4;
-        if (n4 > 0) {
+        // groups of kBatchRB, then of four right-hand sides share every pair
+        // evaluation; the remainder goes one at a time
+        const int n8 = kBatchRB == 8 ? nrhs / 8 * 8 : 0;
+        if (n8 > 0) {
             a.r_begin = 0;
+            a.r_end = n8;
+            const dim3 grid(grid8, n8 / 8);
+            if (kind == 0) hipLaunchKernelGGL((k_real<0, 8, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
+            else hipLaunchKernelGGL((k_real<1, 8, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
+        }
+        const int n4 = n8 + (nrhs - n8) / 4 * 4;
+        if (n4 > n8) {
+            a.r_begin = n8;
             a.r_end = n4;
-            const dim3 grid(grid8, n4 / 4);
+            const dim3 grid(grid8, (n4 - n8) / 4);
             if (kind == 0) hipLaunchKernelGGL((k_real<0, 4, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
             else hipLaunchKernelGGL((k_real<1, 4, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
         }
