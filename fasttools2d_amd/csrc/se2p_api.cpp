@@ -289,6 +289,10 @@ int se2p_create(se2p_ctx **out, int device)
         return SE2P_ERR_HIP;
     }
     ctx->own_stream = true;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        ncu > 0)
+        ctx->ncu = ncu;
     // fused-sort key counts: zero once here, cleared again by every sort
     const size_t fs_bytes = sizeof(int) * kFsMaxKeys;
     if (hipMalloc((void **)&ctx->fs, fs_bytes) != hipSuccess ||

@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include "e1_table.h"
+#include "exp2_table.h"
 
 #define SE2P_PI 3.1415926535897932385
 #define SE2P_EULER_GAMMA 0.57721566490153286061
@@ -29,44 +30,23 @@ __host__ __device__ inline double se2p_sqsum(double dx, double dy)
     return a + b;
 }
 
-// 2^(-j/64), j = 0..63 (correctly rounded; mpmath at 200 bits).
-#define SE2P_EXP2_TABLE \
-    {1.0, 0.9892280131939755, 0.9785720620877001, 0.9680308967461472, \
-     0.9576032806985737, 0.9472879907934828, 0.93708381705515, 0.9269895625416927, \
-     0.9170040432046712, 0.9071260877501994, 0.8973545375015536, 0.8876882462632606, \
-     0.8781260801866497, 0.8686669176368531, 0.859309649061239, 0.8500531768592617, \
-     0.8408964152537145, 0.8318382901633682, 0.8228777390769825, 0.8140137109286739, \
-     0.8052451659746271, 0.7965710756711335, 0.7879904225539432, 0.7795022001189185, \
-     0.7711054127039704, 0.7627990753722692, 0.7545822137967114, 0.7464538641456324, \
-     0.7384130729697497, 0.7304588970903235, 0.7225904034885233, 0.714806669195985, \
-     0.7071067811865476, 0.6994898362691556, 0.691954940981916, 0.6845012114872953, \
-     0.6771277734684463, 0.6698337620266515, 0.6626183215798707, 0.6554806057623822, \
-     0.6484197773255048, 0.6414350080393891, 0.6345254785958666, 0.6276903785123455, \
-     0.620928906036742, 0.614240268053435, 0.6076236799902345, 0.6010783657263515, \
-     0.5946035575013605, 0.5881984958251406, 0.5818624293887887, 0.5755946149764913, \
-     0.5693943173783458, 0.5632608093041209, 0.5571933712979462, 0.5511912916539204, \
-     0.5452538663326288, 0.5393803988785599, 0.5335702003384118, 0.5278225891802786, \
-     0.5221368912137069, 0.5165124395106142, 0.5109485743270583, 0.5054446430258502}
-
-// exp(-x) for finite x >= 0, table driven (Tang's method with 64 table
-// entries): k = rint(64 x / ln 2), r = x - k ln2/64 with ln2/64 split into a
-// double and its remainder (Cody-Waite, each product exact inside its fused
-// multiply-add), so |r| <= ln2/128; then
-// exp(-x) = 2^-(k>>6) * 2^-((k&63)/64) * exp(-r), exp(-r) by its degree-5
-// Taylor polynomial (truncation |r|^6/6! < 3.5e-17).  A few ulp; the sums
-// that use it are compared with libm's exp at 1e-12.  tab64 = SE2P_EXP2_TABLE.
-constexpr int SE2P_EXP2_N = 64;
-__host__ __device__ inline double se2p_exp_neg(double x, const double *tab64)
+// exp(-x) for finite x >= 0, table driven (Tang's method, N = 256 table
+// entries, exp2_table.h): k = rint(x N / ln 2), r = x - k ln2/N with ln2/N
+// split into a double and its remainder (each product exact inside its
+// fused multiply-add), so |r| <= ln2/(2N); then
+// exp(-x) = 2^-(k>>8) * 2^-((k&255)/256) * exp(-r), exp(-r) by its degree-4
+// Taylor polynomial (truncation |r|^5/5! < 3.8e-17).  A few ulp; the sums
+// that use it are compared with libm's exp at 1e-12.
+// tab256 = SE2P_EXP2_TABLE (kernels hold it in LDS).
+__host__ __device__ inline double se2p_exp_neg(double x, const double *tab256)
 {
-    const double k = rint(x * 92.332482616893656);     // 64 / ln 2
-    double r = fma(-k, 0.010830424696249145, x);        // ln2/64, leading part
-    r = fma(-k, 3.6235106466348430e-19, r);             // and the remainder
-    // Estrin's scheme: dependency depth 3
+    const double k = rint(x * SE2P_EXP2_SCALE);
+    double r = fma(-k, SE2P_EXP2_LN2N_HI, x);
+    r = fma(-k, SE2P_EXP2_LN2N_LO, r);
     const double r2 = r * r;
-    const double a = 1.0 - r, b = fma(r, -1.0 / 6.0, 0.5), c = fma(r, -1.0 / 120.0, 1.0 / 24.0);
-    const double p = fma(r2 * r2, c, fma(r2, b, a));
+    const double p = fma(r2, fma(r2, 1.0 / 24.0, fma(r, -1.0 / 6.0, 0.5)), 1.0 - r);
     const int ki = (int)k;
-    return ldexp(tab64[ki & 63] * p, -(ki >> 6));
+    return ldexp(tab256[ki & (SE2P_EXP2_N - 1)] * p, -(ki >> 8));
 }
 
 // se2p_sqsum that also returns the two squares (same rounding).

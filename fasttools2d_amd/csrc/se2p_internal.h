@@ -90,6 +90,7 @@ struct se2p_ctx {
     // This device's address of the fixed scratch of se2p_dlp_real_noalloc
     // (looked up once per device at se2p_create; null if that failed).
     char *scratch = nullptr;
+    int ncu = 256;            // compute units of the device (persistent launches)
     // se2p_set_frozen: calls that would allocate or free device memory fail
     // with SE2P_ERR_FROZEN instead (captured graphs keep their addresses)
     bool frozen = false;

@@ -21,7 +21,7 @@
 // wave-uniform and arrives through the scalar cache.  The four partial sums
 // are added in a fixed order through LDS, so results are deterministic (no
 // atomics).  E1 for the Stokeslet comes from a piecewise polynomial table in
-// LDS, exp(-x) from a 16-entry table (se2p_exp_neg).
+// LDS, exp(-x) from a 256-entry table (se2p_exp_neg).
 //
 // Neighbour cells are pruned by sub-cell: the bounding box of every
 // (cell, sub-cell) run of the sorted sources is computed once per call, and a
@@ -44,6 +44,8 @@
 namespace se2p {
 
 static __constant__ double c_e1_table[SE2P_E1_NPIECE * SE2P_E1_ROW];
+// 2^(-j/256), the table of se2p_exp_neg; each workgroup copies it to LDS
+static __constant__ double c_exp2_table[SE2P_EXP2_N] = SE2P_EXP2_TABLE;
 
 // ---- fixed scratch of the no-allocation entry point (se2p_dlp_real_noalloc)
 // The sorted state of that entry point (cell lists, source records, chunk
@@ -366,10 +368,7 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (KIND == 0)
         for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
-    if (threadIdx.x < SE2P_EXP2_N) {
-        const double t[SE2P_EXP2_N] = SE2P_EXP2_TABLE;
-        etab[threadIdx.x] = t[threadIdx.x];
-    }
+    for (int i = threadIdx.x; i < SE2P_EXP2_N; i += blockDim.x) etab[i] = c_exp2_table[i];
     __syncthreads();
 
     const int b = xcd_chunk(a.chunk_off[a.ncell]);
@@ -570,6 +569,425 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
     }
 }
 
+// ---- the single-density kernel over windows of a chunk's sources --------
+// A chunk's source sequence is its own cell's sources followed by the kept
+// runs of its eight neighbour cells in (neighbour, sub-cell) order.  It is
+// cut into windows of kWin positions.  In a window, wave w sums the
+// positions p = w (mod 4) in order from zero; the four wave sums are added
+// in wave order, and the window sums in window order from zero.  That sum
+// is the same whether one workgroup walks every window of a chunk or a
+// chunk is split into one work item per window (its window sums go to
+// slots that k_real_win_final adds), so the split, which balances the
+// launch, never changes a result, and neither does the order in which
+// work items run.  Work items are numbered in chunk order; persistent
+// workgroups take them from eight queues, one per XCD, each a contiguous
+// range of items, and drain the other queues when their own is empty.
+constexpr int kWin = 256;
+constexpr int kQueueStride = 32;   // ints between queue counters (own cache lines)
+constexpr int kWinSlots = 16384;   // window slots of split chunks (16 MiB of sums)
+constexpr int kWinBlocksPerCU = 8;
+
+struct WinPlan {
+    unsigned long long *keep;   // [2 max_chunks] kept neighbour runs (keep_mask bits)
+    int *npos;                  // [max_chunks] sequence length
+    int2 *where;                // [max_chunks] (first target, cell)
+    int *item_off;              // [max_chunks] first work item of each chunk
+    int *slot;                  // [max_chunks] first window slot of a split chunk, -1 if unsplit
+    int *item_chunk;            // [max_chunks + slots] work item -> chunk
+    int4 *split;                // [max_chunks] split chunks: (slot, windows, first target, cell)
+    int *meta;                  // [0] work items, [1] split threshold, [2] split chunks,
+                                // queue x at kQueueStride (x + 1)
+    double *part;               // [slots][2][64] window sums of split chunks
+    int slots;
+};
+
+__device__ __forceinline__ int win_count(int n) { return n > kWin ? (n + kWin - 1) / kWin : 1; }
+
+// Sequence length and kept runs of every chunk: one wavefront per chunk.
+__global__ __launch_bounds__(256) void k_real_win_plan(RealArgs a, const int *__restrict__ soff,
+                                                       const double4 *__restrict__ sbox, WinPlan w)
+{
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const int nch = a.chunk_off[a.ncell];
+    if (c >= nch) return;   // uniform over the wavefront
+    const int cell = a.block_cell[c];
+    const RealBox &B = a.boxes.get(a.cell_box ? a.cell_box[cell] : 0);
+    const int nsx = B.nsx, nsy = B.nsy, cbase = B.cell_base;
+    const int sub = 1 << (2 * kSubLog);
+    const int t0 = a.toff[cell * sub] + 64 * (c - a.chunk_off[cell]);
+    const bool active = t0 + lane < a.toff[(cell + 1) * sub];
+    double tx = 0.0, ty = 0.0;
+    if (active) {
+        const double2 p = a.tpos[t0 + lane];
+        tx = p.x;
+        ty = p.y;
+    }
+    const int lcell = cell - cbase;
+    const int cx = lcell % nsx, cy = lcell / nsx;
+    double bx0, by0, bx1, by1;
+    wave_bbox(active, tx, ty, bx0, by0, bx1, by1);
+    const double lim = B.cut2 * (1.0 + 1e-9);
+    unsigned long long keep[2];
+    int len = 0;
+#pragma unroll
+    for (int rd = 0; rd < 2; ++rd) {
+        keep[rd] = keep_mask(sbox, rd, lane, cbase, cx, cy, nsx, nsy, B.Lx, B.Ly, lim, bx0, by0,
+                             bx1, by1);
+        const int j = 4 * rd + (lane >> 4);
+        const int wx = (cx + nb_dx(j) + nsx) % nsx, wy = (cy + nb_dy(j) + nsy) % nsy;
+        const int k = ((cbase + wy * nsx + wx) << 4) + (lane & 15);
+        if ((keep[rd] >> lane) & 1) len += soff[k + 1] - soff[k];
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) len += __shfl_xor(len, d);
+    if (lane == 0) {
+        w.keep[2 * c] = keep[0];
+        w.keep[2 * c + 1] = keep[1];
+        w.npos[c] = soff[(cell + 1) * sub] - soff[cell * sub] + len;
+        w.where[c] = make_int2(t0, cell);
+    }
+}
+
+// Work items: a chunk with at least `theta` windows is split into one item
+// per window, the others are one item each.  theta is the smallest of
+// 2, 4, ..., 64 whose split chunks fit the window slots (none split if even
+// 64 does not).  Split chunks are also listed, with what k_real_win_final
+// needs, for that kernel.  One workgroup; thread t owns a contiguous
+// segment of chunks.
+constexpr int kWinScanWaves = 16;
+constexpr int kWinTheta = 6;     // candidate thresholds 2 << i
+constexpr int kWinLds = 4096;    // chunks whose window counts the scan stages in LDS
+__global__ __launch_bounds__(64 * kWinScanWaves) void k_real_win_scan(const int *__restrict__ chunk_off,
+                                                                      int ncell, WinPlan w)
+{
+    __shared__ int wsum[3][kWinScanWaves];
+    __shared__ unsigned long long wtot[kWinScanWaves][kWinTheta];
+    __shared__ int lnw[kWinLds];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nch = chunk_off[ncell];
+    const bool staged = nch <= kWinLds;
+    // window counts: staged in LDS by strided (independent, pipelined) loads
+    if (staged) {
+#pragma unroll 4
+        for (int c = threadIdx.x; c < nch; c += blockDim.x) lnw[c] = win_count(w.npos[c]);
+        __syncthreads();
+    }
+    auto nw_of = [&](int c) { return staged ? lnw[c] : win_count(w.npos[c]); };
+    // thread t owns the contiguous segment [c0, c1)
+    const int per = (nch + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int c0 = min((int)threadIdx.x * per, nch), c1 = min(c0 + per, nch);
+    // windows of the chunks with at least 2 << q windows
+    unsigned long long sum[kWinTheta] = {};
+    for (int c = c0; c < c1; ++c) {
+        const int nw = nw_of(c);
+#pragma unroll
+        for (int q = 0; q < kWinTheta; ++q) sum[q] += nw >= (2 << q) ? nw : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kWinTheta; ++q) {
+        for (int d = 32; d > 0; d >>= 1) sum[q] += __shfl_xor(sum[q], d);
+        if (lane == 0) wtot[wave][q] = sum[q];
+    }
+    __syncthreads();
+    int theta = 0x7fffffff;
+    for (int q = kWinTheta - 1; q >= 0; --q) {
+        unsigned long long t = 0;
+        for (int v = 0; v < kWinScanWaves; ++v) t += wtot[v][q];
+        if (t <= (unsigned long long)w.slots) theta = 2 << q;
+    }
+    int ni = 0, ns = 0, nl = 0;
+    for (int c = c0; c < c1; ++c) {
+        const int nw = nw_of(c);
+        ni += nw >= theta ? nw : 1;
+        ns += nw >= theta ? nw : 0;
+        nl += nw >= theta ? 1 : 0;
+    }
+    int ti, ts, tl;
+    int ci = block_scan_excl<kWinScanWaves>(ni, wsum[0], &ti);
+    int cs = block_scan_excl<kWinScanWaves>(ns, wsum[1], &ts);
+    int cl = block_scan_excl<kWinScanWaves>(nl, wsum[2], &tl);
+    for (int c = c0; c < c1; ++c) {
+        const int nw = nw_of(c);
+        const bool split = nw >= theta;
+        const int n = split ? nw : 1;
+        w.item_off[c] = ci;
+        w.slot[c] = split ? cs : -1;
+        for (int k = 0; k < n; ++k) w.item_chunk[ci + k] = c;
+        if (split) {
+            const int2 at = w.where[c];
+            w.split[cl++] = make_int4(cs, nw, at.x, at.y);
+            cs += nw;
+        }
+        ci += n;
+    }
+    if (threadIdx.x == 0) {
+        w.meta[0] = ti;
+        w.meta[1] = theta;
+        w.meta[2] = tl;
+    }
+    if (threadIdx.x < 8) w.meta[kQueueStride * (threadIdx.x + 1)] = 0;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_real_win(RealArgs a, const double *__restrict__ srec,
+                                                  const int *__restrict__ soff, WinPlan w)
+{
+    __shared__ double e1tab[KIND == 0 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
+    __shared__ double etab[SE2P_EXP2_N];
+    __shared__ double red[2][kWaves - 1][2][64];
+    __shared__ int s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (KIND == 0)
+        for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
+    for (int i = threadIdx.x; i < SE2P_EXP2_N; i += blockDim.x) etab[i] = c_exp2_table[i];
+    const int total = w.meta[0];
+    const int per = (total + 7) >> 3;
+    const int xcd = blockIdx.x & 7;
+    const int sub = 1 << (2 * kSubLog);
+    int qi = 0;   // thread 0: queues found empty so far
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int item = -1;
+            for (; qi < 8; ++qi) {
+                const int x = (xcd + qi) & 7;
+                const int lo = x * per, nq = min(per, total - lo);
+                if (nq <= 0) continue;
+                const int it = atomicAdd(&w.meta[kQueueStride * (x + 1)], 1);
+                if (it < nq) {
+                    item = lo + it;
+                    break;
+                }
+            }
+            s_item = item;
+        }
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item < 0) break;   // uniform over the workgroup
+
+        const int c = w.item_chunk[item];
+        const int sl = w.slot[c];
+        const int cell = a.block_cell[c];
+        const RealBox &B = a.boxes.get(a.cell_box ? a.cell_box[cell] : 0);
+        const int nsx = B.nsx, nsy = B.nsy, cbase = B.cell_base;
+        const double Lx = B.Lx, Ly = B.Ly, cut2 = B.cut2;
+        const double xi = B.xi, xi2 = B.xi2, self = B.self;
+        const int t = a.toff[cell * sub] + 64 * (c - a.chunk_off[cell]) + lane;
+        const bool active = t < a.toff[(cell + 1) * sub];
+        double tx, ty;
+        if (active) {
+            const double2 p = a.tpos[t];
+            tx = p.x;
+            ty = p.y;
+        } else {
+            tx = ty = __builtin_nan("");   // inactive lanes compute garbage, never stored
+        }
+        const int lcell = cell - cbase;
+        const int cx = lcell % nsx, cy = lcell / nsx;
+        const int n = w.npos[c];
+        const int kb = sl >= 0 ? item - w.item_off[c] : 0;
+        const int ke = sl >= 0 ? kb + 1 : win_count(n);
+        const unsigned long long keep0 = w.keep[2 * c], keep1 = w.keep[2 * c + 1];
+
+        double acc0 = 0.0, acc1 = 0.0;
+        struct Src {
+            double x, y, q[6];
+        };
+        auto load = [&](int s) {
+            Src o;
+            const double *rp = srec + (int64_t)s * SrcRec<KIND>::n;
+            const double4 v0 = *(const double4 *)rp;
+            o.x = v0.x;
+            o.y = v0.y;
+            o.q[0] = v0.z;
+            o.q[1] = v0.w;
+            if (KIND == 1) {
+                const double4 v1 = *(const double4 *)(rp + 4);
+                o.q[2] = v1.x;
+                o.q[3] = v1.y;
+                o.q[4] = v1.z;
+                o.q[5] = v1.w;
+            }
+            return o;
+        };
+        // own: the own cell (no cutoff); img: a neighbour seen through a
+        // periodic image (zx, zy not both zero)
+        auto pair = [&](const Src &src, double zx, double zy, bool own, bool img) {
+            const double *q0 = src.q;
+            double d1, d2;
+            if (img) {
+                diff<KIND>(src.x, src.y, tx, ty, zx, zy, d1, d2);
+            } else {
+                // no image offset: both kernels' difference is src - tgt (up
+                // to the sign of a zero, which adding a zero offset can flip
+                // and which no sum sees)
+                d1 = src.x - tx;
+                d2 = src.y - ty;
+            }
+            double d11, d22;
+            const double r2 = se2p_sqsum2(d1, d2, &d11, &d22);
+            if (own ? (r2 == 0.0) : !(r2 < cut2)) {
+                if (KIND == 0 && own) {   // coincident points: the self term
+                    acc0 += self * q0[0];
+                    acc1 += self * q0[1];
+                }
+                return;
+            }
+            if (KIND == 0) {
+                double hE, g;
+                slp_factors(r2, xi, e1tab, etab, hE, g);
+                slp_add(d1, d2, hE, g, q0[0], q0[1], acc0, acc1);
+            } else {
+                // record values carry -4 and 2 xi^2:
+                //   u_i += e2 (H d_i + l'_i),  H = (1 + x)/r^4 (d.S'd),
+                // with (1 + x)/r^4 = (1/r^2)(1/r^2 + xi^2)
+                const double x = xi2 * r2;
+                const double e2 = se2p_exp_neg(x, etab);
+                const double ir2 = rcp_nr(r2);
+                const double H = (ir2 * (ir2 + xi2)) * (d11 * q0[0] + (d1 * d2) * q0[1] + d22 * q0[2]);
+                acc0 = fma(e2, fma(H, d1, d1 * q0[3] + d2 * q0[5]), acc0);
+                acc1 = fma(e2, fma(H, d2, d1 * q0[5] + d2 * q0[4]), acc1);
+            }
+        };
+        // sources s, s + kWaves, ... < s1, two records loaded together
+        auto run = [&](int s, int s1, double zx, double zy, bool own, bool img) {
+            for (; s + kWaves < s1; s += 2 * kWaves) {
+                const Src a0 = load(s), a1 = load(s + kWaves);
+                pair(a0, zx, zy, own, img);
+                pair(a1, zx, zy, own, img);
+            }
+            if (s < s1) pair(load(s), zx, zy, own, img);
+        };
+
+        // the walk: current run [s0, s1) at sequence position pos; j = -1 is
+        // the own cell, j = 0..7 the neighbours (mex_stokes_slp_real.cpp:61-62)
+        int j = -1, s0 = soff[cell * sub], s1 = soff[(cell + 1) * sub], pos = 0, sb = 0;
+        unsigned bits = 0;
+        double zx = 0.0, zy = 0.0;
+        bool more = true;
+        auto advance = [&]() {
+            pos += s1 - s0;
+            while (bits == 0) {
+                if (++j == 8) {
+                    more = false;
+                    return;
+                }
+                bits = (unsigned)(((j < 4) ? keep0 : keep1) >> (16 * (j & 3))) & 0xffffu;
+                if (bits) {
+                    const int qx = cx + nb_dx(j), qy = cy + nb_dy(j);
+                    const int wx = (qx + nsx) % nsx, wy = (qy + nsy) % nsy;
+                    sb = cbase + wy * nsx + wx;
+                    zx = (Lx * (qx - wx)) / nsx;   // :248-249
+                    zy = (Ly * (qy - wy)) / nsy;
+                }
+            }
+            const int ra = __builtin_ctz(bits);
+            const int rb = ra + __builtin_ctz(~(bits >> ra));
+            bits &= ~((1u << rb) - 1u);
+            s0 = soff[sb * sub + ra];
+            s1 = soff[sb * sub + rb];
+        };
+        double tot0 = 0.0, tot1 = 0.0;
+        for (int k = kb; k < ke; ++k) {
+            const int wa = k * kWin, wb = min(wa + kWin, n);
+            acc0 = acc1 = 0.0;
+            while (more) {
+                const int pe = pos + (s1 - s0);
+                const int pa = max(pos, wa), pb = min(pe, wb);
+                if (pa < pb) {
+                    const int sa = s0 + (pa - pos) + ((wave - pa) & (kWaves - 1));
+                    const int se = s0 + (pb - pos);
+                    if (j < 0) run(sa, se, 0.0, 0.0, true, false);
+                    else if (zx == 0.0 && zy == 0.0) run(sa, se, 0.0, 0.0, false, false);
+                    else run(sa, se, zx, zy, false, true);
+                }
+                if (pe > wb) break;   // the run goes on in the next window
+                advance();
+            }
+            // the window's sum: wave sums added in wave order
+            if (wave > 0) {
+                red[k & 1][wave - 1][0][lane] = acc0;
+                red[k & 1][wave - 1][1][lane] = acc1;
+            }
+            __syncthreads();
+            if (wave == 0) {
+                double v0 = acc0, v1 = acc1;
+#pragma unroll
+                for (int q = 0; q < kWaves - 1; ++q) {
+                    v0 += red[k & 1][q][0][lane];
+                    v1 += red[k & 1][q][1][lane];
+                }
+                if (sl >= 0) {
+                    double *pp = w.part + (int64_t)(sl + k) * 128;
+                    pp[lane] = v0;
+                    pp[64 + lane] = v1;
+                } else {
+                    tot0 += v0;
+                    tot1 += v1;
+                }
+            }
+        }
+        if (sl < 0 && wave == 0 && active) {
+            const int o = a.tperm[t] - B.t_base;
+            if (KIND == 0) {   // :344-345
+                B.u[2 * o] = tot0 / (4 * SE2P_PI);
+                B.u[2 * o + 1] = tot1 / (4 * SE2P_PI);
+            } else {           // mex_stokes_dlp_real.cpp:198-199
+                B.u[2 * o] = tot0 / 4 / SE2P_PI;
+                B.u[2 * o + 1] = tot1 / 4 / SE2P_PI;
+            }
+        }
+    }
+}
+
+// Split chunks: window sums added in window order from zero, then scaled
+// and scattered like k_real_win's own results.  One wavefront per split
+// chunk, the wavefronts of the launch striding over the list.
+constexpr int kWinFinalBlocks = 256;
+template <int KIND>
+__global__ __launch_bounds__(256) void k_real_win_final(RealArgs a, WinPlan w)
+{
+    const int lane = threadIdx.x & 63;
+    const int nsplit = w.meta[2];
+    const int sub = 1 << (2 * kSubLog);
+    for (int i = blockIdx.x * kWaves + (threadIdx.x >> 6); i < nsplit; i += gridDim.x * kWaves) {
+        const int4 r = w.split[i];   // (slot, windows, first target, cell)
+        const int t = r.z + lane;
+        if (t >= a.toff[(r.w + 1) * sub]) continue;
+        const double *pp = w.part + (int64_t)r.x * 128 + lane;
+        const int nw = r.y;
+        double v0 = 0.0, v1 = 0.0;
+        int k = 0;
+        for (; k + 8 <= nw; k += 8) {   // eight windows' loads in flight, added in order
+            double x0[8], x1[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                x0[q] = pp[(k + q) * 128];
+                x1[q] = pp[(k + q) * 128 + 64];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                v0 += x0[q];
+                v1 += x1[q];
+            }
+        }
+        for (; k < nw; ++k) {
+            v0 += pp[k * 128];
+            v1 += pp[k * 128 + 64];
+        }
+        const RealBox &B = a.boxes.get(a.cell_box ? a.cell_box[r.w] : 0);
+        const int o = a.tperm[t] - B.t_base;
+        if (KIND == 0) {
+            B.u[2 * o] = v0 / (4 * SE2P_PI);
+            B.u[2 * o + 1] = v1 / (4 * SE2P_PI);
+        } else {
+            B.u[2 * o] = v0 / 4 / SE2P_PI;
+            B.u[2 * o + 1] = v1 / 4 / SE2P_PI;
+        }
+    }
+}
+
 // Pair statistics: one wave per chunk, counts pairs evaluated / rejected
 // (every neighbour-cell pair the reference examines).
 __global__ __launch_bounds__(64) void k_real_count(RealArgs a)
@@ -713,10 +1131,7 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (KIND == 0 && SIDE >= 1)
         for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
-    if (threadIdx.x < SE2P_EXP2_N) {
-        const double t[SE2P_EXP2_N] = SE2P_EXP2_TABLE;
-        etab[threadIdx.x] = t[threadIdx.x];
-    }
+    for (int i = threadIdx.x; i < SE2P_EXP2_N; i += blockDim.x) etab[i] = c_exp2_table[i];
     __syncthreads();
 
     const int b = xcd_chunk(a.chunk_off[a.ncell]);
@@ -938,10 +1353,7 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (KIND == 0)
         for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
-    if (threadIdx.x < SE2P_EXP2_N) {
-        const double t[SE2P_EXP2_N] = SE2P_EXP2_TABLE;
-        etab[threadIdx.x] = t[threadIdx.x];
-    }
+    for (int i = threadIdx.x; i < SE2P_EXP2_N; i += blockDim.x) etab[i] = c_exp2_table[i];
     __syncthreads();
 
     const int b = xcd_chunk(a.chunk_off[a.ncell]);
@@ -1447,6 +1859,8 @@ struct RealPlan {
     double4 *sbox = nullptr;
     RealBox *dbox = nullptr;
     BoxTable btab{};
+    WinPlan win{};   // single density: windows, work items and split slots
+    int win_slots = 0;
 };
 
 // Box bases, derived parameters and the plan's sizes; 0 or an error code.
@@ -1504,6 +1918,18 @@ static void real_layout(Carve &cv, RealPlan &p)
     p.cnt = cv.take<unsigned long long>(2);
     p.bmap = cv.take<int>(p.max_chunks);
     p.sbox = cv.take<double4>(p.nkeys);
+    if (p.nrhs == 1) {
+        p.win.slots = p.win_slots;
+        p.win.keep = cv.take<unsigned long long>(2 * (size_t)p.max_chunks);
+        p.win.npos = cv.take<int>(p.max_chunks);
+        p.win.where = cv.take<int2>(p.max_chunks);
+        p.win.item_off = cv.take<int>(p.max_chunks);
+        p.win.slot = cv.take<int>(p.max_chunks);
+        p.win.item_chunk = cv.take<int>(p.max_chunks + p.win_slots);
+        p.win.split = cv.take<int4>(p.max_chunks);
+        p.win.meta = cv.take<int>(kQueueStride * 9);
+        p.win.part = cv.take<double>(128 * (size_t)p.win_slots);
+    }
     if (p.nbox > 1) {
         p.dbox = cv.take<RealBox>(p.nbox);
         p.offs = cv.take<int>(3 * (size_t)(p.nbox + 1));
@@ -1656,8 +2082,23 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
     // a call whose backward may follow keeps its plan in a buffer of its own
     const bool keep = ctx->keep && !ctx->host_ptrs && nbox == 1 && nrhs == 1 && !counting &&
                       p.same_points && !fixed;
+    p.win_slots = nrhs == 1 && !counting ? kWinSlots : 0;
     Carve sizing;
     real_layout(sizing, p);
+    if (fixed && sizing.off > fixed_bytes && p.win_slots > 0) {
+        // the fixed scratch holds fewer window slots (results do not depend
+        // on how many chunks are split)
+        p.win_slots = 0;
+        Carve base;
+        real_layout(base, p);
+        if (base.off <= fixed_bytes) {
+            const size_t spare = fixed_bytes - base.off;
+            const size_t per_slot = 128 * sizeof(double) + sizeof(int);   // sums + item entry
+            p.win_slots = spare > 1024 ? (int)std::min<size_t>(kWinSlots, (spare - 1024) / per_slot) : 0;
+        }
+        sizing = Carve{};
+        real_layout(sizing, p);
+    }
     Carve cv;
     if (fixed) {
         // caller-provided scratch: nothing is allocated
@@ -1759,9 +2200,23 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
             a.r_begin = n4;
             a.r_end = nrhs;
             const dim3 grid(grid8, nrhs - n4);
-            if (rec) {   // single density: source records
-                if (kind == 0) hipLaunchKernelGGL((k_real<0, 1, true>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
-                else hipLaunchKernelGGL((k_real<1, 1, true>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
+            if (rec) {   // single density: source records, windows of the chunks
+                const WinPlan &wp = p.win;
+                const unsigned cgrid = (unsigned)((p.max_chunks + kWaves - 1) / kWaves);
+                hipLaunchKernelGGL(k_real_win_plan, dim3(cgrid), dim3(256), 0, st, a, soff, sbox, wp);
+                hipLaunchKernelGGL(k_real_win_scan, dim3(1), dim3(64 * kWinScanWaves), 0, st, p.coff,
+                                   (int)p.ncell, wp);
+                const int64_t items = p.max_chunks + p.win_slots;
+                const int64_t cap = (int64_t)ctx->ncu * kWinBlocksPerCU;
+                const unsigned wgrid = (unsigned)((std::min(items, cap) + 7) & ~int64_t(7));
+                const unsigned fgrid = std::min<unsigned>(cgrid, kWinFinalBlocks);
+                if (kind == 0) {
+                    hipLaunchKernelGGL(k_real_win<0>, dim3(wgrid), dim3(256), 0, st, a, rec, soff, wp);
+                    hipLaunchKernelGGL(k_real_win_final<0>, dim3(fgrid), dim3(256), 0, st, a, wp);
+                } else {
+                    hipLaunchKernelGGL(k_real_win<1>, dim3(wgrid), dim3(256), 0, st, a, rec, soff, wp);
+                    hipLaunchKernelGGL(k_real_win_final<1>, dim3(fgrid), dim3(256), 0, st, a, wp);
+                }
             } else {
                 if (kind == 0) hipLaunchKernelGGL((k_real<0, 1, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
                 else hipLaunchKernelGGL((k_real<1, 1, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
