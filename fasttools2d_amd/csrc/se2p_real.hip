@@ -168,6 +168,24 @@ struct RealArgs {
 };
 
 // ---- pair kernels --------------------------------------------------------
+// E1(x) as se2p_e1 computes it (same table, same operations), reading the
+// coefficients from LDS in groups of four: a scheduling barrier after each
+// group keeps the compiler from holding all fifteen in registers at once,
+// which would cost the Stokeslet kernels a third of their occupancy.
+__device__ __forceinline__ double e1_lds(double x, double emx, const double *tab)
+{
+    const double *row = tab + SE2P_E1_ROW * se2p_e1_piece(x);
+    double s = x * row[0] + row[1];
+    double v = row[2 + SE2P_E1_NCOEF - 1];
+#pragma unroll
+    for (int k = SE2P_E1_NCOEF - 2; k >= 0; --k) {
+        v = v * s + row[2 + k];
+        if (k % 4 == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+    if (x < 1.0) return x * v - SE2P_EULER_GAMMA - log(x);
+    return emx * v;
+}
+
 // Stokeslet pair, before 1/4pi: 0.5 E1(xi^2 r^2) f + e^{-xi^2 r^2}((d.f) d/r^2 - f),
 // arranged as in mex_stokes_slp_real.cpp:207-225: c = d1 f2 - d2 f1,
 // u1 += E/2 f1 + (e/r2) d2 c,  u2 += E/2 f2 - (e/r2) d1 c.
@@ -177,7 +195,7 @@ __device__ __forceinline__ void slp_factors(double r2, double xi, const double *
 {
     const double x = r2 * (xi * xi);   // xi^2 is loop-invariant: one multiply per pair
     const double e = se2p_exp_neg(x, etab);
-    hE = 0.5 * se2p_e1(x, e, e1tab);
+    hE = 0.5 * e1_lds(x, e, e1tab);
     g = e * rcp_nr(r2);
 }
 
@@ -603,14 +621,12 @@ struct WinPlan {
 
 __device__ __forceinline__ int win_count(int n) { return n > kWin ? (n + kWin - 1) / kWin : 1; }
 
-// Sequence length and kept runs of every chunk: one wavefront per chunk.
-__global__ __launch_bounds__(256) void k_real_win_plan(RealArgs a, const int *__restrict__ soff,
-                                                       const double4 *__restrict__ sbox, WinPlan w)
+// Sequence length and kept runs of chunk c (one wavefront).
+__device__ __forceinline__ void win_plan_chunk(const RealArgs &a, const int *__restrict__ soff,
+                                               const double4 *__restrict__ sbox, const WinPlan &w,
+                                               int c)
 {
     const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * kWaves + (threadIdx.x >> 6);
-    const int nch = a.chunk_off[a.ncell];
-    if (c >= nch) return;   // uniform over the wavefront
     const int cell = a.block_cell[c];
     const RealBox &B = a.boxes.get(a.cell_box ? a.cell_box[cell] : 0);
     const int nsx = B.nsx, nsy = B.nsy, cbase = B.cell_base;
@@ -653,34 +669,25 @@ __global__ __launch_bounds__(256) void k_real_win_plan(RealArgs a, const int *__
 // per window, the others are one item each.  theta is the smallest of
 // 2, 4, ..., 64 whose split chunks fit the window slots (none split if even
 // 64 does not).  Split chunks are also listed, with what k_real_win_final
-// needs, for that kernel.  One workgroup; thread t owns a contiguous
-// segment of chunks.
+// needs, for that kernel.  One workgroup of kWinScanWaves wavefronts;
+// wavefront v owns a contiguous range of chunks and walks it 64 chunks
+// (one per lane) at a time, so every load is coalesced.
 constexpr int kWinScanWaves = 16;
-constexpr int kWinTheta = 6;     // candidate thresholds 2 << i
-constexpr int kWinLds = 4096;    // chunks whose window counts the scan stages in LDS
+constexpr int kWinTheta = 6;   // candidate thresholds 2 << i
 __global__ __launch_bounds__(64 * kWinScanWaves) void k_real_win_scan(const int *__restrict__ chunk_off,
                                                                       int ncell, WinPlan w)
 {
-    __shared__ int wsum[3][kWinScanWaves];
     __shared__ unsigned long long wtot[kWinScanWaves][kWinTheta];
-    __shared__ int lnw[kWinLds];
+    __shared__ int wcnt[kWinScanWaves][3];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nch = chunk_off[ncell];
-    const bool staged = nch <= kWinLds;
-    // window counts: staged in LDS by strided (independent, pipelined) loads
-    if (staged) {
-#pragma unroll 4
-        for (int c = threadIdx.x; c < nch; c += blockDim.x) lnw[c] = win_count(w.npos[c]);
-        __syncthreads();
-    }
-    auto nw_of = [&](int c) { return staged ? lnw[c] : win_count(w.npos[c]); };
-    // thread t owns the contiguous segment [c0, c1)
-    const int per = (nch + (int)blockDim.x - 1) / (int)blockDim.x;
-    const int c0 = min((int)threadIdx.x * per, nch), c1 = min(c0 + per, nch);
+    const int range = (nch + kWinScanWaves - 1) / kWinScanWaves;
+    const int r0 = min(wave * range, nch), r1 = min(r0 + range, nch);
     // windows of the chunks with at least 2 << q windows
     unsigned long long sum[kWinTheta] = {};
-    for (int c = c0; c < c1; ++c) {
-        const int nw = nw_of(c);
+#pragma unroll 4
+    for (int c = r0 + lane; c < r1; c += 64) {
+        const int nw = win_count(w.npos[c]);
 #pragma unroll
         for (int q = 0; q < kWinTheta; ++q) sum[q] += nw >= (2 << q) ? nw : 0;
     }
@@ -696,37 +703,89 @@ __global__ __launch_bounds__(64 * kWinScanWaves) void k_real_win_scan(const int 
         for (int v = 0; v < kWinScanWaves; ++v) t += wtot[v][q];
         if (t <= (unsigned long long)w.slots) theta = 2 << q;
     }
-    int ni = 0, ns = 0, nl = 0;
-    for (int c = c0; c < c1; ++c) {
-        const int nw = nw_of(c);
-        ni += nw >= theta ? nw : 1;
-        ns += nw >= theta ? nw : 0;
-        nl += nw >= theta ? 1 : 0;
-    }
-    int ti, ts, tl;
-    int ci = block_scan_excl<kWinScanWaves>(ni, wsum[0], &ti);
-    int cs = block_scan_excl<kWinScanWaves>(ns, wsum[1], &ts);
-    int cl = block_scan_excl<kWinScanWaves>(nl, wsum[2], &tl);
-    for (int c = c0; c < c1; ++c) {
-        const int nw = nw_of(c);
+    // per chunk: work items, window slots, split-list entries
+    auto counts = [&](int nw, int &ni, int &ns, int &nl) {
         const bool split = nw >= theta;
-        const int n = split ? nw : 1;
-        w.item_off[c] = ci;
-        w.slot[c] = split ? cs : -1;
-        for (int k = 0; k < n; ++k) w.item_chunk[ci + k] = c;
-        if (split) {
-            const int2 at = w.where[c];
-            w.split[cl++] = make_int4(cs, nw, at.x, at.y);
-            cs += nw;
+        ni = split ? nw : 1;
+        ns = split ? nw : 0;
+        nl = split ? 1 : 0;
+    };
+    int ti = 0, ts = 0, tl = 0;
+#pragma unroll 4
+    for (int c = r0 + lane; c < r1; c += 64) {
+        int ni, ns, nl;
+        counts(win_count(w.npos[c]), ni, ns, nl);
+        ti += ni;
+        ts += ns;
+        tl += nl;
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        ti += __shfl_xor(ti, d);
+        ts += __shfl_xor(ts, d);
+        tl += __shfl_xor(tl, d);
+    }
+    if (lane == 0) {
+        wcnt[wave][0] = ti;
+        wcnt[wave][1] = ts;
+        wcnt[wave][2] = tl;
+    }
+    __syncthreads();
+    int ci = 0, cs = 0, cl = 0;   // this wavefront's first item, slot, split-list entry
+    for (int v = 0; v < wave; ++v) {
+        ci += wcnt[v][0];
+        cs += wcnt[v][1];
+        cl += wcnt[v][2];
+    }
+    for (int base = r0; base < r1; base += 64) {
+        const int c = base + lane;
+        int nw = 0, ni = 0, ns = 0, nl = 0;
+        if (c < r1) {
+            nw = win_count(w.npos[c]);
+            counts(nw, ni, ns, nl);
         }
-        ci += n;
+        // inclusive scans across the wavefront
+        int si = ni, ss = ns, sl = nl;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int a = __shfl_up(si, d), b = __shfl_up(ss, d), e = __shfl_up(sl, d);
+            if (lane >= d) {
+                si += a;
+                ss += b;
+                sl += e;
+            }
+        }
+        if (c < r1) {
+            const int io = ci + si - ni, so = cs + ss - ns, lo = cl + sl - nl;
+            w.item_off[c] = io;
+            w.slot[c] = ns ? so : -1;
+            for (int k = 0; k < ni; ++k) w.item_chunk[io + k] = c;
+            if (ns) {
+                const int2 at = w.where[c];
+                w.split[lo] = make_int4(so, nw, at.x, at.y);
+            }
+        }
+        ci += __shfl(si, 63);
+        cs += __shfl(ss, 63);
+        cl += __shfl(sl, 63);
     }
     if (threadIdx.x == 0) {
-        w.meta[0] = ti;
+        int t0 = 0, t2 = 0;
+        for (int v = 0; v < kWinScanWaves; ++v) {
+            t0 += wcnt[v][0];
+            t2 += wcnt[v][2];
+        }
+        w.meta[0] = t0;
         w.meta[1] = theta;
-        w.meta[2] = tl;
+        w.meta[2] = t2;
     }
     if (threadIdx.x < 8) w.meta[kQueueStride * (threadIdx.x + 1)] = 0;
+}
+
+// Sequence length and kept runs of every chunk: one wavefront per chunk.
+__global__ __launch_bounds__(256) void k_real_win_plan(RealArgs a, const int *__restrict__ soff,
+                                                       const double4 *__restrict__ sbox, WinPlan w)
+{
+    const int c = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (c < a.chunk_off[a.ncell]) win_plan_chunk(a, soff, sbox, w, c);   // uniform over the wavefront
 }
 
 template <int KIND>
