@@ -1258,7 +1258,7 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
             if (KIND == 0) {
                 slp_grad(d1, d2, a.xi2, e, ir2, a1, a2, ld[0], ld[1], acc[0], acc[1]);
                 // G(d) du_t, arranged as the forward pair
-                const double E = se2p_e1(x, e, e1tab);
+                const double E = e1_lds(x, e, e1tab);
                 const double hE = 0.5 * E, g = e * ir2;
                 const double c = d1 * a2 - d2 * a1;
                 acc[2] += hE * a1 + g * d2 * c;
@@ -1468,14 +1468,17 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
         o.b2 = d.y;
         return o;
     };
-    auto pair = [&](const U &u, double zx, double zy, bool own) {
+    // own: the own cell (no cutoff); img: a neighbour seen through a
+    // periodic image (zx, zy not both zero; without one the differences are
+    // the plain ones, up to the sign of a zero)
+    auto pair = [&](const U &u, double zx, double zy, bool own, bool img) {
         if constexpr (KIND == 1) {
             double t1, t2;
-            if (own) {   // own cell, no image
+            if (img) {
+                diff<1>(u.x, u.y, lx, ly, zx, zy, t1, t2);
+            } else {
                 t1 = u.x - lx;
                 t2 = u.y - ly;
-            } else {
-                diff<1>(u.x, u.y, lx, ly, zx, zy, t1, t2);
             }
             double d11, d22;
             const double r2 = se2p_sqsum2(t1, t2, &d11, &d22);
@@ -1520,9 +1523,16 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
             acc[12] = fma(-u.b2, tt3, fma(-u.b1, tt2, acc[12]));
         } else {
             double t1, t2, s1, s2;
-            diff<0>(u.x, u.y, lx, ly, zx, zy, t1, t2);   // lane as target
-            s1 = lx - (u.x + zx);                         // lane as source
-            s2 = ly - (u.y + zy);
+            if (img) {
+                diff<0>(u.x, u.y, lx, ly, zx, zy, t1, t2);   // lane as target
+                s1 = lx - (u.x + zx);                         // lane as source
+                s2 = ly - (u.y + zy);
+            } else {
+                t1 = u.x - lx;
+                t2 = u.y - ly;
+                s1 = lx - u.x;
+                s2 = ly - u.y;
+            }
             const double rt = se2p_sqsum(t1, t2), rs = se2p_sqsum(s1, s2);
             const bool in_t = own ? rt != 0.0 : rt < cut2;
             const bool in_s = own ? rs != 0.0 : rs < cut2;
@@ -1538,7 +1548,7 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
             if (in_t) slp_grad(t1, t2, xi2, e, ir2, la1, la2, u.q0, u.q1, acc[0], acc[1]);
             if (in_s) {
                 slp_grad(s1, s2, xi2, e, ir2, u.b1, u.b2, lq0, lq1, acc[2], acc[3]);
-                const double E = se2p_e1(x, e, e1tab);
+                const double E = e1_lds(x, e, e1tab);
                 const double hE = 0.5 * E, g = e * ir2;
                 const double c = s1 * u.b2 - s2 * u.b1;
                 acc[4] += hE * u.b1 + g * s2 * c;
@@ -1546,38 +1556,32 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
             }
         }
     };
-    // uniform points u, u + kWaves, ... < u1; the next record is loaded
-    // before the current pair is evaluated (the last load repeats the current
-    // point and is unused)
-    auto run = [&](int u, int u1, double zx, double zy, bool own) {
-        if (u >= u1) return;
-        U cur = load(u);
-        for (;;) {
-            const int un = u + kWaves;
-            const U nxt = load(un < u1 ? un : u);
-            pair(cur, zx, zy, own);
-            if (un >= u1) break;
-            cur = nxt;
-            u = un;
+    // uniform points u, u + kWaves, ... < u1, two records loaded together
+    auto run = [&](int u, int u1, double zx, double zy, bool own, bool img) {
+        for (; u + kWaves < u1; u += 2 * kWaves) {
+            const U c0 = load(u), c1 = load(u + kWaves);
+            pair(c0, zx, zy, own, img);
+            pair(c1, zx, zy, own, img);
         }
+        if (u < u1) pair(load(u), zx, zy, own, img);
     };
     // TG = 16: points ub + row, ub = u0, u0 + 4, ... < u1 (lanes past u1 idle;
     // their loads are clamped to u1 - 1)
-    auto run_rows = [&](int u0, int u1, double zx, double zy, bool own) {
+    auto run_rows = [&](int u0, int u1, double zx, double zy, bool own, bool img) {
         if (u0 >= u1) return;
         int ub = u0;
         U cur = load(min(ub + row, u1 - 1));
         for (;;) {
             const int un = ub + 4;
             const U nxt = load(min(un + row, u1 - 1));
-            if (ub + row < u1) pair(cur, zx, zy, own);
+            if (ub + row < u1) pair(cur, zx, zy, own, img);
             if (un >= u1) break;
             cur = nxt;
             ub = un;
         }
     };
-    if (TG == 64) run(a.off[cell * sub] + wave, a.off[(cell + 1) * sub], 0.0, 0.0, true);
-    else run_rows(a.off[cell * sub], a.off[(cell + 1) * sub], 0.0, 0.0, true);
+    if (TG == 64) run(a.off[cell * sub] + wave, a.off[(cell + 1) * sub], 0.0, 0.0, true, false);
+    else run_rows(a.off[cell * sub], a.off[(cell + 1) * sub], 0.0, 0.0, true, false);
     double bx0, by0, bx1, by1;
     wave_bbox(active, lx, ly, bx0, by0, bx1, by1);
     const double lim = cut2 * (1.0 + 1e-9);
@@ -1600,8 +1604,14 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
             const int rb = ra + __builtin_ctz(~(bits >> ra));
             bits &= ~((1u << rb) - 1u);
             const int u0 = a.off[ub * sub + ra], u1 = a.off[ub * sub + rb];
-            if (TG == 64) run(u0 + ((wave - cnt) & (kWaves - 1)), u1, zx, zy, false);
-            else run_rows(u0, u1, zx, zy, false);
+            const bool img = zx != 0.0 || zy != 0.0;
+            if (TG == 64) {
+                if (img) run(u0 + ((wave - cnt) & (kWaves - 1)), u1, zx, zy, false, true);
+                else run(u0 + ((wave - cnt) & (kWaves - 1)), u1, 0.0, 0.0, false, false);
+            } else {
+                if (img) run_rows(u0, u1, zx, zy, false, true);
+                else run_rows(u0, u1, 0.0, 0.0, false, false);
+            }
             cnt += u1 - u0;
         }
     }
@@ -1664,6 +1674,263 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
             a.dn[2 * o] = m1 * f1 + m2 * f2;
             a.dn[2 * o + 1] = m2 * f1 + m3 * f2;
         }
+    }
+}
+
+// ---- windowed stresslet adjoint (targets are the sources) ------------------
+// The pass of k_real_adj_same<1, 64> over work items: a chunk with at least
+// the scan's threshold of windows (k_real_win_scan, its own slot budget) is
+// split into one item per window of its source sequence, whose 13 sums per
+// lane go to slots that k_real_adj_win_final adds in window order; other
+// chunks walk their whole sequence in one item.  Kept and rebuilt backward
+// calls plan the same items from the same data, so their results agree bit
+// for bit.
+constexpr int kAdjNV = 13;          // stresslet adjoint sums per lane
+constexpr int kAdjSlots = 16384;    // window slots (kAdjNV x 64 doubles each, 104 MiB)
+constexpr int kAdjBlocksPerCU = 4;
+
+// The lane point's gradients from its 13 sums (stresslet, k_real_adj_same's
+// epilogue): the lane's du and densities multiplied back in.
+__device__ __forceinline__ void adj_dlp_out(const RealAdjSameArgs &a, int l, const double *acc)
+{
+    const double *rp = a.rec + (int64_t)l * SrcRec<1>::n;
+    const double lq0 = -0.25 * rp[2], lq1 = -0.25 * rp[3], lq2 = -0.25 * rp[4];
+    const double2 du = a.du[l];
+    const double la1 = du.x, la2 = du.y;
+    const int o = a.perm[l];
+    const double s4 = 1.0 / (4 * SE2P_PI);
+    if (a.dptar) {
+        const double t0 = -0.25 * acc[0], t1 = -0.25 * acc[1], t2 = -0.25 * acc[2],
+                     t3 = -0.25 * acc[3];
+        a.dptar[2 * o] = -(la1 * t0 + la2 * t1) * s4;
+        a.dptar[2 * o + 1] = -(la1 * t2 + la2 * t3) * s4;
+    }
+    if (a.dpsrc) {
+        a.dpsrc[2 * o] = (lq0 * acc[4] + lq1 * acc[5] + lq2 * acc[6]) * s4;
+        a.dpsrc[2 * o + 1] = (lq0 * acc[7] + lq1 * acc[8] + lq2 * acc[9]) * s4;
+    }
+    const double m1 = acc[10] * s4, m2 = acc[11] * s4, m3 = acc[12] * s4;
+    const double f1 = a.f[2 * o], f2 = a.f[2 * o + 1];
+    const double n1 = a.nv[2 * o], n2 = a.nv[2 * o + 1];
+    if (a.df) {
+        a.df[2 * o] = m1 * n1 + m2 * n2;
+        a.df[2 * o + 1] = m2 * n1 + m3 * n2;
+    }
+    if (a.dn) {
+        a.dn[2 * o] = m1 * f1 + m2 * f2;
+        a.dn[2 * o + 1] = m2 * f1 + m3 * f2;
+    }
+}
+
+__global__ __launch_bounds__(256, 4) void k_real_adj_win(RealAdjSameArgs a, WinPlan w)
+{
+    constexpr int RN = SrcRec<1>::n;
+    constexpr int NV = kAdjNV;
+    __shared__ double etab[SE2P_EXP2_N];
+    __shared__ double red[kWaves - 1][NV][64];
+    __shared__ int s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int i = threadIdx.x; i < SE2P_EXP2_N; i += blockDim.x) etab[i] = c_exp2_table[i];
+    const int total = w.meta[0];
+    const int per = (total + 7) >> 3;
+    const int xcd = blockIdx.x & 7;
+    const int sub = 1 << (2 * kSubLog);
+    const double xi2 = a.xi2, cut2 = a.cut2;
+    int qi = 0;   // thread 0: queues found empty so far
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int item = -1;
+            for (; qi < 8; ++qi) {
+                const int x = (xcd + qi) & 7;
+                const int lo = x * per, nq = min(per, total - lo);
+                if (nq <= 0) continue;
+                const int it = atomicAdd(&w.meta[kQueueStride * (x + 1)], 1);
+                if (it < nq) {
+                    item = lo + it;
+                    break;
+                }
+            }
+            s_item = item;
+        }
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item < 0) break;   // uniform over the workgroup
+
+        const int c = w.item_chunk[item];
+        const int sl = w.slot[c];
+        const int cell = a.block_cell[c];
+        const int l = a.off[cell * sub] + 64 * (c - a.chunk_off[cell]) + lane;
+        const bool active = l < a.off[(cell + 1) * sub];
+        double lx, ly;
+        if (active) {
+            const double2 pl = *(const double2 *)(a.rec + (int64_t)l * RN);
+            lx = pl.x;
+            ly = pl.y;
+        } else {
+            lx = ly = __builtin_nan("");   // inactive lanes compute garbage, never stored
+        }
+        const int cx = cell % a.nsx, cy = cell / a.nsx;
+        const int n = w.npos[c];
+        const int kb = sl >= 0 ? item - w.item_off[c] : 0;
+        const int wa = sl >= 0 ? kb * kWin : 0, wb = sl >= 0 ? min(wa + kWin, n) : n;
+        const unsigned long long keep0 = w.keep[2 * c], keep1 = w.keep[2 * c + 1];
+        double acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+
+        struct U {
+            double x, y, q0, q1, q2, b1, b2;
+        };
+        auto load = [&](int u) {
+            U o;
+            const double *rp = a.rec + (int64_t)u * RN;
+            const double4 v0 = *(const double4 *)rp;
+            o.x = v0.x;
+            o.y = v0.y;
+            o.q0 = v0.z;
+            o.q1 = v0.w;
+            o.q2 = rp[4];
+            const double2 d = a.du[u];
+            o.b1 = d.x;
+            o.b2 = d.y;
+            return o;
+        };
+        // the pair of k_real_adj_same<1, 64>; img: a neighbour seen through a
+        // periodic image (without one the difference is the plain one, up to
+        // the sign of a zero)
+        auto pair = [&](const U &u, double zx, double zy, bool own, bool img) {
+            double t1, t2;
+            if (img) {
+                diff<1>(u.x, u.y, lx, ly, zx, zy, t1, t2);
+            } else {
+                t1 = u.x - lx;
+                t2 = u.y - ly;
+            }
+            double d11, d22;
+            const double r2 = se2p_sqsum2(t1, t2, &d11, &d22);
+            if (own ? (r2 == 0.0) : !(r2 < cut2)) return;
+            const double x = xi2 * r2;
+            const double e2 = se2p_exp_neg(x, etab);
+            const double ir2 = rcp_nr(r2);
+            const double ir4 = ir2 * ir2;
+            const double al = -4.0 * e2 * (1.0 + x) * ir4;
+            const double be = 2.0 * xi2 * e2;
+            const double g1 = 8.0 * e2 * (x * (x + 2.0) + 2.0) * (ir4 * ir2);
+            const double g2 = -2.0 * xi2 * be;
+            const double A11 = al * d11, A22 = al * d22, A12 = al * (t1 * t2);
+            const double b1 = A11 + be, b3 = A22 + be;
+            const double tt0 = t1 * (b1 + 2.0 * be), tt1 = t2 * b1, tt2 = t1 * b3,
+                         tt3 = t2 * (b3 + 2.0 * be);
+            const double G11 = g1 * d11, G22 = g1 * d22;
+            const double h0 = t1 * (G11 + 3.0 * g2), h1 = t2 * (G11 + g2), h2 = t1 * (G22 + g2),
+                         h3 = t2 * (G22 + 3.0 * g2);
+            const double D10 = t1 * h0 + 3.0 * b1, D11 = t1 * h1 + 2.0 * A12, D12 = t1 * h2 + b3,
+                         D13 = t1 * h3;
+            const double D20 = t2 * h0, D21 = t2 * h1 + b1, D22 = t2 * h2 + 2.0 * A12,
+                         D23 = t2 * h3 + 3.0 * b3;
+            acc[0] = fma(D12, u.q2, fma(D11, u.q1, fma(D10, u.q0, acc[0])));
+            acc[1] = fma(D13, u.q2, fma(D12, u.q1, fma(D11, u.q0, acc[1])));
+            acc[2] = fma(D22, u.q2, fma(D21, u.q1, fma(D20, u.q0, acc[2])));
+            acc[3] = fma(D23, u.q2, fma(D22, u.q1, fma(D21, u.q0, acc[3])));
+            acc[4] = fma(D11, u.b2, fma(D10, u.b1, acc[4]));
+            acc[5] = fma(D12, u.b2, fma(D11, u.b1, acc[5]));
+            acc[6] = fma(D13, u.b2, fma(D12, u.b1, acc[6]));
+            acc[7] = fma(D21, u.b2, fma(D20, u.b1, acc[7]));
+            acc[8] = fma(D22, u.b2, fma(D21, u.b1, acc[8]));
+            acc[9] = fma(D23, u.b2, fma(D22, u.b1, acc[9]));
+            acc[10] = fma(-u.b2, tt1, fma(-u.b1, tt0, acc[10]));
+            acc[11] = fma(-u.b2, tt2, fma(-u.b1, tt1, acc[11]));
+            acc[12] = fma(-u.b2, tt3, fma(-u.b1, tt2, acc[12]));
+        };
+        auto run = [&](int u, int u1, double zx, double zy, bool own, bool img) {
+            for (; u + kWaves < u1; u += 2 * kWaves) {
+                const U c0 = load(u), c1 = load(u + kWaves);
+                pair(c0, zx, zy, own, img);
+                pair(c1, zx, zy, own, img);
+            }
+            if (u < u1) pair(load(u), zx, zy, own, img);
+        };
+        // the walk over [wa, wb) of the sequence (k_real_win's walk)
+        int j = -1, s0 = a.off[cell * sub], s1 = a.off[(cell + 1) * sub], pos = 0, sb = 0;
+        unsigned bits = 0;
+        double zx = 0.0, zy = 0.0;
+        bool more = true;
+        while (more) {
+            const int pe = pos + (s1 - s0);
+            const int pa = max(pos, wa), pb = min(pe, wb);
+            if (pa < pb) {
+                const int sa = s0 + (pa - pos) + ((wave - pa) & (kWaves - 1));
+                const int se = s0 + (pb - pos);
+                if (j < 0) run(sa, se, 0.0, 0.0, true, false);
+                else if (zx == 0.0 && zy == 0.0) run(sa, se, 0.0, 0.0, false, false);
+                else run(sa, se, zx, zy, false, true);
+            }
+            if (pe >= wb) break;
+            pos = pe;
+            while (bits == 0) {
+                if (++j == 8) {
+                    more = false;
+                    break;
+                }
+                bits = (unsigned)(((j < 4) ? keep0 : keep1) >> (16 * (j & 3))) & 0xffffu;
+                if (bits) {
+                    const int qx = cx + nb_dx(j), qy = cy + nb_dy(j);
+                    const int wx = (qx + a.nsx) % a.nsx, wy = (qy + a.nsy) % a.nsy;
+                    sb = wy * a.nsx + wx;
+                    zx = (a.Lx * (qx - wx)) / a.nsx;   // :248-249
+                    zy = (a.Ly * (qy - wy)) / a.nsy;
+                }
+            }
+            if (!more) break;
+            const int ra = __builtin_ctz(bits);
+            const int rb = ra + __builtin_ctz(~(bits >> ra));
+            bits &= ~((1u << rb) - 1u);
+            s0 = a.off[sb * sub + ra];
+            s1 = a.off[sb * sub + rb];
+        }
+        // the four wavefronts' sums in wave order
+        if (wave > 0)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) red[wave - 1][v][lane] = acc[v];
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int q = 0; q < kWaves - 1; ++q) acc[v] += red[q][v][lane];
+            if (sl >= 0) {
+                double *pp = w.part + (int64_t)(sl + kb) * (NV * 64) + lane;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) pp[v * 64] = acc[v];
+            } else if (active) {
+                adj_dlp_out(a, l, acc);
+            }
+        }
+        __syncthreads();   // red and s_item are reused
+    }
+}
+
+// Split chunks of k_real_adj_win: window sums added in window order from
+// zero, then the lane's gradients.  One wavefront per split chunk.
+__global__ __launch_bounds__(256) void k_real_adj_win_final(RealAdjSameArgs a, WinPlan w)
+{
+    constexpr int NV = kAdjNV;
+    const int lane = threadIdx.x & 63;
+    const int nsplit = w.meta[2];
+    const int sub = 1 << (2 * kSubLog);
+    for (int i = blockIdx.x * kWaves + (threadIdx.x >> 6); i < nsplit; i += gridDim.x * kWaves) {
+        const int4 r = w.split[i];   // (slot, windows, first lane point, cell)
+        const int l = r.z + lane;
+        if (l >= a.off[(r.w + 1) * sub]) continue;
+        double acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+        const double *pp = w.part + (int64_t)r.x * (NV * 64) + lane;
+        for (int k = 0; k < r.y; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] += pp[(int64_t)k * (NV * 64) + v * 64];
+        adj_dlp_out(a, l, acc);
     }
 }
 
@@ -2387,9 +2654,21 @@ static int real_adjoint_same(se2p_ctx *ctx, int kind, const RealGeom &g, const d
     if (kept) p = ctx->kept_real->plan;
     else if (int rc = real_plan(ctx, kind, &box, 1, 1, 2 * g.ns, &p)) return rc;
     double2 *dut = nullptr;
+    // the stresslet's work items: the forward plan's chunk windows (kept, or
+    // planned here), its own items, slots and counters
+    WinPlan aw{};
     auto layout = [&](Carve &cv) {
         if (!kept) real_layout(cv, p);
         dut = cv.take<double2>(g.nt);
+        if (kind == 1) {
+            aw.slots = kAdjSlots;
+            aw.item_off = cv.take<int>(p.max_chunks);
+            aw.slot = cv.take<int>(p.max_chunks);
+            aw.item_chunk = cv.take<int>(p.max_chunks + kAdjSlots);
+            aw.split = cv.take<int4>(p.max_chunks);
+            aw.meta = cv.take<int>(kQueueStride * 9);
+            aw.part = cv.take<double>((size_t)kAdjSlots * kAdjNV * 64);
+        }
     };
     Carve sizing;
     layout(sizing);
@@ -2429,8 +2708,34 @@ static int real_adjoint_same(se2p_ctx *ctx, int kind, const RealGeom &g, const d
     a.df = df;
     a.dn = dn;
     stage_begin(ctx, ST_REAL);
-    if (kind == 0) hipLaunchKernelGGL((k_real_adj_same<0, 16>), dim3(p.grid8), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_real_adj_same<1, 64>), dim3(p.grid8), dim3(256), 0, st, a);
+    if (kind == 0) {
+        hipLaunchKernelGGL((k_real_adj_same<0, 16>), dim3(p.grid8), dim3(256), 0, st, a);
+    } else {
+        aw.keep = p.win.keep;
+        aw.npos = p.win.npos;
+        aw.where = p.win.where;
+        const unsigned cgrid = (unsigned)((p.max_chunks + kWaves - 1) / kWaves);
+        if (!kept) {   // the chunk windows, as the forward plans them
+            RealArgs ra{};
+            ra.tpos = p.tpos;
+            ra.toff = p.kss.off;
+            ra.chunk_off = p.coff;
+            ra.block_cell = p.bmap;
+            ra.ncell = (int)p.ncell;
+            ra.boxes = p.btab;
+            ra.cell_box = p.cell_box;
+            hipLaunchKernelGGL(k_real_win_plan, dim3(cgrid), dim3(256), 0, st, ra, p.kss.off, p.sbox,
+                               p.win);
+        }
+        hipLaunchKernelGGL(k_real_win_scan, dim3(1), dim3(64 * kWinScanWaves), 0, st, p.coff,
+                           (int)p.ncell, aw);
+        const int64_t items = p.max_chunks + kAdjSlots;
+        const int64_t cap = (int64_t)ctx->ncu * kAdjBlocksPerCU;
+        const unsigned wgrid = (unsigned)((std::min(items, cap) + 7) & ~int64_t(7));
+        hipLaunchKernelGGL(k_real_adj_win, dim3(wgrid), dim3(256), 0, st, a, aw);
+        hipLaunchKernelGGL(k_real_adj_win_final, dim3(std::min<unsigned>(cgrid, kWinFinalBlocks)),
+                           dim3(256), 0, st, a, aw);
+    }
     stage_end(ctx, ST_REAL);
     SE2P_HIP(hipGetLastError());
     return 0;
