@@ -777,7 +777,10 @@ def main() -> int:
 
     pairs_in, pairs_out = pe.count_pairs(ps, pt, p.nside_x, p.nside_y, w.Lx, w.Ly, ctx=ctx)
     flops_real = pairs_in * F_PAIR_IN + pairs_out * F_PAIR_OUT
-    real_ms = stage_ms["real_call.real"]
+    # the dominant kernel alone (k_real_win<1>), and the whole real-space
+    # stage around it (chunk planning, work items, split-chunk sums)
+    real_ms = stage_ms.get("real_call.pair", stage_ms["real_call.real"])
+    real_stage_ms = stage_ms["real_call.real"]
     achieved_tflops = flops_real / (real_ms * 1e-3) / 1e12
 
     ms_per_step = gpu_ms / args.steps
@@ -861,7 +864,7 @@ def main() -> int:
     if not args.no_adjoint:
         adjoint = _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, ms_per_step)
 
-    traffic = _pmc_traffic("k_real_dlp")
+    traffic = _pmc_traffic("k_real_win_dlp")
     out = {
         "metric": "target evals/s at 1e-10 rel err, 1 GPU (stresslet DLP, plain forward)",
         "value": value,
@@ -879,11 +882,17 @@ def main() -> int:
                                "P=24, tol 1e-10",
                    "N": N, "Mx": p.Mx, "My": p.My, "P": p.P, "xi": p.xi,
                    "nside": [p.nside_x, p.nside_y], "tol": p.tol},
-        "roofline": {"bound": "valu", "kernel": "k_real<1,1,true> (real-space stresslet pair sum)",
+        "roofline": {"bound": "valu",
+                     "kernel": "k_real_win<1> (real-space stresslet pair sum over chunk windows)",
                      "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "flops_per_launch": flops_real, "pairs_in": pairs_in, "pairs_out": pairs_out,
-                     "avg_kernel_ms": real_ms},
+                     "avg_kernel_ms": real_ms,
+                     "real_stage_ms": real_stage_ms,
+                     "real_stage_frac": flops_real / (real_stage_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                     "note": "achieved = pair-count flops / the pair kernel's HIP-event time; "
+                             "real_stage_* counts the whole real-space stage (chunk plan, "
+                             "work-item scan, pair kernel, split-chunk sums)"},
         "stage_ms": stage_ms,
         "stage_roofline": {"stages": stages_model, "model_ms": t_model,
                            "frac": t_model / ms_per_step,

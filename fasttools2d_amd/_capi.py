@@ -143,7 +143,7 @@ SIGNATURES.update({
                                     ctypes.POINTER(Se2pTuneBoxesInfo)]),
 })
 
-STAGES = ("sort", "real", "spread", "fft_fwd", "filter", "fft_inv", "interp")
+STAGES = ("sort", "real", "spread", "fft_fwd", "filter", "fft_inv", "interp", "pair")
 
 _lib = None
 

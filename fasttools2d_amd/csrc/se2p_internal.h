@@ -25,6 +25,7 @@ enum Stage {
     ST_FILTER,        // k-space filter
     ST_FFT_INV,       // inverse transforms
     ST_INTERP,        // Gaussian interpolation
+    ST_PAIR,          // the real-space pair kernel alone (inside ST_REAL)
     ST_COUNT
 };
 

@@ -371,10 +371,9 @@ __global__ __launch_bounds__(256) void k_run_bbox(const double2 *__restrict__ po
 // Right-hand sides per pass of the several-density pair kernel (8 or 4).
 constexpr int kBatchRB = 8;
 
-template <int KIND, int RB, bool REC>
+template <int KIND, int RB>
 __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restrict__ spos,
                                               const double *__restrict__ qsrc,
-                                              const double *__restrict__ srec,
                                               const int *__restrict__ soff,
                                               const double4 *__restrict__ sbox)
 {
@@ -430,37 +429,16 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
 #pragma unroll
         for (int r = 0; r < RB; ++r) acc[r][0] = acc[r][1] = 0.0;
         const double *qb = qsrc + (int64_t)r0 * ncomp;   // + s * qrow: source s's group
-        // one source: its record (single density) or position + density array
-        // a source's position and (one right-hand side) its record values
+        // a source's position (its densities are read per pair from qb)
         struct Src {
-            double x, y, q[kDlpComp + 1];
+            double x, y;
         };
         auto load = [&](int s) {
-            Src o;
-            if (REC) {
-                const double *rp = srec + (int64_t)s * SrcRec<KIND>::n;
-                const double4 v0 = *(const double4 *)rp;
-                o.x = v0.x;
-                o.y = v0.y;
-                o.q[0] = v0.z;
-                o.q[1] = v0.w;
-                if (KIND == 1) {
-                    const double4 v1 = *(const double4 *)(rp + 4);
-                    o.q[2] = v1.x;
-                    o.q[3] = v1.y;
-                    o.q[4] = v1.z;
-                    o.q[5] = v1.w;
-                }
-            } else {
-                const double2 sp = spos[s];
-                o.x = sp.x;
-                o.y = sp.y;
-            }
-            return o;
+            const double2 sp = spos[s];
+            return Src{sp.x, sp.y};
         };
         auto pair = [&](const Src &src, int s, double zx, double zy, bool own) {
             const double sx = src.x, sy = src.y;
-            const double *q0 = src.q;
             double d1, d2;
             if (own) {
                 // own cell, no image: both kernels' difference is src - tgt
@@ -474,61 +452,22 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
             const double r2 = se2p_sqsum2(d1, d2, &d11, &d22);
             if (own ? (r2 == 0.0) : !(r2 < cut2)) {
                 if (KIND == 0 && own) {   // coincident points: the self term
-                    if (REC) {
-                        acc[0][0] += self * q0[0];
-                        acc[0][1] += self * q0[1];
-                    } else {
 #pragma unroll
-                        for (int r = 0; r < RB; ++r) {
-                            acc[r][0] += self * qb[s * qrow + 2 * r];
-                            acc[r][1] += self * qb[s * qrow + 2 * r + 1];
-                        }
+                    for (int r = 0; r < RB; ++r) {
+                        acc[r][0] += self * qb[s * qrow + 2 * r];
+                        acc[r][1] += self * qb[s * qrow + 2 * r + 1];
                     }
                 }
                 return;
             }
-            if (REC) {
-                if (KIND == 0) {
-                    double hE, g;
-                    slp_factors(r2, xi, e1tab, etab, hE, g);
-                    slp_add(d1, d2, hE, g, q0[0], q0[1], acc[0][0], acc[0][1]);
-                } else {
-                    // record values carry -4 and 2 xi^2:
-                    //   u_i += e2 (H d_i + l'_i),  H = (1 + x)/r^4 (d.S'd),
-                    // with (1 + x)/r^4 = (1/r^2)(1/r^2 + xi^2)
-                    const double x = xi2 * r2;
-                    const double e2 = se2p_exp_neg(x, etab);
-                    const double ir2 = rcp_nr(r2);
-                    const double H = (ir2 * (ir2 + xi2)) *
-                                     (d11 * q0[0] + (d1 * d2) * q0[1] + d22 * q0[2]);
-                    acc[0][0] = fma(e2, fma(H, d1, d1 * q0[3] + d2 * q0[5]), acc[0][0]);
-                    acc[0][1] = fma(e2, fma(H, d2, d1 * q0[5] + d2 * q0[4]), acc[0][1]);
-                }
-            } else {
-                if (KIND == 0) slp_pair<RB>(d1, d2, r2, xi, e1tab, etab, qb + s * qrow, 1, acc);
-                else dlp_pair<RB>(d1, d2, d11, d22, r2, xi2, etab, qb + s * qrow, 1, acc);
-            }
+            if (KIND == 0) slp_pair<RB>(d1, d2, r2, xi, e1tab, etab, qb + s * qrow, 1, acc);
+            else dlp_pair<RB>(d1, d2, d11, d22, r2, xi2, etab, qb + s * qrow, 1, acc);
         };
 
-        // sources s0 + w, s0 + w + kWaves, ... < s1; with records, the next
-        // source's values are loaded before the current pair is evaluated
-        // (the last load repeats the current source and is unused)
+        // sources s0 + w, s0 + w + kWaves, ... < s1
         auto run = [&](int s, int s1, double zx, double zy, bool own) {
-            if constexpr (!REC) {   // several right-hand sides: plain loop
 #pragma unroll 2
-                for (; s < s1; s += kWaves) pair(load(s), s, zx, zy, own);
-                return;
-            }
-            if (s >= s1) return;
-            Src cur = load(s);
-            for (;;) {
-                const int sn = s + kWaves;
-                const Src nxt = load(sn < s1 ? sn : s);
-                pair(cur, s, zx, zy, own);
-                if (sn >= s1) break;
-                cur = nxt;
-                s = sn;
-            }
+            for (; s < s1; s += kWaves) pair(load(s), s, zx, zy, own);
         };
         // own cell: every pair, no cutoff
         run(soff[cell * sub] + wave, soff[(cell + 1) * sub], 0.0, 0.0, true);
@@ -1358,9 +1297,10 @@ __global__ __launch_bounds__(256) void k_real_adj(RealAdjArgs a, const double2 *
 // (the source side, dL/dpsrc and the density gradient).  Each uses the
 // difference vector and inclusion test the forward sums use.
 //
-// Stresslet: the lane-as-source difference is exactly the negated
-// lane-as-target one (IEEE subtraction is antisymmetric and the image offsets
-// of the two visits are negatives), so one r^2 and one test serve both.  With
+// Stresslet (k_real_adj_win): the lane-as-source difference is exactly the
+// negated lane-as-target one (IEEE subtraction is antisymmetric and the image
+// offsets of the two visits are negatives), so one r^2 and one test serve
+// both.  With
 // t = e2 T (the three-index stresslet tensor, fully symmetric; components
 // t111, t112, t122, t222) and the merged densities q = (S11, S12+S21, S22),
 //   phi(a, q) = sum_k w_k(a, q) t_k,
@@ -1386,32 +1326,23 @@ struct RealAdjSameArgs {
     double *dpsrc, *dptar, *df, *dn;   // caller order; may be null
 };
 
-// TG: targets per wavefront.  TG = 64: the wavefronts of a workgroup share
-// its 64 targets and split every source run four ways (partial sums added
-// through LDS).  TG = 16: each wavefront owns 16 of the targets and its four
+// Stokeslet: each wavefront owns 16 of the chunk's 64 targets and its four
 // 16-lane rows take four consecutive sources per step (lane = target + 16 x
 // row); a wavefront's reach is then a quarter of the chunk, so fewer of its
-// steps carry no pair in range.  The rows' sums are added across lanes in a
-// fixed order.  The Stokeslet uses TG = 16 (5% faster at N = 4e6), the
-// stresslet TG = 64 (TG = 16 measured no faster).
-template <int KIND, int TG>
-__global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
+// steps carry no pair in range (5% faster at N = 4e6 than 64 targets per
+// wavefront).  The rows' sums are added across lanes in a fixed order.
+constexpr int kAdjTG = 16;
+__global__ __launch_bounds__(256) void k_real_adj_slp(RealAdjSameArgs a)
 {
-    constexpr int RN = SrcRec<KIND>::n;
-    static_assert(TG == 64 || TG == 16, "targets per wavefront");
-    // accumulators.  Stokeslet: target grad (2), source grad (2), density (2).
-    // Stresslet, with the lane's du and densities factored out of the sums
-    // (they are constant over the pass): T[2l + a] = sum_j (D q_j)-terms of
-    // target component l and lane du component a (4), U[2 + 3l + c] likewise
-    // for the source side and lane density c (6), then the density terms (3).
-    constexpr int NV = KIND == 0 ? 6 : 13;
-    __shared__ double e1tab[KIND == 0 ? SE2P_E1_NPIECE * SE2P_E1_ROW : 1];
+    constexpr int RN = SrcRec<0>::n;
+    constexpr int TG = kAdjTG;
+    // accumulators: target grad (2), source grad (2), density (2)
+    constexpr int NV = 6;
+    __shared__ double e1tab[SE2P_E1_NPIECE * SE2P_E1_ROW];
     __shared__ double etab[SE2P_EXP2_N];
-    __shared__ double red[TG == 64 ? kWaves - 1 : 1][NV][64];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (KIND == 0)
-        for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
+    for (int i = threadIdx.x; i < SE2P_E1_NPIECE * SE2P_E1_ROW; i += 256) e1tab[i] = c_e1_table[i];
     for (int i = threadIdx.x; i < SE2P_EXP2_N; i += blockDim.x) etab[i] = c_exp2_table[i];
     __syncthreads();
 
@@ -1421,12 +1352,12 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
     const int sub = 1 << (2 * kSubLog);
     const int l0 = a.off[cell * sub] + 64 * (b - a.chunk_off[cell]);
     const int l1 = a.off[(cell + 1) * sub];
-    if (TG == 16 && l0 + TG * wave >= l1) return;   // no target in this wavefront
-    const int l = TG == 64 ? l0 + lane : l0 + TG * wave + (lane % TG);
-    const int row = lane / TG;   // TG = 16: source phase of the lane
+    if (l0 + TG * wave >= l1) return;   // no target in this wavefront
+    const int l = l0 + TG * wave + (lane % TG);
+    const int row = lane / TG;   // source phase of the lane
     const bool active = l < l1;
     // lane point: position, densities, du
-    double lx, ly, lq0 = 0.0, lq1 = 0.0, lq2 = 0.0, la1 = 0.0, la2 = 0.0;
+    double lx, ly, lq0 = 0.0, lq1 = 0.0, la1 = 0.0, la2 = 0.0;
     if (active) {
         const double *rp = a.rec + (int64_t)l * RN;
         const double4 v0 = *(const double4 *)rp;
@@ -1434,11 +1365,6 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
         ly = v0.y;
         lq0 = v0.z;
         lq1 = v0.w;
-        if (KIND == 1) {   // the record holds -4 S: exact scaling back
-            lq0 = -0.25 * v0.z;
-            lq1 = -0.25 * v0.w;
-            lq2 = -0.25 * rp[4];
-        }
         const double2 d = a.du[l];
         la1 = d.x;
         la2 = d.y;
@@ -1452,7 +1378,7 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
     for (int v = 0; v < NV; ++v) acc[v] = 0.0;
 
     struct U {
-        double x, y, q0, q1, q2, b1, b2;
+        double x, y, q0, q1, b1, b2;
     };
     auto load = [&](int u) {
         U o;
@@ -1462,7 +1388,6 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
         o.y = v0.y;
         o.q0 = v0.z;
         o.q1 = v0.w;
-        o.q2 = KIND == 1 ? rp[4] : 0.0;
         const double2 d = a.du[u];
         o.b1 = d.x;
         o.b2 = d.y;
@@ -1472,101 +1397,41 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
     // periodic image (zx, zy not both zero; without one the differences are
     // the plain ones, up to the sign of a zero)
     auto pair = [&](const U &u, double zx, double zy, bool own, bool img) {
-        if constexpr (KIND == 1) {
-            double t1, t2;
-            if (img) {
-                diff<1>(u.x, u.y, lx, ly, zx, zy, t1, t2);
-            } else {
-                t1 = u.x - lx;
-                t2 = u.y - ly;
-            }
-            double d11, d22;
-            const double r2 = se2p_sqsum2(t1, t2, &d11, &d22);
-            if (own ? (r2 == 0.0) : !(r2 < cut2)) return;
-            const double x = xi2 * r2;
-            const double e2 = se2p_exp_neg(x, etab);
-            const double ir2 = rcp_nr(r2);
-            const double ir4 = ir2 * ir2;
-            const double al = -4.0 * e2 * (1.0 + x) * ir4;
-            const double be = 2.0 * xi2 * e2;
-            const double g1 = 8.0 * e2 * (x * (x + 2.0) + 2.0) * (ir4 * ir2);
-            const double g2 = -2.0 * xi2 * be;
-            const double A11 = al * d11, A22 = al * d22, A12 = al * (t1 * t2);
-            const double b1 = A11 + be, b3 = A22 + be;
-            // t (lane as target); the source side's tensor is -t
-            const double tt0 = t1 * (b1 + 2.0 * be), tt1 = t2 * b1, tt2 = t1 * b3,
-                         tt3 = t2 * (b3 + 2.0 * be);
-            const double G11 = g1 * d11, G22 = g1 * d22;
-            const double h0 = t1 * (G11 + 3.0 * g2), h1 = t2 * (G11 + g2), h2 = t1 * (G22 + g2),
-                         h3 = t2 * (G22 + 3.0 * g2);
-            const double D10 = t1 * h0 + 3.0 * b1, D11 = t1 * h1 + 2.0 * A12, D12 = t1 * h2 + b3,
-                         D13 = t1 * h3;
-            const double D20 = t2 * h0, D21 = t2 * h1 + b1, D22 = t2 * h2 + 2.0 * A12,
-                         D23 = t2 * h3 + 3.0 * b3;
-            // target side (uniform densities q): component l, lane du a:
-            //   sum_k w_k D_lk = a1 (D_l0 q0 + D_l1 q1 + D_l2 q2)
-            //                  + a2 (D_l1 q0 + D_l2 q1 + D_l3 q2)
-            acc[0] = fma(D12, u.q2, fma(D11, u.q1, fma(D10, u.q0, acc[0])));
-            acc[1] = fma(D13, u.q2, fma(D12, u.q1, fma(D11, u.q0, acc[1])));
-            acc[2] = fma(D22, u.q2, fma(D21, u.q1, fma(D20, u.q0, acc[2])));
-            acc[3] = fma(D23, u.q2, fma(D22, u.q1, fma(D21, u.q0, acc[3])));
-            // source side (uniform du b), lane density c: D_lc b1 + D_l,c+1 b2
-            acc[4] = fma(D11, u.b2, fma(D10, u.b1, acc[4]));
-            acc[5] = fma(D12, u.b2, fma(D11, u.b1, acc[5]));
-            acc[6] = fma(D13, u.b2, fma(D12, u.b1, acc[6]));
-            acc[7] = fma(D21, u.b2, fma(D20, u.b1, acc[7]));
-            acc[8] = fma(D22, u.b2, fma(D21, u.b1, acc[8]));
-            acc[9] = fma(D23, u.b2, fma(D22, u.b1, acc[9]));
-            // density gradient of the lane as source (tensor -t)
-            acc[10] = fma(-u.b2, tt1, fma(-u.b1, tt0, acc[10]));
-            acc[11] = fma(-u.b2, tt2, fma(-u.b1, tt1, acc[11]));
-            acc[12] = fma(-u.b2, tt3, fma(-u.b1, tt2, acc[12]));
+        double t1, t2, s1, s2;
+        if (img) {
+            diff<0>(u.x, u.y, lx, ly, zx, zy, t1, t2);   // lane as target
+            s1 = lx - (u.x + zx);                         // lane as source
+            s2 = ly - (u.y + zy);
         } else {
-            double t1, t2, s1, s2;
-            if (img) {
-                diff<0>(u.x, u.y, lx, ly, zx, zy, t1, t2);   // lane as target
-                s1 = lx - (u.x + zx);                         // lane as source
-                s2 = ly - (u.y + zy);
-            } else {
-                t1 = u.x - lx;
-                t2 = u.y - ly;
-                s1 = lx - u.x;
-                s2 = ly - u.y;
-            }
-            const double rt = se2p_sqsum(t1, t2), rs = se2p_sqsum(s1, s2);
-            const bool in_t = own ? rt != 0.0 : rt < cut2;
-            const bool in_s = own ? rs != 0.0 : rs < cut2;
-            if (own && rs == 0.0) {   // coincident: self term, no gradient
-                acc[4] += a.self * u.b1;
-                acc[5] += a.self * u.b2;
-            }
-            if (!(in_t || in_s)) return;
-            const double r2 = in_t ? rt : rs;
-            const double x = r2 * xi2;
-            const double e = se2p_exp_neg(x, etab);
-            const double ir2 = rcp_nr(r2);
-            if (in_t) slp_grad(t1, t2, xi2, e, ir2, la1, la2, u.q0, u.q1, acc[0], acc[1]);
-            if (in_s) {
-                slp_grad(s1, s2, xi2, e, ir2, u.b1, u.b2, lq0, lq1, acc[2], acc[3]);
-                const double E = e1_lds(x, e, e1tab);
-                const double hE = 0.5 * E, g = e * ir2;
-                const double c = s1 * u.b2 - s2 * u.b1;
-                acc[4] += hE * u.b1 + g * s2 * c;
-                acc[5] += hE * u.b2 - g * s1 * c;
-            }
+            t1 = u.x - lx;
+            t2 = u.y - ly;
+            s1 = lx - u.x;
+            s2 = ly - u.y;
+        }
+        const double rt = se2p_sqsum(t1, t2), rs = se2p_sqsum(s1, s2);
+        const bool in_t = own ? rt != 0.0 : rt < cut2;
+        const bool in_s = own ? rs != 0.0 : rs < cut2;
+        if (own && rs == 0.0) {   // coincident: self term, no gradient
+            acc[4] += a.self * u.b1;
+            acc[5] += a.self * u.b2;
+        }
+        if (!(in_t || in_s)) return;
+        const double r2 = in_t ? rt : rs;
+        const double x = r2 * xi2;
+        const double e = se2p_exp_neg(x, etab);
+        const double ir2 = rcp_nr(r2);
+        if (in_t) slp_grad(t1, t2, xi2, e, ir2, la1, la2, u.q0, u.q1, acc[0], acc[1]);
+        if (in_s) {
+            slp_grad(s1, s2, xi2, e, ir2, u.b1, u.b2, lq0, lq1, acc[2], acc[3]);
+            const double E = e1_lds(x, e, e1tab);
+            const double hE = 0.5 * E, g = e * ir2;
+            const double c = s1 * u.b2 - s2 * u.b1;
+            acc[4] += hE * u.b1 + g * s2 * c;
+            acc[5] += hE * u.b2 - g * s1 * c;
         }
     };
-    // uniform points u, u + kWaves, ... < u1, two records loaded together
-    auto run = [&](int u, int u1, double zx, double zy, bool own, bool img) {
-        for (; u + kWaves < u1; u += 2 * kWaves) {
-            const U c0 = load(u), c1 = load(u + kWaves);
-            pair(c0, zx, zy, own, img);
-            pair(c1, zx, zy, own, img);
-        }
-        if (u < u1) pair(load(u), zx, zy, own, img);
-    };
-    // TG = 16: points ub + row, ub = u0, u0 + 4, ... < u1 (lanes past u1 idle;
-    // their loads are clamped to u1 - 1)
+    // points ub + row, ub = u0, u0 + 4, ... < u1 (lanes past u1 idle; their
+    // loads are clamped to u1 - 1)
     auto run_rows = [&](int u0, int u1, double zx, double zy, bool own, bool img) {
         if (u0 >= u1) return;
         int ub = u0;
@@ -1580,8 +1445,7 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
             ub = un;
         }
     };
-    if (TG == 64) run(a.off[cell * sub] + wave, a.off[(cell + 1) * sub], 0.0, 0.0, true, false);
-    else run_rows(a.off[cell * sub], a.off[(cell + 1) * sub], 0.0, 0.0, true, false);
+    run_rows(a.off[cell * sub], a.off[(cell + 1) * sub], 0.0, 0.0, true, false);
     double bx0, by0, bx1, by1;
     wave_bbox(active, lx, ly, bx0, by0, bx1, by1);
     const double lim = cut2 * (1.0 + 1e-9);
@@ -1589,7 +1453,6 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
         keep_mask(a.box, 0, lane, 0, cx, cy, a.nsx, a.nsy, a.Lx, a.Ly, lim, bx0, by0, bx1, by1);
     const unsigned long long keep1 =
         keep_mask(a.box, 1, lane, 0, cx, cy, a.nsx, a.nsy, a.Lx, a.Ly, lim, bx0, by0, bx1, by1);
-    int cnt = 0;
 #pragma unroll 1
     for (int j = 0; j < 8; ++j) {
         unsigned bits = (unsigned)(((j < 4) ? keep0 : keep1) >> (16 * (j & 3))) & 0xffffu;
@@ -1604,81 +1467,35 @@ __global__ __launch_bounds__(256) void k_real_adj_same(RealAdjSameArgs a)
             const int rb = ra + __builtin_ctz(~(bits >> ra));
             bits &= ~((1u << rb) - 1u);
             const int u0 = a.off[ub * sub + ra], u1 = a.off[ub * sub + rb];
-            const bool img = zx != 0.0 || zy != 0.0;
-            if (TG == 64) {
-                if (img) run(u0 + ((wave - cnt) & (kWaves - 1)), u1, zx, zy, false, true);
-                else run(u0 + ((wave - cnt) & (kWaves - 1)), u1, 0.0, 0.0, false, false);
-            } else {
-                if (img) run_rows(u0, u1, zx, zy, false, true);
-                else run_rows(u0, u1, 0.0, 0.0, false, false);
-            }
-            cnt += u1 - u0;
+            if (zx != 0.0 || zy != 0.0) run_rows(u0, u1, zx, zy, false, true);
+            else run_rows(u0, u1, 0.0, 0.0, false, false);
         }
     }
-    if (TG == 64) {
-        if (wave > 0)
+    // rows 0..3: (row 0 + row 2) + (row 1 + row 3), in lanes 0..15
 #pragma unroll
-            for (int v = 0; v < NV; ++v) red[wave - 1][v][lane] = acc[v];
-        __syncthreads();
-        if (wave != 0 || !active) return;
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-#pragma unroll
-            for (int w = 0; w < kWaves - 1; ++w) acc[v] += red[w][v][lane];
-    } else {
-        // rows 0..3: (row 0 + row 2) + (row 1 + row 3), in lanes 0..15
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            acc[v] += __shfl_down(acc[v], 32);
-            acc[v] += __shfl_down(acc[v], 16);
-        }
-        if (row != 0 || !active) return;
+    for (int v = 0; v < NV; ++v) {
+        acc[v] += __shfl_down(acc[v], 32);
+        acc[v] += __shfl_down(acc[v], 16);
     }
+    if (row != 0 || !active) return;
     const int o = a.perm[l];
     const double s4 = 1.0 / (4 * SE2P_PI);
-    if (KIND == 0) {
-        if (a.dptar) {
-            a.dptar[2 * o] = -acc[0] * s4;
-            a.dptar[2 * o + 1] = -acc[1] * s4;
-        }
-        if (a.dpsrc) {
-            a.dpsrc[2 * o] = acc[2] * s4;
-            a.dpsrc[2 * o + 1] = acc[3] * s4;
-        }
-        if (a.df) {
-            a.df[2 * o] = acc[4] * s4;
-            a.df[2 * o + 1] = acc[5] * s4;
-        }
-    } else {
-        // the lane's du and densities multiplied back in; the target-side
-        // sums used the records' -4 S (exact scaling back)
-        if (a.dptar) {
-            const double t0 = -0.25 * acc[0], t1 = -0.25 * acc[1], t2 = -0.25 * acc[2],
-                         t3 = -0.25 * acc[3];
-            a.dptar[2 * o] = -(la1 * t0 + la2 * t1) * s4;
-            a.dptar[2 * o + 1] = -(la1 * t2 + la2 * t3) * s4;
-        }
-        if (a.dpsrc) {
-            a.dpsrc[2 * o] = (lq0 * acc[4] + lq1 * acc[5] + lq2 * acc[6]) * s4;
-            a.dpsrc[2 * o + 1] = (lq0 * acc[7] + lq1 * acc[8] + lq2 * acc[9]) * s4;
-        }
-        // m = dL/d(S11, S12 + S21, S22), S = f (x) n
-        const double m1 = acc[10] * s4, m2 = acc[11] * s4, m3 = acc[12] * s4;
-        const double f1 = a.f[2 * o], f2 = a.f[2 * o + 1];
-        const double n1 = a.nv[2 * o], n2 = a.nv[2 * o + 1];
-        if (a.df) {
-            a.df[2 * o] = m1 * n1 + m2 * n2;
-            a.df[2 * o + 1] = m2 * n1 + m3 * n2;
-        }
-        if (a.dn) {
-            a.dn[2 * o] = m1 * f1 + m2 * f2;
-            a.dn[2 * o + 1] = m2 * f1 + m3 * f2;
-        }
+    if (a.dptar) {
+        a.dptar[2 * o] = -acc[0] * s4;
+        a.dptar[2 * o + 1] = -acc[1] * s4;
+    }
+    if (a.dpsrc) {
+        a.dpsrc[2 * o] = acc[2] * s4;
+        a.dpsrc[2 * o + 1] = acc[3] * s4;
+    }
+    if (a.df) {
+        a.df[2 * o] = acc[4] * s4;
+        a.df[2 * o + 1] = acc[5] * s4;
     }
 }
 
 // ---- windowed stresslet adjoint (targets are the sources) ------------------
-// The pass of k_real_adj_same<1, 64> over work items: a chunk with at least
+// The stresslet adjoint pass over work items: a chunk with at least
 // the scan's threshold of windows (k_real_win_scan, its own slot budget) is
 // split into one item per window of its source sequence, whose 13 sums per
 // lane go to slots that k_real_adj_win_final adds in window order; other
@@ -1689,8 +1506,8 @@ constexpr int kAdjNV = 13;          // stresslet adjoint sums per lane
 constexpr int kAdjSlots = 16384;    // window slots (kAdjNV x 64 doubles each, 104 MiB)
 constexpr int kAdjBlocksPerCU = 4;
 
-// The lane point's gradients from its 13 sums (stresslet, k_real_adj_same's
-// epilogue): the lane's du and densities multiplied back in.
+// The lane point's gradients from its 13 sums (stresslet): the lane's du
+// and densities multiplied back in.
 __device__ __forceinline__ void adj_dlp_out(const RealAdjSameArgs &a, int l, const double *acc)
 {
     const double *rp = a.rec + (int64_t)l * SrcRec<1>::n;
@@ -1796,7 +1613,7 @@ __global__ __launch_bounds__(256, 4) void k_real_adj_win(RealAdjSameArgs a, WinP
             o.b2 = d.y;
             return o;
         };
-        // the pair of k_real_adj_same<1, 64>; img: a neighbour seen through a
+        // the stresslet pair (see above); img: a neighbour seen through a
         // periodic image (without one the difference is the plain one, up to
         // the sign of a zero)
         auto pair = [&](const U &u, double zx, double zy, bool own, bool img) {
@@ -2511,16 +2328,16 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
             a.r_begin = 0;
             a.r_end = n8;
             const dim3 grid(grid8, n8 / 8);
-            if (kind == 0) hipLaunchKernelGGL((k_real<0, 8, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
-            else hipLaunchKernelGGL((k_real<1, 8, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
+            if (kind == 0) hipLaunchKernelGGL((k_real<0, 8>), grid, dim3(256), 0, st, a, spos, q, soff, sbox);
+            else hipLaunchKernelGGL((k_real<1, 8>), grid, dim3(256), 0, st, a, spos, q, soff, sbox);
         }
         const int n4 = n8 + (nrhs - n8) / 4 * 4;
         if (n4 > n8) {
             a.r_begin = n8;
             a.r_end = n4;
             const dim3 grid(grid8, (n4 - n8) / 4);
-            if (kind == 0) hipLaunchKernelGGL((k_real<0, 4, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
-            else hipLaunchKernelGGL((k_real<1, 4, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
+            if (kind == 0) hipLaunchKernelGGL((k_real<0, 4>), grid, dim3(256), 0, st, a, spos, q, soff, sbox);
+            else hipLaunchKernelGGL((k_real<1, 4>), grid, dim3(256), 0, st, a, spos, q, soff, sbox);
         }
         if (n4 < nrhs) {
             a.r_begin = n4;
@@ -2536,16 +2353,15 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
                 const int64_t cap = (int64_t)ctx->ncu * kWinBlocksPerCU;
                 const unsigned wgrid = (unsigned)((std::min(items, cap) + 7) & ~int64_t(7));
                 const unsigned fgrid = std::min<unsigned>(cgrid, kWinFinalBlocks);
-                if (kind == 0) {
-                    hipLaunchKernelGGL(k_real_win<0>, dim3(wgrid), dim3(256), 0, st, a, rec, soff, wp);
-                    hipLaunchKernelGGL(k_real_win_final<0>, dim3(fgrid), dim3(256), 0, st, a, wp);
-                } else {
-                    hipLaunchKernelGGL(k_real_win<1>, dim3(wgrid), dim3(256), 0, st, a, rec, soff, wp);
-                    hipLaunchKernelGGL(k_real_win_final<1>, dim3(fgrid), dim3(256), 0, st, a, wp);
-                }
+                stage_begin(ctx, ST_PAIR);
+                if (kind == 0) hipLaunchKernelGGL(k_real_win<0>, dim3(wgrid), dim3(256), 0, st, a, rec, soff, wp);
+                else hipLaunchKernelGGL(k_real_win<1>, dim3(wgrid), dim3(256), 0, st, a, rec, soff, wp);
+                stage_end(ctx, ST_PAIR);
+                if (kind == 0) hipLaunchKernelGGL(k_real_win_final<0>, dim3(fgrid), dim3(256), 0, st, a, wp);
+                else hipLaunchKernelGGL(k_real_win_final<1>, dim3(fgrid), dim3(256), 0, st, a, wp);
             } else {
-                if (kind == 0) hipLaunchKernelGGL((k_real<0, 1, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
-                else hipLaunchKernelGGL((k_real<1, 1, false>), grid, dim3(256), 0, st, a, spos, q, rec, soff, sbox);
+                if (kind == 0) hipLaunchKernelGGL((k_real<0, 1>), grid, dim3(256), 0, st, a, spos, q, soff, sbox);
+                else hipLaunchKernelGGL((k_real<1, 1>), grid, dim3(256), 0, st, a, spos, q, soff, sbox);
             }
         }
     }
@@ -2630,7 +2446,8 @@ int real_sum_fixed(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f, 
 
 // Adjoint when the targets are the sources: the forward's plan (kept by the
 // forward call when it matches, else rebuilt the same way), du gathered into
-// its order, one pass of k_real_adj_same.
+// its order, one pass of k_real_adj_slp (Stokeslet) or the windowed
+// k_real_adj_win (stresslet).
 static int real_adjoint_same(se2p_ctx *ctx, int kind, const RealGeom &g, const double *f,
                              const double *n, const double *du, double *dpsrc, double *dptar,
                              double *df, double *dn)
@@ -2709,7 +2526,7 @@ static int real_adjoint_same(se2p_ctx *ctx, int kind, const RealGeom &g, const d
     a.dn = dn;
     stage_begin(ctx, ST_REAL);
     if (kind == 0) {
-        hipLaunchKernelGGL((k_real_adj_same<0, 16>), dim3(p.grid8), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(k_real_adj_slp, dim3(p.grid8), dim3(256), 0, st, a);
     } else {
         aw.keep = p.win.keep;
         aw.npos = p.win.npos;
@@ -2732,7 +2549,9 @@ static int real_adjoint_same(se2p_ctx *ctx, int kind, const RealGeom &g, const d
         const int64_t items = p.max_chunks + kAdjSlots;
         const int64_t cap = (int64_t)ctx->ncu * kAdjBlocksPerCU;
         const unsigned wgrid = (unsigned)((std::min(items, cap) + 7) & ~int64_t(7));
+        stage_begin(ctx, ST_PAIR);
         hipLaunchKernelGGL(k_real_adj_win, dim3(wgrid), dim3(256), 0, st, a, aw);
+        stage_end(ctx, ST_PAIR);
         hipLaunchKernelGGL(k_real_adj_win_final, dim3(std::min<unsigned>(cgrid, kWinFinalBlocks)),
                            dim3(256), 0, st, a, aw);
     }

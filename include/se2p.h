@@ -137,8 +137,10 @@ int se2p_set_spread_chunk(se2p_ctx *ctx, int chunk);
 int se2p_set_frozen(se2p_ctx *ctx, int enable);
 /* Per-stage device timing with HIP events on the context stream.  When
  * enabled, se2p_stage_times fills up to n floats (milliseconds) for the
- * most recent call: sort, real, spread, fft_fwd, filter, fft_inv, interp
- * (-1 for a stage the call did not run). */
+ * most recent call: sort, real, spread, fft_fwd, filter, fft_inv, interp,
+ * pair (-1 for a stage the call did not run).  `real` is the whole
+ * real-space stage; `pair` is its pair-sum kernel alone (without the
+ * chunk planning and split-chunk sums around it). */
 int se2p_set_timing(se2p_ctx *ctx, int enable);
 int se2p_stage_times(se2p_ctx *ctx, float *ms, int n);
 
