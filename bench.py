@@ -7,7 +7,10 @@ smooth densities, Ewald parameters from the reference driver's formulas at
 tol 1e-10 with the grid fixed at 512^2 (P = 24).  One step = the real-space sum
 plus the Fourier-space sum (the two kernels the reference driver calls,
 mex_stokes_dlp_real + mex_stokes_dlp_kspace) over all targets, inputs resident
-in HBM.  The targets are the sources and are passed as the same array.
+in HBM.  The targets are the sources and are passed as the same array.  The
+two calls are independent and run as one concurrent group
+(se2p_group_begin / se2p_group_end; --sequential times them one after the
+other, and the line carries both step times).
 
 Prints ONE JSON line: target evaluations per second (one evaluation = one
 2-vector u(x_t) for one density), the roofline of the dominant kernel (the
@@ -422,28 +425,54 @@ def _graph_variant(pe, torch, step, ctx, stream, N, reps=10):
 def _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, fwd_ms):
     """Forward + adjoint on the headline workload: one step is the two forward
     sums plus the two backward entry points with every gradient (dL/dpsrc,
-    dL/dptar, dL/df, dL/dn) for a seeded du.  Measured twice: with the
-    forward calls keeping their sorted state for the backward calls
-    (se2p_set_keep, the training pattern; the reported step), and with the
-    backward calls rebuilding it."""
+    dL/dptar, dL/df, dL/dn) for a seeded du.  Measured with the forward calls
+    keeping their sorted state for the backward calls (se2p_set_keep, the
+    training pattern; the reported step) and with the backward calls
+    rebuilding it; each as the headline step is formed (the forward pair
+    and the backward pair each one concurrent group, or, with --sequential,
+    one call after the other) and in the other form.  ``fwd_ms`` maps each
+    form to its forward step time."""
     N = ps.shape[0]
     du = torch.from_numpy(np.random.default_rng(77).standard_normal((N, 2))).to(ps.device)
     ur = torch.empty_like(du)
     uk = torch.empty_like(du)
+    form = "sequential" if args.sequential else "grouped"
+    other = "grouped" if args.sequential else "sequential"
 
-    def backward():
-        pe.mex_stokes_dlp_real_backward(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly, du,
-                                        ctx=ctx)
-        pe.mex_stokes_dlp_kspace_backward(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, w.Lx, w.Ly, p.w,
-                                          p.P, du, ctx=ctx)
+    def pair(first, second, grouped):
+        if grouped:
+            with ctx.group():
+                first()
+                second()
+        else:
+            second()
+            first()
 
-    def step():
-        pe.mex_stokes_dlp_real(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly, ctx=ctx, out=ur)
-        pe.mex_stokes_dlp_kspace(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, w.Lx, w.Ly, p.w, p.P,
-                                 ctx=ctx, out=uk)
-        backward()
+    def make_step(grouped):
+        def fwd_k():
+            pe.mex_stokes_dlp_kspace(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, w.Lx, w.Ly, p.w,
+                                     p.P, ctx=ctx, out=uk)
 
-    def timed_steps():
+        def fwd_r():
+            pe.mex_stokes_dlp_real(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly,
+                                   ctx=ctx, out=ur)
+
+        def bwd_k():
+            pe.mex_stokes_dlp_kspace_backward(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, w.Lx,
+                                              w.Ly, p.w, p.P, du, ctx=ctx)
+
+        def bwd_r():
+            pe.mex_stokes_dlp_real_backward(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx,
+                                            w.Ly, du, ctx=ctx)
+
+        def step():
+            pair(fwd_k, fwd_r, grouped)
+            pair(bwd_k, bwd_r, grouped)
+        return step
+
+    step = make_step(not args.sequential)
+
+    def timed_steps(step=step):
         for _ in range(max(1, args.warmup)):
             step()
         torch.cuda.synchronize()
@@ -479,29 +508,41 @@ def _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, fwd_
             ctx.set_timing(False)
         return {k: statistics.mean(v) for k, v in stages.items()}
 
+    other_step = make_step(args.sequential)
     with torch.cuda.stream(stream):
         ctx.bind_torch_stream(stream)
         ctx.set_async(True)
         ms_rebuild = timed_steps()
+        ms_rebuild_other = timed_steps(other_step)
         st_rebuild = stage_split()
         ctx.set_keep(True)
         ctx.set_async(True)
         ms = timed_steps()
+        ms_other = timed_steps(other_step)
         st_keep = stage_split()
         ctx.set_keep(False)
         ctx.set_async(False)
     torch.cuda.synchronize()
-    print(f"# forward+adjoint: {ms:.3f} ms per step (kept forward state), {ms_rebuild:.3f} ms "
-          f"(rebuilt)", file=sys.stderr, flush=True)
+    f0, f1 = fwd_ms[form], fwd_ms[other]
+    print(f"# forward+adjoint ({form}): {ms:.3f} ms per step (kept forward state, "
+          f"{ms / f0:.2f}x the forward), {ms_rebuild:.3f} ms (rebuilt, {ms_rebuild / f0:.2f}x); "
+          f"{other}: {ms_other:.3f} ms ({ms_other / f1:.2f}x), {ms_rebuild_other:.3f} ms "
+          f"({ms_rebuild_other / f1:.2f}x)", file=sys.stderr, flush=True)
     return {"ms_per_step": ms, "target_evals_per_s": N / (ms * 1e-3),
-            "over_forward": ms / fwd_ms,
-            "adjoint_over_forward_time": (ms - fwd_ms) / fwd_ms,
+            "over_forward": ms / f0,
+            "adjoint_over_forward_time": (ms - f0) / f0,
+            "step_form": form,
             "stage_ms": st_keep,
-            "rebuilt": {"ms_per_step": ms_rebuild, "over_forward": ms_rebuild / fwd_ms,
+            "rebuilt": {"ms_per_step": ms_rebuild, "over_forward": ms_rebuild / f0,
                         "stage_ms": st_rebuild},
+            other: {"ms_per_step": ms_other, "over_forward": ms_other / f1,
+                    "forward_ms": f1,
+                    "rebuilt": {"ms_per_step": ms_rebuild_other,
+                                "over_forward": ms_rebuild_other / f1}},
             "workload": "same as the headline; step = forward real + Fourier sums, then both "
                         "backward entry points with all four gradients; the forward calls keep "
-                        "their sorted state for the backward calls (se2p_set_keep)"}
+                        "their sorted state for the backward calls (se2p_set_keep); "
+                        "over_forward divides by the forward step of the same form"}
 
 
 def _autotune_variant(args, pe, np, torch, w, ps, f, nv, stream, fwd_ms):
@@ -589,29 +630,41 @@ def _at_scale_variant(args, pe, W, np, torch, dev):
         ctx = pe.Context(0)
         stream = torch.cuda.Stream(dev)
 
+        def pair(real, fourier):
+            # as the headline step: one concurrent group, Fourier-space call first
+            if args.sequential:
+                real()
+                fourier()
+            else:
+                with ctx.group():
+                    fourier()
+                    real()
+
         def fwd():
             if kind == "slp":
-                pe.mex_stokes_slp_real(ps, ps, f, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, ctx=ctx, out=ur)
-                pe.mex_stokes_slp_kspace(ps, ps, p.xi, p.eta, f, p.Mx, p.My, 1.0, 1.0, p.w, p.P,
-                                         ctx=ctx, out=uk)
+                pair(lambda: pe.mex_stokes_slp_real(ps, ps, f, p.xi, p.nside_x, p.nside_y, 1.0, 1.0,
+                                                    ctx=ctx, out=ur),
+                     lambda: pe.mex_stokes_slp_kspace(ps, ps, p.xi, p.eta, f, p.Mx, p.My, 1.0, 1.0,
+                                                      p.w, p.P, ctx=ctx, out=uk))
             else:
-                pe.mex_stokes_dlp_real(ps, ps, f, nv, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, ctx=ctx,
-                                       out=ur)
-                pe.mex_stokes_dlp_kspace(ps, ps, p.xi, p.eta, f, nv, p.Mx, p.My, 1.0, 1.0, p.w, p.P,
-                                         ctx=ctx, out=uk)
+                pair(lambda: pe.mex_stokes_dlp_real(ps, ps, f, nv, p.xi, p.nside_x, p.nside_y, 1.0,
+                                                    1.0, ctx=ctx, out=ur),
+                     lambda: pe.mex_stokes_dlp_kspace(ps, ps, p.xi, p.eta, f, nv, p.Mx, p.My, 1.0,
+                                                      1.0, p.w, p.P, ctx=ctx, out=uk))
 
         def fwd_adj():
             fwd()
             if kind == "slp":
-                pe.mex_stokes_slp_real_backward(ps, ps, f, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, du,
-                                                ctx=ctx)
-                pe.mex_stokes_slp_kspace_backward(ps, ps, p.xi, p.eta, f, p.Mx, p.My, 1.0, 1.0,
-                                                  p.w, p.P, du, ctx=ctx)
+                pair(lambda: pe.mex_stokes_slp_real_backward(ps, ps, f, p.xi, p.nside_x, p.nside_y,
+                                                             1.0, 1.0, du, ctx=ctx),
+                     lambda: pe.mex_stokes_slp_kspace_backward(ps, ps, p.xi, p.eta, f, p.Mx, p.My,
+                                                               1.0, 1.0, p.w, p.P, du, ctx=ctx))
             else:
-                pe.mex_stokes_dlp_real_backward(ps, ps, f, nv, p.xi, p.nside_x, p.nside_y, 1.0, 1.0,
-                                                du, ctx=ctx)
-                pe.mex_stokes_dlp_kspace_backward(ps, ps, p.xi, p.eta, f, nv, p.Mx, p.My, 1.0, 1.0,
-                                                  p.w, p.P, du, ctx=ctx)
+                pair(lambda: pe.mex_stokes_dlp_real_backward(ps, ps, f, nv, p.xi, p.nside_x,
+                                                             p.nside_y, 1.0, 1.0, du, ctx=ctx),
+                     lambda: pe.mex_stokes_dlp_kspace_backward(ps, ps, p.xi, p.eta, f, nv, p.Mx,
+                                                               p.My, 1.0, 1.0, p.w, p.P, du,
+                                                               ctx=ctx))
 
         def timed(fn, reps=3):
             fn()
@@ -652,6 +705,7 @@ def _at_scale_variant(args, pe, W, np, torch, dev):
                                 "P": p.P, "tol": 1e-12, "mode": "fixed grid"}}
     out["workload"] = ("N = 4e6 uniform points, unit box, targets = sources, tol 1e-12, "
                        "grid 2048^2 for both kernels (fixed-grid parameters)")
+    out["step_form"] = "sequential" if args.sequential else "grouped"
     out["error_reference"] = ("forward total (real + Fourier) on 256 targets against the oracle "
                               "at tol 1e-14 on a 4096^2 grid with the drivers' 121 x 121 cells "
                               "(tests/golden/truth_uniform_N4e6.json)")
@@ -671,6 +725,9 @@ def main() -> int:
                     help="skip the forward+adjoint variant")
     ap.add_argument("--no-scale", action="store_true",
                     help="skip the N = 4e6 forward / forward+adjoint variant")
+    ap.add_argument("--sequential", action="store_true",
+                    help="time the two sums one after the other instead of as one concurrent "
+                         "group (se2p_group_begin / se2p_group_end)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (scales HBM counters)")
     ap.add_argument("--variant", choices=["batched", "batched_ellipses", "multibox", "at_scale"],
@@ -720,10 +777,27 @@ def main() -> int:
     stream = torch.cuda.Stream(dev)
     ctx.bind_torch_stream(stream)
 
-    def step():
+    def call_real():
         pe.mex_stokes_dlp_real(ps, pt, f, nv, p.xi, p.nside_x, p.nside_y, w.Lx, w.Ly, ctx=ctx, out=ur)
+
+    def call_kspace():
         pe.mex_stokes_dlp_kspace(ps, pt, p.xi, p.eta, f, nv, p.Mx, p.My, w.Lx, w.Ly, p.w, p.P,
                                  ctx=ctx, out=uk)
+
+    def step_sequential():
+        call_real()
+        call_kspace()
+
+    def step_grouped():
+        # neither sum reads the other's result: one concurrent group runs them
+        # side by side (the Fourier-space call on the context stream, the
+        # real-space call on a stream slot)
+        with ctx.group():
+            call_kspace()
+            call_real()
+
+    step, other_step = ((step_sequential, step_grouped) if args.sequential
+                        else (step_grouped, step_sequential))
 
     torch.cuda.synchronize()
     with torch.cuda.stream(stream):
@@ -744,6 +818,10 @@ def main() -> int:
         wall = time.perf_counter() - wall0
         gpu_ms = ev0.elapsed_time(ev1)
         print(f"# timed {args.steps} steps: {gpu_ms:.3f} ms (wall {wall * 1e3:.3f} ms)",
+              file=sys.stderr, flush=True)
+        # the other step form, for comparison (same outputs, bitwise)
+        other_ms = _timed_ms(torch, stream, other_step, reps=args.steps)
+        print(f"# {'grouped' if args.sequential else 'sequential'} step: {other_ms:.3f} ms",
               file=sys.stderr, flush=True)
 
         # per-stage kernel durations (HIP events on the context stream)
@@ -862,7 +940,9 @@ def main() -> int:
             variants["at_scale"] = _at_scale_variant(args, pe, W, np, torch, dev)
     adjoint = None
     if not args.no_adjoint:
-        adjoint = _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, ms_per_step)
+        fwd_forms = {"sequential" if args.sequential else "grouped": ms_per_step,
+                     "grouped" if args.sequential else "sequential": other_ms}
+        adjoint = _adjoint_variant(args, pe, np, torch, w, p, ps, pt, f, nv, ctx, stream, fwd_forms)
 
     traffic = _pmc_traffic("k_real_win_dlp")
     out = {
@@ -890,9 +970,16 @@ def main() -> int:
                      "avg_kernel_ms": real_ms,
                      "real_stage_ms": real_stage_ms,
                      "real_stage_frac": flops_real / (real_stage_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
-                     "note": "achieved = pair-count flops / the pair kernel's HIP-event time; "
+                     "note": "achieved = pair-count flops / the pair kernel's HIP-event time "
+                             "in the stage-timed calls, where it runs alone on the GPU (in the "
+                             "grouped step it shares the GPU with the Fourier-space call); "
                              "real_stage_* counts the whole real-space stage (chunk plan, "
                              "work-item scan, pair kernel, split-chunk sums)"},
+        "step_form": ("sequential: real-space call, then Fourier-space call" if args.sequential
+                      else "grouped: the Fourier-space and real-space calls in one concurrent "
+                           "group (se2p_group_begin / se2p_group_end), side by side on the GPU"),
+        "step_forms_ms": {"grouped": other_ms if args.sequential else ms_per_step,
+                          "sequential": ms_per_step if args.sequential else other_ms},
         "stage_ms": stage_ms,
         "stage_roofline": {"stages": stages_model, "model_ms": t_model,
                            "frac": t_model / ms_per_step,

@@ -47,6 +47,8 @@ SIGNATURES = {
     "se2p_set_timing": (_I, [_P, _I]),
     "se2p_set_frozen": (_I, [_P, _I]),
     "se2p_stage_times": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
+    "se2p_group_begin": (_I, [_P]),
+    "se2p_group_end": (_I, [_P]),
     "se2p_slp_real": (_I, [_P, _P, _I64, _P, _I64, _P, _D, _I, _I, _D, _D, _P]),
     "se2p_slp_kspace": (_I, [_P, _P, _I64, _P, _I64, _D, _D, _P, _I, _I, _D, _D, _D, _I, _P]),
     "se2p_dlp_real": (_I, [_P, _P, _I64, _P, _I64, _P, _P, _D, _I, _I, _D, _D, _P]),

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 #include <mutex>
+#include <optional>
 
 #include "se2p_device.h"
 #include "se2p_internal.h"
@@ -71,6 +72,8 @@ void stage_end(se2p_ctx *ctx, Stage s)
 int finish(se2p_ctx *ctx)
 {
     SE2P_HIP(hipGetLastError());
+    // a grouped call only enqueues; se2p_group_end joins and waits
+    if (ctx->group >= 0) return SE2P_OK;
     if (!ctx->async || ctx->timing) SE2P_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->timing) {
         for (int s = 0; s < ST_COUNT; ++s) {
@@ -249,7 +252,13 @@ int ensure_slots(se2p_ctx *ctx, int n)
     if (!ctx->fork_ev) SE2P_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     while ((int)ctx->slots.size() < n) {
         se2p_ctx::Slot sl;
-        SE2P_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
+        // A stream created with a CU mask gets a hardware queue of its own;
+        // ordinary streams share the runtime's few queues (four by default)
+        // round-robin, and two slots on one queue run one after the other.
+        // The mask names every CU.
+        std::vector<uint32_t> mask((size_t)(ctx->ncu + 31) / 32, 0u);
+        for (int c = 0; c < ctx->ncu; ++c) mask[(size_t)c / 32] |= 1u << (c % 32);
+        SE2P_HIP(hipExtStreamCreateWithCUMask(&sl.st, (uint32_t)mask.size(), mask.data()));
         SE2P_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
         const size_t fs_bytes = sizeof(int) * kFsMaxKeys;
         SE2P_HIP(hipMalloc((void **)&sl.fs, fs_bytes));
@@ -259,6 +268,64 @@ int ensure_slots(se2p_ctx *ctx, int n)
     }
     return 0;
 }
+
+// SE2P_ERR_ARG while a group (se2p_group_begin) is open, naming `what`.
+static int refuse_in_group(se2p_ctx *ctx, const char *what)
+{
+    if (ctx->group < 0) return 0;
+    return fail(ctx, SE2P_ERR_ARG, "%s is not allowed between se2p_group_begin and se2p_group_end",
+                what);
+}
+
+// A call made while a group is open: the group's first call runs on the
+// context stream itself (which is where group_ev was recorded, so it needs
+// no wait), call k >= 1 on slot k - 1, whose stream waits for group_ev and
+// whose stream, workspace, sort state and plan set are the context's until
+// the call returns; the slot's completion event is recorded on every exit
+// path (se2p_group_end waits for it).  Stage timing is off for grouped
+// calls.  Outside a group enter() does nothing.
+class GroupCall {
+public:
+    explicit GroupCall(se2p_ctx *c) : ctx(c) {}
+    int enter()
+    {
+        if (ctx->group < 0) return 0;
+        if (ctx->host_ptrs)
+            return fail(ctx, SE2P_ERR_ARG,
+                        "host-pointer calls are synchronous and cannot run in a group");
+        if (ctx->group >= SE2P_GROUP_MAX_CALLS)
+            return fail(ctx, SE2P_ERR_ARG, "a group holds at most %d calls", SE2P_GROUP_MAX_CALLS);
+        const int k = ctx->group;
+        if (k > 0) {
+            if (int rc = ensure_slots(ctx, k)) return rc;
+            SE2P_HIP(hipStreamWaitEvent(ctx->slots[(size_t)k - 1].st, ctx->group_ev, 0));
+        }
+        ctx->group = k + 1;
+        k_ = k;
+        if (ctx->timing)
+            for (float &ms : ctx->stage_ms) ms = -1.0f;
+        timing_ = ctx->timing;
+        if (k > 0) scope_.emplace(ctx, k - 1);   // it turns timing off itself
+        else ctx->timing = false;
+        return 0;
+    }
+    ~GroupCall()
+    {
+        if (k_ < 0) return;
+        if (k_ == 0) {
+            ctx->timing = timing_;
+            return;
+        }
+        scope_.reset();
+        hipEventRecord(ctx->slots[(size_t)k_ - 1].done, ctx->slots[(size_t)k_ - 1].st);
+    }
+
+private:
+    se2p_ctx *ctx;
+    int k_ = -1;
+    bool timing_ = false;
+    std::optional<SlotScope> scope_;
+};
 
 }  // namespace se2p
 
@@ -323,6 +390,8 @@ int se2p_destroy(se2p_ctx *ctx)
     }
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto &sl : ctx->slots)   // a group left open may still run there
+        if (sl.st) hipStreamSynchronize(sl.st);
     for (auto &kv : ctx->plans) {
         FftPlans &p = kv.second;
         if (p.fwd) rocfft_plan_destroy(p.fwd);
@@ -346,6 +415,7 @@ int se2p_destroy(se2p_ctx *ctx)
     }
     if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
     if (ctx->switch_ev) hipEventDestroy(ctx->switch_ev);
+    if (ctx->group_ev) hipEventDestroy(ctx->group_ev);
     for (auto &stage : ctx->ev)
         for (auto &visit : stage)
             for (hipEvent_t &e : visit)
@@ -367,6 +437,7 @@ int se2p_set_stream(se2p_ctx *ctx, void *s)
     if (ctx->device == SE2P_HOST_ONLY) return fail(ctx, SE2P_ERR_HIP, "host-only context");
     if (s && (hipStream_t)s == ctx->stream) return SE2P_OK;
     if (!s && ctx->own_stream) return SE2P_OK;
+    if (int rc = refuse_in_group(ctx, "binding another stream (se2p_set_stream)")) return rc;
     SE2P_HIP(hipSetDevice(ctx->device));
     // The context's workspace, sort state and kept buffers are shared by all
     // its calls: work enqueued on the new stream must not overlap what is
@@ -430,6 +501,8 @@ int se2p_set_host_pointers(se2p_ctx *ctx, int enable)
 int se2p_set_keep(se2p_ctx *ctx, int enable)
 {
     if (!ctx) return SE2P_ERR_ARG;
+    if (enable == 0)
+        if (int rc = refuse_in_group(ctx, "se2p_set_keep(ctx, 0)")) return rc;
     if (enable == 0 && (ctx->kept_real || ctx->kept_k))
         if (int rc = refuse_if_frozen(ctx, "freeing the kept forward state", 0)) return rc;
     ctx->keep = enable != 0;
@@ -447,6 +520,8 @@ int se2p_set_spread_chunk(se2p_ctx *ctx, int chunk)
     if (chunk != 0 && (chunk < SE2P_SPREAD_CHUNK_MIN || chunk > SE2P_SPREAD_CHUNK_MAX || chunk % 16 != 0))
         return fail(ctx, SE2P_ERR_ARG, "spread chunk %d: 0, or a multiple of 16 in [%d, %d]", chunk,
                     SE2P_SPREAD_CHUNK_MIN, SE2P_SPREAD_CHUNK_MAX);
+    if (chunk != ctx->spread_chunk)
+        if (int rc = refuse_in_group(ctx, "changing the spreading split")) return rc;
     if (chunk != ctx->spread_chunk && ctx->kept_k)
         if (int rc = refuse_if_frozen(ctx, "changing the split (it frees the kept state)", 0))
             return rc;
@@ -480,6 +555,32 @@ int se2p_stage_times(se2p_ctx *ctx, float *ms, int n)
     return SE2P_OK;
 }
 
+int se2p_group_begin(se2p_ctx *ctx)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    ctx->err.clear();
+    if (ctx->group >= 0) return fail(ctx, SE2P_ERR_ARG, "a group is already open");
+    if (int rc = set_device(ctx)) return rc;
+    if (!ctx->group_ev) SE2P_HIP(hipEventCreateWithFlags(&ctx->group_ev, hipEventDisableTiming));
+    SE2P_HIP(hipEventRecord(ctx->group_ev, ctx->stream));
+    ctx->group = 0;
+    return SE2P_OK;
+}
+
+int se2p_group_end(se2p_ctx *ctx)
+{
+    if (!ctx) return SE2P_ERR_ARG;
+    ctx->err.clear();
+    if (ctx->group < 0) return fail(ctx, SE2P_ERR_ARG, "no group is open");
+    const int used = ctx->group;
+    ctx->group = -1;
+    SE2P_HIP(hipSetDevice(ctx->device));
+    for (int k = 1; k < used; ++k)   // the first call ran on the context stream
+        SE2P_HIP(hipStreamWaitEvent(ctx->stream, ctx->slots[(size_t)k - 1].done, 0));
+    if (!ctx->async) SE2P_HIP(hipStreamSynchronize(ctx->stream));
+    return SE2P_OK;
+}
+
 int se2p_slp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
                   const double *f, double xi, int nside_x, int nside_y, double Lx, double Ly,
                   double *u)
@@ -490,6 +591,8 @@ int se2p_slp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *p
     if (ns > 0 && !f) return fail(ctx, SE2P_ERR_ARG, "null density pointer");
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -515,6 +618,8 @@ int se2p_dlp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *p
     if (ns > 0 && (!f || !n)) return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -538,6 +643,7 @@ int se2p_dlp_real_noalloc(se2p_ctx *ctx, const double *psrc, int64_t ns, const d
 {
     if (!ctx) return SE2P_ERR_ARG;
     begin_call(ctx);
+    if (int rc = refuse_in_group(ctx, "se2p_dlp_real_noalloc")) return rc;
     if (int rc = check_common(ctx, psrc, ns, ptar, nt, u)) return rc;
     if (ns > 0 && (!f || !n)) return fail(ctx, SE2P_ERR_ARG, "null density or normal pointer");
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
@@ -573,6 +679,8 @@ int se2p_slp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double 
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -600,6 +708,8 @@ int se2p_dlp_kspace(se2p_ctx *ctx, const double *psrc, int64_t ns, const double 
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -639,6 +749,8 @@ int se2p_slp_real_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const d
     if (int rc = check_batch(ctx, nrhs, f_stride, ns, u_stride, nt)) return rc;
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -666,6 +778,8 @@ int se2p_dlp_real_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const d
     if (int rc = check_batch(ctx, nrhs, f_stride, ns, u_stride, nt)) return rc;
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -696,6 +810,8 @@ int se2p_slp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -725,6 +841,8 @@ int se2p_dlp_kspace_batched(se2p_ctx *ctx, const double *psrc, int64_t ns, const
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -775,6 +893,7 @@ static int multibox(se2p_ctx *ctx, const se2p_box_desc *boxes, int nbox, int kin
 {
     if (!ctx) return SE2P_ERR_ARG;
     begin_call(ctx);
+    if (int rc = refuse_in_group(ctx, "a multi-box call")) return rc;
     if (nbox < 0 || (nbox > 0 && !boxes)) return fail(ctx, SE2P_ERR_ARG, "bad box list");
     for (int i = 0; i < nbox; ++i)
         if (int rc = check_box(ctx, boxes[i], i, kind == 1, kspace)) return rc;
@@ -840,6 +959,8 @@ int se2p_slp_real_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const 
     if (int rc = check_adjoint(ctx, psrc, ns, ptar, nt, f, nullptr, false, du)) return rc;
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -871,6 +992,8 @@ int se2p_dlp_real_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, const 
     if (int rc = check_adjoint(ctx, psrc, ns, ptar, nt, f, n, true, du)) return rc;
     if (int rc = check_real(ctx, xi, nside_x, nside_y, Lx, Ly)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -907,6 +1030,8 @@ int se2p_slp_kspace_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, cons
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -941,6 +1066,8 @@ int se2p_dlp_kspace_backward(se2p_ctx *ctx, const double *psrc, int64_t ns, cons
     KGeom g{psrc, ptar, ns, nt, xi, eta, Lx, Ly, w, Mx, My, P};
     if (int rc = check_kspace_params(ctx, g)) return rc;
     if (int rc = set_device(ctx)) return rc;
+    GroupCall gc(ctx);
+    if (int rc = gc.enter()) return rc;
     HostIo io(ctx);
     io.want(psrc, 16 * (size_t)ns);
     io.want(ptar, 16 * (size_t)nt);
@@ -1036,6 +1163,7 @@ static int ewald_full(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, c
                       int64_t nt, const double *f, const double *n, double Lx, double Ly,
                       se2p_params *prm, double *u, double *ur_out, double *uk_out)
 {
+    if (int rc = refuse_in_group(ctx, "a driver call")) return rc;
     if (int rc = check_driver(ctx, ns, nt, Lx, Ly, prm)) return rc;
     if (int rc = set_device(ctx)) return rc;
     HostIo io(ctx);
@@ -1325,6 +1453,7 @@ static int autotune(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, con
                     int64_t nt, const double *f, const double *n, double Lx, double Ly,
                     se2p_params *prm, se2p_tune_info *info)
 {
+    if (int rc = refuse_in_group(ctx, "autotuning")) return rc;
     if (int rc = check_driver(ctx, ns, nt, Lx, Ly, prm)) return rc;
     if (!(prm->tol > 0) || !(prm->tol < 1))
         return fail(ctx, SE2P_ERR_PARAM, "tol must be in (0, 1)");
@@ -1393,6 +1522,7 @@ static int smooth_even_at_least(int M)
 static int autotune_boxes(se2p_ctx *ctx, int kind, se2p_box_desc *boxes, int nbox, double tol,
                           se2p_tune_boxes_info *info)
 {
+    if (int rc = refuse_in_group(ctx, "autotuning")) return rc;
     if (!(tol > 0) || !(tol < 1)) return fail(ctx, SE2P_ERR_PARAM, "tol must be in (0, 1)");
     if (nbox < 0 || (nbox > 0 && !boxes)) return fail(ctx, SE2P_ERR_ARG, "bad box list");
     if (ctx->host_ptrs) return fail(ctx, SE2P_ERR_ARG, "multi-box tuning needs device arrays");
@@ -1621,6 +1751,7 @@ int se2p_count_pairs(se2p_ctx *ctx, const double *psrc, int64_t ns, const double
 {
     if (!ctx || !pairs_in || !pairs_out) return SE2P_ERR_ARG;
     begin_call(ctx);
+    if (int rc = refuse_in_group(ctx, "se2p_count_pairs")) return rc;
     double dummy_u = 0.0;
     if (int rc = check_common(ctx, psrc, ns, ptar, nt, nt > 0 ? &dummy_u : nullptr)) return rc;
     if (int rc = check_real(ctx, 1.0, nside_x, nside_y, Lx, Ly)) return rc;
@@ -1647,6 +1778,7 @@ int se2p_selftest_keysort(se2p_ctx *ctx, const int *keys, int64_t n, int nkeys, 
 {
     if (!ctx) return SE2P_ERR_ARG;
     begin_call(ctx);
+    if (int rc = refuse_in_group(ctx, "se2p_selftest_keysort")) return rc;
     if (n < 0 || n > 0x7fffffffLL || nkeys < 1 || (n > 0 && (!keys || !perm)) || !off)
         return fail(ctx, SE2P_ERR_ARG, "bad keysort arguments");
     if (int rc = set_device(ctx)) return rc;

@@ -83,6 +83,11 @@ struct se2p_ctx {
     std::vector<Slot> slots;
     hipEvent_t fork_ev = nullptr;
     hipEvent_t switch_ev = nullptr;   // orders a newly bound stream after the previous one
+    // se2p_group_begin .. se2p_group_end: -1 outside a group, else the number
+    // of calls made in the open group; call k runs on slot k, which waits for
+    // group_ev (recorded on `stream` at se2p_group_begin)
+    int group = -1;
+    hipEvent_t group_ev = nullptr;
     int plan_slot = 0;        // 0: the context's own plans
     bool keep = false;        // forward calls keep their sorted state (se2p_set_keep)
     bool split_adjoint = false;   // se2p_set_adjoint_passes(ctx, 2)

@@ -22,6 +22,7 @@ codes returned by the C ABI.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 from typing import Optional
 
@@ -117,6 +118,38 @@ class Context:
         other (se2p_set_adjoint_passes).  Results agree to rounding."""
         self._check(self._lib.se2p_set_adjoint_passes(self.handle, int(passes)))
 
+    def group_begin(self) -> None:
+        """Open a concurrent group (se2p_group_begin): each following call
+        runs on a stream slot of its own, side by side with the others,
+        until group_end.  The calls must be independent.  The context is
+        bound to torch's current stream first (a group cannot change it)."""
+        self.bind_torch_stream(torch.cuda.current_stream(self.device))
+        self._check(self._lib.se2p_group_begin(self.handle))
+        self._hold = []
+        _group_holds.append(self._hold)
+
+    def group_end(self) -> None:
+        """Join the group's calls into the context stream (se2p_group_end);
+        their outputs are defined from here on."""
+        hold = getattr(self, "_hold", None)
+        try:
+            self._check(self._lib.se2p_group_end(self.handle))
+        finally:
+            if hold is not None:
+                # by identity: == on lists would compare the tensors
+                _group_holds[:] = [h for h in _group_holds if h is not hold]
+                self._hold = None
+
+    @contextlib.contextmanager
+    def group(self):
+        """``with ctx.group(): ...`` -- group_begin, the calls, group_end
+        (also when a call raises, so that the context stays usable)."""
+        self.group_begin()
+        try:
+            yield self
+        finally:
+            self.group_end()
+
     def stage_times(self) -> dict:
         buf = (ctypes.c_float * len(_capi.STAGES))()
         self._check(self._lib.se2p_stage_times(self.handle, buf, len(_capi.STAGES)))
@@ -181,7 +214,17 @@ def _same_size(ps, a, name: str) -> None:
         raise ValueError(f"psrc and {name} must be the same size.")
 
 
+# Tensors handed to the library while a group is open: a grouped call reads
+# and writes them on a stream slot after the wrapper has returned, so they
+# (temporary copies of numpy inputs included) stay referenced until
+# group_end; otherwise torch's allocator could hand their memory to the next
+# call of the group while the first still uses it.
+_group_holds: list = []
+
+
 def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    for hold in _group_holds:
+        hold.append(t)
     return ctypes.c_void_p(t.data_ptr() if t.numel() > 0 else 0)
 
 

@@ -143,6 +143,29 @@ int se2p_set_frozen(se2p_ctx *ctx, int enable);
  * chunk planning and split-chunk sums around it). */
 int se2p_set_timing(se2p_ctx *ctx, int enable);
 int se2p_stage_times(se2p_ctx *ctx, float *ms, int n);
+/* Concurrent groups.  The reference's drivers call the real-space and the
+ * Fourier-space sum one after the other (StokesSLP_ewald_2p.m:119,126;
+ * StokesDLP_ewald_2p.m:140,147), although neither reads the other's result.
+ * Between se2p_group_begin and se2p_group_end, each call of a forward,
+ * batched or backward entry point (the twelve se2p_{slp,dlp}_{real,kspace}
+ * entries and their _batched and _backward forms) runs on a stream slot of
+ * its own, with its own workspace, sort state and FFT plans, and the calls
+ * of a group run side by side on the GPU.  Each waits for the work the
+ * context stream held at se2p_group_begin; se2p_group_end makes the context
+ * stream wait for all of them and, unless the context is in async mode,
+ * waits for them on the host.  Outputs are defined only after
+ * se2p_group_end, and are bitwise equal to the same calls made outside a
+ * group.  The calls of one group must be independent: no call may write an
+ * array another call of the group reads or writes.  At most
+ * SE2P_GROUP_MAX_CALLS calls per group.  Inside a group, host-pointer mode,
+ * the multi-box, driver, autotuning, pair-count and no-allocation entries
+ * and the settings that bind a stream or free kept state fail with
+ * SE2P_ERR_ARG, and stage times are not recorded.  A kept forward state
+ * (se2p_set_keep) made in a group serves a backward call after
+ * se2p_group_end.  No reference counterpart. */
+#define SE2P_GROUP_MAX_CALLS 4
+int se2p_group_begin(se2p_ctx *ctx);
+int se2p_group_end(se2p_ctx *ctx);
 
 /* ---- forward entry points (MEX replacements) ---------------------------- */
 int se2p_slp_real(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar,
