@@ -1201,10 +1201,26 @@ static int ewald_full(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, c
     launch_wrap(st, psrc, ns, Lx, Ly, b.src);
     if (!same) launch_wrap(st, ptar, nt, Lx, Ly, b.tar);
     RealGeom rg{b.src, b.tar, ns, nt, prm->xi, Lx, Ly, prm->nside_x, prm->nside_y};
-    if (int rc = real_sum(ctx, kind, rg, f, n, 1, 2 * ns, 2 * nt, ur, nullptr)) return rc;
     KGeom kg{b.src, b.tar, ns, nt, prm->xi, prm->eta, Lx, Ly, prm->w, prm->Mx, prm->My, prm->P};
     if (int rc = check_kspace_params(ctx, kg)) return rc;
-    if (int rc = kspace_sum(ctx, kind, kg, f, n, 1, 2 * ns, 2 * nt, uk)) return rc;
+    // The two sums read the wrapped points and write ur and uk only: as the
+    // two calls of a concurrent group, the Fourier-space sum runs on the
+    // context stream and the real-space sum beside it on slot 0, joined
+    // before the composition (also after an error, so the stream stays
+    // ordered).
+    if (int rc = ensure_slots(ctx, 1)) return rc;
+    SE2P_HIP(hipEventRecord(ctx->fork_ev, st));
+    SE2P_HIP(hipStreamWaitEvent(ctx->slots[0].st, ctx->fork_ev, 0));
+    ctx->beside = true;
+    int rc = kspace_sum(ctx, kind, kg, f, n, 1, 2 * ns, 2 * nt, uk);
+    if (rc == 0) {
+        SlotScope scope(ctx, 0);
+        rc = real_sum(ctx, kind, rg, f, n, 1, 2 * ns, 2 * nt, ur, nullptr);
+    }
+    ctx->beside = false;
+    SE2P_HIP(hipEventRecord(ctx->slots[0].done, ctx->slots[0].st));
+    SE2P_HIP(hipStreamWaitEvent(st, ctx->slots[0].done, 0));
+    if (rc) return rc;
     launch_compose(st, ur, uk, nt, kind, b.sums, Lx * Ly, u);
     return io.done(finish(ctx));
 }

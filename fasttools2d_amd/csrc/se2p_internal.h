@@ -70,7 +70,8 @@ struct se2p_ctx {
     int sort_mode = 0;        // SE2P_SORT_* bits
     int spread_chunk = 0;     // candidate sources per spreading workgroup (0: the default)
     // Stream slots for independent work run side by side (multi-box Fourier
-    // sums): each has its own stream, workspace, sort state and FFT plans
+    // sums, concurrent groups): each has its own stream (with a hardware
+    // queue of its own), workspace, sort state and FFT plans
     // (plan_slot in the plan key).  Slot work forks from and joins into
     // `stream` through events.
     struct Slot {
@@ -84,10 +85,12 @@ struct se2p_ctx {
     hipEvent_t fork_ev = nullptr;
     hipEvent_t switch_ev = nullptr;   // orders a newly bound stream after the previous one
     // se2p_group_begin .. se2p_group_end: -1 outside a group, else the number
-    // of calls made in the open group; call k runs on slot k, which waits for
-    // group_ev (recorded on `stream` at se2p_group_begin)
+    // of calls made in the open group; call 0 runs on `stream`, call k >= 1 on
+    // slot k - 1, which waits for group_ev (recorded on `stream` at
+    // se2p_group_begin)
     int group = -1;
     hipEvent_t group_ev = nullptr;
+    bool beside = false;      // a driver call runs its two sums side by side
     int plan_slot = 0;        // 0: the context's own plans
     bool keep = false;        // forward calls keep their sorted state (se2p_set_keep)
     bool split_adjoint = false;   // se2p_set_adjoint_passes(ctx, 2)
@@ -152,6 +155,11 @@ struct SlotScope {
         ctx->plan_slot = 0;
     }
 };
+
+// Whether other work of the caller runs beside this call on the GPU (an open
+// concurrent group, or the driver entries' two sums): persistent kernels then
+// leave part of each CU to it.
+inline bool shares_gpu(const se2p_ctx *ctx) { return ctx->group >= 0 || ctx->beside; }
 
 // Timing helpers (no-ops unless ctx->timing).
 void stage_begin(se2p_ctx *ctx, Stage s);

@@ -543,9 +543,10 @@ constexpr int kWin = 256;
 constexpr int kQueueStride = 32;   // ints between queue counters (own cache lines)
 constexpr int kWinSlots = 16384;   // window slots of split chunks (16 MiB of sums)
 constexpr int kWinBlocksPerCU = 8;
-// In a concurrent group (se2p_group_begin) the other calls of the group run
-// beside the pair kernel: its persistent workgroups leave a quarter of each
-// CU's wave slots to them instead of taking them all until the last item.
+// When other work runs beside the pair kernel (shares_gpu: a concurrent
+// group, se2p_group_begin, or a driver call's two sums), its persistent
+// workgroups leave a quarter of each CU's wave slots to that work instead of
+// taking them all until the last item.
 constexpr int kWinBlocksPerCUGroup = 6;
 
 struct WinPlan {
@@ -1510,7 +1511,7 @@ __global__ __launch_bounds__(256) void k_real_adj_slp(RealAdjSameArgs a)
 constexpr int kAdjNV = 13;          // stresslet adjoint sums per lane
 constexpr int kAdjSlots = 16384;    // window slots (kAdjNV x 64 doubles each, 104 MiB)
 constexpr int kAdjBlocksPerCU = 4;
-constexpr int kAdjBlocksPerCUGroup = 3;   // in a concurrent group (as kWinBlocksPerCUGroup)
+constexpr int kAdjBlocksPerCUGroup = 3;   // when shares_gpu (as kWinBlocksPerCUGroup)
 
 // The lane point's gradients from its 13 sums (stresslet): the lane's du
 // and densities multiplied back in.
@@ -2356,7 +2357,7 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
                 hipLaunchKernelGGL(k_real_win_scan, dim3(1), dim3(64 * kWinScanWaves), 0, st, p.coff,
                                    (int)p.ncell, wp);
                 const int64_t items = p.max_chunks + p.win_slots;
-                const int bpc = ctx->group >= 0 ? kWinBlocksPerCUGroup : kWinBlocksPerCU;
+                const int bpc = shares_gpu(ctx) ? kWinBlocksPerCUGroup : kWinBlocksPerCU;
                 const int64_t cap = (int64_t)ctx->ncu * bpc;
                 const unsigned wgrid = (unsigned)((std::min(items, cap) + 7) & ~int64_t(7));
                 const unsigned fgrid = std::min<unsigned>(cgrid, kWinFinalBlocks);
@@ -2554,7 +2555,7 @@ static int real_adjoint_same(se2p_ctx *ctx, int kind, const RealGeom &g, const d
         hipLaunchKernelGGL(k_real_win_scan, dim3(1), dim3(64 * kWinScanWaves), 0, st, p.coff,
                            (int)p.ncell, aw);
         const int64_t items = p.max_chunks + kAdjSlots;
-        const int64_t cap = (int64_t)ctx->ncu * (ctx->group >= 0 ? kAdjBlocksPerCUGroup : kAdjBlocksPerCU);
+        const int64_t cap = (int64_t)ctx->ncu * (shares_gpu(ctx) ? kAdjBlocksPerCUGroup : kAdjBlocksPerCU);
         const unsigned wgrid = (unsigned)((std::min(items, cap) + 7) & ~int64_t(7));
         stage_begin(ctx, ST_PAIR);
         hipLaunchKernelGGL(k_real_adj_win, dim3(wgrid), dim3(256), 0, st, a, aw);

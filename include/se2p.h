@@ -329,7 +329,11 @@ int se2p_select_params(int kind, int64_t ns, int64_t nt, double Q, double Lx, do
  * (PeriodicEwald/src/matlab/StokesSLP_ewald_2p.m:1-138): parameters from
  * *prm (filled on return), points wrapped into [-L/2, L/2), real + Fourier
  * sums, u = ur + uk.  psrc/ptar are the caller's (unwrapped) points; ur and
- * uk may be NULL.  All arrays are device arrays. */
+ * uk may be NULL.  All arrays are device arrays.  The two sums run side by
+ * side on the GPU, as the two calls of a concurrent group would (the
+ * Fourier-space sum on the context stream, the real-space sum on a stream
+ * slot), and give bitwise the results of the same sums run one after the
+ * other. */
 int se2p_slp_ewald(se2p_ctx *ctx, const double *psrc, int64_t ns, const double *ptar, int64_t nt,
                    const double *f, double Lx, double Ly, se2p_params *prm, double *u, double *ur,
                    double *uk);
