@@ -245,26 +245,31 @@ private:
     se2p_ctx *ctx = ctx_;   // for SE2P_HIP
 };
 
-int ensure_slots(se2p_ctx *ctx, int n)
+int ensure_slots(se2p_ctx *ctx, int n, bool own_queue)
 {
-    if ((int)ctx->slots.size() < n)
+    std::vector<se2p_ctx::Slot> &pool = own_queue ? ctx->gslots : ctx->slots;
+    if ((int)pool.size() < n)
         if (int rc = refuse_if_frozen(ctx, "creating stream slots", sizeof(int) * kFsMaxKeys)) return rc;
     if (!ctx->fork_ev) SE2P_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
-    while ((int)ctx->slots.size() < n) {
+    while ((int)pool.size() < n) {
         se2p_ctx::Slot sl;
-        // A stream created with a CU mask gets a hardware queue of its own;
-        // ordinary streams share the runtime's few queues (four by default)
-        // round-robin, and two slots on one queue run one after the other.
-        // The mask names every CU.
-        std::vector<uint32_t> mask((size_t)(ctx->ncu + 31) / 32, 0u);
-        for (int c = 0; c < ctx->ncu; ++c) mask[(size_t)c / 32] |= 1u << (c % 32);
-        SE2P_HIP(hipExtStreamCreateWithCUMask(&sl.st, (uint32_t)mask.size(), mask.data()));
+        if (own_queue) {
+            // A stream created with a CU mask gets a hardware queue of its
+            // own; ordinary streams share the runtime's few queues (four by
+            // default) round-robin, and two streams on one queue run one
+            // after the other.  The mask names every CU.
+            std::vector<uint32_t> mask((size_t)(ctx->ncu + 31) / 32, 0u);
+            for (int c = 0; c < ctx->ncu; ++c) mask[(size_t)c / 32] |= 1u << (c % 32);
+            SE2P_HIP(hipExtStreamCreateWithCUMask(&sl.st, (uint32_t)mask.size(), mask.data()));
+        } else {
+            SE2P_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
+        }
         SE2P_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
         const size_t fs_bytes = sizeof(int) * kFsMaxKeys;
         SE2P_HIP(hipMalloc((void **)&sl.fs, fs_bytes));
         SE2P_HIP(hipMemsetAsync(sl.fs, 0, fs_bytes, sl.st));
         SE2P_HIP(hipStreamSynchronize(sl.st));
-        ctx->slots.push_back(sl);
+        pool.push_back(sl);
     }
     return 0;
 }
@@ -297,15 +302,15 @@ public:
             return fail(ctx, SE2P_ERR_ARG, "a group holds at most %d calls", SE2P_GROUP_MAX_CALLS);
         const int k = ctx->group;
         if (k > 0) {
-            if (int rc = ensure_slots(ctx, k)) return rc;
-            SE2P_HIP(hipStreamWaitEvent(ctx->slots[(size_t)k - 1].st, ctx->group_ev, 0));
+            if (int rc = ensure_slots(ctx, k, true)) return rc;
+            SE2P_HIP(hipStreamWaitEvent(ctx->gslots[(size_t)k - 1].st, ctx->group_ev, 0));
         }
         ctx->group = k + 1;
         k_ = k;
         if (ctx->timing)
             for (float &ms : ctx->stage_ms) ms = -1.0f;
         timing_ = ctx->timing;
-        if (k > 0) scope_.emplace(ctx, k - 1);   // it turns timing off itself
+        if (k > 0) scope_.emplace(ctx, k - 1, true);   // it turns timing off itself
         else ctx->timing = false;
         return 0;
     }
@@ -317,7 +322,7 @@ public:
             return;
         }
         scope_.reset();
-        hipEventRecord(ctx->slots[(size_t)k_ - 1].done, ctx->slots[(size_t)k_ - 1].st);
+        hipEventRecord(ctx->gslots[(size_t)k_ - 1].done, ctx->gslots[(size_t)k_ - 1].st);
     }
 
 private:
@@ -390,8 +395,9 @@ int se2p_destroy(se2p_ctx *ctx)
     }
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    for (auto &sl : ctx->slots)   // a group left open may still run there
-        if (sl.st) hipStreamSynchronize(sl.st);
+    for (auto *pool : {&ctx->slots, &ctx->gslots})   // a group left open may still run there
+        for (auto &sl : *pool)
+            if (sl.st) hipStreamSynchronize(sl.st);
     for (auto &kv : ctx->plans) {
         FftPlans &p = kv.second;
         if (p.fwd) rocfft_plan_destroy(p.fwd);
@@ -406,13 +412,13 @@ int se2p_destroy(se2p_ctx *ctx)
     if (ctx->drv) hipFree(ctx->drv);
     if (ctx->io) hipFree(ctx->io);
     if (ctx->fs) hipFree(ctx->fs);
-    for (auto &sl : ctx->slots) {
-        if (sl.st) hipStreamSynchronize(sl.st);
-        if (sl.ws) hipFree(sl.ws);
-        if (sl.fs) hipFree(sl.fs);
-        if (sl.done) hipEventDestroy(sl.done);
-        if (sl.st) hipStreamDestroy(sl.st);
-    }
+    for (auto *pool : {&ctx->slots, &ctx->gslots})
+        for (auto &sl : *pool) {
+            if (sl.ws) hipFree(sl.ws);
+            if (sl.fs) hipFree(sl.fs);
+            if (sl.done) hipEventDestroy(sl.done);
+            if (sl.st) hipStreamDestroy(sl.st);
+        }
     if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
     if (ctx->switch_ev) hipEventDestroy(ctx->switch_ev);
     if (ctx->group_ev) hipEventDestroy(ctx->group_ev);
@@ -576,7 +582,7 @@ int se2p_group_end(se2p_ctx *ctx)
     ctx->group = -1;
     SE2P_HIP(hipSetDevice(ctx->device));
     for (int k = 1; k < used; ++k)   // the first call ran on the context stream
-        SE2P_HIP(hipStreamWaitEvent(ctx->stream, ctx->slots[(size_t)k - 1].done, 0));
+        SE2P_HIP(hipStreamWaitEvent(ctx->stream, ctx->gslots[(size_t)k - 1].done, 0));
     if (!ctx->async) SE2P_HIP(hipStreamSynchronize(ctx->stream));
     return SE2P_OK;
 }
@@ -1208,18 +1214,19 @@ static int ewald_full(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, c
     // context stream and the real-space sum beside it on slot 0, joined
     // before the composition (also after an error, so the stream stays
     // ordered).
-    if (int rc = ensure_slots(ctx, 1)) return rc;
+    if (int rc = ensure_slots(ctx, 1, true)) return rc;
+    se2p_ctx::Slot &sl = ctx->gslots[0];
     SE2P_HIP(hipEventRecord(ctx->fork_ev, st));
-    SE2P_HIP(hipStreamWaitEvent(ctx->slots[0].st, ctx->fork_ev, 0));
+    SE2P_HIP(hipStreamWaitEvent(sl.st, ctx->fork_ev, 0));
     ctx->beside = true;
     int rc = kspace_sum(ctx, kind, kg, f, n, 1, 2 * ns, 2 * nt, uk);
     if (rc == 0) {
-        SlotScope scope(ctx, 0);
+        SlotScope scope(ctx, 0, true);
         rc = real_sum(ctx, kind, rg, f, n, 1, 2 * ns, 2 * nt, ur, nullptr);
     }
     ctx->beside = false;
-    SE2P_HIP(hipEventRecord(ctx->slots[0].done, ctx->slots[0].st));
-    SE2P_HIP(hipStreamWaitEvent(st, ctx->slots[0].done, 0));
+    SE2P_HIP(hipEventRecord(sl.done, sl.st));
+    SE2P_HIP(hipStreamWaitEvent(st, sl.done, 0));
     if (rc) return rc;
     launch_compose(st, ur, uk, nt, kind, b.sums, Lx * Ly, u);
     return io.done(finish(ctx));

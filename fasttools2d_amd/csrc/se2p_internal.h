@@ -70,8 +70,8 @@ struct se2p_ctx {
     int sort_mode = 0;        // SE2P_SORT_* bits
     int spread_chunk = 0;     // candidate sources per spreading workgroup (0: the default)
     // Stream slots for independent work run side by side (multi-box Fourier
-    // sums, concurrent groups): each has its own stream (with a hardware
-    // queue of its own), workspace, sort state and FFT plans
+    // sums, concurrent groups): each has its own stream, workspace, sort
+    // state and FFT plans
     // (plan_slot in the plan key).  Slot work forks from and joins into
     // `stream` through events.
     struct Slot {
@@ -81,12 +81,14 @@ struct se2p_ctx {
         int *fs = nullptr;
         hipEvent_t done = nullptr;
     };
-    std::vector<Slot> slots;
+    std::vector<Slot> slots;    // multi-box Fourier sums: ordinary streams
+    std::vector<Slot> gslots;   // concurrent groups and driver calls: streams
+                                // with hardware queues of their own
     hipEvent_t fork_ev = nullptr;
     hipEvent_t switch_ev = nullptr;   // orders a newly bound stream after the previous one
     // se2p_group_begin .. se2p_group_end: -1 outside a group, else the number
     // of calls made in the open group; call 0 runs on `stream`, call k >= 1 on
-    // slot k - 1, which waits for group_ev (recorded on `stream` at
+    // gslots[k - 1], which waits for group_ev (recorded on `stream` at
     // se2p_group_begin)
     int group = -1;
     hipEvent_t group_ev = nullptr;
@@ -118,9 +120,18 @@ int refuse_if_frozen(se2p_ctx *ctx, const char *what, size_t bytes);
 int ensure_workspace(se2p_ctx *ctx, size_t bytes);
 
 // Stream slots: at least n of them, each with its stream, event and sort
-// state.  Multi-box Fourier-space sums use kKspaceSlots.
-int ensure_slots(se2p_ctx *ctx, int n);
+// state.  Multi-box Fourier-space sums use kKspaceSlots ordinary streams
+// (their many short launches, forked and joined often, run best on the
+// runtime's shared queues); concurrent groups and the driver entries use
+// `own_queue` slots (ctx->gslots), whose streams get hardware queues of
+// their own so that their work really runs beside the context stream's.
+int ensure_slots(se2p_ctx *ctx, int n, bool own_queue = false);
 constexpr int kKspaceSlots = 4;
+
+inline se2p_ctx::Slot &slot_at(se2p_ctx *ctx, int k, bool own_queue = false)
+{
+    return own_queue ? ctx->gslots[(size_t)k] : ctx->slots[(size_t)k];
+}
 
 // The context's stream, workspace, sort state and plan set swapped for slot
 // k's while in scope (restored on every exit path).
@@ -132,8 +143,8 @@ struct SlotScope {
     size_t ws_bytes;
     int *fs;
     bool timing;
-    SlotScope(se2p_ctx *c, int k)
-        : ctx(c), slot(c->slots[(size_t)k]), st(c->stream), ws(c->ws), ws_bytes(c->ws_bytes),
+    SlotScope(se2p_ctx *c, int k, bool own_queue = false)
+        : ctx(c), slot(slot_at(c, k, own_queue)), st(c->stream), ws(c->ws), ws_bytes(c->ws_bytes),
           fs(c->fs), timing(c->timing)
     {
         ctx->stream = slot.st;
@@ -141,7 +152,7 @@ struct SlotScope {
         ctx->ws_bytes = slot.ws_bytes;
         ctx->fs = slot.fs;
         ctx->timing = false;
-        ctx->plan_slot = k + 1;
+        ctx->plan_slot = own_queue ? kKspaceSlots + 1 + k : k + 1;
     }
     ~SlotScope()
     {
