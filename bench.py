@@ -551,7 +551,7 @@ def _autotune_variant(args, pe, np, torch, w, ps, f, nv, stream, fwd_ms):
     support/grid (P 16, 20 or the drivers' 24, on oversampled grids, checked
     against P = 24) by timing, on a context of its own (the split is a
     context setting).  One step is the same two MEX-equivalent calls at the
-    tuned parameters; the output is checked against the FP64 direct Ewald sum on
+    tuned parameters, formed as the headline step; the output is checked against the FP64 direct Ewald sum on
     the 256 truth targets (the real + Fourier total does not depend on the
     split, so the truth at the headline's xi serves any xi)."""
     N = ps.shape[0]
@@ -562,11 +562,23 @@ def _autotune_variant(args, pe, np, torch, w, ps, f, nv, stream, fwd_ms):
     ur = torch.empty((N, 2), dtype=torch.float64, device=ps.device)
     uk = torch.empty_like(ur)
 
-    def step():
+    def real():
         pe.mex_stokes_dlp_real(ps, ps, f, nv, prm.xi, prm.nside_x, prm.nside_y, w.Lx, w.Ly,
                                ctx=ctx, out=ur)
+
+    def fourier():
         pe.mex_stokes_dlp_kspace(ps, ps, prm.xi, prm.eta, f, nv, prm.Mx, prm.My, w.Lx, w.Ly,
                                  prm.w, prm.P, ctx=ctx, out=uk)
+
+    def step():
+        # the headline's step form; the tuner times the two sums side by side
+        if args.sequential:
+            real()
+            fourier()
+        else:
+            with ctx.group():
+                fourier()
+                real()
 
     with torch.cuda.stream(stream):
         ctx.bind_torch_stream(stream)

@@ -332,6 +332,31 @@ private:
     std::optional<SlotScope> scope_;
 };
 
+// The Fourier-space sum `fourier` on the context stream and the real-space
+// sum `real` beside it on own-queue slot 0, as the two calls of a concurrent
+// group (Fourier-space call first) would run; joined into the context stream
+// also after an error, so that the stream stays ordered.
+template <class F, class R>
+static int sums_side_by_side(se2p_ctx *ctx, F &&fourier, R &&real)
+{
+    hipStream_t st = ctx->stream;
+    if (int rc = ensure_slots(ctx, 1, true)) return rc;
+    se2p_ctx::Slot &sl = ctx->gslots[0];
+    SE2P_HIP(hipEventRecord(ctx->fork_ev, st));
+    SE2P_HIP(hipStreamWaitEvent(sl.st, ctx->fork_ev, 0));
+    const bool beside = ctx->beside;
+    ctx->beside = true;
+    int rc = fourier();
+    if (rc == 0) {
+        SlotScope scope(ctx, 0, true);
+        rc = real();
+    }
+    ctx->beside = beside;
+    SE2P_HIP(hipEventRecord(sl.done, sl.st));
+    SE2P_HIP(hipStreamWaitEvent(st, sl.done, 0));
+    return rc;
+}
+
 }  // namespace se2p
 
 using namespace se2p;
@@ -1209,25 +1234,11 @@ static int ewald_full(se2p_ctx *ctx, int kind, const double *psrc, int64_t ns, c
     RealGeom rg{b.src, b.tar, ns, nt, prm->xi, Lx, Ly, prm->nside_x, prm->nside_y};
     KGeom kg{b.src, b.tar, ns, nt, prm->xi, prm->eta, Lx, Ly, prm->w, prm->Mx, prm->My, prm->P};
     if (int rc = check_kspace_params(ctx, kg)) return rc;
-    // The two sums read the wrapped points and write ur and uk only: as the
-    // two calls of a concurrent group, the Fourier-space sum runs on the
-    // context stream and the real-space sum beside it on slot 0, joined
-    // before the composition (also after an error, so the stream stays
-    // ordered).
-    if (int rc = ensure_slots(ctx, 1, true)) return rc;
-    se2p_ctx::Slot &sl = ctx->gslots[0];
-    SE2P_HIP(hipEventRecord(ctx->fork_ev, st));
-    SE2P_HIP(hipStreamWaitEvent(sl.st, ctx->fork_ev, 0));
-    ctx->beside = true;
-    int rc = kspace_sum(ctx, kind, kg, f, n, 1, 2 * ns, 2 * nt, uk);
-    if (rc == 0) {
-        SlotScope scope(ctx, 0, true);
-        rc = real_sum(ctx, kind, rg, f, n, 1, 2 * ns, 2 * nt, ur, nullptr);
-    }
-    ctx->beside = false;
-    SE2P_HIP(hipEventRecord(sl.done, sl.st));
-    SE2P_HIP(hipStreamWaitEvent(st, sl.done, 0));
-    if (rc) return rc;
+    // the two sums read the wrapped points and write uk and ur only
+    if (int rc = sums_side_by_side(
+            ctx, [&] { return kspace_sum(ctx, kind, kg, f, n, 1, 2 * ns, 2 * nt, uk); },
+            [&] { return real_sum(ctx, kind, rg, f, n, 1, 2 * ns, 2 * nt, ur, nullptr); }))
+        return rc;
     launch_compose(st, ur, uk, nt, kind, b.sums, Lx * Ly, u);
     return io.done(finish(ctx));
 }
@@ -1308,9 +1319,11 @@ static int tune_time(se2p_ctx *ctx, const TuneGeom &tg, const se2p_params &c, Ev
     const RealGeom rg{tg.src, tg.tar, tg.ns, tg.nt, c.xi, tg.Lx, tg.Ly, c.nside_x, c.nside_y};
     const KGeom kg{tg.src, tg.tar, tg.ns, tg.nt, c.xi, c.eta, tg.Lx, tg.Ly, c.w, c.Mx, c.My, c.P};
     if (int rc = check_kspace_params(ctx, kg)) return rc;
+    // the step the driver entries run: both sums side by side
     auto step = [&]() -> int {
-        if (int rc = real_sum(ctx, tg.kind, rg, tg.f, tg.n, 1, 2 * tg.ns, 2 * tg.nt, tg.ur, nullptr)) return rc;
-        return kspace_sum(ctx, tg.kind, kg, tg.f, tg.n, 1, 2 * tg.ns, 2 * tg.nt, tg.uk);
+        return sums_side_by_side(
+            ctx, [&] { return kspace_sum(ctx, tg.kind, kg, tg.f, tg.n, 1, 2 * tg.ns, 2 * tg.nt, tg.uk); },
+            [&] { return real_sum(ctx, tg.kind, rg, tg.f, tg.n, 1, 2 * tg.ns, 2 * tg.nt, tg.ur, nullptr); });
     };
     if (int rc = step()) return rc;
     SE2P_HIP(hipStreamSynchronize(st));

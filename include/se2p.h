@@ -372,8 +372,9 @@ typedef struct se2p_tune_info {
 
 /* Chooses the drivers' free parameters for one geometry at prm->tol
  * (required, 0 < tol < 1) by timing the real- plus Fourier-space sums on
- * this context (each candidate: a warm-up step, then the best of three
- * blocks of three back-to-back steps).  Three stages:
+ * this context, side by side as the driver entries run them (each
+ * candidate: a warm-up step, then the best of three blocks of three
+ * back-to-back steps).  Three stages:
  *  1. the points per cell Nb, which sets the cell grid (nside), the
  *     splitting xi and the Fourier grid through the driver formulas
  *     (StokesSLP_ewald_2p.m:33-104, StokesDLP_ewald_2p.m:33-125):
