@@ -876,34 +876,34 @@ __device__ inline double wavenumber(int j, int M, double L)
     return (j <= M / 2) ? 2.0 * SE2P_PI / L * j : 2.0 * SE2P_PI / L * (j - M);
 }
 
-// SLP: input grids [2r], [2r+1] -> output in place.
-__global__ void k_filter_slp(FilterArgs a)
+// Stokeslet filter coefficients at half-spectrum index (j, k), the inverse
+// transform's 1/(Mx My) included (mex_stokes_slp_kspace.cpp:200): e times
+// the matrix (a11, a12; a12, a22); a12 is dropped where exactly one index is
+// on its Nyquist line (the filter's Hermitian part).  Zero at (0, 0).
+struct SlpCoef {
+    double e, a11, a22, a12;
+};
+__device__ inline SlpCoef slp_coef(const FilterArgs &a, int j, int k)
 {
-    const int Mh = a.My / 2 + 1;
-    const int64_t n = (int64_t)a.Mx * Mh;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int j = (int)(p / Mh), k = (int)(p % Mh);
+    SlpCoef c{0.0, 0.0, 0.0, 0.0};
+    if (j == 0 && k == 0) return c;
     const double k1 = wavenumber(j, a.Mx, a.Lx);
     const double k2 = 2.0 * SE2P_PI / a.Ly * k;
     const double K2 = k1 * k1 + k2 * k2;
-    double e = 0.0, a11 = 0.0, a22 = 0.0, a12 = 0.0;
-    if (p != 0) {
-        // mex_stokes_slp_kspace.cpp:200
-        e = (1.0 / (K2 * K2) + 0.25 / (K2 * a.xi * a.xi)) *
-            exp(-0.25 * (1 - a.eta) / (a.xi * a.xi) * K2) * a.scale;
-        const bool nyx = (2 * j == a.Mx), nyy = (2 * k == a.My);
-        a11 = k2 * k2;
-        a22 = k1 * k1;
-        a12 = (nyx != nyy) ? 0.0 : k1 * k2;   // Hermitian part on Nyquist lines
-    }
-    for (int r = 0; r < a.nrhs; ++r) {
-        double2 *H1 = a.spec + (int64_t)(2 * r) * a.sstride;
-        double2 *H2 = H1 + a.sstride;
-        const double2 h1 = H1[p], h2 = H2[p];
-        H1[p] = make_double2((a11 * h1.x - a12 * h2.x) * e, (a11 * h1.y - a12 * h2.y) * e);
-        H2[p] = make_double2((a22 * h2.x - a12 * h1.x) * e, (a22 * h2.y - a12 * h1.y) * e);
-    }
+    c.e = (1.0 / (K2 * K2) + 0.25 / (K2 * a.xi * a.xi)) *
+          exp(-0.25 * (1 - a.eta) / (a.xi * a.xi) * K2) * a.scale;
+    const bool nyx = (2 * j == a.Mx), nyy = (2 * k == a.My);
+    c.a11 = k2 * k2;
+    c.a22 = k1 * k1;
+    c.a12 = (nyx != nyy) ? 0.0 : k1 * k2;
+    return c;
+}
+
+__device__ __forceinline__ void slp_apply(const SlpCoef &c, double2 h1, double2 h2, double2 &o1,
+                                          double2 &o2)
+{
+    o1 = make_double2((c.a11 * h1.x - c.a12 * h2.x) * c.e, (c.a11 * h1.y - c.a12 * h2.y) * c.e);
+    o2 = make_double2((c.a22 * h2.x - c.a12 * h1.x) * c.e, (c.a22 * h2.y - c.a12 * h1.y) * c.e);
 }
 
 // Stresslet filter coefficients at half-spectrum index (j, k), the inverse
@@ -934,109 +934,134 @@ __device__ inline DlpCoef dlp_coef(const FilterArgs &a, int j, int k)
     return c;
 }
 
-// DLP: input grids [3r], [3r+1], [3r+2] (A = n1f1, B = n2f1+n1f2, C = n2f2)
-// -> output grids [2r], [2r+1], multiplied by -i so that the real inverse
-// transform yields the imaginary part of the reference's inverse.
-__global__ void k_filter_dlp(FilterArgs a)
+// The stresslet filter on one spectrum value of A = n1f1, B = n2f1+n1f2,
+// C = n2f2: two outputs multiplied by -i, so that the real inverse transform
+// yields the imaginary part of the reference's inverse.
+__device__ __forceinline__ void dlp_apply(const DlpCoef &c, double2 A, double2 B, double2 C,
+                                          double2 &o1, double2 &o2)
 {
-    const int Mh = a.My / 2 + 1;
-    const int64_t n = (int64_t)a.Mx * Mh;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const DlpCoef c = dlp_coef(a, (int)(p / Mh), (int)(p % Mh));
-    for (int r = 0; r < a.nrhs; ++r) {
-        const double2 A = a.spec[(int64_t)(3 * r) * a.sstride + p];
-        const double2 B = a.spec[(int64_t)(3 * r + 1) * a.sstride + p];
-        const double2 C = a.spec[(int64_t)(3 * r + 2) * a.sstride + p];
-        const double o1r = c.et * (c.b111 * A.x + c.b112 * B.x + c.b122 * C.x);
-        const double o1i = c.et * (c.b111 * A.y + c.b112 * B.y + c.b122 * C.y);
-        const double o2r = c.et * (c.b112 * A.x + c.b122 * B.x + c.b222 * C.x);
-        const double o2i = c.et * (c.b112 * A.y + c.b122 * B.y + c.b222 * C.y);
-        // all threads read their own element before writing it; grid 2r and
-        // 2r+1 never alias an unread input of another thread
-        a.spec[(int64_t)(2 * r) * a.sstride + p] = make_double2(o1i, -o1r);
-        a.spec[(int64_t)(2 * r + 1) * a.sstride + p] = make_double2(o2i, -o2r);
-    }
+    const double o1r = c.et * (c.b111 * A.x + c.b112 * B.x + c.b122 * C.x);
+    const double o1i = c.et * (c.b111 * A.y + c.b112 * B.y + c.b122 * C.y);
+    const double o2r = c.et * (c.b112 * A.x + c.b122 * B.x + c.b222 * C.x);
+    const double o2i = c.et * (c.b112 * A.y + c.b122 * B.y + c.b222 * C.y);
+    o1 = make_double2(o1i, -o1r);
+    o2 = make_double2(o2i, -o2r);
 }
 
-// Transpose of k_filter_dlp (one right-hand side): the filter is a real
-// convolution with an odd kernel, so its transpose multiplies by the complex
-// conjugate of -i et b, i.e. +i et b, with the 2x3 coefficient matrix
-// transposed.  Input grids [0], [1] (spectra of the spread du), output grids
-// [0], [1], [2] (the three stresslet densities).
-__global__ void k_filter_dlp_adj(FilterArgs a)
+// Its transpose: the filter is a real convolution with an odd kernel, so the
+// transpose multiplies by the complex conjugate of -i et b, i.e. +i et b,
+// with the 2x3 coefficient matrix transposed (two inputs, three outputs).
+__device__ __forceinline__ void dlp_adj_apply(const DlpCoef &c, double2 Z1, double2 Z2,
+                                              double2 &o1, double2 &o2, double2 &o3)
 {
-    const int Mh = a.My / 2 + 1;
-    const int64_t n = (int64_t)a.Mx * Mh;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const DlpCoef c = dlp_coef(a, (int)(p / Mh), (int)(p % Mh));
-    const double2 Z1 = a.spec[p], Z2 = a.spec[a.sstride + p];
     const double o1r = c.et * (c.b111 * Z1.x + c.b112 * Z2.x);
     const double o1i = c.et * (c.b111 * Z1.y + c.b112 * Z2.y);
     const double o2r = c.et * (c.b112 * Z1.x + c.b122 * Z2.x);
     const double o2i = c.et * (c.b112 * Z1.y + c.b122 * Z2.y);
     const double o3r = c.et * (c.b122 * Z1.x + c.b222 * Z2.x);
     const double o3i = c.et * (c.b122 * Z1.y + c.b222 * Z2.y);
-    a.spec[p] = make_double2(-o1i, o1r);
-    a.spec[a.sstride + p] = make_double2(-o2i, o2r);
-    a.spec[2 * a.sstride + p] = make_double2(-o3i, o3r);
+    o1 = make_double2(-o1i, o1r);
+    o2 = make_double2(-o2i, o2r);
+    o3 = make_double2(-o3i, o3r);
 }
 
-// Forward and transposed filters in one pass over the spectra of one
-// spreading of 2 + 2 (Stokeslet) or 3 + 2 (stresslet) grids at the same
-// points: the densities' spectra [0, ncomp) filtered as k_filter_slp /
-// k_filter_dlp into [0, 2), the spread du's spectra [ncomp, ncomp + 2) by the
-// transposed filter (k_filter_slp again, the Stokeslet filter being its own
-// transpose; k_filter_dlp_adj for the stresslet) into [2, 2 + ncomp).  Each
-// thread reads its element of every input before it writes any output.
-template <int KIND>
-__global__ void k_filter_both(FilterArgs a)
+// The filters as maps from input spectra to output spectra at one index
+// (j, k) of the half spectrum, per right-hand side r: input grid in(r, c),
+// c < nin, output grid out(r, o), o < nout, all in the same spectrum array.
+//   FILTER_SLP      [2r], [2r+1]         -> [2r], [2r+1]
+//   FILTER_DLP      [3r], [3r+1], [3r+2] -> [2r], [2r+1]  (A = n1f1,
+//                   B = n2f1 + n1f2, C = n2f2)
+//   FILTER_DLP_ADJ  [0], [1] (spectra of the spread du) -> [0], [1], [2]
+//                   (the three stresslet densities), one right-hand side
+//   FILTER_BOTH_*   forward and transposed filters of one spreading of
+//                   2 + 2 (Stokeslet) or 3 + 2 (stresslet) grids at the same
+//                   points: the densities' spectra [0, ncomp) filtered into
+//                   [0, 2), the spread du's [ncomp, ncomp + 2) by the
+//                   transposed filter (the Stokeslet filter is its own
+//                   transpose) into [2, 2 + ncomp); one right-hand side.
+// Every output of r is a grid no later right-hand side reads (2r + 1 <
+// 3r' for r' > r), so right-hand sides may run one after the other in place.
+enum FilterKind { FILTER_SLP, FILTER_DLP, FILTER_DLP_ADJ, FILTER_BOTH_SLP, FILTER_BOTH_DLP };
+
+template <int FK>
+struct FilterOp;
+template <>
+struct FilterOp<FILTER_SLP> {
+    static constexpr int nin = 2, nout = 2;
+    __device__ static int in(int r, int c) { return 2 * r + c; }
+    __device__ static int out(int r, int o) { return 2 * r + o; }
+    __device__ static SlpCoef coef(const FilterArgs &a, int j, int k) { return slp_coef(a, j, k); }
+    __device__ static void apply(const SlpCoef &c, const double2 *x, double2 *y)
+    {
+        slp_apply(c, x[0], x[1], y[0], y[1]);
+    }
+};
+template <>
+struct FilterOp<FILTER_DLP> {
+    static constexpr int nin = 3, nout = 2;
+    __device__ static int in(int r, int c) { return 3 * r + c; }
+    __device__ static int out(int r, int o) { return 2 * r + o; }
+    __device__ static DlpCoef coef(const FilterArgs &a, int j, int k) { return dlp_coef(a, j, k); }
+    __device__ static void apply(const DlpCoef &c, const double2 *x, double2 *y)
+    {
+        dlp_apply(c, x[0], x[1], x[2], y[0], y[1]);
+    }
+};
+template <>
+struct FilterOp<FILTER_DLP_ADJ> {
+    static constexpr int nin = 2, nout = 3;
+    __device__ static int in(int, int c) { return c; }
+    __device__ static int out(int, int o) { return o; }
+    __device__ static DlpCoef coef(const FilterArgs &a, int j, int k) { return dlp_coef(a, j, k); }
+    __device__ static void apply(const DlpCoef &c, const double2 *x, double2 *y)
+    {
+        dlp_adj_apply(c, x[0], x[1], y[0], y[1], y[2]);
+    }
+};
+template <>
+struct FilterOp<FILTER_BOTH_SLP> {
+    static constexpr int nin = 4, nout = 4;
+    __device__ static int in(int, int c) { return c; }
+    __device__ static int out(int, int o) { return o; }
+    __device__ static SlpCoef coef(const FilterArgs &a, int j, int k) { return slp_coef(a, j, k); }
+    __device__ static void apply(const SlpCoef &c, const double2 *x, double2 *y)
+    {
+        slp_apply(c, x[0], x[1], y[0], y[1]);
+        slp_apply(c, x[2], x[3], y[2], y[3]);
+    }
+};
+template <>
+struct FilterOp<FILTER_BOTH_DLP> {
+    static constexpr int nin = 5, nout = 5;
+    __device__ static int in(int, int c) { return c; }
+    __device__ static int out(int, int o) { return o; }
+    __device__ static DlpCoef coef(const FilterArgs &a, int j, int k) { return dlp_coef(a, j, k); }
+    __device__ static void apply(const DlpCoef &c, const double2 *x, double2 *y)
+    {
+        dlp_apply(c, x[0], x[1], x[2], y[0], y[1]);
+        dlp_adj_apply(c, x[3], x[4], y[2], y[3], y[4]);
+    }
+};
+
+// The filter on spectra in place, one thread per half-spectrum index; each
+// thread reads its element of every input of a right-hand side before it
+// writes any output.
+template <int FK>
+__global__ void k_filter(FilterArgs a)
 {
+    using Op = FilterOp<FK>;
     const int Mh = a.My / 2 + 1;
     const int64_t n = (int64_t)a.Mx * Mh;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    double2 *S = a.spec + p;
-    const int64_t ss = a.sstride;
-    if constexpr (KIND == 0) {
-        const int j = (int)(p / Mh), k = (int)(p % Mh);
-        const double k1 = wavenumber(j, a.Mx, a.Lx);
-        const double k2 = 2.0 * SE2P_PI / a.Ly * k;
-        const double K2 = k1 * k1 + k2 * k2;
-        double e = 0.0, a11 = 0.0, a22 = 0.0, a12 = 0.0;
-        if (p != 0) {
-            e = (1.0 / (K2 * K2) + 0.25 / (K2 * a.xi * a.xi)) *
-                exp(-0.25 * (1 - a.eta) / (a.xi * a.xi) * K2) * a.scale;
-            const bool nyx = (2 * j == a.Mx), nyy = (2 * k == a.My);
-            a11 = k2 * k2;
-            a22 = k1 * k1;
-            a12 = (nyx != nyy) ? 0.0 : k1 * k2;
-        }
+    const auto c = Op::coef(a, (int)(p / Mh), (int)(p % Mh));
+    for (int r = 0; r < a.nrhs; ++r) {
+        double2 x[Op::nin], y[Op::nout];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const double2 h1 = S[(2 * r) * ss], h2 = S[(2 * r + 1) * ss];
-            S[(2 * r) * ss] = make_double2((a11 * h1.x - a12 * h2.x) * e, (a11 * h1.y - a12 * h2.y) * e);
-            S[(2 * r + 1) * ss] = make_double2((a22 * h2.x - a12 * h1.x) * e, (a22 * h2.y - a12 * h1.y) * e);
-        }
-    } else {
-        const DlpCoef c = dlp_coef(a, (int)(p / Mh), (int)(p % Mh));
-        const double2 A = S[0], B = S[ss], C = S[2 * ss], Z1 = S[3 * ss], Z2 = S[4 * ss];
-        const double o1r = c.et * (c.b111 * A.x + c.b112 * B.x + c.b122 * C.x);
-        const double o1i = c.et * (c.b111 * A.y + c.b112 * B.y + c.b122 * C.y);
-        const double o2r = c.et * (c.b112 * A.x + c.b122 * B.x + c.b222 * C.x);
-        const double o2i = c.et * (c.b112 * A.y + c.b122 * B.y + c.b222 * C.y);
-        const double p1r = c.et * (c.b111 * Z1.x + c.b112 * Z2.x);
-        const double p1i = c.et * (c.b111 * Z1.y + c.b112 * Z2.y);
-        const double p2r = c.et * (c.b112 * Z1.x + c.b122 * Z2.x);
-        const double p2i = c.et * (c.b112 * Z1.y + c.b122 * Z2.y);
-        const double p3r = c.et * (c.b122 * Z1.x + c.b222 * Z2.x);
-        const double p3i = c.et * (c.b122 * Z1.y + c.b222 * Z2.y);
-        S[0] = make_double2(o1i, -o1r);
-        S[ss] = make_double2(o2i, -o2r);
-        S[2 * ss] = make_double2(-p1i, p1r);
-        S[3 * ss] = make_double2(-p2i, p2r);
-        S[4 * ss] = make_double2(-p3i, p3r);
+        for (int i = 0; i < Op::nin; ++i) x[i] = a.spec[(int64_t)Op::in(r, i) * a.sstride + p];
+        Op::apply(c, x, y);
+#pragma unroll
+        for (int o = 0; o < Op::nout; ++o) a.spec[(int64_t)Op::out(r, o) * a.sstride + p] = y[o];
     }
 }
 
@@ -1802,8 +1827,6 @@ static int spread(se2p_ctx *ctx, const KLayout &L, const GridPass &gp, const Spr
     return 0;
 }
 
-enum FilterKind { FILTER_SLP, FILTER_DLP, FILTER_DLP_ADJ, FILTER_BOTH_SLP, FILTER_BOTH_DLP };
-
 // Forward transforms, filter, inverse transforms; grids in, filtered real
 // grids out (spec is scratch).
 static int transform_filter(se2p_ctx *ctx, const KLayout &L, const KGeom &g, FftPlans *plans,
@@ -1833,11 +1856,11 @@ static int transform_filter(se2p_ctx *ctx, const KLayout &L, const KGeom &g, Fft
     fa.eta = g.eta;
     fa.scale = 1.0 / ((double)L.Mx * (double)L.My);
     const unsigned fblocks = (unsigned)((L.GH + 255) / 256);
-    if (fk == FILTER_SLP) hipLaunchKernelGGL(k_filter_slp, dim3(fblocks), dim3(256), 0, st, fa);
-    else if (fk == FILTER_DLP) hipLaunchKernelGGL(k_filter_dlp, dim3(fblocks), dim3(256), 0, st, fa);
-    else if (fk == FILTER_DLP_ADJ) hipLaunchKernelGGL(k_filter_dlp_adj, dim3(fblocks), dim3(256), 0, st, fa);
-    else if (fk == FILTER_BOTH_SLP) hipLaunchKernelGGL(k_filter_both<0>, dim3(fblocks), dim3(256), 0, st, fa);
-    else hipLaunchKernelGGL(k_filter_both<1>, dim3(fblocks), dim3(256), 0, st, fa);
+    if (fk == FILTER_SLP) hipLaunchKernelGGL(k_filter<FILTER_SLP>, dim3(fblocks), dim3(256), 0, st, fa);
+    else if (fk == FILTER_DLP) hipLaunchKernelGGL(k_filter<FILTER_DLP>, dim3(fblocks), dim3(256), 0, st, fa);
+    else if (fk == FILTER_DLP_ADJ) hipLaunchKernelGGL(k_filter<FILTER_DLP_ADJ>, dim3(fblocks), dim3(256), 0, st, fa);
+    else if (fk == FILTER_BOTH_SLP) hipLaunchKernelGGL(k_filter<FILTER_BOTH_SLP>, dim3(fblocks), dim3(256), 0, st, fa);
+    else hipLaunchKernelGGL(k_filter<FILTER_BOTH_DLP>, dim3(fblocks), dim3(256), 0, st, fa);
     SE2P_HIP(hipGetLastError());
     stage_end(ctx, ST_FILTER);
 
@@ -2623,7 +2646,7 @@ static void launch_interp_adj(hipStream_t st, const KLayout &L, const GridPass &
 // passes share their points, so they run as one.  The densities' ncomp
 // gridding columns and du's two are spread together (one sort and gather, one
 // spreading of ncomp + 2 grids), one batched transform pair carries both
-// filters (k_filter_both), and the interpolation forms the target part from
+// filters (k_filter<FILTER_BOTH_*>), and the interpolation forms the target part from
 // grids 0, 1 and the source part from grids 2.. .
 static int kspace_adjoint_same(se2p_ctx *ctx, int kind, const KGeom &g, const double *f,
                                const double *nv, const double *du, double *dpsrc, double *dptar,
