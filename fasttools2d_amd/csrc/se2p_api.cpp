@@ -245,25 +245,15 @@ private:
     se2p_ctx *ctx = ctx_;   // for SE2P_HIP
 };
 
-int ensure_slots(se2p_ctx *ctx, int n, bool own_queue)
+int ensure_slots(se2p_ctx *ctx, int n, bool group)
 {
-    std::vector<se2p_ctx::Slot> &pool = own_queue ? ctx->gslots : ctx->slots;
+    std::vector<se2p_ctx::Slot> &pool = group ? ctx->gslots : ctx->slots;
     if ((int)pool.size() < n)
         if (int rc = refuse_if_frozen(ctx, "creating stream slots", sizeof(int) * kFsMaxKeys)) return rc;
     if (!ctx->fork_ev) SE2P_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     while ((int)pool.size() < n) {
         se2p_ctx::Slot sl;
-        if (own_queue) {
-            // A stream created with a CU mask gets a hardware queue of its
-            // own; ordinary streams share the runtime's few queues (four by
-            // default) round-robin, and two streams on one queue run one
-            // after the other.  The mask names every CU.
-            std::vector<uint32_t> mask((size_t)(ctx->ncu + 31) / 32, 0u);
-            for (int c = 0; c < ctx->ncu; ++c) mask[(size_t)c / 32] |= 1u << (c % 32);
-            SE2P_HIP(hipExtStreamCreateWithCUMask(&sl.st, (uint32_t)mask.size(), mask.data()));
-        } else {
-            SE2P_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
-        }
+        SE2P_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
         SE2P_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
         const size_t fs_bytes = sizeof(int) * kFsMaxKeys;
         SE2P_HIP(hipMalloc((void **)&sl.fs, fs_bytes));
@@ -333,7 +323,7 @@ private:
 };
 
 // The Fourier-space sum `fourier` on the context stream and the real-space
-// sum `real` beside it on own-queue slot 0, as the two calls of a concurrent
+// sum `real` beside it on group slot 0, as the two calls of a concurrent
 // group (Fourier-space call first) would run; joined into the context stream
 // also after an error, so that the stream stays ordered.
 template <class F, class R>

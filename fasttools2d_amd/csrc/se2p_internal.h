@@ -81,9 +81,8 @@ struct se2p_ctx {
         int *fs = nullptr;
         hipEvent_t done = nullptr;
     };
-    std::vector<Slot> slots;    // multi-box Fourier sums: ordinary streams
-    std::vector<Slot> gslots;   // concurrent groups and driver calls: streams
-                                // with hardware queues of their own
+    std::vector<Slot> slots;    // multi-box Fourier sums
+    std::vector<Slot> gslots;   // concurrent groups and driver calls
     hipEvent_t fork_ev = nullptr;
     hipEvent_t switch_ev = nullptr;   // orders a newly bound stream after the previous one
     // se2p_group_begin .. se2p_group_end: -1 outside a group, else the number
@@ -120,17 +119,15 @@ int refuse_if_frozen(se2p_ctx *ctx, const char *what, size_t bytes);
 int ensure_workspace(se2p_ctx *ctx, size_t bytes);
 
 // Stream slots: at least n of them, each with its stream, event and sort
-// state.  Multi-box Fourier-space sums use kKspaceSlots ordinary streams
-// (their many short launches, forked and joined often, run best on the
-// runtime's shared queues); concurrent groups and the driver entries use
-// `own_queue` slots (ctx->gslots), whose streams get hardware queues of
-// their own so that their work really runs beside the context stream's.
-int ensure_slots(se2p_ctx *ctx, int n, bool own_queue = false);
+// state.  Multi-box Fourier-space sums use kKspaceSlots of ctx->slots;
+// concurrent groups and the driver entries use ctx->gslots (`group`), so
+// that neither overwrites the other's workspace or plans.
+int ensure_slots(se2p_ctx *ctx, int n, bool group = false);
 constexpr int kKspaceSlots = 4;
 
-inline se2p_ctx::Slot &slot_at(se2p_ctx *ctx, int k, bool own_queue = false)
+inline se2p_ctx::Slot &slot_at(se2p_ctx *ctx, int k, bool group = false)
 {
-    return own_queue ? ctx->gslots[(size_t)k] : ctx->slots[(size_t)k];
+    return group ? ctx->gslots[(size_t)k] : ctx->slots[(size_t)k];
 }
 
 // The context's stream, workspace, sort state and plan set swapped for slot
@@ -143,8 +140,8 @@ struct SlotScope {
     size_t ws_bytes;
     int *fs;
     bool timing;
-    SlotScope(se2p_ctx *c, int k, bool own_queue = false)
-        : ctx(c), slot(slot_at(c, k, own_queue)), st(c->stream), ws(c->ws), ws_bytes(c->ws_bytes),
+    SlotScope(se2p_ctx *c, int k, bool group = false)
+        : ctx(c), slot(slot_at(c, k, group)), st(c->stream), ws(c->ws), ws_bytes(c->ws_bytes),
           fs(c->fs), timing(c->timing)
     {
         ctx->stream = slot.st;
@@ -152,7 +149,7 @@ struct SlotScope {
         ctx->ws_bytes = slot.ws_bytes;
         ctx->fs = slot.fs;
         ctx->timing = false;
-        ctx->plan_slot = own_queue ? kKspaceSlots + 1 + k : k + 1;
+        ctx->plan_slot = group ? kKspaceSlots + 1 + k : k + 1;
     }
     ~SlotScope()
     {
