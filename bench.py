@@ -254,7 +254,8 @@ def _fixed_grid_variant(args, pe, np, torch, w, ps, f, nv, stream, fwd_ms):
     k_inf = 256 and the cells from their real-space estimate (the headline
     uses the drivers' cell size with the grid raised to 512^2, which their
     own rule would size at 368^2).  Step time and error against the FP64
-    direct Ewald truth on 256 targets (the total is independent of xi)."""
+    direct Ewald truth on 256 targets (the total is independent of xi).  The
+    step is formed as the headline step."""
     from fasttools2d_amd import params as Pm
 
     N = len(w.psrc)
@@ -263,10 +264,22 @@ def _fixed_grid_variant(args, pe, np, torch, w, ps, f, nv, stream, fwd_ms):
     ur = torch.empty((N, 2), dtype=torch.float64, device=ps.device)
     uk = torch.empty_like(ur)
 
-    def step():
+    def real():
         pe.mex_stokes_dlp_real(ps, ps, f, nv, p.xi, p.nside_x, p.nside_y, 1.0, 1.0, ctx=ctx, out=ur)
+
+    def fourier():
         pe.mex_stokes_dlp_kspace(ps, ps, p.xi, p.eta, f, nv, p.Mx, p.My, 1.0, 1.0, p.w, p.P,
                                  ctx=ctx, out=uk)
+
+    def step():
+        # the headline's step form
+        if args.sequential:
+            real()
+            fourier()
+        else:
+            with ctx.group():
+                fourier()
+                real()
 
     with torch.cuda.stream(stream):
         ctx.bind_torch_stream(stream)
