@@ -4,13 +4,15 @@ table-driven exp(-x) used by the real-space kernels (se2p_exp_neg).
 
 Method (Tang, "Table-driven implementation of the exponential function in
 IEEE floating-point arithmetic", ACM TOMS 15, 1989): with N = 256,
-k = rint(x N / ln 2) and r = x - k ln2/N, so |r| <= ln2/(2N) = 1.35e-3, and
+k = rint(-x N / ln 2) <= 0 and r = x + k ln2/N, so |r| <= ln2/(2N) =
+1.35e-3, and, with k = N floor(k/N) + j, 0 <= j < N,
 
-    exp(-x) = 2^-(k div N) * 2^-((k mod N)/N) * exp(-r).
+    exp(-x) = 2^floor(k/N) * 2^(j/N) * exp(-r).
 
-The table holds 2^(-j/N), j = 0..N-1, correctly rounded to double; ln2/N is
+The table holds 2^(j/N), j = 0..N-1, correctly rounded to double; ln2/N is
 split into a double and the double nearest its remainder (each product
-k * part is formed exactly inside a fused multiply-add).  exp(-r) is the
+k * part is formed exactly inside a fused multiply-add).  k is formed as
+fma(x, -N/ln2, 1.5 * 2^52), whose low word then holds k.  exp(-r) is the
 degree-4 Taylor polynomial, whose truncation |r|^5/5! is below 3.8e-17.
 Values are computed with mpmath at 200 bits.
 
@@ -31,7 +33,7 @@ def main() -> None:
     scale = float(N / ln2)
     hi = float(ln2 / N)
     lo = float(ln2 / N - mp.mpf(hi))
-    tab = [float(mp.power(2, -mp.mpf(j) / N)) for j in range(N)]
+    tab = [float(mp.power(2, mp.mpf(j) / N)) for j in range(N)]
     lines = [
         "// exp2_table.h -- generated by gen_exp_table.py; do not edit.",
         "// Table and reduction constants of se2p_exp_neg (se2p_device.h).",
@@ -41,7 +43,8 @@ def main() -> None:
         f"#define SE2P_EXP2_SCALE {scale!r}   // N / ln 2",
         f"#define SE2P_EXP2_LN2N_HI {hi!r}   // ln 2 / N, leading part",
         f"#define SE2P_EXP2_LN2N_LO {lo!r}   // and the remainder",
-        "// 2^(-j/N), j = 0..N-1, correctly rounded",
+        "#define SE2P_EXP2_SHIFTER 6755399441055744.0   // 1.5 * 2^52",
+        "// 2^(j/N), j = 0..N-1, correctly rounded",
         "#define SE2P_EXP2_TABLE \\",
         "    { \\",
     ]

@@ -31,22 +31,26 @@ __host__ __device__ inline double se2p_sqsum(double dx, double dy)
 }
 
 // exp(-x) for finite x >= 0, table driven (Tang's method, N = 256 table
-// entries, exp2_table.h): k = rint(x N / ln 2), r = x - k ln2/N with ln2/N
-// split into a double and its remainder (each product exact inside its
-// fused multiply-add), so |r| <= ln2/(2N); then
-// exp(-x) = 2^-(k>>8) * 2^-((k&255)/256) * exp(-r), exp(-r) by its degree-4
-// Taylor polynomial (truncation |r|^5/5! < 3.8e-17).  A few ulp; the sums
-// that use it are compared with libm's exp at 1e-12.
-// tab256 = SE2P_EXP2_TABLE (kernels hold it in LDS).
+// entries, exp2_table.h).  k = rint(-x N / ln 2) <= 0 comes from one fused
+// multiply-add onto 1.5 * 2^52, which leaves k in the low word of the sum
+// (no separate rounding or conversion); r = x + k ln2/N with ln2/N split
+// into a double and its remainder (each product exact inside its fused
+// multiply-add), so |r| <= ln2/(2N); then, with k = N floor(k/N) + j,
+// 0 <= j < N,  exp(-x) = 2^floor(k/N) * 2^(j/N) * exp(-r), exp(-r) by its
+// degree-4 Taylor polynomial in Horner form (truncation |r|^5/5! < 3.8e-17).
+// A few ulp; the sums that use it are compared with libm's exp at 1e-12.
+// tab256 = SE2P_EXP2_TABLE, 2^(j/N) (kernels hold it in LDS).
 __host__ __device__ inline double se2p_exp_neg(double x, const double *tab256)
 {
-    const double k = rint(x * SE2P_EXP2_SCALE);
-    double r = fma(-k, SE2P_EXP2_LN2N_HI, x);
-    r = fma(-k, SE2P_EXP2_LN2N_LO, r);
-    const double r2 = r * r;
-    const double p = fma(r2, fma(r2, 1.0 / 24.0, fma(r, -1.0 / 6.0, 0.5)), 1.0 - r);
-    const int ki = (int)k;
-    return ldexp(tab256[ki & (SE2P_EXP2_N - 1)] * p, -(ki >> 8));
+    const double kd = fma(x, -SE2P_EXP2_SCALE, SE2P_EXP2_SHIFTER);
+    const double k = kd - SE2P_EXP2_SHIFTER;
+    uint64_t kb;
+    __builtin_memcpy(&kb, &kd, sizeof(kb));
+    const int ki = (int)(uint32_t)kb;
+    double r = fma(k, SE2P_EXP2_LN2N_HI, x);
+    r = fma(k, SE2P_EXP2_LN2N_LO, r);
+    const double p = fma(fma(fma(fma(r, 1.0 / 24.0, -1.0 / 6.0), r, 0.5), r, -1.0), r, 1.0);
+    return ldexp(tab256[ki & (SE2P_EXP2_N - 1)] * p, ki >> 8);
 }
 
 // se2p_sqsum that also returns the two squares (same rounding).

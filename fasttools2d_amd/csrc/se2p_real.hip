@@ -44,7 +44,7 @@
 namespace se2p {
 
 static __constant__ double c_e1_table[SE2P_E1_NPIECE * SE2P_E1_ROW];
-// 2^(-j/256), the table of se2p_exp_neg; each workgroup copies it to LDS
+// 2^(j/256), the table of se2p_exp_neg; each workgroup copies it to LDS
 static __constant__ double c_exp2_table[SE2P_EXP2_N] = SE2P_EXP2_TABLE;
 
 // ---- fixed scratch of the no-allocation entry point (se2p_dlp_real_noalloc)
