@@ -313,6 +313,19 @@ def test_frozen_flag_setter(hctx):
     assert lib.se2p_set_frozen(None, 1) == _capi.SE2P_ERR_ARG
 
 
+def test_group_entry_validation(hctx):
+    """se2p_group_begin / se2p_group_end on a context without a device, and
+    their null-context and not-open errors (the GPU behaviour is in
+    tests/test_gpu_group.py)."""
+    lib, h = hctx
+    assert lib.se2p_group_end(h) == _capi.SE2P_ERR_ARG
+    assert b"no group is open" in lib.se2p_last_error(h)
+    assert lib.se2p_group_begin(h) == _capi.SE2P_ERR_HIP
+    assert lib.se2p_group_end(h) == _capi.SE2P_ERR_ARG   # the failed begin opened nothing
+    assert lib.se2p_group_begin(None) == _capi.SE2P_ERR_ARG
+    assert lib.se2p_group_end(None) == _capi.SE2P_ERR_ARG
+
+
 def test_python_out_argument_checked():
     """out= is written through its data pointer by the library, so anything but
     a contiguous float64 (Nt, 2) tensor on the inputs' device is refused."""
