@@ -924,9 +924,12 @@ __global__ __launch_bounds__(256, KIND == 1 ? 8 : 5) void k_real_win(RealArgs a,
                     v1 += red[k & 1][q][1][lane];
                 }
                 if (sl >= 0) {
+                    // the lane index recomputed here (mbcnt): a lane offset
+                    // kept live across the item loop spilled
+                    const int lo = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
                     double *pp = w.part + (int64_t)(sl + k) * 128;
-                    pp[lane] = v0;
-                    pp[64 + lane] = v1;
+                    pp[lo] = v0;
+                    pp[64 + lo] = v1;
                 } else {
                     tot0 += v0;
                     tot1 += v1;
