@@ -388,8 +388,13 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
     for (int i = threadIdx.x; i < SE2P_EXP2_N; i += blockDim.x) etab[i] = c_exp2_table[i];
     __syncthreads();
 
-    const int b = xcd_chunk(a.chunk_off[a.ncell]);
-    if (b < 0) return;   // uniform over the workgroup
+    // workgroup -> (chunk, group of RB right-hand sides), groups fastest: a
+    // chunk's groups are dispatched together, so the launch does not end on
+    // the heavy chunks of the last group (5% on 64 densities)
+    const int ngroup = (a.r_end - a.r_begin) / RB;
+    const int b = (int)blockIdx.x / ngroup;
+    const int grp = (int)blockIdx.x - b * ngroup;
+    if (b >= a.chunk_off[a.ncell]) return;   // uniform over the workgroup
     const int cell = a.block_cell[b];
     const RealBox &B = a.boxes.get(a.cell_box ? a.cell_box[cell] : 0);
     const int nsx = B.nsx, nsy = B.nsy, cbase = B.cell_base;
@@ -422,9 +427,8 @@ __global__ __launch_bounds__(256) void k_real(RealArgs a, const double2 *__restr
     const unsigned long long keep1 =
         keep_mask(sbox, 1, lane, cbase, cx, cy, nsx, nsy, Lx, Ly, lim, bx0, by0, bx1, by1);
 
-    // blockIdx.y selects this workgroup's group of RB right-hand sides
-    for (int r0 = a.r_begin + blockIdx.y * RB; r0 < a.r_begin + (blockIdx.y + 1) * RB &&
-                                               r0 + RB <= a.r_end; r0 += RB) {
+    for (int r0 = a.r_begin + grp * RB; r0 < a.r_begin + (grp + 1) * RB && r0 + RB <= a.r_end;
+         r0 += RB) {
         double acc[RB][2];
 #pragma unroll
         for (int r = 0; r < RB; ++r) acc[r][0] = acc[r][1] = 0.0;
@@ -2325,7 +2329,6 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
     const double *q = p.q, *rec = p.rec;
     const int *soff = p.kss.off;
     const double4 *sbox = p.sbox;
-    const unsigned grid8 = p.grid8;
 
     stage_begin(ctx, ST_REAL);
     if (counting) {
@@ -2338,7 +2341,7 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
         if (n8 > 0) {
             a.r_begin = 0;
             a.r_end = n8;
-            const dim3 grid(grid8, n8 / 8);
+            const dim3 grid((unsigned)(p.max_chunks * (n8 / 8)));
             if (kind == 0) hipLaunchKernelGGL((k_real<0, 8>), grid, dim3(256), 0, st, a, spos, q, soff, sbox);
             else hipLaunchKernelGGL((k_real<1, 8>), grid, dim3(256), 0, st, a, spos, q, soff, sbox);
         }
@@ -2346,14 +2349,14 @@ int real_sum_boxes(se2p_ctx *ctx, int kind, RealBox *boxes, int nbox, int nrhs, 
         if (n4 > n8) {
             a.r_begin = n8;
             a.r_end = n4;
-            const dim3 grid(grid8, (n4 - n8) / 4);
+            const dim3 grid((unsigned)(p.max_chunks * ((n4 - n8) / 4)));
             if (kind == 0) hipLaunchKernelGGL((k_real<0, 4>), grid, dim3(256), 0, st, a, spos, q, soff, sbox);
             else hipLaunchKernelGGL((k_real<1, 4>), grid, dim3(256), 0, st, a, spos, q, soff, sbox);
         }
         if (n4 < nrhs) {
             a.r_begin = n4;
             a.r_end = nrhs;
-            const dim3 grid(grid8, nrhs - n4);
+            const dim3 grid((unsigned)(p.max_chunks * (nrhs - n4)));
             if (rec) {   // single density: source records, windows of the chunks
                 const WinPlan &wp = p.win;
                 const unsigned cgrid = (unsigned)((p.max_chunks + kWaves - 1) / kWaves);
