@@ -738,10 +738,15 @@ __device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b
 // Strip spreading on grids with Mx, My >= P + 16 uses the matrix cores.
 constexpr bool kSpreadMfma = true;
 
+// One-dimensional grid of max_chunks x groups, groups fastest: a chunk's
+// groups of grids are dispatched together, so with several groups (batched
+// densities) the launch does not end on the heavy tiles of the last group.
 template <int NQ, bool WIDE>
 __global__ __launch_bounds__(256) void k_spread_strip(SpreadArgs a)
 {
-    spread_strip_wg<NQ, WIDE, WIDE && kSpreadMfma>(a, blockIdx.x, blockIdx.y);
+    const int ngrp = (int)(gridDim.x / (unsigned)a.max_chunks);
+    const int b = (int)blockIdx.x / ngrp;
+    spread_strip_wg<NQ, WIDE, WIDE && kSpreadMfma>(a, b, (int)blockIdx.x - b * ngrp);
 }
 
 // Many boxes in one launch: box j owns workgroups [off[j], off[j+1]) and each
@@ -1801,12 +1806,14 @@ static int spread(se2p_ctx *ctx, const KLayout &L, const GridPass &gp, const Spr
     const bool wide = L.Mx >= L.P + 16 && L.My >= L.P + 16;
 #define SE2P_SPREAD(NQV)                                                                           \
     if (gp.strip && wide) {                                                                        \
-        hipLaunchKernelGGL((k_spread_strip<NQV, true>), dim3((unsigned)gp.max_spread_chunks,       \
-                           gp.ngrp), dim3(256), gp.lds_strip, st, sa);                             \
+        hipLaunchKernelGGL((k_spread_strip<NQV, true>),                                            \
+                           dim3((unsigned)(gp.max_spread_chunks * gp.ngrp)), dim3(256), gp.lds_strip, \
+                           st, sa);                                                                \
         hipLaunchKernelGGL((k_spread_combine<NQV>), dim3(L.ntiles, gp.ngrp), dim3(64), 0, st, sa); \
     } else if (gp.strip) {                                                                         \
-        hipLaunchKernelGGL((k_spread_strip<NQV, false>), dim3((unsigned)gp.max_spread_chunks,      \
-                           gp.ngrp), dim3(256), gp.lds_strip, st, sa);                             \
+        hipLaunchKernelGGL((k_spread_strip<NQV, false>),                                           \
+                           dim3((unsigned)(gp.max_spread_chunks * gp.ngrp)), dim3(256), gp.lds_strip, \
+                           st, sa);                                                                \
         hipLaunchKernelGGL((k_spread_combine<NQV>), dim3(L.ntiles, gp.ngrp), dim3(64), 0, st, sa); \
     } else if (L.wrap) {                                                                           \
         hipLaunchKernelGGL((k_spread<NQV, true>), tiles2d, dim3(256), gp.lds_gen, st, sa);        \
