@@ -1305,6 +1305,7 @@ struct InterpTileArgs {
     double *u;
     int64_t ustride;
     double *jac;           // DERIV: [4 nt] output gradients, caller order
+    int64_t max_chunks;    // workgroups per right-hand side (k_interp_tile's grid)
 };
 
 // PT > 0: the support P = PT is a compile-time constant; each row of the
@@ -1409,10 +1410,14 @@ __device__ __forceinline__ void interp_tile_wg(const InterpTileArgs &a, const in
     }
 }
 
+// One-dimensional grid of max_chunks x right-hand sides, right-hand sides
+// fastest (as k_spread_strip's groups).
 template <int PT, bool DERIV>
 __global__ __launch_bounds__(256) void k_interp_tile(InterpTileArgs a)
 {
-    interp_tile_wg<PT, DERIV>(a, blockIdx.x, blockIdx.y);
+    const int nrhs = (int)(gridDim.x / (unsigned)a.max_chunks);
+    const int b = (int)blockIdx.x / nrhs;
+    interp_tile_wg<PT, DERIV>(a, b, (int)blockIdx.x - b * nrhs);
 }
 
 struct InterpPack {
@@ -2093,6 +2098,7 @@ static InterpTileArgs interp_tile_args(const KRun &r)
     ia.u = r.u;
     ia.ustride = r.ustride;
     ia.jac = r.jac;
+    ia.max_chunks = r.gp.max_interp_chunks;
     return ia;
 }
 
@@ -2106,13 +2112,13 @@ static int krun_interp(se2p_ctx *ctx, const KRun &r)
     if (L.itile) {
         const InterpTileArgs ia = interp_tile_args(r);
         if (L.P == 24 && r.jac)
-            hipLaunchKernelGGL((k_interp_tile<24, true>), dim3((unsigned)gp.max_interp_chunks, nrhs),
+            hipLaunchKernelGGL((k_interp_tile<24, true>), dim3((unsigned)(gp.max_interp_chunks * nrhs)),
                                dim3(256), L.lds_itile, st, ia);
         else if (L.P == 24)
-            hipLaunchKernelGGL((k_interp_tile<24, false>), dim3((unsigned)gp.max_interp_chunks, nrhs),
+            hipLaunchKernelGGL((k_interp_tile<24, false>), dim3((unsigned)(gp.max_interp_chunks * nrhs)),
                                dim3(256), L.lds_itile, st, ia);
         else
-            hipLaunchKernelGGL((k_interp_tile<0, false>), dim3((unsigned)gp.max_interp_chunks, nrhs),
+            hipLaunchKernelGGL((k_interp_tile<0, false>), dim3((unsigned)(gp.max_interp_chunks * nrhs)),
                                dim3(256), L.lds_itile, st, ia);
     } else {
         for (int rr = 0; rr < nrhs; ++rr) {
