@@ -496,10 +496,14 @@ __device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b
         while (rpref[rcur + 1] <= base) ++rcur;
         if (WIDE) {
             // Fill with every lane: LPR lanes per weight row (rows 0..kSB-1:
-            // x rows, then the y rows), each lane one segment of the row,
+            // x rows, then the y rows), each lane one segment of the part of
+            // the row that the tile uses (support nodes d .. d + kTile - 1
+            // inside [0, P], d the source's tile offset on that axis),
             // starting from e3^j0 by squaring; the first lane of an x row
-            // also writes the source's tile offsets and densities.  The y
-            // rows' zero padding was written once at the start.
+            // also writes the source's tile offsets and densities.  Row
+            // entries outside that part keep older values: the product loops
+            // read them only under a false support test, which selects 0.
+            // The y rows' zero padding was written once at the start.
             constexpr int LPR = 64 / (2 * kSB);
             const int row = lane / LPR, part = lane % LPR;
             const bool isx = row < kSB;
@@ -509,8 +513,16 @@ __device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b
                 int r = rcur;
                 while (rpref[r + 1] <= v) ++r;
                 const int k = rstart[r] + (v - rpref[r]);
-                const int seg = (P1 + LPR - 1) / LPR;
-                const int j0 = part * seg, j1 = min(P1, j0 + seg);
+                const int2 an = a.s.anchor[k];
+                // tile-relative offset d = (T0 - anchor) mod M taken in (P - M, P]:
+                // grid line T0 + l is node l + d of the support
+                const int M = isx ? a.Mx : a.My;
+                int d = (isx ? X0 - an.x : Y0 - an.y);
+                d += d < 0 ? M : 0;
+                d = d <= a.P ? d : d - M;
+                constexpr int seg = (kTile + LPR - 1) / LPR;
+                const int hi = min(a.P + 1, d + kTile);
+                const int j0 = max(d, 0) + part * seg, j1 = min(hi, j0 + seg);
                 const double e3 = a.s.fgg[(isx ? 1 : 2) * a.s.ns + k];
                 double w = isx ? a.s.fgg[k] : 1.0;
                 double pw = e3;
@@ -524,13 +536,9 @@ __device__ __forceinline__ void spread_strip_wg(const SpreadArgs &a, const int b
                     w *= e3;
                 }
                 if (isx && part == 0) {
-                    // tile-relative offsets d = (X0 - ax) mod Mx taken in
-                    // (P - Mx, P]: column X0 + l is node l + d of the support
-                    const int2 an = a.s.anchor[k];
-                    int d = X0 - an.x, e = Y0 - an.y;
-                    d += d < 0 ? a.Mx : 0;
+                    int e = Y0 - an.y;
                     e += e < 0 ? a.My : 0;
-                    AXY[2 * sl] = d <= a.P ? d : d - a.Mx;
+                    AXY[2 * sl] = d;
                     AXY[2 * sl + 1] = e <= a.P ? e : e - a.My;
 #pragma unroll
                     for (int c = 0; c < NQ; ++c) Q[sl * NQ + c] = qg[(int64_t)c * a.s.ns + k];
