@@ -764,7 +764,7 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
+        print(json.dumps({"error": "the spectral Ewald path runs on one GPU; run with --gpus 1",
                           "n_gpus": args.gpus}), flush=True)
         return 2
 
